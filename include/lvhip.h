@@ -1,0 +1,188 @@
+/*
+ * lvhip.h -- C ABI of liblvhip.so, the MI355X (gfx950) implementation of LagrangianVoronoi.jl's
+ * per-step hot path: remesh (cell list + Voronoi clipping), the pressure Poisson solve and the
+ * particle updates.
+ *
+ * Conventions
+ *   - Every entry point returns an int status (LV_OK = 0).  No C++ exception crosses the boundary.
+ *   - Particle labels are 0-based.  Boundary edge labels are exactly the reference's
+ *     BDARY_UP = -1, BDARY_RIGHT = -2, BDARY_DOWN = -3, BDARY_LEFT = -4 (src/polygon.jl:4-7).
+ *   - Host arrays are structure-of-arrays planes: a 2-vector field of n particles is
+ *     double[2*n] with component k of particle i at [k*n + i]; the 2x2 tensor D is double[4*n] in
+ *     Julia's column-major order (D11, D21, D12, D22).
+ *   - Host buffers are copied during the call and never retained.  Device memory belongs to the
+ *     context.  A context is not thread-safe; it owns one HIP stream of default priority.
+ *   - Citations are file:line of OndrejKincl/LagrangianVoronoi.jl v0.1.0.
+ */
+#ifndef LVHIP_H
+#define LVHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------------------------------- */
+#define LV_OK 0
+#define LV_E_MESH_DESTROYED 1 /* throw("The Voronoi Mesh has been destroyed."), src/voronoigrid.jl:63-65 */
+#define LV_E_NAN_VELOCITY 2   /* throw("Velocity field invalidated."), src/move.jl:24-26 */
+#define LV_E_POLY_OVERFLOW 3  /* a polygon needs more than LV_MAX_EDGES edges */
+#define LV_E_NOT_CONVERGED 4  /* pressure solve missed its tolerance (the reference ignores this) */
+#define LV_E_HIP 5            /* HIP runtime error; lv_last_error() has the text */
+#define LV_E_INVALID 6        /* invalid argument */
+#define LV_E_NOMEM 7          /* device allocation failed */
+#define LV_E_NO_MESH 8        /* an operation needs a mesh and remesh has not run since x changed */
+
+#define LV_MAX_EDGES 16       /* edges a polygon may have (the reference's FastVector grows freely) */
+
+/* ---- field ids for lv_upload / lv_download (PolygonNS fields, src/celldefs.jl:7-39) ------------ */
+#define LV_FIELD_X 0         /* double[2n]  generating seed x */
+#define LV_FIELD_V 1         /* double[2n]  velocity v */
+#define LV_FIELD_DV 2        /* double[2n]  repair velocity dv */
+#define LV_FIELD_MOMENTUM 3  /* double[2n] */
+#define LV_FIELD_RHO 4       /* double[n] */
+#define LV_FIELD_E 5         /* double[n]   specific energy */
+#define LV_FIELD_P 6         /* double[n] */
+#define LV_FIELD_C2 7        /* double[n] */
+#define LV_FIELD_MASS 8      /* double[n] */
+#define LV_FIELD_ENERGY 9    /* double[n] */
+#define LV_FIELD_QUALITY 10  /* double[n] */
+#define LV_FIELD_MU 11       /* double[n] */
+#define LV_FIELD_D 12        /* double[4n] */
+#define LV_FIELD_PHASE 13    /* int32[n] */
+#define LV_FIELD_AREA 14     /* double[n]   area(p) of the current mesh (read-only) */
+#define LV_FIELD_CENTROID 15 /* double[2n]  centroid(p) of the current mesh (read-only) */
+#define LV_NFIELDS 16
+
+/* ---- pressure solver methods ------------------------------------------------------------------ */
+#define LV_SOLVER_PCG64 0  /* FP64 Jacobi-PCG */
+#define LV_SOLVER_MIXED 1  /* FP32 Jacobi-PCG inside FP64 iterative refinement */
+#define LV_SOLVER_SPAI 2   /* FP32 PCG with the learned SPAI preconditioner inside FP64 refinement;
+                              falls back to LV_SOLVER_MIXED when it stalls */
+
+typedef struct lv_ctx lv_ctx;
+
+/* VoronoiGrid{T}(boundary_rect, dr; h = 2dr, r_max = 10dr, xperiodic, yperiodic),
+ * src/voronoigrid.jl:26-49.  h <= 0 or r_max <= 0 select the reference defaults. */
+typedef struct {
+    double xmin[2], xmax[2]; /* boundary_rect corners */
+    double dr;
+    double h;
+    double r_max;
+    int32_t xperiodic, yperiodic;
+    int32_t capacity;        /* most particles the context will hold */
+} lv_grid_desc;
+
+typedef struct {
+    int32_t method;       /* LV_SOLVER_* */
+    double tol;           /* stop at ||b - A P||_2 <= tol * ||b||_2 (FP64-evaluated) */
+    int32_t itmax;        /* iteration cap per linear solve (inner iterations for MIXED/SPAI) */
+    double inner_tol;     /* MIXED/SPAI: relative reduction asked of each FP32 inner solve */
+    int32_t outer_max;    /* MIXED/SPAI: cap on refinement sweeps */
+    double vbc[8];        /* boundary_velocity per label: vbc[2*(-label-1) + comp], label -1..-4 */
+} lv_solve_opts;
+
+typedef struct {
+    int32_t iters[32];    /* Krylov iterations of each Picard solve */
+    int32_t outer[32];    /* refinement sweeps of each Picard solve (MIXED/SPAI) */
+    double relres[32];    /* final FP64 relative residual of each Picard solve */
+    double solve_ms[32];  /* device time of each linear solve (HIP events on the context stream) */
+    int32_t total_iters;
+    int32_t fallbacks;    /* SPAI solves that fell back to Jacobi */
+    int32_t nsolves;
+} lv_solve_stats;
+
+/* One step!(sim, t) of tests/taylorgreen.jl:61-71 and its relatives (examples/cavity.jl:70-80,
+ * examples/rayleightaylor.jl:90-101 without the multiphase projection):
+ *   move! -> [gravity_step!] -> [stiffened_eos! | ideal_eos!] -> find_pressure! -> pressure_step!
+ *   -> find_D! -> [viscous_step!] -> [bdary_friction!] -> find_dv! -> relaxation_step! */
+typedef struct {
+    double dt;
+    int32_t eos;          /* 0 none, 1 stiffened_eos!(gamma, P0), 2 ideal_eos!(gamma; Pmin) */
+    double gamma, P0, Pmin;
+    int32_t gravity;      /* 1: gravity_step!(g, dt) after move! */
+    double g[2];
+    int32_t niter;        /* Picard iterations of find_pressure! (reference default 10) */
+    lv_solve_opts solve;
+    int32_t viscous;      /* 1: viscous_step!(dt; artificial_viscosity) */
+    int32_t artificial_viscosity;
+    int32_t friction;     /* 1: bdary_friction!(vDirichlet, dt) with vDirichlet per label */
+    double friction_vbc[8];
+    double alpha;         /* find_dv!(dt, alpha) */
+    int32_t rusanov;      /* relaxation_step!(dt; rusanov) */
+} lv_step_desc;
+
+/* ---- context ---------------------------------------------------------------------------------- */
+int lv_create(const lv_grid_desc *desc, lv_ctx **out);  /* VoronoiGrid ctor, src/voronoigrid.jl:26-49 */
+int lv_destroy(lv_ctx *ctx);
+const char *lv_last_error(lv_ctx *ctx);
+int lv_sync(lv_ctx *ctx);
+/* grid geometry: n1, n2 (cell counts), npath (magic-path nodes within r_max) and
+ * out6 = cropping xmin(2), cropping xmax(2), rr of the first node beyond r_max (-1 if none), rr_max */
+int lv_grid_info(lv_ctx *ctx, int32_t *n1, int32_t *n2, int32_t *npath, double *out6);
+/* magic_path prefix, src/neighborlist.jl:29-41: offsets and rr of the nodes with rr <= rr_max */
+int lv_grid_path(lv_ctx *ctx, int32_t *i1, int32_t *i2, double *rr);
+
+/* ---- particle state --------------------------------------------------------------------------- */
+/* push!(grid.polygons, T(x = x)) for n seeds at once (src/populate.jl pattern): sets the particle
+ * count, uploads x, zeroes every other field (PolygonNS defaults, src/celldefs.jl:7-27) and marks
+ * the mesh stale. */
+int lv_set_particles(lv_ctx *ctx, int32_t n, const double *x);
+int lv_num_particles(lv_ctx *ctx, int32_t *n);
+int lv_upload(lv_ctx *ctx, int32_t field, const void *host);
+int lv_download(lv_ctx *ctx, int32_t field, void *host);
+/* device pointer of a field (for zero-copy interop with other device code on the same stream) */
+int lv_field_device_ptr(lv_ctx *ctx, int32_t field, void **dptr);
+
+/* ---- mesh ------------------------------------------------------------------------------------- */
+int lv_remesh(lv_ctx *ctx);                                  /* remesh!, src/voronoigrid.jl:89-108 */
+int lv_mesh_size(lv_ctx *ctx, int64_t *nedges);
+/* edges of every polygon after sort_edges! (src/IO.jl:35-48), CSR by particle:
+ * edge_ptr[n+1], labels[nedges], verts[4*nedges] = (v1x, v1y, v2x, v2y) per edge */
+int lv_get_mesh(lv_ctx *ctx, int32_t *edge_ptr, int32_t *labels, double *verts);
+/* the cell list of the last remesh (CellList cells, src/neighborlist.jl:11-67): cell c (0-based,
+ * c = (i1-1) + (i2-1)*n1) holds cell_labels[cell_start[c] .. cell_start[c+1]) in ascending order */
+int lv_cell_list_size(lv_ctx *ctx, int64_t *nitems);
+int lv_get_cell_list(lv_ctx *ctx, int32_t *cell_start, int32_t *cell_labels);
+
+/* ---- physics operators (each is the reference function of the same name) ---------------------- */
+int lv_move(lv_ctx *ctx, double dt);                          /* move!, src/move.jl:9-21 (remeshes) */
+int lv_gravity_step(lv_ctx *ctx, const double g[2], double dt);  /* src/pressure.jl:77-82 */
+int lv_find_rho(lv_ctx *ctx);                                 /* src/pressure.jl:36-40 */
+int lv_eos_stiffened(lv_ctx *ctx, double gamma, double P0);   /* src/pressure.jl:64-70 */
+int lv_eos_ideal(lv_ctx *ctx, double gamma, double Pmin);     /* src/pressure.jl:49-55 */
+/* find_pressure!(solver, dt, niter; boundary_velocity), src/pressure.jl:215-224 */
+int lv_find_pressure(lv_ctx *ctx, double dt, int32_t niter, const lv_solve_opts *opts, lv_solve_stats *stats);
+int lv_pressure_step(lv_ctx *ctx, double dt);                 /* src/pressure.jl:10-25 */
+int lv_find_D(lv_ctx *ctx);                                   /* src/diffusion.jl:8-19 */
+int lv_viscous_step(lv_ctx *ctx, double dt, int32_t artificial_viscosity); /* src/diffusion.jl:39-53 */
+/* bdary_friction!(grid, vDirichlet, dt), src/diffusion.jl:64-78; vDirichlet as a per-label table */
+int lv_bdary_friction(lv_ctx *ctx, const double vbc[8], double dt);
+int lv_find_dv(lv_ctx *ctx, double dt, double alpha);         /* src/relaxation.jl:10-25 */
+int lv_relaxation_step(lv_ctx *ctx, double dt, int32_t rusanov); /* src/relaxation.jl:36-73 (remeshes) */
+/* the whole step!, device resident; stats may be NULL */
+int lv_step_fused(lv_ctx *ctx, const lv_step_desc *desc, lv_solve_stats *stats);
+
+/* ---- pressure system access (PressureOperator/PressureSolver, src/pressure.jl:89-203) ---------- */
+/* refresh!(A, grid, dt), src/pressure.jl:104-117, into device CSR; returns nnz of the off-diagonal */
+int lv_assemble_pressure(lv_ctx *ctx, double dt, int64_t *nnz);
+/* the assembled operator: diag[n], row_ptr[n+1], cols[nnz], coef[nnz] (y_i = d_i x_i + sum c_ij (x_i - x_j));
+ * cols and coef may be NULL (nnz = row_ptr[n]) */
+int lv_get_pressure_operator(lv_ctx *ctx, double *diag, int32_t *row_ptr, int32_t *cols, double *coef);
+/* refresh!(solver, dt, gp_step, boundary_velocity), src/pressure.jl:162-203: b[n] and GP[2n] */
+int lv_pressure_rhs(lv_ctx *ctx, double dt, int32_t gp_step, const double vbc[8], double *b, double *GP);
+/* solve the assembled operator for b with initial guess x (in/out) */
+int lv_solve_pressure(lv_ctx *ctx, const double *b, double *x, const lv_solve_opts *opts, lv_solve_stats *stats);
+
+/* ---- timing ----------------------------------------------------------------------------------- */
+/* When enabled, the library brackets every launch of the named kernel with HIP events on the
+ * context stream and accumulates the device time. kernel ids: 0 = FP32 SpMV, 1 = FP64 SpMV,
+ * 2 = clipping, 3 = cell-list scatter */
+int lv_timing_enable(lv_ctx *ctx, int32_t kernel_id, int32_t on);
+int lv_timing_read(lv_ctx *ctx, int32_t kernel_id, double *total_ms, int64_t *launches, double *bytes_per_launch);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,480 @@
+"""Python mirror of LagrangianVoronoi.jl's hot-path API over liblvhip.so (MI355X / gfx950).
+
+Names follow the reference with the Julia `!` dropped: `remesh(grid)` is `remesh!(grid)`,
+`find_pressure(solver, dt)` is `find_pressure!(solver, dt)`, and so on.  Every operation runs
+through the HIP library; there is no CPU fallback: importing this package on a machine without
+the built library raises, and the library reports a HIP error when no GPU is present.
+
+Fields use the reference's PolygonNS names (src/celldefs.jl:7-39).  Arrays are numpy, shaped
+(n,) for scalars, (n, 2) for vectors and (n, 2, 2) for the tensor D; labels are 0-based and
+boundary labels are BDARY_UP = -1, BDARY_RIGHT = -2, BDARY_DOWN = -3, BDARY_LEFT = -4.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import inputs  # noqa: F401
+
+__all__ = [
+    "LVError", "Rectangle", "UnitRectangle", "VoronoiGrid", "GridNS", "PressureSolver", "remesh", "move",
+    "stiffened_eos", "ideal_eos", "find_rho", "gravity_step", "find_pressure", "pressure_step", "find_D",
+    "viscous_step", "bdary_friction", "find_dv", "relaxation_step", "step_fused", "StepDesc",
+    "BDARY_UP", "BDARY_RIGHT", "BDARY_DOWN", "BDARY_LEFT", "lib", "library_path",
+]
+
+BDARY_UP, BDARY_RIGHT, BDARY_DOWN, BDARY_LEFT = -1, -2, -3, -4
+
+LV_OK = 0
+LV_E_MESH_DESTROYED = 1
+LV_E_NAN_VELOCITY = 2
+LV_E_POLY_OVERFLOW = 3
+LV_E_NOT_CONVERGED = 4
+LV_E_HIP = 5
+LV_E_INVALID = 6
+LV_E_NOMEM = 7
+LV_E_NO_MESH = 8
+LV_MAX_EDGES = 16
+
+SOLVERS = {"pcg64": 0, "mixed": 1, "spai": 2}
+
+FIELDS = {"x": (0, 2), "v": (1, 2), "dv": (2, 2), "momentum": (3, 2), "rho": (4, 1), "e": (5, 1), "P": (6, 1),
+          "c2": (7, 1), "mass": (8, 1), "energy": (9, 1), "quality": (10, 1), "mu": (11, 1), "D": (12, 4),
+          "phase": (13, 1), "area": (14, 1), "centroid": (15, 2)}
+
+library_path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "liblvhip.so")
+
+
+class LVError(RuntimeError):
+    """An error status of liblvhip; the message of the reference's throw where one exists."""
+
+    def __init__(self, code, message):
+        super().__init__(message)
+        self.code = code
+
+
+# ---- C structures (include/lvhip.h) ---------------------------------------------------------------
+
+class GridDesc(ctypes.Structure):
+    _fields_ = [("xmin", ctypes.c_double * 2), ("xmax", ctypes.c_double * 2), ("dr", ctypes.c_double),
+                ("h", ctypes.c_double), ("r_max", ctypes.c_double), ("xperiodic", ctypes.c_int32),
+                ("yperiodic", ctypes.c_int32), ("capacity", ctypes.c_int32)]
+
+
+class SolveOpts(ctypes.Structure):
+    _fields_ = [("method", ctypes.c_int32), ("tol", ctypes.c_double), ("itmax", ctypes.c_int32),
+                ("inner_tol", ctypes.c_double), ("outer_max", ctypes.c_int32), ("vbc", ctypes.c_double * 8)]
+
+
+class SolveStats(ctypes.Structure):
+    _fields_ = [("iters", ctypes.c_int32 * 32), ("outer", ctypes.c_int32 * 32), ("relres", ctypes.c_double * 32),
+                ("solve_ms", ctypes.c_double * 32), ("total_iters", ctypes.c_int32),
+                ("fallbacks", ctypes.c_int32), ("nsolves", ctypes.c_int32)]
+
+    def as_dict(self):
+        k = min(self.nsolves, 32)
+        return dict(iters=list(self.iters[:k]), outer=list(self.outer[:k]), relres=list(self.relres[:k]),
+                    solve_ms=list(self.solve_ms[:k]), total_iters=self.total_iters, fallbacks=self.fallbacks,
+                    nsolves=self.nsolves)
+
+
+class StepDesc(ctypes.Structure):
+    _fields_ = [("dt", ctypes.c_double), ("eos", ctypes.c_int32), ("gamma", ctypes.c_double),
+                ("P0", ctypes.c_double), ("Pmin", ctypes.c_double), ("gravity", ctypes.c_int32),
+                ("g", ctypes.c_double * 2), ("niter", ctypes.c_int32), ("solve", SolveOpts),
+                ("viscous", ctypes.c_int32), ("artificial_viscosity", ctypes.c_int32), ("friction", ctypes.c_int32),
+                ("friction_vbc", ctypes.c_double * 8), ("alpha", ctypes.c_double), ("rusanov", ctypes.c_int32)]
+
+
+_P = ctypes.c_void_p
+_D = ctypes.c_double
+_I = ctypes.c_int32
+
+_SIGNATURES = {
+    "lv_create": [ctypes.POINTER(GridDesc), ctypes.POINTER(ctypes.c_void_p)],
+    "lv_destroy": [_P],
+    "lv_sync": [_P],
+    "lv_grid_info": [_P, _P, _P, _P, _P],
+    "lv_grid_path": [_P, _P, _P, _P],
+    "lv_set_particles": [_P, _I, _P],
+    "lv_num_particles": [_P, _P],
+    "lv_upload": [_P, _I, _P],
+    "lv_download": [_P, _I, _P],
+    "lv_field_device_ptr": [_P, _I, ctypes.POINTER(ctypes.c_void_p)],
+    "lv_remesh": [_P],
+    "lv_mesh_size": [_P, _P],
+    "lv_get_mesh": [_P, _P, _P, _P],
+    "lv_cell_list_size": [_P, _P],
+    "lv_get_cell_list": [_P, _P, _P],
+    "lv_move": [_P, _D],
+    "lv_gravity_step": [_P, _P, _D],
+    "lv_find_rho": [_P],
+    "lv_eos_stiffened": [_P, _D, _D],
+    "lv_eos_ideal": [_P, _D, _D],
+    "lv_find_pressure": [_P, _D, _I, ctypes.POINTER(SolveOpts), ctypes.POINTER(SolveStats)],
+    "lv_pressure_step": [_P, _D],
+    "lv_find_D": [_P],
+    "lv_viscous_step": [_P, _D, _I],
+    "lv_bdary_friction": [_P, _P, _D],
+    "lv_find_dv": [_P, _D, _D],
+    "lv_relaxation_step": [_P, _D, _I],
+    "lv_step_fused": [_P, ctypes.POINTER(StepDesc), ctypes.POINTER(SolveStats)],
+    "lv_assemble_pressure": [_P, _D, _P],
+    "lv_get_pressure_operator": [_P, _P, _P, _P, _P],
+    "lv_pressure_rhs": [_P, _D, _I, _P, _P, _P],
+    "lv_solve_pressure": [_P, _P, _P, ctypes.POINTER(SolveOpts), ctypes.POINTER(SolveStats)],
+    "lv_timing_enable": [_P, _I, _I],
+    "lv_timing_read": [_P, _I, _P, _P, _P],
+}
+
+_lib = None
+
+
+def lib():
+    """The loaded liblvhip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(library_path):
+            raise ImportError("liblvhip.so is not built: run `python __graft_entry__.py` or "
+                              "`python lagrangianvoronoi.jl_amd/build.py` first")
+        L = ctypes.CDLL(library_path)
+        for name, args in _SIGNATURES.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = ctypes.c_int
+        L.lv_last_error.argtypes = [_P]
+        L.lv_last_error.restype = ctypes.c_char_p
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _check(ctx, rc):
+    if rc != LV_OK:
+        msg = lib().lv_last_error(ctx).decode() if ctx else "invalid argument"
+        raise LVError(rc, msg or "liblvhip error %d" % rc)
+    return rc
+
+
+# ---- geometry -------------------------------------------------------------------------------------
+
+class Rectangle:
+    """Rectangle(xmin, xmax), src/geometry.jl:94-111"""
+
+    def __init__(self, xmin=None, xmax=None, xlims=None, ylims=None):
+        if xlims is not None:
+            xmin, xmax = (xlims[0], ylims[0]), (xlims[1], ylims[1])
+        self.xmin = (float(xmin[0]), float(xmin[1]))
+        self.xmax = (float(xmax[0]), float(xmax[1]))
+
+
+def UnitRectangle():
+    return Rectangle((0.0, 0.0), (1.0, 1.0))
+
+
+def _vbc_table(spec):
+    """Per-label boundary velocity table vbc[2*(-label-1) + comp] from a dict {label: (vx, vy)},
+    an (8,) array, or a callable f(x, label) that is constant on each boundary side."""
+    t = np.zeros(8)
+    if spec is None:
+        return t
+    if isinstance(spec, dict):
+        for lab, val in spec.items():
+            t[2 * (-lab - 1):2 * (-lab - 1) + 2] = val
+        return t
+    if callable(spec):
+        for lab in (BDARY_UP, BDARY_RIGHT, BDARY_DOWN, BDARY_LEFT):
+            t[2 * (-lab - 1):2 * (-lab - 1) + 2] = spec(None, lab)
+        return t
+    return np.asarray(spec, dtype=np.float64).reshape(8).copy()
+
+
+class VoronoiGrid:
+    """VoronoiGrid{T}(boundary_rect, dr; h = 2dr, r_max = 10dr, xperiodic, yperiodic),
+    src/voronoigrid.jl:14-50, with its polygons held on the GPU."""
+
+    def __init__(self, boundary_rect, dr, h=None, r_max=None, xperiodic=False, yperiodic=False, capacity=None,
+                 positions=None):
+        self.boundary_rect = boundary_rect
+        self.dr = float(dr)
+        self.h = 2.0 * self.dr if h is None else float(h)
+        self.r_max = 10.0 * self.dr if r_max is None else float(r_max)
+        self.xperiodic, self.yperiodic = bool(xperiodic), bool(yperiodic)
+        if capacity is None:
+            capacity = 0 if positions is None else np.asarray(positions).size // 2
+        d = GridDesc()
+        d.xmin[:] = boundary_rect.xmin
+        d.xmax[:] = boundary_rect.xmax
+        d.dr, d.h, d.r_max = self.dr, self.h, self.r_max
+        d.xperiodic, d.yperiodic = int(self.xperiodic), int(self.yperiodic)
+        d.capacity = max(int(capacity), 1)
+        self.capacity = d.capacity
+        h_ = ctypes.c_void_p()
+        rc = lib().lv_create(ctypes.byref(d), ctypes.byref(h_))
+        if rc != LV_OK:
+            raise LVError(rc, "lv_create failed with status %d" % rc)
+        self._ctx = h_
+        self.n = 0
+        if positions is not None:
+            self.set_positions(positions)
+
+    def __del__(self):
+        ctx = getattr(self, "_ctx", None)
+        if ctx and _lib is not None:
+            _lib.lv_destroy(ctx)
+            self._ctx = None
+
+    @property
+    def ctx(self):
+        return self._ctx
+
+    # push! of many seeds at once, then the caller remeshes (src/populate.jl pattern)
+    def set_positions(self, x):
+        x = np.asarray(x, dtype=np.float64)
+        if x.ndim == 2:
+            x = np.ascontiguousarray(x.T).reshape(-1)
+        x = np.ascontiguousarray(x)
+        n = x.size // 2
+        _check(self._ctx, lib().lv_set_particles(self._ctx, n, _ptr(x)))
+        self.n = n
+
+    def get(self, name):
+        fid, comps = FIELDS[name]
+        dtype = np.int32 if name == "phase" else np.float64
+        out = np.empty(comps * self.n, dtype=dtype)
+        _check(self._ctx, lib().lv_download(self._ctx, fid, _ptr(out)))
+        if comps == 1:
+            return out
+        if comps == 2:
+            return out.reshape(2, self.n).T.copy()
+        return out.reshape(2, 2, self.n).transpose(2, 1, 0).copy()  # (n, row, col), column-major planes
+
+    def get_soa(self, name):
+        """The raw SoA planes of a field, as the C ABI holds them."""
+        fid, comps = FIELDS[name]
+        dtype = np.int32 if name == "phase" else np.float64
+        out = np.empty(comps * self.n, dtype=dtype)
+        _check(self._ctx, lib().lv_download(self._ctx, fid, _ptr(out)))
+        return out
+
+    def set(self, name, value):
+        fid, comps = FIELDS[name]
+        dtype = np.int32 if name == "phase" else np.float64
+        a = np.asarray(value, dtype=dtype)
+        if a.ndim == 0:
+            a = np.full(comps * self.n, a, dtype=dtype)
+        elif comps == 2 and a.ndim == 2:
+            a = a.T.reshape(-1)
+        elif comps == 4 and a.ndim == 3:
+            a = a.transpose(2, 1, 0).reshape(-1)
+        a = np.ascontiguousarray(a)
+        if a.size != comps * self.n:
+            raise ValueError("field %s expects %d values" % (name, comps * self.n))
+        _check(self._ctx, lib().lv_upload(self._ctx, fid, _ptr(a)))
+
+    def set_soa(self, name, planes):
+        fid, comps = FIELDS[name]
+        dtype = np.int32 if name == "phase" else np.float64
+        a = np.ascontiguousarray(np.asarray(planes, dtype=dtype).reshape(-1))
+        _check(self._ctx, lib().lv_upload(self._ctx, fid, _ptr(a)))
+
+    def mesh(self):
+        """(edge_ptr[n+1], labels[E], verts[E,4]) with every polygon's edges after sort_edges!"""
+        ne = ctypes.c_int64()
+        _check(self._ctx, lib().lv_mesh_size(self._ctx, ctypes.byref(ne)))
+        ptr = np.empty(self.n + 1, dtype=np.int32)
+        lab = np.empty(ne.value, dtype=np.int32)
+        ver = np.empty((ne.value, 4), dtype=np.float64)
+        _check(self._ctx, lib().lv_get_mesh(self._ctx, _ptr(ptr), _ptr(lab), _ptr(ver)))
+        return ptr, lab, ver
+
+    def cell_list(self):
+        """(cell_start[n1*n2+1], cell_labels) of the last remesh"""
+        n1, n2, npath = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        _check(self._ctx, lib().lv_grid_info(self._ctx, ctypes.byref(n1), ctypes.byref(n2), ctypes.byref(npath), None))
+        m = ctypes.c_int64()
+        _check(self._ctx, lib().lv_cell_list_size(self._ctx, ctypes.byref(m)))
+        cs = np.empty(n1.value * n2.value + 1, dtype=np.int32)
+        cl = np.empty(m.value, dtype=np.int32)
+        _check(self._ctx, lib().lv_get_cell_list(self._ctx, _ptr(cs), _ptr(cl)))
+        return cs, cl
+
+    def info(self):
+        n1, n2, npath = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        out6 = np.empty(6)
+        _check(self._ctx, lib().lv_grid_info(self._ctx, ctypes.byref(n1), ctypes.byref(n2), ctypes.byref(npath),
+                                             _ptr(out6)))
+        return dict(n1=n1.value, n2=n2.value, npath=npath.value, cropping_min=tuple(out6[:2]),
+                    cropping_max=tuple(out6[2:4]), next_rr=out6[4], rr_max=out6[5])
+
+    def magic_path(self):
+        npath = self.info()["npath"]
+        i1 = np.empty(npath, dtype=np.int32)
+        i2 = np.empty(npath, dtype=np.int32)
+        rr = np.empty(npath)
+        _check(self._ctx, lib().lv_grid_path(self._ctx, _ptr(i1), _ptr(i2), _ptr(rr)))
+        return i1, i2, rr
+
+    def sync(self):
+        _check(self._ctx, lib().lv_sync(self._ctx))
+
+
+GridNS = VoronoiGrid
+
+
+# ---- operators (src/voronoigrid.jl, src/move.jl, src/pressure.jl, src/diffusion.jl, src/relaxation.jl) --
+
+def remesh(grid):
+    _check(grid.ctx, lib().lv_remesh(grid.ctx))
+
+
+def move(grid, dt):
+    _check(grid.ctx, lib().lv_move(grid.ctx, float(dt)))
+
+
+def gravity_step(grid, g, dt):
+    g = np.ascontiguousarray(np.asarray(g, dtype=np.float64).reshape(2))
+    _check(grid.ctx, lib().lv_gravity_step(grid.ctx, _ptr(g), float(dt)))
+
+
+def find_rho(grid):
+    _check(grid.ctx, lib().lv_find_rho(grid.ctx))
+
+
+def stiffened_eos(grid, gamma=1.4, P0=0.0):
+    _check(grid.ctx, lib().lv_eos_stiffened(grid.ctx, float(gamma), float(P0)))
+
+
+def ideal_eos(grid, gamma=1.4, Pmin=0.0):
+    _check(grid.ctx, lib().lv_eos_ideal(grid.ctx, float(gamma), float(Pmin)))
+
+
+def pressure_step(grid, dt):
+    _check(grid.ctx, lib().lv_pressure_step(grid.ctx, float(dt)))
+
+
+def find_D(grid):
+    _check(grid.ctx, lib().lv_find_D(grid.ctx))
+
+
+def viscous_step(grid, dt, artificial_viscosity=True):
+    _check(grid.ctx, lib().lv_viscous_step(grid.ctx, float(dt), int(bool(artificial_viscosity))))
+
+
+def bdary_friction(grid, vDirichlet, dt):
+    t = _vbc_table(vDirichlet)
+    _check(grid.ctx, lib().lv_bdary_friction(grid.ctx, _ptr(t), float(dt)))
+
+
+def find_dv(grid, dt, alpha=1.0):
+    _check(grid.ctx, lib().lv_find_dv(grid.ctx, float(dt), float(alpha)))
+
+
+def relaxation_step(grid, dt, rusanov=True):
+    _check(grid.ctx, lib().lv_relaxation_step(grid.ctx, float(dt), int(bool(rusanov))))
+
+
+def solve_opts(method="pcg64", tol=1e-10, itmax=1000, inner_tol=1e-4, outer_max=20, boundary_velocity=None):
+    o = SolveOpts()
+    o.method = SOLVERS[method] if isinstance(method, str) else int(method)
+    o.tol, o.itmax, o.inner_tol, o.outer_max = float(tol), int(itmax), float(inner_tol), int(outer_max)
+    o.vbc[:] = _vbc_table(boundary_velocity)
+    return o
+
+
+class PressureSolver:
+    """PressureSolver(grid; verbose), src/pressure.jl:142-159.  The Krylov method and tolerance are
+    this build's (the reference hard-codes MINRES with atol = rtol = 1e-6, :219)."""
+
+    def __init__(self, grid, verbose=False, method="pcg64", tol=1e-10, itmax=1000, inner_tol=1e-4, outer_max=20):
+        self.grid = grid
+        self.verbose = verbose
+        self.method, self.tol, self.itmax = method, tol, itmax
+        self.inner_tol, self.outer_max = inner_tol, outer_max
+        self.last_stats = None
+
+    def opts(self, boundary_velocity=None):
+        return solve_opts(self.method, self.tol, self.itmax, self.inner_tol, self.outer_max, boundary_velocity)
+
+    # PressureOperator access (src/pressure.jl:89-135)
+    def assemble(self, dt):
+        nnz = ctypes.c_int64()
+        _check(self.grid.ctx, lib().lv_assemble_pressure(self.grid.ctx, float(dt), ctypes.byref(nnz)))
+        return nnz.value
+
+    def operator(self):
+        """(diag[n], row_ptr[n+1], cols[nnz], coef[nnz]) of the assembled operator"""
+        n = self.grid.n
+        diag = np.empty(n)
+        rp = np.empty(n + 1, dtype=np.int32)
+        _check(self.grid.ctx, lib().lv_get_pressure_operator(self.grid.ctx, _ptr(diag), _ptr(rp), None, None))
+        nnz = int(rp[-1])
+        cols = np.empty(nnz, dtype=np.int32)
+        coef = np.empty(nnz)
+        _check(self.grid.ctx, lib().lv_get_pressure_operator(self.grid.ctx, _ptr(diag), _ptr(rp), _ptr(cols),
+                                                             _ptr(coef)))
+        return diag, rp, cols, coef
+
+    def rhs(self, dt, gp_step, boundary_velocity=None):
+        n = self.grid.n
+        b = np.empty(n)
+        GP = np.empty(2 * n)
+        t = _vbc_table(boundary_velocity)
+        _check(self.grid.ctx, lib().lv_pressure_rhs(self.grid.ctx, float(dt), int(bool(gp_step)), _ptr(t), _ptr(b),
+                                                    _ptr(GP)))
+        return b, GP
+
+    def solve(self, b, x0, raise_on_fail=True, **kw):
+        o = solve_opts(**{**dict(method=self.method, tol=self.tol, itmax=self.itmax, inner_tol=self.inner_tol,
+                                  outer_max=self.outer_max), **kw})
+        x = np.ascontiguousarray(np.array(x0, dtype=np.float64))
+        b = np.ascontiguousarray(np.asarray(b, dtype=np.float64))
+        st = SolveStats()
+        rc = lib().lv_solve_pressure(self.grid.ctx, _ptr(b), _ptr(x), ctypes.byref(o), ctypes.byref(st))
+        if rc != LV_E_NOT_CONVERGED or raise_on_fail:
+            _check(self.grid.ctx, rc)
+        self.last_stats = st.as_dict()
+        return x
+
+
+def find_pressure(solver, dt, niter=10, boundary_velocity=None):
+    """find_pressure!(solver, dt, niter; boundary_velocity), src/pressure.jl:215-224.  Returns the
+    solve statistics.  Unlike the reference (which ignores MINRES convergence), a missed tolerance
+    raises LVError(LV_E_NOT_CONVERGED) after P has been written."""
+    st = SolveStats()
+    o = solver.opts(boundary_velocity)
+    rc = lib().lv_find_pressure(solver.grid.ctx, float(dt), int(niter), ctypes.byref(o), ctypes.byref(st))
+    solver.last_stats = st.as_dict()
+    _check(solver.grid.ctx, rc)
+    return solver.last_stats
+
+
+def step_fused(grid, desc, stats=True):
+    """One whole step! on the device (move! ... relaxation_step!), see StepDesc."""
+    st = SolveStats()
+    rc = lib().lv_step_fused(grid.ctx, ctypes.byref(desc), ctypes.byref(st) if stats else None)
+    _check(grid.ctx, rc)
+    return st.as_dict()
+
+
+def make_step_desc(dt, eos=None, gamma=1.4, P0=0.0, Pmin=0.0, gravity=None, niter=10, solver=None,
+                   viscous=True, artificial_viscosity=False, friction=None, alpha=1.0, rusanov=True,
+                   boundary_velocity=None):
+    d = StepDesc()
+    d.dt = float(dt)
+    d.eos = {None: 0, "stiffened": 1, "ideal": 2}[eos]
+    d.gamma, d.P0, d.Pmin = float(gamma), float(P0), float(Pmin)
+    d.gravity = int(gravity is not None)
+    d.g[:] = (0.0, 0.0) if gravity is None else tuple(gravity)
+    d.niter = int(niter)
+    d.solve = solver.opts(boundary_velocity) if solver is not None else solve_opts(boundary_velocity=boundary_velocity)
+    d.viscous = int(bool(viscous))
+    d.artificial_viscosity = int(bool(artificial_viscosity))
+    d.friction = int(friction is not None)
+    d.friction_vbc[:] = _vbc_table(friction)
+    d.alpha = float(alpha)
+    d.rusanov = int(bool(rusanov))
+    return d

@@ -1,0 +1,901 @@
+// lv_api.hip -- the C ABI of liblvhip.so: context, particle state, remesh and the step operators.
+//
+// A context is one VoronoiGrid (src/voronoigrid.jl:14-50) whose particles, mesh, cell list and
+// pressure system live in device memory as SoA planes.  Every call enqueues on the context's own
+// stream; calls that can raise one of the reference's errors synchronise once to read the device
+// error word and map it to its status code.
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "lv_solver.h"
+
+struct lv_ctx {
+    lv_grid_params g;
+    int cap = 0, n = 0;
+    hipStream_t s = nullptr;
+    lv_fields F{};
+    // mesh (ELL, slot-major, leading dimension n)
+    int *ne = nullptr;
+    double *ev = nullptr;
+    int *el = nullptr;
+    bool mesh_valid = false;   // the mesh belongs to the current positions
+    // cell list
+    int *cell_start = nullptr, *cell_labels = nullptr;
+    double *cell_x = nullptr;
+    int64_t max_items = 0;
+    int nitems = 0;
+    void *cl_temp = nullptr;
+    int *d_nitems = nullptr;
+    // magic path prefix
+    int *path_i = nullptr;
+    double *path_rr = nullptr;
+    std::vector<int> h_path_i;
+    std::vector<double> h_path_rr;
+    // pressure system
+    lv_pressure_system A{};
+    lv_solver_work W{};
+    double *b = nullptr, *P0 = nullptr, *GP = nullptr;
+    int *deg = nullptr;
+    void *scan_temp = nullptr;
+    int *d_flags = nullptr;   // [0] error word, [1] nonsingular flag
+    lv_timer *timers[LV_NKTIMERS] = {nullptr, nullptr, nullptr, nullptr};
+    char msg[512] = {0};
+};
+
+int lv_fail(lv_ctx *ctx, int code, const char *msg)
+{
+    if (ctx) snprintf(ctx->msg, sizeof(ctx->msg), "%s", msg);
+    return code;
+}
+
+int lv_fail_hip(lv_ctx *ctx, hipError_t e, const char *what, const char *file, int line)
+{
+    if (ctx) snprintf(ctx->msg, sizeof(ctx->msg), "HIP error %d (%s) in %s at %s:%d", (int)e, hipGetErrorString(e),
+                      what, file, line);
+    return (e == hipErrorOutOfMemory) ? LV_E_NOMEM : LV_E_HIP;
+}
+
+// ---- kernel timers ---------------------------------------------------------------------------------
+
+void lv_timer_begin(lv_timer *t, hipStream_t s)
+{
+    if (t->used == LV_TPOOL) lv_timer_flush(t);
+    hipEventRecord(t->ev[2 * t->used], s);
+}
+
+void lv_timer_end(lv_timer *t, hipStream_t s)
+{
+    hipEventRecord(t->ev[2 * t->used + 1], s);
+    t->used++;
+}
+
+void lv_timer_flush(lv_timer *t)
+{
+    for (int k = 0; k < t->used; k++) {
+        float ms = 0.0f;
+        hipEventSynchronize(t->ev[2 * k + 1]);
+        if (hipEventElapsedTime(&ms, t->ev[2 * k], t->ev[2 * k + 1]) == hipSuccess) t->total_ms += ms;
+        t->launches++;
+    }
+    t->used = 0;
+}
+
+static lv_timer *timer_if_on(lv_ctx *ctx, int id) { return (ctx->timers[id] && ctx->timers[id]->on) ? ctx->timers[id] : nullptr; }
+
+// ---- set-up ---------------------------------------------------------------------------------------
+
+namespace {
+
+struct pnode {
+    int i1, i2;
+    long long s, gen;
+    double rr;
+};
+
+// magic_path, src/neighborlist.jl:29-41: nodes sorted by rr = h^2*(max(0,|i1|-1)^2 + max(0,|i2|-1)^2),
+// ties by i1^2 + i2^2, then by generation order (i2 outer over (1-n1):(n1-1), i1 inner over
+// (1-n2):(n2-1)), both sorts stable.  Only the nodes with rr <= rr_max and the first node past it
+// decide anything, so only small offsets are enumerated.
+void build_path(lv_grid_params &g, std::vector<int> &pi, std::vector<double> &prr)
+{
+    const int lim1 = g.n2 - 1, lim2 = g.n1 - 1;
+    const int kmax = (int)ceil(g.r_max / g.h) + 3;
+    const int m1 = std::min(kmax, lim1), m2 = std::min(kmax, lim2);
+    std::vector<pnode> nodes;
+    nodes.reserve((size_t)(2 * m1 + 1) * (2 * m2 + 1));
+    for (int i2 = -m2; i2 <= m2; i2++)
+        for (int i1 = -m1; i1 <= m1; i1++) {
+            long long a = std::max(0, std::abs(i1) - 1), b = std::max(0, std::abs(i2) - 1);
+            pnode p;
+            p.i1 = i1;
+            p.i2 = i2;
+            p.s = (long long)i1 * i1 + (long long)i2 * i2;
+            p.gen = (long long)(i2 - (1 - g.n1)) * (2LL * g.n2 - 1) + (i1 - (1 - g.n2));
+            p.rr = (g.h * g.h) * (double)(a * a + b * b);
+            nodes.push_back(p);
+        }
+    std::stable_sort(nodes.begin(), nodes.end(), [](const pnode &p, const pnode &q) {
+        if (p.rr != q.rr) return p.rr < q.rr;
+        if (p.s != q.s) return p.s < q.s;
+        return p.gen < q.gen;
+    });
+    size_t np = 0;
+    while (np < nodes.size() && !(nodes[np].rr > g.rr_max)) np++;
+    pi.clear();
+    prr.clear();
+    for (size_t k = 0; k < np; k++) {
+        pi.push_back(nodes[k].i1);
+        pi.push_back(nodes[k].i2);
+        prr.push_back(nodes[k].rr);
+    }
+    g.npath = (int)np;
+    g.has_next = (np < nodes.size()) ? 1 : 0;
+    g.next_rr = g.has_next ? nodes[np].rr : INFINITY;
+}
+
+template <typename T>
+hipError_t dalloc(T **p, size_t count)
+{
+    *p = nullptr;
+    if (count == 0) count = 1;
+    return hipMalloc((void **)p, sizeof(T) * count);
+}
+
+void dfree(void *p)
+{
+    if (p) hipFree(p);
+}
+
+double *field_ptr(lv_ctx *c, int f, size_t *count, size_t *elem, bool *readonly)
+{
+    const size_t n = (size_t)c->n;
+    *elem = sizeof(double);
+    *readonly = false;
+    switch (f) {
+    case LV_FIELD_X: *count = 2 * n; return c->F.x;
+    case LV_FIELD_V: *count = 2 * n; return c->F.v;
+    case LV_FIELD_DV: *count = 2 * n; return c->F.dv;
+    case LV_FIELD_MOMENTUM: *count = 2 * n; return c->F.mom;
+    case LV_FIELD_RHO: *count = n; return c->F.rho;
+    case LV_FIELD_E: *count = n; return c->F.e;
+    case LV_FIELD_P: *count = n; return c->F.P;
+    case LV_FIELD_C2: *count = n; return c->F.c2;
+    case LV_FIELD_MASS: *count = n; return c->F.mass;
+    case LV_FIELD_ENERGY: *count = n; return c->F.energy;
+    case LV_FIELD_QUALITY: *count = n; return c->F.quality;
+    case LV_FIELD_MU: *count = n; return c->F.mu;
+    case LV_FIELD_D: *count = 4 * n; return c->F.D;
+    case LV_FIELD_PHASE: *count = n; *elem = sizeof(int); return (double *)c->F.phase;
+    case LV_FIELD_AREA: *count = n; *readonly = true; return c->F.area;
+    case LV_FIELD_CENTROID: *count = 2 * n; *readonly = true; return c->F.cent;
+    default: return nullptr;
+    }
+}
+
+lv_mesh_view mesh_view(lv_ctx *c) { return lv_mesh_view{c->ne, c->ev, c->el, c->n}; }
+
+int read_flags(lv_ctx *ctx, int *flags2)
+{
+    LV_HIP_CHECK(hipMemcpyAsync(flags2, ctx->d_flags, 2 * sizeof(int), hipMemcpyDeviceToHost, ctx->s));
+    LV_HIP_CHECK(hipStreamSynchronize(ctx->s));
+    return LV_OK;
+}
+
+int clear_flags(lv_ctx *ctx)
+{
+    LV_HIP_CHECK(hipMemsetAsync(ctx->d_flags, 0, 2 * sizeof(int), ctx->s));
+    return LV_OK;
+}
+
+int check_ctx(lv_ctx *ctx, bool need_mesh)
+{
+    if (!ctx) return LV_E_INVALID;
+    if (ctx->n <= 0) return lv_fail(ctx, LV_E_INVALID, "no particles");
+    if (need_mesh && !ctx->mesh_valid) return lv_fail(ctx, LV_E_NO_MESH, "positions changed since the last remesh");
+    return LV_OK;
+}
+
+const char *error_text(int code)
+{
+    switch (code) {
+    case LV_E_MESH_DESTROYED: return "The Voronoi Mesh has been destroyed.";
+    case LV_E_NAN_VELOCITY: return "Velocity field invalidated.";
+    case LV_E_POLY_OVERFLOW: return "A Voronoi polygon exceeded LV_MAX_EDGES edges.";
+    default: return "error";
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int lv_create(const lv_grid_desc *d, lv_ctx **out)
+{
+    if (!d || !out) return LV_E_INVALID;
+    *out = nullptr;
+    const double h = (d->h > 0.0) ? d->h : 2.0 * d->dr;            // h = 2dr, src/voronoigrid.jl:28
+    const double r_max = (d->r_max > 0.0) ? d->r_max : 10.0 * d->dr;  // r_max = 10dr
+    if (!(h > 0.0) || !(d->dr > 0.0) || d->capacity <= 0) return LV_E_INVALID;
+    if (!(d->xmax[0] > d->xmin[0]) || !(d->xmax[1] > d->xmin[1])) return LV_E_INVALID;
+    lv_ctx *ctx = new lv_ctx();
+    lv_grid_params &g = ctx->g;
+    memset(&g, 0, sizeof(g));
+    g.bxmin0 = d->xmin[0];
+    g.bxmin1 = d->xmin[1];
+    g.bxmax0 = d->xmax[0];
+    g.bxmax1 = d->xmax[1];
+    g.xperiodic = d->xperiodic ? 1 : 0;
+    g.yperiodic = d->yperiodic ? 1 : 0;
+    // cropping rect, src/voronoigrid.jl:29-30 (r_max*periodic is r_max or +0.0)
+    const double rx = g.xperiodic ? r_max : 0.0, ry = g.yperiodic ? r_max : 0.0;
+    g.cmin0 = (g.bxmin0 - rx * 1.0) - ry * 0.0;
+    g.cmin1 = (g.bxmin1 - rx * 0.0) - ry * 1.0;
+    g.cmax0 = (g.bxmax0 + rx * 1.0) + ry * 0.0;
+    g.cmax1 = (g.bxmax1 + rx * 0.0) + ry * 1.0;
+    g.xperiod = g.bxmax0 - g.bxmin0;
+    g.yperiod = g.bxmax1 - g.bxmin1;
+    g.h = h;
+    g.dr = d->dr;
+    g.r_max = r_max;
+    g.rr_max = r_max * r_max;
+    // CellList, src/neighborlist.jl:23-25
+    g.origin0 = g.cmin0 - h;
+    g.origin1 = g.cmin1 - h;
+    g.n1 = (int)floor((g.cmax0 - g.cmin0) / h) + 3;
+    g.n2 = (int)floor((g.cmax1 - g.cmin1) / h) + 3;
+    build_path(g, ctx->h_path_i, ctx->h_path_rr);
+    if (g.npath > LV_MAXPATH) {
+        delete ctx;
+        return LV_E_INVALID;
+    }
+    ctx->cap = d->capacity;
+    const size_t cap = (size_t)d->capacity;
+    const int64_t ncells = (int64_t)g.n1 * g.n2;
+    hipError_t e = hipStreamCreateWithFlags(&ctx->s, hipStreamNonBlocking);
+#define ALLOC(p, cnt)                                   \
+    if (e == hipSuccess) e = dalloc(&(p), (size_t)(cnt));
+    ALLOC(ctx->F.x, 2 * cap);
+    ALLOC(ctx->F.v, 2 * cap);
+    ALLOC(ctx->F.dv, 2 * cap);
+    ALLOC(ctx->F.mom, 2 * cap);
+    ALLOC(ctx->F.rho, cap);
+    ALLOC(ctx->F.e, cap);
+    ALLOC(ctx->F.P, cap);
+    ALLOC(ctx->F.c2, cap);
+    ALLOC(ctx->F.mass, cap);
+    ALLOC(ctx->F.energy, cap);
+    ALLOC(ctx->F.quality, cap);
+    ALLOC(ctx->F.mu, cap);
+    ALLOC(ctx->F.D, 4 * cap);
+    ALLOC(ctx->F.area, cap);
+    ALLOC(ctx->F.cent, 2 * cap);
+    ALLOC(ctx->F.phase, cap);
+    ALLOC(ctx->ne, cap);
+    ALLOC(ctx->ev, 4 * LV_MAXE * cap);
+    ALLOC(ctx->el, LV_MAXE * cap);
+    ALLOC(ctx->cell_start, ncells + 1);
+    ALLOC(ctx->d_nitems, 1);
+    ALLOC(ctx->d_flags, 2);
+    ALLOC(ctx->path_i, 2 * (size_t)std::max(1, g.npath));
+    ALLOC(ctx->path_rr, (size_t)std::max(1, g.npath));
+    // pressure system and solver work
+    ALLOC(ctx->A.row_ptr, cap + 1);
+    ALLOC(ctx->A.diag64, cap);
+    ALLOC(ctx->A.dinv64, cap);
+    ALLOC(ctx->A.diag32, cap);
+    ALLOC(ctx->A.dinv32, cap);
+    ALLOC(ctx->deg, cap + 1);
+    ALLOC(ctx->b, cap);
+    ALLOC(ctx->P0, cap);
+    ALLOC(ctx->GP, 2 * cap);
+    ALLOC(ctx->W.x, cap);
+    ALLOC(ctx->W.r, cap);
+    ALLOC(ctx->W.z, cap);
+    ALLOC(ctx->W.p, cap);
+    ALLOC(ctx->W.q, cap);
+    ALLOC(ctx->W.e32, cap);
+    ALLOC(ctx->W.r32, cap);
+    ALLOC(ctx->W.z32, cap);
+    ALLOC(ctx->W.p32, cap);
+    ALLOC(ctx->W.q32, cap);
+    ALLOC(ctx->W.S, 1);
+    if (e == hipSuccess) e = hipMalloc(&ctx->W.partials, lv_partials_bytes((int)cap));
+    if (e == hipSuccess) e = hipMalloc(&ctx->scan_temp, lv_scan_temp_bytes((int64_t)cap + 1) + 256);
+#undef ALLOC
+    if (e == hipSuccess && g.npath > 0) {
+        e = hipMemcpy(ctx->path_i, ctx->h_path_i.data(), sizeof(int) * 2 * g.npath, hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = hipMemcpy(ctx->path_rr, ctx->h_path_rr.data(), sizeof(double) * g.npath, hipMemcpyHostToDevice);
+    }
+    if (e == hipSuccess) e = hipMemset(ctx->d_flags, 0, 2 * sizeof(int));
+    for (int k = 0; k < LV_NKTIMERS && e == hipSuccess; k++) {
+        ctx->timers[k] = new lv_timer();
+        memset(ctx->timers[k], 0, sizeof(lv_timer));
+        for (int q = 0; q < 2 * LV_TPOOL && e == hipSuccess; q++) e = hipEventCreate(&ctx->timers[k]->ev[q]);
+    }
+    if (e != hipSuccess) {
+        int code = lv_fail_hip(ctx, e, "lv_create", __FILE__, __LINE__);
+        lv_destroy(ctx);
+        return code;
+    }
+    *out = ctx;
+    return LV_OK;
+}
+
+int lv_destroy(lv_ctx *ctx)
+{
+    if (!ctx) return LV_OK;
+    if (ctx->s) hipStreamSynchronize(ctx->s);
+    void *ptrs[] = {ctx->F.x, ctx->F.v, ctx->F.dv, ctx->F.mom, ctx->F.rho, ctx->F.e, ctx->F.P, ctx->F.c2,
+                    ctx->F.mass, ctx->F.energy, ctx->F.quality, ctx->F.mu, ctx->F.D, ctx->F.area, ctx->F.cent,
+                    ctx->F.phase, ctx->ne, ctx->ev, ctx->el, ctx->cell_start, ctx->cell_labels, ctx->cell_x,
+                    ctx->cl_temp, ctx->d_nitems, ctx->d_flags, ctx->path_i, ctx->path_rr, ctx->A.row_ptr,
+                    ctx->A.cols, ctx->A.val64, ctx->A.val32, ctx->A.diag64, ctx->A.dinv64, ctx->A.diag32,
+                    ctx->A.dinv32, ctx->deg, ctx->b, ctx->P0, ctx->GP, ctx->W.x, ctx->W.r, ctx->W.z, ctx->W.p,
+                    ctx->W.q, ctx->W.e32, ctx->W.r32, ctx->W.z32, ctx->W.p32, ctx->W.q32, ctx->W.S,
+                    ctx->W.partials, ctx->scan_temp};
+    for (void *p : ptrs) dfree(p);
+    for (int k = 0; k < LV_NKTIMERS; k++) {
+        if (!ctx->timers[k]) continue;
+        for (int q = 0; q < 2 * LV_TPOOL; q++)
+            if (ctx->timers[k]->ev[q]) hipEventDestroy(ctx->timers[k]->ev[q]);
+        delete ctx->timers[k];
+    }
+    if (ctx->s) hipStreamDestroy(ctx->s);
+    delete ctx;
+    return LV_OK;
+}
+
+const char *lv_last_error(lv_ctx *ctx) { return ctx ? ctx->msg : "null context"; }
+
+int lv_sync(lv_ctx *ctx)
+{
+    if (!ctx) return LV_E_INVALID;
+    LV_HIP_CHECK(hipStreamSynchronize(ctx->s));
+    return LV_OK;
+}
+
+int lv_grid_info(lv_ctx *ctx, int32_t *n1, int32_t *n2, int32_t *npath, double *out6)
+{
+    if (!ctx) return LV_E_INVALID;
+    const lv_grid_params &g = ctx->g;
+    if (n1) *n1 = g.n1;
+    if (n2) *n2 = g.n2;
+    if (npath) *npath = g.npath;
+    if (out6) {
+        out6[0] = g.cmin0;
+        out6[1] = g.cmin1;
+        out6[2] = g.cmax0;
+        out6[3] = g.cmax1;
+        out6[4] = g.has_next ? g.next_rr : -1.0;
+        out6[5] = g.rr_max;
+    }
+    return LV_OK;
+}
+
+int lv_grid_path(lv_ctx *ctx, int32_t *i1, int32_t *i2, double *rr)
+{
+    if (!ctx) return LV_E_INVALID;
+    for (int k = 0; k < ctx->g.npath; k++) {
+        i1[k] = ctx->h_path_i[2 * k];
+        i2[k] = ctx->h_path_i[2 * k + 1];
+        rr[k] = ctx->h_path_rr[k];
+    }
+    return LV_OK;
+}
+
+int lv_set_particles(lv_ctx *ctx, int32_t n, const double *x)
+{
+    if (!ctx || n < 0 || n > ctx->cap || (n > 0 && !x)) return LV_E_INVALID;
+    ctx->n = n;
+    ctx->mesh_valid = false;
+    ctx->A.valid = 0;
+    const size_t cap = (size_t)ctx->cap;
+    double *zero2[] = {ctx->F.v, ctx->F.dv, ctx->F.mom, ctx->F.cent};
+    for (double *p : zero2) LV_HIP_CHECK(hipMemsetAsync(p, 0, sizeof(double) * 2 * cap, ctx->s));
+    double *zero1[] = {ctx->F.rho, ctx->F.e, ctx->F.P, ctx->F.c2, ctx->F.mass,
+                       ctx->F.energy, ctx->F.quality, ctx->F.mu, ctx->F.area};
+    for (double *p : zero1) LV_HIP_CHECK(hipMemsetAsync(p, 0, sizeof(double) * cap, ctx->s));
+    LV_HIP_CHECK(hipMemsetAsync(ctx->F.D, 0, sizeof(double) * 4 * cap, ctx->s));
+    LV_HIP_CHECK(hipMemsetAsync(ctx->F.phase, 0, sizeof(int) * cap, ctx->s));
+    LV_HIP_CHECK(hipMemsetAsync(ctx->ne, 0, sizeof(int) * cap, ctx->s));
+    if (n > 0) LV_HIP_CHECK(hipMemcpyAsync(ctx->F.x, x, sizeof(double) * 2 * n, hipMemcpyHostToDevice, ctx->s));
+    LV_HIP_CHECK(hipStreamSynchronize(ctx->s));
+    return LV_OK;
+}
+
+int lv_num_particles(lv_ctx *ctx, int32_t *n)
+{
+    if (!ctx || !n) return LV_E_INVALID;
+    *n = ctx->n;
+    return LV_OK;
+}
+
+int lv_upload(lv_ctx *ctx, int32_t field, const void *host)
+{
+    if (!ctx || !host) return LV_E_INVALID;
+    size_t count, elem;
+    bool ro;
+    void *p = field_ptr(ctx, field, &count, &elem, &ro);
+    if (!p || ro) return lv_fail(ctx, LV_E_INVALID, "invalid or read-only field");
+    if (count == 0) return LV_OK;
+    LV_HIP_CHECK(hipMemcpyAsync(p, host, count * elem, hipMemcpyHostToDevice, ctx->s));
+    LV_HIP_CHECK(hipStreamSynchronize(ctx->s));
+    if (field == LV_FIELD_X) ctx->mesh_valid = false;
+    return LV_OK;
+}
+
+int lv_download(lv_ctx *ctx, int32_t field, void *host)
+{
+    if (!ctx || !host) return LV_E_INVALID;
+    size_t count, elem;
+    bool ro;
+    void *p = field_ptr(ctx, field, &count, &elem, &ro);
+    if (!p) return lv_fail(ctx, LV_E_INVALID, "invalid field");
+    if (count == 0) return LV_OK;
+    LV_HIP_CHECK(hipMemcpyAsync(host, p, count * elem, hipMemcpyDeviceToHost, ctx->s));
+    LV_HIP_CHECK(hipStreamSynchronize(ctx->s));
+    return LV_OK;
+}
+
+int lv_field_device_ptr(lv_ctx *ctx, int32_t field, void **dptr)
+{
+    if (!ctx || !dptr) return LV_E_INVALID;
+    size_t count, elem;
+    bool ro;
+    void *p = field_ptr(ctx, field, &count, &elem, &ro);
+    if (!p) return LV_E_INVALID;
+    *dptr = p;
+    return LV_OK;
+}
+
+// ---- remesh ----------------------------------------------------------------------------------------
+
+int lv_remesh(lv_ctx *ctx)
+{
+    int rc = check_ctx(ctx, false);
+    if (rc) return rc;
+    const int n = ctx->n;
+    const int64_t ncells = (int64_t)ctx->g.n1 * ctx->g.n2;
+    if (ctx->max_items == 0) {
+        int64_t want = (int64_t)n + n / 4 + 4096;
+        if (!ctx->g.xperiodic && !ctx->g.yperiodic) want = n + 1;
+        want = std::min<int64_t>(want, 9LL * n);
+        ctx->max_items = want;
+        LV_HIP_CHECK(dalloc(&ctx->cell_labels, want));
+        LV_HIP_CHECK(dalloc(&ctx->cell_x, 2 * want));
+        LV_HIP_CHECK(hipMalloc(&ctx->cl_temp, lv_celllist_temp_bytes(std::max<int64_t>(want, n + 1), ncells)));
+    }
+    rc = clear_flags(ctx);
+    if (rc) return rc;
+    lv_timer *tc = timer_if_on(ctx, 3);
+    if (tc && tc->used == LV_TPOOL) lv_timer_flush(tc);
+    for (int attempt = 0; attempt < 2; attempt++) {
+        hipError_t e = lv_build_cell_list(ctx->g, n, ctx->F.x, ctx->cell_start, ctx->cell_labels, ctx->cell_x,
+                                          ctx->max_items, ctx->d_nitems, ctx->cl_temp, ctx->s,
+                                          tc ? tc->ev[2 * tc->used] : nullptr, tc ? tc->ev[2 * tc->used + 1] : nullptr,
+                                          tc != nullptr);
+        if (e == hipErrorInvalidValue && attempt == 0) {
+            // more periodic images than buffered: grow to the count the scan produced and rebuild
+            int m = 0;
+            LV_HIP_CHECK(hipMemcpy(&m, ctx->d_nitems, sizeof(int), hipMemcpyDeviceToHost));
+            if (m <= 0) return lv_fail(ctx, LV_E_INVALID, "cell list: no items");
+            dfree(ctx->cell_labels);
+            dfree(ctx->cell_x);
+            dfree(ctx->cl_temp);
+            ctx->cell_labels = nullptr;
+            ctx->cell_x = nullptr;
+            ctx->cl_temp = nullptr;
+            int64_t want = (int64_t)m + m / 8 + 4096;
+            ctx->max_items = want;
+            LV_HIP_CHECK(dalloc(&ctx->cell_labels, want));
+            LV_HIP_CHECK(dalloc(&ctx->cell_x, 2 * want));
+            LV_HIP_CHECK(hipMalloc(&ctx->cl_temp, lv_celllist_temp_bytes(std::max<int64_t>(want, n + 1), ncells)));
+            continue;
+        }
+        LV_HIP_CHECK(e);
+        if (tc) tc->used++;
+        break;
+    }
+    LV_HIP_CHECK(hipMemcpyAsync(&ctx->nitems, ctx->d_nitems, sizeof(int), hipMemcpyDeviceToHost, ctx->s));
+    lv_timer *tk = timer_if_on(ctx, 2);
+    if (tk) lv_timer_begin(tk, ctx->s);
+    LV_HIP_CHECK(lv_launch_clip(ctx->g, n, ctx->F.x, ctx->cell_start, ctx->cell_labels, ctx->cell_x, ctx->path_i,
+                                ctx->path_rr, ctx->ne, ctx->ev, ctx->el, n, ctx->F.area, ctx->F.cent, ctx->d_flags,
+                                ctx->s, (int)ctx->max_items));
+    if (tk) lv_timer_end(tk, ctx->s);
+    int flags[2];
+    rc = read_flags(ctx, flags);
+    if (rc) return rc;
+    if (tk) lv_timer_flush(tk);
+    if (tc) lv_timer_flush(tc);
+    ctx->A.valid = 0;
+    if (flags[0] != 0) {
+        ctx->mesh_valid = false;
+        return lv_fail(ctx, flags[0], error_text(flags[0]));
+    }
+    ctx->mesh_valid = true;
+    return LV_OK;
+}
+
+int lv_mesh_size(lv_ctx *ctx, int64_t *nedges)
+{
+    int rc = check_ctx(ctx, true);
+    if (rc) return rc;
+    std::vector<int> ne(ctx->n);
+    LV_HIP_CHECK(hipMemcpy(ne.data(), ctx->ne, sizeof(int) * ctx->n, hipMemcpyDeviceToHost));
+    int64_t t = 0;
+    for (int v : ne) t += v;
+    *nedges = t;
+    return LV_OK;
+}
+
+int lv_get_mesh(lv_ctx *ctx, int32_t *edge_ptr, int32_t *labels, double *verts)
+{
+    int rc = check_ctx(ctx, true);
+    if (rc) return rc;
+    const int n = ctx->n;
+    std::vector<int> ne(n), el((size_t)LV_MAXE * n);
+    std::vector<double> ev((size_t)4 * LV_MAXE * n);
+    LV_HIP_CHECK(hipMemcpy(ne.data(), ctx->ne, sizeof(int) * n, hipMemcpyDeviceToHost));
+    LV_HIP_CHECK(hipMemcpy(el.data(), ctx->el, sizeof(int) * LV_MAXE * n, hipMemcpyDeviceToHost));
+    LV_HIP_CHECK(hipMemcpy(ev.data(), ctx->ev, sizeof(double) * 4 * LV_MAXE * n, hipMemcpyDeviceToHost));
+    int64_t o = 0;
+    for (int i = 0; i < n; i++) {
+        edge_ptr[i] = (int32_t)o;
+        for (int k = 0; k < ne[i]; k++) {
+            labels[o] = el[(size_t)k * n + i];
+            for (int p = 0; p < 4; p++) verts[4 * o + p] = ev[(size_t)(p * LV_MAXE + k) * n + i];
+            o++;
+        }
+    }
+    edge_ptr[n] = (int32_t)o;
+    return LV_OK;
+}
+
+int lv_cell_list_size(lv_ctx *ctx, int64_t *nitems)
+{
+    int rc = check_ctx(ctx, true);
+    if (rc) return rc;
+    *nitems = ctx->nitems;
+    return LV_OK;
+}
+
+int lv_get_cell_list(lv_ctx *ctx, int32_t *cell_start, int32_t *cell_labels)
+{
+    int rc = check_ctx(ctx, true);
+    if (rc) return rc;
+    const int64_t ncells = (int64_t)ctx->g.n1 * ctx->g.n2;
+    LV_HIP_CHECK(hipMemcpy(cell_start, ctx->cell_start, sizeof(int) * (ncells + 1), hipMemcpyDeviceToHost));
+    LV_HIP_CHECK(hipMemcpy(cell_labels, ctx->cell_labels, sizeof(int) * ctx->nitems, hipMemcpyDeviceToHost));
+    return LV_OK;
+}
+
+// ---- operators -------------------------------------------------------------------------------------
+
+int lv_move(lv_ctx *ctx, double dt)
+{
+    int rc = check_ctx(ctx, false);
+    if (rc) return rc;
+    rc = clear_flags(ctx);
+    if (rc) return rc;
+    LV_HIP_CHECK(lv_launch_move(ctx->g, ctx->F, mesh_view(ctx), ctx->n, dt, ctx->d_flags, ctx->s));
+    int flags[2];
+    rc = read_flags(ctx, flags);
+    if (rc) return rc;
+    ctx->mesh_valid = false;
+    if (flags[0]) return lv_fail(ctx, flags[0], error_text(flags[0]));
+    return lv_remesh(ctx);
+}
+
+int lv_gravity_step(lv_ctx *ctx, const double g[2], double dt)
+{
+    int rc = check_ctx(ctx, false);
+    if (rc) return rc;
+    LV_HIP_CHECK(lv_launch_gravity(ctx->F, ctx->n, g[0], g[1], dt, ctx->s));
+    return LV_OK;
+}
+
+int lv_find_rho(lv_ctx *ctx)
+{
+    int rc = check_ctx(ctx, true);
+    if (rc) return rc;
+    LV_HIP_CHECK(lv_launch_find_rho(ctx->F, ctx->n, ctx->s));
+    return LV_OK;
+}
+
+int lv_eos_stiffened(lv_ctx *ctx, double gamma, double P0)
+{
+    int rc = check_ctx(ctx, true);
+    if (rc) return rc;
+    LV_HIP_CHECK(lv_launch_eos(ctx->F, ctx->n, 1, gamma, P0, 0.0, ctx->s));
+    return LV_OK;
+}
+
+int lv_eos_ideal(lv_ctx *ctx, double gamma, double Pmin)
+{
+    int rc = check_ctx(ctx, true);
+    if (rc) return rc;
+    LV_HIP_CHECK(lv_launch_eos(ctx->F, ctx->n, 2, gamma, 0.0, Pmin, ctx->s));
+    return LV_OK;
+}
+
+int lv_assemble_pressure(lv_ctx *ctx, double dt, int64_t *nnz)
+{
+    int rc = check_ctx(ctx, true);
+    if (rc) return rc;
+    const int n = ctx->n;
+    lv_mesh_view M = mesh_view(ctx);
+    LV_HIP_CHECK(lv_launch_degree(M, n, ctx->deg, ctx->s));
+    LV_HIP_CHECK(lv_exclusive_scan_i32(ctx->deg, ctx->A.row_ptr, n + 1, ctx->d_nitems, ctx->scan_temp, ctx->s));
+    int total = 0;
+    LV_HIP_CHECK(hipMemcpyAsync(&total, ctx->d_nitems, sizeof(int), hipMemcpyDeviceToHost, ctx->s));
+    LV_HIP_CHECK(hipStreamSynchronize(ctx->s));
+    if (total > ctx->A.nnz_cap) {
+        dfree(ctx->A.cols);
+        dfree(ctx->A.val64);
+        dfree(ctx->A.val32);
+        int64_t want = (int64_t)total + total / 8 + 1024;
+        LV_HIP_CHECK(dalloc(&ctx->A.cols, want));
+        LV_HIP_CHECK(dalloc(&ctx->A.val64, want));
+        LV_HIP_CHECK(dalloc(&ctx->A.val32, want));
+        ctx->A.nnz_cap = want;
+    }
+    ctx->A.nnz = total;
+    rc = clear_flags(ctx);
+    if (rc) return rc;
+    LV_HIP_CHECK(lv_launch_assemble(ctx->g, ctx->F, M, n, dt, ctx->A, ctx->d_flags + 1, ctx->s));
+    int flags[2];
+    rc = read_flags(ctx, flags);
+    if (rc) return rc;
+    ctx->A.singular = (flags[1] == 0);
+    ctx->A.valid = 1;
+    if (nnz) *nnz = total;
+    return LV_OK;
+}
+
+int lv_get_pressure_operator(lv_ctx *ctx, double *diag, int32_t *row_ptr, int32_t *cols, double *coef)
+{
+    int rc = check_ctx(ctx, true);
+    if (rc) return rc;
+    if (!ctx->A.valid) return lv_fail(ctx, LV_E_INVALID, "pressure operator not assembled");
+    const int n = ctx->n;
+    LV_HIP_CHECK(hipMemcpy(diag, ctx->A.diag64, sizeof(double) * n, hipMemcpyDeviceToHost));
+    LV_HIP_CHECK(hipMemcpy(row_ptr, ctx->A.row_ptr, sizeof(int) * (n + 1), hipMemcpyDeviceToHost));
+    if (cols) LV_HIP_CHECK(hipMemcpy(cols, ctx->A.cols, sizeof(int) * ctx->A.nnz, hipMemcpyDeviceToHost));
+    if (coef) LV_HIP_CHECK(hipMemcpy(coef, ctx->A.val64, sizeof(double) * ctx->A.nnz, hipMemcpyDeviceToHost));
+    return LV_OK;
+}
+
+int lv_pressure_rhs(lv_ctx *ctx, double dt, int32_t gp_step, const double vbc[8], double *b, double *GP)
+{
+    int rc = check_ctx(ctx, true);
+    if (rc) return rc;
+    lv_vbc v;
+    for (int k = 0; k < 8; k++) v.v[k] = vbc ? vbc[k] : 0.0;
+    LV_HIP_CHECK(lv_launch_rhs(ctx->g, ctx->F, mesh_view(ctx), ctx->n, dt, gp_step, v, ctx->b, ctx->P0, ctx->GP,
+                               ctx->s));
+    if (b) LV_HIP_CHECK(hipMemcpyAsync(b, ctx->b, sizeof(double) * ctx->n, hipMemcpyDeviceToHost, ctx->s));
+    if (GP) LV_HIP_CHECK(hipMemcpyAsync(GP, ctx->GP, sizeof(double) * 2 * ctx->n, hipMemcpyDeviceToHost, ctx->s));
+    LV_HIP_CHECK(hipStreamSynchronize(ctx->s));
+    return LV_OK;
+}
+
+static void default_opts(lv_solve_opts &o)
+{
+    memset(&o, 0, sizeof(o));
+    o.method = LV_SOLVER_PCG64;
+    o.tol = 1e-10;
+    o.itmax = 1000;
+    o.inner_tol = 1e-4;
+    o.outer_max = 20;
+}
+
+// one linear solve on the assembled operator, device vectors b and x (x in/out)
+static int solve_on_device(lv_ctx *ctx, const double *db, double *dx, const lv_solve_opts &opt, lv_solve_stats *st)
+{
+    hipEvent_t e0, e1;
+    LV_HIP_CHECK(hipEventCreate(&e0));
+    LV_HIP_CHECK(hipEventCreate(&e1));
+    LV_HIP_CHECK(hipEventRecord(e0, ctx->s));
+    int iters = 0, sweeps = 0;
+    double relres = 0.0;
+    bool conv = false;
+    hipError_t e = lv_solve_device(ctx->A, ctx->W, ctx->n, db, dx, opt, ctx->s, timer_if_on(ctx, 0),
+                                   timer_if_on(ctx, 1), &iters, &sweeps, &relres, &conv);
+    if (e != hipSuccess) {
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+        return lv_fail_hip(ctx, e, "lv_solve_device", __FILE__, __LINE__);
+    }
+    LV_HIP_CHECK(hipEventRecord(e1, ctx->s));
+    LV_HIP_CHECK(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    hipEventElapsedTime(&ms, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (st && st->nsolves < 32) {
+        int k = st->nsolves;
+        st->iters[k] = iters;
+        st->outer[k] = sweeps;
+        st->relres[k] = relres;
+        st->solve_ms[k] = ms;
+    }
+    if (st) {
+        st->nsolves++;
+        st->total_iters += iters;
+    }
+    return conv ? LV_OK : LV_E_NOT_CONVERGED;
+}
+
+int lv_solve_pressure(lv_ctx *ctx, const double *b, double *x, const lv_solve_opts *opts, lv_solve_stats *stats)
+{
+    int rc = check_ctx(ctx, true);
+    if (rc) return rc;
+    if (!ctx->A.valid) return lv_fail(ctx, LV_E_INVALID, "pressure operator not assembled");
+    lv_solve_opts o;
+    if (opts) o = *opts;
+    else default_opts(o);
+    const int n = ctx->n;
+    LV_HIP_CHECK(hipMemcpyAsync(ctx->b, b, sizeof(double) * n, hipMemcpyHostToDevice, ctx->s));
+    LV_HIP_CHECK(hipMemcpyAsync(ctx->P0, x, sizeof(double) * n, hipMemcpyHostToDevice, ctx->s));
+    if (stats) memset(stats, 0, sizeof(*stats));
+    rc = solve_on_device(ctx, ctx->b, ctx->P0, o, stats);
+    LV_HIP_CHECK(hipMemcpyAsync(x, ctx->P0, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->s));
+    LV_HIP_CHECK(hipStreamSynchronize(ctx->s));
+    return rc;
+}
+
+// find_pressure!(solver, dt, niter; boundary_velocity), src/pressure.jl:215-224
+int lv_find_pressure(lv_ctx *ctx, double dt, int32_t niter, const lv_solve_opts *opts, lv_solve_stats *stats)
+{
+    int rc = check_ctx(ctx, true);
+    if (rc) return rc;
+    lv_solve_opts o;
+    if (opts) o = *opts;
+    else default_opts(o);
+    if (stats) memset(stats, 0, sizeof(*stats));
+    rc = lv_assemble_pressure(ctx, dt, nullptr);
+    if (rc) return rc;
+    lv_vbc v;
+    for (int k = 0; k < 8; k++) v.v[k] = o.vbc[k];
+    int status = LV_OK;
+    for (int it = 1; it <= niter; it++) {
+        LV_HIP_CHECK(lv_launch_rhs(ctx->g, ctx->F, mesh_view(ctx), ctx->n, dt, it > 1, v, ctx->b, ctx->P0, ctx->GP,
+                                   ctx->s));
+        rc = solve_on_device(ctx, ctx->b, ctx->P0, o, stats);
+        if (rc == LV_E_NOT_CONVERGED) status = rc;
+        else if (rc) return rc;
+        LV_HIP_CHECK(lv_launch_write_P(ctx->F, ctx->n, ctx->P0, ctx->s));
+    }
+    if (status) lv_fail(ctx, status, "pressure solve did not reach its tolerance");
+    return status;
+}
+
+int lv_pressure_step(lv_ctx *ctx, double dt)
+{
+    int rc = check_ctx(ctx, true);
+    if (rc) return rc;
+    LV_HIP_CHECK(lv_launch_pressure_step(ctx->g, ctx->F, mesh_view(ctx), ctx->n, dt, ctx->s));
+    return LV_OK;
+}
+
+int lv_find_D(lv_ctx *ctx)
+{
+    int rc = check_ctx(ctx, true);
+    if (rc) return rc;
+    LV_HIP_CHECK(lv_launch_find_D(ctx->g, ctx->F, mesh_view(ctx), ctx->n, ctx->s));
+    return LV_OK;
+}
+
+int lv_viscous_step(lv_ctx *ctx, double dt, int32_t artificial_viscosity)
+{
+    int rc = check_ctx(ctx, true);
+    if (rc) return rc;
+    const double avdr = artificial_viscosity ? ctx->g.dr : 0.0;  // src/diffusion.jl:40
+    LV_HIP_CHECK(lv_launch_viscous(ctx->g, ctx->F, mesh_view(ctx), ctx->n, dt, avdr, ctx->s));
+    return LV_OK;
+}
+
+int lv_bdary_friction(lv_ctx *ctx, const double vbc[8], double dt)
+{
+    int rc = check_ctx(ctx, true);
+    if (rc) return rc;
+    lv_vbc v;
+    for (int k = 0; k < 8; k++) v.v[k] = vbc ? vbc[k] : 0.0;
+    LV_HIP_CHECK(lv_launch_friction(ctx->F, mesh_view(ctx), ctx->n, v, dt, ctx->s));
+    return LV_OK;
+}
+
+int lv_find_dv(lv_ctx *ctx, double dt, double alpha)
+{
+    int rc = check_ctx(ctx, true);
+    if (rc) return rc;
+    LV_HIP_CHECK(lv_launch_find_dv(ctx->g, ctx->F, mesh_view(ctx), ctx->n, dt, alpha, ctx->s));
+    return LV_OK;
+}
+
+int lv_relaxation_step(lv_ctx *ctx, double dt, int32_t rusanov)
+{
+    int rc = check_ctx(ctx, true);
+    if (rc) return rc;
+    LV_HIP_CHECK(lv_launch_relaxation(ctx->g, ctx->F, mesh_view(ctx), ctx->n, dt, rusanov, ctx->s));
+    ctx->mesh_valid = false;
+    return lv_remesh(ctx);  // src/relaxation.jl:72
+}
+
+int lv_step_fused(lv_ctx *ctx, const lv_step_desc *d, lv_solve_stats *stats)
+{
+    if (!d) return LV_E_INVALID;
+    int rc = lv_move(ctx, d->dt);
+    if (rc) return rc;
+    if (d->gravity) {
+        rc = lv_gravity_step(ctx, d->g, d->dt);
+        if (rc) return rc;
+    }
+    if (d->eos == 1) rc = lv_eos_stiffened(ctx, d->gamma, d->P0);
+    else if (d->eos == 2) rc = lv_eos_ideal(ctx, d->gamma, d->Pmin);
+    if (rc) return rc;
+    int prc = lv_find_pressure(ctx, d->dt, d->niter, &d->solve, stats);
+    if (prc && prc != LV_E_NOT_CONVERGED) return prc;
+    rc = lv_pressure_step(ctx, d->dt);
+    if (rc) return rc;
+    rc = lv_find_D(ctx);
+    if (rc) return rc;
+    if (d->viscous) {
+        rc = lv_viscous_step(ctx, d->dt, d->artificial_viscosity);
+        if (rc) return rc;
+    }
+    if (d->friction) {
+        rc = lv_bdary_friction(ctx, d->friction_vbc, d->dt);
+        if (rc) return rc;
+    }
+    rc = lv_find_dv(ctx, d->dt, d->alpha);
+    if (rc) return rc;
+    rc = lv_relaxation_step(ctx, d->dt, d->rusanov);
+    if (rc) return rc;
+    return prc;
+}
+
+// ---- timing ------------------------------------------------------------------------------------
+
+int lv_timing_enable(lv_ctx *ctx, int32_t kernel_id, int32_t on)
+{
+    if (!ctx || kernel_id < 0 || kernel_id >= LV_NKTIMERS) return LV_E_INVALID;
+    lv_timer *t = ctx->timers[kernel_id];
+    LV_HIP_CHECK(hipStreamSynchronize(ctx->s));
+    lv_timer_flush(t);
+    t->on = on ? 1 : 0;
+    t->total_ms = 0.0;
+    t->launches = 0;
+    return LV_OK;
+}
+
+int lv_timing_read(lv_ctx *ctx, int32_t kernel_id, double *total_ms, int64_t *launches, double *bytes_per_launch)
+{
+    if (!ctx || kernel_id < 0 || kernel_id >= LV_NKTIMERS) return LV_E_INVALID;
+    lv_timer *t = ctx->timers[kernel_id];
+    LV_HIP_CHECK(hipStreamSynchronize(ctx->s));
+    lv_timer_flush(t);
+    if (total_ms) *total_ms = t->total_ms;
+    if (launches) *launches = t->launches;
+    if (bytes_per_launch) {
+        // algorithmic (compulsory) bytes of one launch at the current sizes
+        const double n = ctx->n, nnz = (double)ctx->A.nnz, m = ctx->nitems;
+        const double nc = (double)ctx->g.n1 * ctx->g.n2;
+        double b = 0.0;
+        if (kernel_id == 0) b = 4.0 * (n + 1) + 8.0 * nnz + 4.0 * n * 3;       // row_ptr, cols+vals, diag, p, q
+        else if (kernel_id == 1) b = 4.0 * (n + 1) + 12.0 * nnz + 8.0 * n * 3;  // FP64 values
+        else if (kernel_id == 2) b = 16.0 * n + 20.0 * m + 4.0 * (nc + 1) + 4.0 * n + 36.0 * 6.0 * n + 24.0 * n;
+        else if (kernel_id == 3) b = 3.0 * (8.0 + 8.0 + 4.0) * m;              // 3 radix passes over (key, label)
+        *bytes_per_launch = b;
+    }
+    return LV_OK;
+}
+
+}  // extern "C"

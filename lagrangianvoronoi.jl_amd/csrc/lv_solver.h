@@ -1,0 +1,54 @@
+// lv_solver.h -- pressure-system storage, solver workspace and kernel timers of liblvhip.so.
+#pragma once
+
+#include "lv_internal.h"
+
+// Device-resident CG scalars: written by the last block of each reduction, read by the next launch.
+struct lv_cg_scalars {
+    double rz, rr, pq, alpha, beta, bb, thr2, tol2, inner_tol2, rr64, bmean, xmean0, xmean;
+    int iters, itmax, done, singular;
+    unsigned ticket;
+    int pad;
+};
+
+// PressureOperator (src/pressure.jl:89-101) in CSR: off-diagonal coefficients c_ij in the edge order
+// of row i, the compressibility diagonal d_i, and the Jacobi inverse 1/(d_i + sum_j c_ij), each in
+// FP64 and FP32.
+struct lv_pressure_system {
+    int *row_ptr, *cols;
+    double *val64, *diag64, *dinv64;
+    float *val32, *diag32, *dinv32;
+    int64_t nnz, nnz_cap;
+    int singular;  // every d_i == 0 (c2 = Inf): a graph Laplacian with the constants as null space
+    int valid;
+};
+
+struct lv_solver_work {
+    double *x, *r, *z, *p, *q;
+    float *e32, *r32, *z32, *p32, *q32;
+    double *partials;
+    lv_cg_scalars *S;
+};
+
+#define LV_TPOOL 1024
+struct lv_timer {
+    int on, used;
+    double total_ms;
+    int64_t launches;
+    double bytes_per_launch;
+    hipEvent_t ev[2 * LV_TPOOL];
+};
+void lv_timer_begin(lv_timer *t, hipStream_t s);
+void lv_timer_end(lv_timer *t, hipStream_t s);
+void lv_timer_flush(lv_timer *t);
+
+size_t lv_partials_bytes(int n);
+hipError_t lv_launch_degree(lv_mesh_view M, int n, int *deg, hipStream_t s);
+hipError_t lv_launch_assemble(const lv_grid_params &g, lv_fields F, lv_mesh_view M, int n, double dt,
+                              lv_pressure_system &A, int *nonsingular, hipStream_t s);
+hipError_t lv_launch_rhs(const lv_grid_params &g, lv_fields F, lv_mesh_view M, int n, double dt, int gp_step,
+                         lv_vbc vbc, double *b, double *P0, double *GP, hipStream_t s);
+hipError_t lv_launch_write_P(lv_fields F, int n, const double *P, hipStream_t s);
+hipError_t lv_solve_device(const lv_pressure_system &A, lv_solver_work &W, int n, const double *b, double *x,
+                           const lv_solve_opts &opt, hipStream_t s, lv_timer *t32, lv_timer *t64, int *iters,
+                           int *sweeps, double *relres, bool *converged);

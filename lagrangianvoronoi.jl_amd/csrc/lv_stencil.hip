@@ -1,0 +1,488 @@
+// lv_stencil.hip -- per-particle physics operators over the Voronoi mesh: move!, gravity_step!,
+// find_rho!, the equations of state, pressure_step!, find_D!, viscous_step!, bdary_friction!,
+// find_dv! and relaxation_step!.
+//
+// One thread per particle; each walks its own ELL edge slots (coalesced across the wavefront) and
+// gathers neighbour fields by label.  Every two-pass operator is two launches, because the second
+// pass of the reference reads neighbour values the first pass wrote (e.g. pressure_step! reads the
+// updated q.v).  A first pass only ever writes fields of its own particle that no neighbour reads in
+// that pass, so it updates in place exactly as the reference's @batch loops do.  Expression order
+// follows the reference line cited at each kernel; norms are sqrt(a*a + b*b).
+#include "lv_internal.h"
+
+namespace {
+
+constexpr int B = 256;
+
+__device__ __forceinline__ void nbr_y(const lv_grid_params &g, const lv_fields &F, int n, int j, double x1, double x2,
+                                      double &y1, double &y2)
+{
+    lv_arrow_target(g, F.x[j], F.x[n + j], x1, x2, y1, y2);
+}
+
+// least_positive_residue, src/voronoigrid.jl:181-183
+__device__ __forceinline__ double lpr(double x, double d) { return fmod(fmod(x, d) + d, d); }
+
+// try_move!, src/move.jl:23-33: 1 moved, 0 not inside, -1 NaN velocity
+__device__ __forceinline__ int try_move(const lv_grid_params &g, double &x1, double &x2, double v1, double v2,
+                                        double dt)
+{
+    if (isnan(v1) || isnan(v2)) return -1;
+    const double a1 = x1 + dt * v1, a2 = x2 + dt * v2;
+    // periodic_wrap, src/voronoigrid.jl:186-192 (Bool*Float64 gives a signed zero when false)
+    const double wx = lpr(a1 - g.bxmin0, g.xperiod) + g.bxmin0;
+    const double wy = lpr(a2 - g.bxmin1, g.yperiod) + g.bxmin1;
+    const double tx = g.xperiodic ? (wx - a1) : copysign(0.0, wx - a1);
+    const double ty = g.yperiodic ? (wy - a2) : copysign(0.0, wy - a2);
+    const double r1 = (a1 + tx * 1.0) + ty * 0.0;
+    const double r2 = (a2 + tx * 0.0) + ty * 1.0;
+    if ((g.bxmin0 <= r1 && r1 <= g.bxmax0) && (g.bxmin1 <= r2 && r2 <= g.bxmax1)) {
+        x1 = r1;
+        x2 = r2;
+        return 1;
+    }
+    return 0;
+}
+
+// move!, src/move.jl:9-21 (the position update; the caller remeshes)
+__global__ __launch_bounds__(B) void k_move(lv_grid_params g, lv_fields F, lv_mesh_view M, int n, double dt,
+                                            int *err)
+{
+    const int i = blockIdx.x * B + threadIdx.x;
+    if (i >= n) return;
+    double x1 = F.x[i], x2 = F.x[n + i];
+    double v1 = F.v[i], v2 = F.v[n + i];
+    int r = try_move(g, x1, x2, v1, v2, dt);
+    if (r == 0) {
+        const int ne = M.ne[i];
+        for (int k = 0; k < ne; k++) {  // boundaries(p): project v to the tangent space
+            if (LV_EL(M, k, i) >= 0) continue;
+            double n1 = LV_EV(M, 1, k, i) - LV_EV(M, 3, k, i);
+            double n2 = LV_EV(M, 2, k, i) - LV_EV(M, 0, k, i);
+            const double nn = lv_norm2(n1, n2);  // normal_vector(e), src/polygon.jl:153-156
+            n1 = n1 / nn;
+            n2 = n2 / nn;
+            const double vn = v1 * n1 + v2 * n2;
+            v1 = v1 - vn * n1;
+            v2 = v2 - vn * n2;
+        }
+        r = try_move(g, x1, x2, v1, v2, dt);
+        if (r == 0) {
+            v1 = 0.0;
+            v2 = 0.0;
+            r = try_move(g, x1, x2, v1, v2, dt);
+        }
+    }
+    if (r < 0) atomicCAS(err, 0, LV_E_NAN_VELOCITY);
+    F.x[i] = x1;
+    F.x[n + i] = x2;
+    F.v[i] = v1;
+    F.v[n + i] = v2;
+}
+
+// gravity_step!, src/pressure.jl:77-82
+__global__ __launch_bounds__(B) void k_gravity(lv_fields F, int n, double g1, double g2, double dt)
+{
+    const int i = blockIdx.x * B + threadIdx.x;
+    if (i >= n) return;
+    F.v[i] += dt * g1;
+    F.v[n + i] += dt * g2;
+}
+
+// find_rho!, src/pressure.jl:36-40
+__global__ __launch_bounds__(B) void k_find_rho(lv_fields F, int n)
+{
+    const int i = blockIdx.x * B + threadIdx.x;
+    if (i >= n) return;
+    F.rho[i] = F.mass[i] / F.area[i];
+}
+
+// stiffened_eos! (kind 1, src/pressure.jl:64-70) and ideal_eos! (kind 2, :49-55)
+__global__ __launch_bounds__(B) void k_eos(lv_fields F, int n, int kind, double gamma, double P0, double Pmin)
+{
+    const int i = blockIdx.x * B + threadIdx.x;
+    if (i >= n) return;
+    const double rho = F.mass[i] / F.area[i];
+    const double v1 = F.v[i], v2 = F.v[n + i];
+    const double ein = F.e[i] - 0.5 * (v1 * v1 + v2 * v2);  // eint, :32-34
+    const double P = ((gamma - 1.0) * rho) * ein;
+    F.rho[i] = rho;
+    F.P[i] = P;
+    if (kind == 1) {
+        F.c2[i] = (gamma * (P + P0)) / rho;
+    } else {
+        const double pm = (Pmin > P) ? Pmin : P;
+        F.c2[i] = (gamma * pm) / rho;
+    }
+}
+
+// pressure_step! pass 1, src/pressure.jl:11-17: v += dt/m * lrr * (P_i - P_j) * (m - x)
+__global__ __launch_bounds__(B) void k_pstep1(lv_grid_params g, lv_fields F, lv_mesh_view M, int n, double dt)
+{
+    const int i = blockIdx.x * B + threadIdx.x;
+    if (i >= n) return;
+    const double x1 = F.x[i], x2 = F.x[n + i];
+    double v1 = F.v[i], v2 = F.v[n + i];
+    const double Pi = F.P[i], dtm = dt / F.mass[i];
+    const int ne = M.ne[i];
+    for (int k = 0; k < ne; k++) {
+        const int j = LV_EL(M, k, i);
+        if (j < 0) continue;
+        const double e1x = LV_EV(M, 0, k, i), e1y = LV_EV(M, 1, k, i), e2x = LV_EV(M, 2, k, i), e2y = LV_EV(M, 3, k, i);
+        double y1, y2;
+        nbr_y(g, F, n, j, x1, x2, y1, y2);
+        const double lrr = lv_lr_ratio(x1 - y1, x2 - y2, e1x, e1y, e2x, e2y);
+        const double m1 = 0.5 * (e1x + e2x), m2 = 0.5 * (e1y + e2y);
+        const double w = (dtm * lrr) * (Pi - F.P[j]);
+        v1 = v1 + w * (m1 - x1);
+        v2 = v2 + w * (m2 - x2);
+    }
+    F.v[i] = v1;
+    F.v[n + i] = v2;
+}
+
+// pressure_step! pass 2, src/pressure.jl:18-24: e -= dt*lrr/m * ((m-x).(P_i v_i) - (m-y).(P_j v_j))
+__global__ __launch_bounds__(B) void k_pstep2(lv_grid_params g, lv_fields F, lv_mesh_view M, int n, double dt)
+{
+    const int i = blockIdx.x * B + threadIdx.x;
+    if (i >= n) return;
+    const double x1 = F.x[i], x2 = F.x[n + i];
+    const double Pi = F.P[i], mi = F.mass[i];
+    const double pv1 = Pi * F.v[i], pv2 = Pi * F.v[n + i];
+    double ei = F.e[i];
+    const int ne = M.ne[i];
+    for (int k = 0; k < ne; k++) {
+        const int j = LV_EL(M, k, i);
+        if (j < 0) continue;
+        const double e1x = LV_EV(M, 0, k, i), e1y = LV_EV(M, 1, k, i), e2x = LV_EV(M, 2, k, i), e2y = LV_EV(M, 3, k, i);
+        double y1, y2;
+        nbr_y(g, F, n, j, x1, x2, y1, y2);
+        const double lrr = lv_lr_ratio(x1 - y1, x2 - y2, e1x, e1y, e2x, e2y);
+        const double m1 = 0.5 * (e1x + e2x), m2 = 0.5 * (e1y + e2y);
+        const double Pj = F.P[j];
+        const double qv1 = Pj * F.v[j], qv2 = Pj * F.v[n + j];
+        const double a = (m1 - x1) * pv1 + (m2 - x2) * pv2;
+        const double b = (m1 - y1) * qv1 + (m2 - y2) * qv2;
+        ei = ei - ((dt * lrr) / mi) * (a - b);
+    }
+    F.e[i] = ei;
+}
+
+// find_D!, src/diffusion.jl:8-19
+__global__ __launch_bounds__(B) void k_find_D(lv_grid_params g, lv_fields F, lv_mesh_view M, int n)
+{
+    const int i = blockIdx.x * B + threadIdx.x;
+    if (i >= n) return;
+    const double x1 = F.x[i], x2 = F.x[n + i];
+    const double pv1 = F.v[i], pv2 = F.v[n + i];
+    double D11 = 0.0, D21 = 0.0, D12 = 0.0, D22 = 0.0;
+    const int ne = M.ne[i];
+    for (int k = 0; k < ne; k++) {
+        const int j = LV_EL(M, k, i);
+        if (j < 0) continue;
+        const double e1x = LV_EV(M, 0, k, i), e1y = LV_EV(M, 1, k, i), e2x = LV_EV(M, 2, k, i), e2y = LV_EV(M, 3, k, i);
+        double y1, y2;
+        nbr_y(g, F, n, j, x1, x2, y1, y2);
+        const double m1 = 0.5 * (e1x + e2x), m2 = 0.5 * (e1y + e2y);
+        const double lrr = lv_lr_ratio(x1 - y1, x2 - y2, e1x, e1y, e2x, e2y);
+        const double a1 = pv1 - F.v[j], a2 = pv2 - F.v[n + j];
+        const double b1 = m1 - y1, b2 = m2 - y2;
+        D11 = D11 + lrr * (a1 * b1);
+        D21 = D21 + lrr * (a2 * b1);
+        D12 = D12 + lrr * (a1 * b2);
+        D22 = D22 + lrr * (a2 * b2);
+    }
+    const double A = F.area[i];
+    D11 = D11 / A;
+    D21 = D21 / A;
+    D12 = D12 / A;
+    D22 = D22 / A;
+    F.D[i] = 0.5 * (D11 + D11);
+    F.D[n + i] = 0.5 * (D21 + D12);
+    F.D[2 * n + i] = 0.5 * (D12 + D21);
+    F.D[3 * n + i] = 0.5 * (D22 + D22);
+}
+
+// getS, src/diffusion.jl:22-29 (column-major S)
+__device__ __forceinline__ void getS(const lv_fields &F, int n, int i, double dr, double S[4])
+{
+    const double D11 = F.D[i], D21 = F.D[n + i], D12 = F.D[2 * n + i], D22 = F.D[3 * n + i];
+    const double divv = ((D11 * 1.0 + D21 * 0.0) + D12 * 0.0) + D22 * 1.0;
+    double mu = F.mu[i];
+    if (divv < 0.0) mu -= (divv * F.rho[i]) * (dr * dr);
+    const double tm = 2.0 * mu;
+    S[0] = tm * (D11 - (divv * 1.0) / 3.0);
+    S[1] = tm * (D21 - (divv * 0.0) / 3.0);
+    S[2] = tm * (D12 - (divv * 0.0) / 3.0);
+    S[3] = tm * (D22 - (divv * 1.0) / 3.0);
+}
+
+// viscous_step! pass 1, src/diffusion.jl:41-46: v -= dt*lrr/m * (S_p - S_q) * (m - x)
+// (three-argument s*M*w evaluated as M*(w*s))
+__global__ __launch_bounds__(B) void k_visc1(lv_grid_params g, lv_fields F, lv_mesh_view M, int n, double dt,
+                                             double avdr)
+{
+    const int i = blockIdx.x * B + threadIdx.x;
+    if (i >= n) return;
+    const double x1 = F.x[i], x2 = F.x[n + i];
+    double v1 = F.v[i], v2 = F.v[n + i];
+    double Sp[4];
+    getS(F, n, i, avdr, Sp);
+    const double mi = F.mass[i];
+    const int ne = M.ne[i];
+    for (int k = 0; k < ne; k++) {
+        const int j = LV_EL(M, k, i);
+        if (j < 0) continue;
+        const double e1x = LV_EV(M, 0, k, i), e1y = LV_EV(M, 1, k, i), e2x = LV_EV(M, 2, k, i), e2y = LV_EV(M, 3, k, i);
+        double y1, y2;
+        nbr_y(g, F, n, j, x1, x2, y1, y2);
+        const double m1 = 0.5 * (e1x + e2x), m2 = 0.5 * (e1y + e2y);
+        double Sq[4];
+        getS(F, n, j, avdr, Sq);
+        const double sc = (dt * lv_lr_ratio(x1 - y1, x2 - y2, e1x, e1y, e2x, e2y)) / mi;
+        const double M11 = Sp[0] - Sq[0], M21 = Sp[1] - Sq[1], M12 = Sp[2] - Sq[2], M22 = Sp[3] - Sq[3];
+        const double w1 = (m1 - x1) * sc, w2 = (m2 - x2) * sc;
+        v1 = v1 - (M11 * w1 + M12 * w2);
+        v2 = v2 - (M21 * w1 + M22 * w2);
+    }
+    F.v[i] = v1;
+    F.v[n + i] = v2;
+}
+
+// viscous_step! pass 2, src/diffusion.jl:47-52
+__global__ __launch_bounds__(B) void k_visc2(lv_grid_params g, lv_fields F, lv_mesh_view M, int n, double dt,
+                                             double avdr)
+{
+    const int i = blockIdx.x * B + threadIdx.x;
+    if (i >= n) return;
+    const double x1 = F.x[i], x2 = F.x[n + i];
+    double ei = F.e[i];
+    double Sp[4];
+    getS(F, n, i, avdr, Sp);
+    const double pv1 = F.v[i], pv2 = F.v[n + i];
+    const double sv1 = Sp[0] * pv1 + Sp[2] * pv2;
+    const double sv2 = Sp[1] * pv1 + Sp[3] * pv2;
+    const double mi = F.mass[i];
+    const int ne = M.ne[i];
+    for (int k = 0; k < ne; k++) {
+        const int j = LV_EL(M, k, i);
+        if (j < 0) continue;
+        const double e1x = LV_EV(M, 0, k, i), e1y = LV_EV(M, 1, k, i), e2x = LV_EV(M, 2, k, i), e2y = LV_EV(M, 3, k, i);
+        double y1, y2;
+        nbr_y(g, F, n, j, x1, x2, y1, y2);
+        const double m1 = 0.5 * (e1x + e2x), m2 = 0.5 * (e1y + e2y);
+        double Sq[4];
+        getS(F, n, j, avdr, Sq);
+        const double qv1 = Sq[0] * F.v[j] + Sq[2] * F.v[n + j];
+        const double qv2 = Sq[1] * F.v[j] + Sq[3] * F.v[n + j];
+        const double a = (m1 - x1) * sv1 + (m2 - x2) * sv2;
+        const double b = (m1 - y1) * qv1 + (m2 - y2) * qv2;
+        ei = ei + ((dt * lv_lr_ratio(x1 - y1, x2 - y2, e1x, e1y, e2x, e2y)) / mi) * (a - b);
+    }
+    F.e[i] = ei;
+}
+
+// bdary_friction!, src/diffusion.jl:64-78, vDirichlet(m) = vbc[label] and charfun = everywhere
+__global__ __launch_bounds__(B) void k_friction(lv_fields F, lv_mesh_view M, int n, lv_vbc vbc, double dt)
+{
+    const int i = blockIdx.x * B + threadIdx.x;
+    if (i >= n) return;
+    const double x1 = F.x[i], x2 = F.x[n + i];
+    double v1 = F.v[i], v2 = F.v[n + i];
+    double ei = F.e[i];
+    const double mu = F.mu[i], mi = F.mass[i];
+    double tmp = 1.0;
+    const int ne = M.ne[i];
+    for (int k = 0; k < ne; k++) {
+        const int lab = LV_EL(M, k, i);
+        if (lab >= 0) continue;
+        const double e1x = LV_EV(M, 0, k, i), e1y = LV_EV(M, 1, k, i), e2x = LV_EV(M, 2, k, i), e2y = LV_EV(M, 3, k, i);
+        const double m1 = 0.5 * (e1x + e2x), m2 = 0.5 * (e1y + e2y);
+        double n1 = e1y - e2y, n2 = e2x - e1x;
+        const double nn = lv_norm2(n1, n2);
+        n1 = n1 / nn;
+        n2 = n2 / nn;
+        const double len = lv_norm2(e1x - e2x, e1y - e2y);
+        const double lrr = len / fabs((m1 - x1) * n1 + (m2 - x2) * n2);
+        const int slot = -lab - 1;
+        const double vd1 = vbc.v[2 * slot], vd2 = vbc.v[2 * slot + 1];
+        const double ml = mu * lrr;
+        const double f1 = (ml * vd1) / mi, f2 = (ml * vd2) / mi;
+        tmp += ((dt * mu) * lrr) / mi;
+        ei += dt * (f1 * v1 + f2 * v2);
+        v1 = v1 + dt * f1;
+        v2 = v2 + dt * f2;
+    }
+    F.e[i] = ei;
+    F.v[i] = v1 / tmp;
+    F.v[n + i] = v2 / tmp;
+}
+
+// find_dv!, src/relaxation.jl:10-25 (centroid(p) cached by the clipper, same expression order)
+__global__ __launch_bounds__(B) void k_find_dv(lv_grid_params g, lv_fields F, lv_mesh_view M, int n, double dt,
+                                               double alpha)
+{
+    const int i = blockIdx.x * B + threadIdx.x;
+    if (i >= n) return;
+    const double x1 = F.x[i], x2 = F.x[n + i];
+    double rmax = 0.0, rmin = __builtin_inf();
+    const int ne = M.ne[i];
+    for (int k = 0; k < ne; k++) {
+        const int j = LV_EL(M, k, i);
+        if (j < 0) continue;
+        double y1, y2;
+        nbr_y(g, F, n, j, x1, x2, y1, y2);
+        const double r = lv_norm2(x1 - y1, x2 - y2);
+        if (r > rmax) rmax = r;
+        if (r < rmin) rmin = r;
+    }
+    const double q = rmin / rmax;
+    F.quality[i] = q;
+    const double D11 = F.D[i], D21 = F.D[n + i], D12 = F.D[2 * n + i], D22 = F.D[3 * n + i];
+    const double nD = sqrt(((D11 * D11 + D21 * D21) + D12 * D12) + D22 * D22);
+    const double lambda = (alpha * nD) / (q * q);
+    const double f = lambda / (1.0 + dt * lambda);
+    F.dv[i] = f * (F.cent[i] - x1);
+    F.dv[n + i] = f * (F.cent[n + i] - x2);
+}
+
+// relaxation_step! pass 1, src/relaxation.jl:37-59 (writes mass, momentum, energy of p only)
+__global__ __launch_bounds__(B) void k_relax1(lv_grid_params g, lv_fields F, lv_mesh_view M, int n, double dt,
+                                              int rusanov)
+{
+    const int i = blockIdx.x * B + threadIdx.x;
+    if (i >= n) return;
+    const double x1 = F.x[i], x2 = F.x[n + i];
+    const double rp = F.rho[i], ep = F.e[i];
+    const double pv1 = F.v[i], pv2 = F.v[n + i];
+    const double pd1 = F.dv[i], pd2 = F.dv[n + i];
+    const int php = F.phase[i];
+    double mass = F.mass[i];
+    double mo1 = mass * pv1, mo2 = mass * pv2;
+    double en = mass * ep;
+    const double na = lv_norm2(pd1, pd2);
+    const int ne = M.ne[i];
+    for (int k = 0; k < ne; k++) {
+        const int j = LV_EL(M, k, i);
+        if (j < 0) continue;
+        if (!(php == F.phase[j])) continue;
+        const double e1x = LV_EV(M, 0, k, i), e1y = LV_EV(M, 1, k, i), e2x = LV_EV(M, 2, k, i), e2y = LV_EV(M, 3, k, i);
+        double y1, y2;
+        nbr_y(g, F, n, j, x1, x2, y1, y2);
+        const double lrr = lv_lr_ratio(x1 - y1, x2 - y2, e1x, e1y, e2x, e2y);
+        const double m1 = 0.5 * (e1x + e2x), m2 = 0.5 * (e1y + e2y);
+        const double z1 = 0.5 * (x1 + y1), z2 = 0.5 * (x2 + y2);
+        const double rq = F.rho[j], eq = F.e[j];
+        const double qv1 = F.v[j], qv2 = F.v[n + j];
+        const double qd1 = F.dv[j], qd2 = F.dv[n + j];
+        const double pdvpq = pd1 * (x1 - y1) + pd2 * (x2 - y2);
+        const double qdvpq = qd1 * (x1 - y1) + qd2 * (x2 - y2);
+        const double pdvmz = pd1 * (m1 - z1) + pd2 * (m2 - z2);
+        const double qdvmz = qd1 * (m1 - z1) + qd2 * (m2 - z2);
+        const double dl = dt * lrr;
+        const double pr_mz = pdvmz * rp, qr_mz = qdvmz * rq, pr_pq = pdvpq * rp, qr_pq = qdvpq * rq;
+        mass += dl * ((pr_mz - qr_mz) - 0.5 * (pr_pq + qr_pq));
+        mo1 += dl * ((pr_mz * pv1 - qr_mz * qv1) - 0.5 * (pr_pq * pv1 + qr_pq * qv1));
+        mo2 += dl * ((pr_mz * pv2 - qr_mz * qv2) - 0.5 * (pr_pq * pv2 + qr_pq * qv2));
+        en += dl * ((pr_mz * ep - qr_mz * eq) - 0.5 * (pr_pq * ep + qr_pq * eq));
+        if (rusanov) {  // src/relaxation.jl:52-58
+            const double nb = lv_norm2(qd1, qd2);
+            const double a = (nb > na) ? nb : na;
+            const double l = lv_norm2(e1x - e2x, e1y - e2y);
+            const double w = ((0.5 * dt) * l) * a;
+            mass += w * (rq - rp);
+            mo1 += w * (rq * qv1 - rp * pv1);
+            mo2 += w * (rq * qv2 - rp * pv2);
+            en += w * (rq * eq - rp * ep);
+        }
+    }
+    F.mass[i] = mass;
+    F.mom[i] = mo1;
+    F.mom[n + i] = mo2;
+    F.energy[i] = en;
+}
+
+// relaxation_step! pass 2, src/relaxation.jl:60-64
+__global__ __launch_bounds__(B) void k_relax2(lv_fields F, int n, double dt)
+{
+    const int i = blockIdx.x * B + threadIdx.x;
+    if (i >= n) return;
+    const double m = F.mass[i];
+    F.v[i] = F.mom[i] / m;
+    F.v[n + i] = F.mom[n + i] / m;
+    F.e[i] = F.energy[i] / m;
+    F.x[i] += dt * F.dv[i];
+    F.x[n + i] += dt * F.dv[n + i];
+}
+
+inline dim3 grid_of(int n) { return dim3((n + B - 1) / B); }
+
+}  // namespace
+
+hipError_t lv_launch_move(const lv_grid_params &g, lv_fields F, lv_mesh_view M, int n, double dt, int *err,
+                          hipStream_t s)
+{
+    hipLaunchKernelGGL(k_move, grid_of(n), dim3(B), 0, s, g, F, M, n, dt, err);
+    return hipGetLastError();
+}
+
+hipError_t lv_launch_gravity(lv_fields F, int n, double g1, double g2, double dt, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_gravity, grid_of(n), dim3(B), 0, s, F, n, g1, g2, dt);
+    return hipGetLastError();
+}
+
+hipError_t lv_launch_find_rho(lv_fields F, int n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_find_rho, grid_of(n), dim3(B), 0, s, F, n);
+    return hipGetLastError();
+}
+
+hipError_t lv_launch_eos(lv_fields F, int n, int kind, double gamma, double P0, double Pmin, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_eos, grid_of(n), dim3(B), 0, s, F, n, kind, gamma, P0, Pmin);
+    return hipGetLastError();
+}
+
+hipError_t lv_launch_pressure_step(const lv_grid_params &g, lv_fields F, lv_mesh_view M, int n, double dt,
+                                   hipStream_t s)
+{
+    hipLaunchKernelGGL(k_pstep1, grid_of(n), dim3(B), 0, s, g, F, M, n, dt);
+    hipLaunchKernelGGL(k_pstep2, grid_of(n), dim3(B), 0, s, g, F, M, n, dt);
+    return hipGetLastError();
+}
+
+hipError_t lv_launch_find_D(const lv_grid_params &g, lv_fields F, lv_mesh_view M, int n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_find_D, grid_of(n), dim3(B), 0, s, g, F, M, n);
+    return hipGetLastError();
+}
+
+hipError_t lv_launch_viscous(const lv_grid_params &g, lv_fields F, lv_mesh_view M, int n, double dt, double avdr,
+                             hipStream_t s)
+{
+    hipLaunchKernelGGL(k_visc1, grid_of(n), dim3(B), 0, s, g, F, M, n, dt, avdr);
+    hipLaunchKernelGGL(k_visc2, grid_of(n), dim3(B), 0, s, g, F, M, n, dt, avdr);
+    return hipGetLastError();
+}
+
+hipError_t lv_launch_friction(lv_fields F, lv_mesh_view M, int n, lv_vbc vbc, double dt, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_friction, grid_of(n), dim3(B), 0, s, F, M, n, vbc, dt);
+    return hipGetLastError();
+}
+
+hipError_t lv_launch_find_dv(const lv_grid_params &g, lv_fields F, lv_mesh_view M, int n, double dt, double alpha,
+                             hipStream_t s)
+{
+    hipLaunchKernelGGL(k_find_dv, grid_of(n), dim3(B), 0, s, g, F, M, n, dt, alpha);
+    return hipGetLastError();
+}
+
+hipError_t lv_launch_relaxation(const lv_grid_params &g, lv_fields F, lv_mesh_view M, int n, double dt, int rusanov,
+                                hipStream_t s)
+{
+    hipLaunchKernelGGL(k_relax1, grid_of(n), dim3(B), 0, s, g, F, M, n, dt, rusanov);
+    hipLaunchKernelGGL(k_relax2, grid_of(n), dim3(B), 0, s, F, n, dt);
+    return hipGetLastError();
+}

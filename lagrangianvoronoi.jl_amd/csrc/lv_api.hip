@@ -632,9 +632,11 @@ int lv_assemble_pressure(lv_ctx *ctx, double dt, int64_t *nnz)
     const int n = ctx->n;
     lv_mesh_view M = mesh_view(ctx);
     LV_HIP_CHECK(lv_launch_degree(M, n, ctx->deg, ctx->s));
-    LV_HIP_CHECK(lv_exclusive_scan_i32(ctx->deg, ctx->A.row_ptr, n + 1, ctx->d_nitems, ctx->scan_temp, ctx->s));
+    // row_ptr[0..n] = exclusive scan of the n degrees followed by one extra slot: deg[n] is 0
+    LV_HIP_CHECK(hipMemsetAsync(ctx->deg + n, 0, sizeof(int), ctx->s));
+    LV_HIP_CHECK(lv_exclusive_scan_i32(ctx->deg, ctx->A.row_ptr, n + 1, nullptr, ctx->scan_temp, ctx->s));
     int total = 0;
-    LV_HIP_CHECK(hipMemcpyAsync(&total, ctx->d_nitems, sizeof(int), hipMemcpyDeviceToHost, ctx->s));
+    LV_HIP_CHECK(hipMemcpyAsync(&total, ctx->A.row_ptr + n, sizeof(int), hipMemcpyDeviceToHost, ctx->s));
     LV_HIP_CHECK(hipStreamSynchronize(ctx->s));
     if (total > ctx->A.nnz_cap) {
         dfree(ctx->A.cols);

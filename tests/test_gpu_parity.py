@@ -1,0 +1,223 @@
+"""GPU parity: every hot-path kernel through the C ABI against the oracle on identical seeded input.
+
+Bars: cell lists, neighbour topology and edge order bit-exact; vertices bit-exact (same cut order,
+no FMA); areas, centroids, velocities, energies, tensors to 1e-12 relative (they come out bitwise
+equal here, the bound is what the contract asks); pressure to the solver tolerance.
+"""
+import numpy as np
+import pytest
+
+from pairs import make_pair, rel_err
+
+pytestmark = pytest.mark.gpu
+
+KINDS = ["periodic", "walls", "xperiodic", "tall"]
+
+
+def _flush(msg):
+    print(msg, flush=True)
+
+
+@pytest.mark.parametrize("kind", KINDS)
+def test_cell_list_bit_exact(lv, oracle, gpu, kind):
+    _flush("cell list %s" % kind)
+    grid, op, x, dr = make_pair(lv, oracle, kind, K=48, seed=0x5EED0010, ic=None)
+    cs_g, cl_g = grid.cell_list()
+    cs_o, cl_o = op.grid.cell_list(x)
+    assert np.array_equal(cs_g, cs_o)
+    assert np.array_equal(cl_g, cl_o)
+
+
+@pytest.mark.parametrize("kind", KINDS)
+def test_mesh_bit_exact(lv, oracle, gpu, kind):
+    _flush("mesh %s" % kind)
+    grid, op, x, dr = make_pair(lv, oracle, kind, K=48, seed=0x5EED0011, ic=None)
+    pg, lg, vg = grid.mesh()
+    po, lo, vo = op.mesh()
+    assert np.array_equal(pg, po), "edge counts differ"
+    assert np.array_equal(lg, lo), "neighbour labels / edge order differ"
+    assert np.array_equal(vg, vo), "vertices differ"
+    assert rel_err(grid.get("area"), op.area()) <= 1e-12
+    assert rel_err(grid.get_soa("centroid"), op.centroid()) <= 1e-12
+
+
+def test_mesh_bit_exact_large(lv, oracle, gpu):
+    _flush("mesh periodic 256^2")
+    grid, op, x, dr = make_pair(lv, oracle, "periodic", K=256, seed=0x5EED0012, ic=None)
+    pg, lg, vg = grid.mesh()
+    po, lo, vo = op.mesh()
+    assert np.array_equal(pg, po) and np.array_equal(lg, lo) and np.array_equal(vg, vo)
+
+
+def test_remesh_deterministic(lv, oracle, gpu):
+    grid, op, x, dr = make_pair(lv, oracle, "periodic", K=64, seed=0x5EED0013, ic=None)
+    a = grid.mesh()
+    lv.remesh(grid)
+    b = grid.mesh()
+    for u, w in zip(a, b):
+        assert np.array_equal(u, w)
+
+
+def test_magic_path_matches_oracle(lv, oracle, gpu):
+    grid, op, x, dr = make_pair(lv, oracle, "walls", K=20, seed=1, ic=None, remesh=False)
+    g1, g2, grr = grid.magic_path()
+    o1, o2, orr = op.grid.magic_path()
+    assert np.array_equal(g1, o1) and np.array_equal(g2, o2) and np.array_equal(grr, orr)
+    assert grid.info()["next_rr"] == op.grid.next_rr
+
+
+@pytest.mark.parametrize("kind,ic", [("periodic", "tg"), ("walls", "cavity")])
+def test_pressure_operator_and_rhs(lv, oracle, gpu, kind, ic):
+    grid, op, x, dr = make_pair(lv, oracle, kind, K=40, seed=0x5EED0020, ic=ic)
+    dt = 2.5e-4
+    solver = lv.PressureSolver(grid)
+    solver.assemble(dt)
+    dg, rpg, cg, vg = solver.operator()
+    do, rpo, co, vo = op.csr(*op.pressure_operator(dt))
+    assert np.array_equal(rpg, rpo) and np.array_equal(cg, co)
+    assert rel_err(vg, vo) <= 1e-14
+    assert np.array_equal(dg, do) or rel_err(dg, do) <= 1e-14
+    for gp_step in (False, True):
+        bg, GPg = solver.rhs(dt, gp_step)
+        bo, P0o, GPo = op.rhs(dt, gp_step)
+        assert rel_err(bg, bo) <= 1e-12
+        assert rel_err(GPg, GPo) <= 1e-12
+
+
+def _host_residual(op, dt, b, P):
+    diag, coef = op.pressure_operator(dt)
+    r = b - op.apply_A(diag, coef, P)
+    return np.linalg.norm(r) / np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("method,tol", [("pcg64", 1e-10), ("mixed", 1e-8), ("mixed", 1e-10)])
+def test_solver_reaches_tolerance(lv, oracle, gpu, method, tol):
+    _flush("solver %s %g" % (method, tol))
+    grid, op, x, dr = make_pair(lv, oracle, "periodic", K=64, seed=0x5EED0021, ic="tg")
+    dt = 2.5e-4
+    lv.stiffened_eos(grid, 1.4, 2500.0 / 1.4)
+    op.eos_stiffened(1.4, 2500.0 / 1.4)
+    solver = lv.PressureSolver(grid, method=method, tol=tol)
+    solver.assemble(dt)
+    b, _ = solver.rhs(dt, False)
+    P0 = grid.get("P")
+    P = solver.solve(b, P0)
+    st = solver.last_stats
+    # FP64 residual evaluated on the host with the oracle's operator
+    assert _host_residual(op, dt, b, P) <= tol * 1.05, st
+    diag, coef = op.pressure_operator(dt)
+    Po, it, rr = op.pcg(diag, coef, b, P0, 1e-14)
+    assert rel_err(P, Po) <= 1e3 * tol
+
+
+def test_singular_solve_keeps_mean(lv, oracle, gpu):
+    grid, op, x, dr = make_pair(lv, oracle, "walls", K=40, seed=0x5EED0022, ic="cavity")
+    dt = 1e-3
+    solver = lv.PressureSolver(grid, tol=1e-10)
+    solver.assemble(dt)
+    b, _ = solver.rhs(dt, False)
+    P0 = grid.get("P")
+    P = solver.solve(b, P0)
+    assert abs(P.mean() - P0.mean()) < 1e-12
+    bp = b - b.mean()
+    diag, coef = op.pressure_operator(dt)
+    r = bp - op.apply_A(diag, coef, P)
+    assert np.linalg.norm(r) <= 1.05e-10 * np.linalg.norm(bp)
+
+
+def test_find_pressure_picard(lv, oracle, gpu):
+    grid, op, x, dr = make_pair(lv, oracle, "periodic", K=48, seed=0x5EED0023, ic="tg")
+    dt = 2.5e-4
+    lv.stiffened_eos(grid, 1.4, 2500.0 / 1.4)
+    op.eos_stiffened(1.4, 2500.0 / 1.4)
+    solver = lv.PressureSolver(grid, tol=1e-12)
+    lv.find_pressure(solver, dt, 10)
+    op.find_pressure(dt, 10, tol=1e-13)
+    assert rel_err(grid.get("P"), op["P"]) <= 1e-8
+
+
+def _compare_fields(grid, op, names, tol=1e-12):
+    for k in names:
+        g = grid.get_soa(k if k != "mom" else "momentum")
+        o = op[k]
+        assert rel_err(g, o) <= tol, k
+
+
+@pytest.mark.parametrize("kind,ic", [("periodic", "tg"), ("walls", "cavity"), ("xperiodic", "tg")])
+def test_stencil_operators(lv, oracle, gpu, kind, ic):
+    _flush("stencils %s" % kind)
+    grid, op, x, dr = make_pair(lv, oracle, kind, K=40, seed=0x5EED0030, ic=ic)
+    dt = 1e-3
+    lv.stiffened_eos(grid, 1.4, 100.0)
+    op.eos_stiffened(1.4, 100.0)
+    _compare_fields(grid, op, ["rho", "P", "c2"])
+    lv.pressure_step(grid, dt)
+    op.pressure_step(dt)
+    _compare_fields(grid, op, ["v", "e"])
+    lv.find_D(grid)
+    op.find_D()
+    _compare_fields(grid, op, ["D"])
+    lv.viscous_step(grid, dt, artificial_viscosity=True)
+    op.viscous_step(dt, True)
+    _compare_fields(grid, op, ["v", "e"])
+    vbc = np.array([1.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0])
+    lv.bdary_friction(grid, vbc, dt)
+    op.bdary_friction(vbc, dt)
+    _compare_fields(grid, op, ["v", "e"])
+    lv.find_dv(grid, dt, 1.0)
+    op.find_dv(dt, 1.0)
+    _compare_fields(grid, op, ["dv", "quality"])
+    lv.relaxation_step(grid, dt, rusanov=True)
+    op.relaxation_step(dt, True)
+    _compare_fields(grid, op, ["mass", "mom", "energy", "v", "e", "x"])
+    pg, lg, vg = grid.mesh()
+    po, lo, vo = op.mesh()
+    assert np.array_equal(lg, lo) and np.array_equal(vg, vo)
+    lv.gravity_step(grid, (0.0, -1.0), dt)
+    op.gravity((0.0, -1.0), dt)
+    lv.move(grid, dt)
+    op.move(dt)
+    _compare_fields(grid, op, ["x", "v"])
+    pg, lg, vg = grid.mesh()
+    po, lo, vo = op.mesh()
+    assert np.array_equal(lg, lo) and np.array_equal(vg, vo)
+
+
+def test_full_steps_match_oracle(lv, oracle, gpu):
+    """Five full Taylor-Green steps; the only difference is the iterative pressure, solved to 1e-12
+    on both sides, so trajectories agree to ~1e-9."""
+    _flush("full steps")
+    grid, op, x, dr = make_pair(lv, oracle, "periodic", K=48, seed=0x5EED0040, ic="tg")
+    dt = 2.5e-4
+    solver = lv.PressureSolver(grid, tol=1e-12)
+    desc = lv.make_step_desc(dt, eos="stiffened", gamma=1.4, P0=2500.0 / 1.4, solver=solver)
+    for _ in range(5):
+        lv.step_fused(grid, desc)
+        op.step(dt, eos="stiffened", gamma=1.4, P0=2500.0 / 1.4, tol=1e-13)
+    for k in ("x", "v", "P", "e", "mass"):
+        assert rel_err(grid.get_soa(k), op[k]) <= 1e-8, k
+    pg, lg, vg = grid.mesh()
+    po, lo, vo = op.mesh()
+    assert np.array_equal(lg, lo)
+
+
+def test_nan_velocity_raises(lv, oracle, gpu):
+    grid, op, x, dr = make_pair(lv, oracle, "periodic", K=16, seed=3, ic="tg")
+    v = grid.get_soa("v")
+    v[5] = np.nan
+    grid.set_soa("v", v)
+    with pytest.raises(lv.LVError, match="Velocity field invalidated."):
+        lv.move(grid, 1e-3)
+
+
+def test_mesh_destroyed_raises(lv, oracle, gpu):
+    """A seed whose empty neighbourhood is wider than r_max (src/voronoigrid.jl:63-65)."""
+    dr = 1.0 / 64
+    x = np.array([0.5, 0.5 + 30 * dr, 0.5, 0.5])
+    grid = lv.VoronoiGrid(lv.UnitRectangle(), dr, positions=x)
+    with pytest.raises(lv.LVError, match="The Voronoi Mesh has been destroyed."):
+        lv.remesh(grid)
+    og = oracle.Grid((0.0, 0.0), (1.0, 1.0), dr)
+    op = oracle.Particles(og, x)
+    with pytest.raises(oracle.OracleError):
+        op.remesh()

@@ -1,0 +1,167 @@
+"""Benchmark: full Lagrangian-Voronoi steps on one MI355X.
+
+Workload (the metric's 4M-particle configuration): Taylor-Green vortex on the periodic unit square,
+2048 x 2048 particles on a jittered lattice (+-0.1 dr, SplitMix64 seed 0x5EED0002), u = sin 2pi x
+cos 2pi y, v = -cos 2pi x sin 2pi y, nu = 1e-3, stiffened gas c0 = 50, dt = 2.5e-4.  One step is the
+reference test's step! (tests/taylorgreen.jl:61-71): move! + remesh, stiffened_eos!, find_pressure!
+(10 Picard solves, mixed-precision PCG to 1e-8 FP64 relative residual), pressure_step!, find_D!,
+viscous_step!, find_dv!, relaxation_step! + remesh.  Inputs are resident on the device before the
+timed region.
+
+Prints one JSON line: particle-steps/s (value), the first Picard solve's time to 1e-8, the roofline
+of the dominant kernel (FP32 SpMV of the inner PCG, timed with HIP events on the library's stream)
+and the oracle timed on this host's cores (cpu_baseline).
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "particle-steps/s at 4M particles; pressure-solve time to 1e-8 rel. residual"
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--K", type=int, default=2048, help="lattice side (K*K particles)")
+    ap.add_argument("--solver", default="mixed", choices=["pcg64", "mixed", "spai"])
+    ap.add_argument("--tol", type=float, default=1e-8)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-K", type=int, default=1024, help="lattice side of the CPU sample (one full step)")
+    return ap.parse_args()
+
+
+def setup(lv, K, seed=0x5EED0002):
+    dr = 1.0 / K
+    x = lv.inputs.jittered_lattice(K, K, dr, seed=seed)
+    grid = lv.VoronoiGrid(lv.UnitRectangle(), dr, xperiodic=True, yperiodic=True, positions=x)
+    lv.remesh(grid)
+    area = grid.get("area")
+    ic = lv.inputs.taylor_green(x, area)
+    for k in ("v", "rho", "mass", "P", "e", "mu"):
+        grid.set_soa(k, ic[k])
+    return grid, x, ic
+
+
+def traffic_from_profiles(kernel):
+    """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE, copy-kernel
+    calibrated), committed under profiles/ as JSON; None if absent."""
+    path = os.path.join(ROOT, "profiles", "traffic_%s.json" % kernel)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("bytes_per_launch")
+
+
+def cpu_baseline(K, seed, dt, tol):
+    """One full oracle step (same algorithm, FP64 Jacobi-PCG) at lattice side K on all host cores."""
+    import numpy as np
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("lvoracle", os.path.join(ROOT, "oracle", "lvoracle.py"))
+    O = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(O)
+    from __graft_entry__ import load_package
+    lv = load_package()
+    dr = 1.0 / K
+    x = lv.inputs.jittered_lattice(K, K, dr, seed=seed)
+    g = O.Grid((0.0, 0.0), (1.0, 1.0), dr, xperiodic=True, yperiodic=True)
+    p = O.Particles(g, x)
+    p.remesh()
+    ic = lv.inputs.taylor_green(x, p.area())
+    for k in ("v", "rho", "mass", "P", "e", "mu"):
+        p.set(k, ic[k])
+    t0 = time.perf_counter()
+    p.step(dt, eos="stiffened", gamma=ic["gamma"], P0=ic["P0"], tol=tol)
+    t1 = time.perf_counter()
+    # one pressure solve (first Picard iteration, warm-started from the EOS pressure) to tol
+    p.eos_stiffened(ic["gamma"], ic["P0"])
+    diag, coef = p.pressure_operator(dt)
+    b, P0, GP = p.rhs(dt, False)
+    t2 = time.perf_counter()
+    _, iters, _ = p.pcg(diag, coef, b, P0, tol)
+    t3 = time.perf_counter()
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    n = K * K
+    return dict(value=n / (t1 - t0), unit="particle-steps/s", cores=threads, kind="port",
+                sample="one full step of the same flow at %dx%d (%d particles), FP64 Jacobi-PCG to %g; "
+                       "C restatement of the reference algorithm with OpenMP static chunks" % (K, K, n, tol),
+                step_s=t1 - t0, pressure_solve_ms=(t3 - t2) * 1e3, pressure_solve_iters=int(iters))
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    torch.cuda.set_device(0)
+    from __graft_entry__ import load_package
+
+    lv = load_package()
+    K = args.K
+    n = K * K
+    dt = 2.5e-4
+    grid, x, ic = setup(lv, K)
+    solver = lv.PressureSolver(grid, method=args.solver, tol=args.tol, itmax=2000, inner_tol=1e-4)
+    desc = lv.make_step_desc(dt, eos="stiffened", gamma=ic["gamma"], P0=ic["P0"], solver=solver,
+                             viscous=True, artificial_viscosity=False, alpha=1.0, rusanov=True)
+    for _ in range(args.warmup):
+        lv.step_fused(grid, desc)
+    L = lv.lib()
+    dominant = 0  # FP32 SpMV of the inner PCG
+    if args.solver == "pcg64":
+        dominant = 1
+    L.lv_timing_enable(grid.ctx, dominant, 1)
+    grid.sync()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    stats = []
+    for _ in range(args.steps):
+        stats.append(lv.step_fused(grid, desc))
+    grid.sync()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    import ctypes
+    tot_ms, launches, bpl = ctypes.c_double(), ctypes.c_int64(), ctypes.c_double()
+    L.lv_timing_read(grid.ctx, dominant, ctypes.byref(tot_ms), ctypes.byref(launches), ctypes.byref(bpl))
+    L.lv_timing_enable(grid.ctx, dominant, 0)
+    elapsed = t1 - t0
+    value = n * args.steps / elapsed
+    first_solve_ms = sum(s["solve_ms"][0] for s in stats) / len(stats)
+    first_solve_it = sum(s["iters"][0] for s in stats) / len(stats)
+    relres = max(max(s["relres"]) for s in stats)
+    avg_launch_ms = tot_ms.value / max(launches.value, 1)
+    achieved = bpl.value / (avg_launch_ms * 1e-3) / 1e9
+    kname = "k_cg32_spmv" if dominant == 0 else "k_cg64_spmv"
+    roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic_from_profiles(kname),
+                    kernel=kname, avg_launch_us=round(avg_launch_ms * 1e3, 2), launches=launches.value,
+                    algorithmic_bytes_per_launch=bpl.value)
+    out = dict(metric=METRIC, value=round(value, 1), unit="particle-steps/s", n_gpus=1, steps=args.steps,
+               warmup=args.warmup, ms_per_step=round(elapsed / args.steps * 1e3, 3), higher_is_better=True,
+               scaling="weak", vs_baseline=None, dtype="f64 geometry/stencils + f32 inner PCG (f64 dots/refinement)",
+               data="synthetic",
+               config=dict(workload="taylor-green %dx%d periodic, jitter +-0.1dr seed 0x5EED0002, dt=2.5e-4, "
+                                    "c0=50 stiffened, nu=1e-3, 10 Picard solves/step, %s PCG to %g" %
+                                    (K, K, args.solver, args.tol),
+                           particles=n, solver=args.solver, tol=args.tol),
+               pressure_solve_ms=round(first_solve_ms, 3), pressure_solve_iters=first_solve_it,
+               max_relres=relres, picard_iters=stats[-1]["iters"], roofline=roofline)
+    if not args.no_cpu:
+        cK = args.cpu_K
+        out["cpu_baseline"] = cpu_baseline(cK, 0x5EED0002, dt, args.tol)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -4,11 +4,11 @@
 #include "lv_internal.h"
 
 // Device-resident CG scalars: written by the last block of each reduction, read by the next launch.
+// rz is double-buffered by iteration parity: the p-update of iteration k reads rz[k&1] in every
+// block while block 0 writes rz[(k+1)&1].
 struct lv_cg_scalars {
-    double rz, rr, pq, alpha, beta, bb, thr2, tol2, inner_tol2, rr64, bmean, xmean0, xmean;
+    double rz[2], rr, pq, alpha, beta, bb, thr2, tol2, rr64, bmean, xmean0, xmean;
     int iters, itmax, done, singular;
-    unsigned ticket;
-    int pad;
 };
 
 // PressureOperator (src/pressure.jl:89-101) in CSR: off-diagonal coefficients c_ij in the edge order

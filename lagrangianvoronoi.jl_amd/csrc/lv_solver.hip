@@ -143,19 +143,21 @@ __global__ __launch_bounds__(B) void k_write_P(lv_fields F, int n, const double 
 }
 
 // ---- reductions --------------------------------------------------------------------------------
+//
+// A kernel that produces dot products writes one partial sum per block with plain stores.  The
+// kernel that consumes them re-reduces all NB partials in every block, in the same fixed order,
+// so every block obtains bit-identical totals without atomics or device-scope fences (on a
+// multi-XCD GPU a release fence per block costs an L2 write-back).  The kernel boundary makes the
+// partials visible.  Block 0 also stores the totals in lv_cg_scalars for the host and later kernels.
 
-// Publish this block's NV partial sums; returns true in the last block to arrive, after it has
-// summed all partials into out[0..NV) in a fixed order.
 template <int NV>
-__device__ __forceinline__ bool block_reduce_last(double (&v)[NV], double *__restrict__ partials, int nb,
-                                                  unsigned *__restrict__ ticket, double (&out)[NV])
+__device__ __forceinline__ void block_partials(const double (&v)[NV], double *__restrict__ partials, int nb)
 {
     __shared__ double sm[NV][B / 64];
-    __shared__ bool last;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
     for (int q = 0; q < NV; q++) {
-        double s = lv_wave_sum(v[q]);
+        const double s = lv_wave_sum(v[q]);
         if (lane == 0) sm[q][w] = s;
     }
     __syncthreads();
@@ -166,46 +168,89 @@ __device__ __forceinline__ bool block_reduce_last(double (&v)[NV], double *__res
             for (int k = 0; k < B / 64; k++) s += sm[q][k];
             partials[(size_t)q * nb + blockIdx.x] = s;
         }
-        __threadfence();
-        unsigned t = atomicAdd(ticket, 1u);
-        last = (t == (unsigned)nb - 1);
     }
-    __syncthreads();
-    if (!last) return false;
-    __threadfence();
+}
+
+template <int NV>
+__device__ __forceinline__ void reduce_all(const double *__restrict__ partials, int nb, double (&out)[NV])
+{
+    __shared__ double sm[NV][B / 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
     for (int q = 0; q < NV; q++) {
         double s = 0.0;
         for (int k = threadIdx.x; k < nb; k += B) s += partials[(size_t)q * nb + k];
         s = lv_wave_sum(s);
-        __syncthreads();
         if (lane == 0) sm[q][w] = s;
-        __syncthreads();
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NV; q++) {
         double t = 0.0;
         for (int k = 0; k < B / 64; k++) t += sm[q][k];
         out[q] = t;
     }
-    if (threadIdx.x == 0) *ticket = 0u;
-    return true;
+    __syncthreads();
+}
+
+// single-block finalisers of the one-off reductions
+enum { FIN_MEANS = 0, FIN_INIT64 = 1, FIN_RESID = 2, FIN_SUM1 = 3, FIN_INIT32 = 4 };
+
+__global__ __launch_bounds__(B) void k_finalize(const double *__restrict__ partials, int nb, int mode, int n,
+                                                double inner_tol2, lv_cg_scalars *S)
+{
+    double o[3];
+    {
+        double t3[3];
+        reduce_all<3>(partials, nb, t3);
+        o[0] = t3[0];
+        o[1] = t3[1];
+        o[2] = t3[2];
+    }
+    if (threadIdx.x != 0) return;
+    switch (mode) {
+    case FIN_MEANS:
+        S->bmean = S->singular ? o[0] / n : 0.0;
+        S->xmean0 = o[1] / n;
+        break;
+    case FIN_INIT64:
+        S->rz[0] = o[0];
+        S->rr = o[1];
+        S->bb = o[2];
+        S->thr2 = S->tol2 * o[2];
+        S->iters = 0;
+        S->done = (o[1] <= S->thr2) ? 1 : 0;
+        break;
+    case FIN_RESID:
+        S->rr64 = o[0];
+        S->bb = o[1];
+        S->thr2 = S->tol2 * o[1];
+        break;
+    case FIN_SUM1:
+        S->xmean = o[0] / n;
+        break;
+    case FIN_INIT32:
+        S->rz[0] = o[0];
+        S->rr = o[1];
+        S->thr2 = inner_tol2 * o[1];
+        S->iters = 0;
+        S->done = (o[1] <= S->thr2) ? 1 : 0;
+        break;
+    }
 }
 
 // ---- FP64 Jacobi-PCG ----------------------------------------------------------------------------
 
 // sums of b and x (singular systems: b is projected on 1-perp, x keeps its mean)
 __global__ __launch_bounds__(B) void k_means(int n, const double *__restrict__ b, const double *__restrict__ x,
-                                             double *__restrict__ partials, lv_cg_scalars *S)
+                                             double *__restrict__ partials)
 {
-    const int i = blockIdx.x * B + threadIdx.x;
-    double v[2] = {0.0, 0.0};
-    if (i < n) {
-        v[0] = b[i];
-        v[1] = x[i];
+    double v[3] = {0.0, 0.0, 0.0};
+    for (int i = blockIdx.x * B + threadIdx.x; i < n; i += gridDim.x * B) {
+        v[0] += b[i];
+        v[1] += x[i];
     }
-    double o[2];
-    if (block_reduce_last<2>(v, partials, gridDim.x, &S->ticket, o) && threadIdx.x == 0) {
-        S->bmean = S->singular ? o[0] / n : 0.0;
-        S->xmean0 = o[1] / n;
-    }
+    block_partials<3>(v, partials, gridDim.x);
 }
 
 // y = A x in the reference's difference form
@@ -220,80 +265,89 @@ __device__ __forceinline__ T spmv_row(int i, const int *__restrict__ rp, const i
     return acc;
 }
 
-// r = (b - bmean) - A x; z = dinv r; p = z; dots r.z, r.r, b'.b'
+// r = (b - bmean) - A x; z = dinv r; p = z; partials r.z, r.r, b'.b'
 __global__ __launch_bounds__(B) void k_cg64_init(int n, const int *__restrict__ rp, const int *__restrict__ cols,
                                                  const double *__restrict__ vals, const double *__restrict__ diag,
                                                  const double *__restrict__ dinv, const double *__restrict__ b,
                                                  const double *__restrict__ x, double *__restrict__ r,
                                                  double *__restrict__ z, double *__restrict__ p,
-                                                 double *__restrict__ partials, lv_cg_scalars *S)
+                                                 double *__restrict__ partials, const lv_cg_scalars *S)
 {
-    const int i = blockIdx.x * B + threadIdx.x;
+    const double bmean = S->bmean;
     double v[3] = {0.0, 0.0, 0.0};
-    if (i < n) {
-        const double bi = b[i] - S->bmean;
+    for (int i = blockIdx.x * B + threadIdx.x; i < n; i += gridDim.x * B) {
+        const double bi = b[i] - bmean;
         const double ri = bi - spmv_row<double>(i, rp, cols, vals, diag, x);
         const double zi = dinv[i] * ri;
         r[i] = ri;
         z[i] = zi;
         p[i] = zi;
-        v[0] = ri * zi;
-        v[1] = ri * ri;
-        v[2] = bi * bi;
+        v[0] += ri * zi;
+        v[1] += ri * ri;
+        v[2] += bi * bi;
     }
-    double o[3];
-    if (block_reduce_last<3>(v, partials, gridDim.x, &S->ticket, o) && threadIdx.x == 0) {
-        S->rz = o[0];
-        S->rr = o[1];
-        S->bb = o[2];
-        S->thr2 = S->tol2 * o[2];
-        S->iters = 0;
-        S->done = (o[1] <= S->thr2) ? 1 : 0;
-    }
+    block_partials<3>(v, partials, gridDim.x);
 }
 
-__global__ __launch_bounds__(B) void k_cg64_spmv(int n, const int *__restrict__ rp, const int *__restrict__ cols,
-                                                 const double *__restrict__ vals, const double *__restrict__ diag,
-                                                 const double *__restrict__ p, double *__restrict__ q,
-                                                 double *__restrict__ partials, lv_cg_scalars *S)
+template <typename T>
+__global__ __launch_bounds__(B) void k_cg_spmv(int n, const int *__restrict__ rp, const int *__restrict__ cols,
+                                               const T *__restrict__ vals, const T *__restrict__ diag,
+                                               const T *__restrict__ p, T *__restrict__ q,
+                                               double *__restrict__ Ppq, const lv_cg_scalars *S)
 {
     if (S->done) return;
-    const int i = blockIdx.x * B + threadIdx.x;
     double v[1] = {0.0};
-    if (i < n) {
-        const double qi = spmv_row<double>(i, rp, cols, vals, diag, p);
+    for (int i = blockIdx.x * B + threadIdx.x; i < n; i += gridDim.x * B) {
+        const T qi = spmv_row<T>(i, rp, cols, vals, diag, p);
         q[i] = qi;
-        v[0] = p[i] * qi;
+        v[0] += (double)p[i] * (double)qi;
     }
-    double o[1];
-    if (block_reduce_last<1>(v, partials, gridDim.x, &S->ticket, o) && threadIdx.x == 0) {
-        S->pq = o[0];
-        S->alpha = S->rz / o[0];
-    }
+    block_partials<1>(v, Ppq, gridDim.x);
 }
 
-__global__ __launch_bounds__(B) void k_cg64_update(int n, double *__restrict__ x, double *__restrict__ r,
-                                                   double *__restrict__ z, const double *__restrict__ p,
-                                                   const double *__restrict__ q, const double *__restrict__ dinv,
-                                                   double *__restrict__ partials, lv_cg_scalars *S)
+// alpha = rz/pq; x += alpha p; r -= alpha q; z = dinv r; partials r.z, r.r
+template <typename T>
+__global__ __launch_bounds__(B) void k_cg_update(int n, T *__restrict__ x, T *__restrict__ r, T *__restrict__ z,
+                                                 const T *__restrict__ p, const T *__restrict__ q,
+                                                 const T *__restrict__ dinv, const double *__restrict__ Ppq,
+                                                 double *__restrict__ Prr, lv_cg_scalars *S, int par)
 {
     if (S->done) return;
-    const int i = blockIdx.x * B + threadIdx.x;
-    const double alpha = S->alpha;
+    double pq[1];
+    reduce_all<1>(Ppq, gridDim.x, pq);
+    const double alpha = S->rz[par] / pq[0];
+    const T a = (T)alpha;
     double v[2] = {0.0, 0.0};
-    if (i < n) {
-        x[i] += alpha * p[i];
-        const double ri = r[i] - alpha * q[i];
-        const double zi = dinv[i] * ri;
+    for (int i = blockIdx.x * B + threadIdx.x; i < n; i += gridDim.x * B) {
+        x[i] += a * p[i];
+        const T ri = r[i] - a * q[i];
+        const T zi = dinv[i] * ri;
         r[i] = ri;
         z[i] = zi;
-        v[0] = ri * zi;
-        v[1] = ri * ri;
+        v[0] += (double)ri * (double)zi;
+        v[1] += (double)ri * (double)ri;
     }
+    block_partials<2>(v, Prr, gridDim.x);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        S->pq = pq[0];
+        S->alpha = alpha;
+    }
+}
+
+// beta = rz_new/rz; p = z + beta p; convergence bookkeeping in block 0
+template <typename T>
+__global__ __launch_bounds__(B) void k_cg_pupdate(int n, const T *__restrict__ z, T *__restrict__ p,
+                                                  const double *__restrict__ Prr, lv_cg_scalars *S, int par)
+{
+    if (S->done) return;
     double o[2];
-    if (block_reduce_last<2>(v, partials, gridDim.x, &S->ticket, o) && threadIdx.x == 0) {
-        S->beta = o[0] / S->rz;
-        S->rz = o[0];
+    reduce_all<2>(Prr, gridDim.x, o);
+    const double beta = o[0] / S->rz[par];
+    const T bt = (T)beta;
+    for (int i = blockIdx.x * B + threadIdx.x; i < n; i += gridDim.x * B) p[i] = z[i] + bt * p[i];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        S->rz[par ^ 1] = o[0];
+        S->beta = beta;
         S->rr = o[1];
         S->iters += 1;
         if (o[1] <= S->thr2) S->done = 1;
@@ -301,153 +355,80 @@ __global__ __launch_bounds__(B) void k_cg64_update(int n, double *__restrict__ x
     }
 }
 
-__global__ __launch_bounds__(B) void k_cg64_pupdate(int n, const double *__restrict__ z, double *__restrict__ p,
-                                                    const lv_cg_scalars *S)
-{
-    if (S->done) return;
-    const int i = blockIdx.x * B + threadIdx.x;
-    if (i < n) p[i] = z[i] + S->beta * p[i];
-}
-
-// true residual r = (b - bmean) - A x and |r|^2 (FP64), for refinement sweeps and final checks
+// true residual r = (b - bmean) - A x; partials |r|^2, |b'|^2 (FP64)
 __global__ __launch_bounds__(B) void k_resid64(int n, const int *__restrict__ rp, const int *__restrict__ cols,
                                                const double *__restrict__ vals, const double *__restrict__ diag,
                                                const double *__restrict__ b, const double *__restrict__ x,
                                                double *__restrict__ r, double *__restrict__ partials,
-                                               lv_cg_scalars *S)
+                                               const lv_cg_scalars *S)
 {
-    const int i = blockIdx.x * B + threadIdx.x;
-    double v[2] = {0.0, 0.0};
-    if (i < n) {
-        const double bi = b[i] - S->bmean;
+    const double bmean = S->bmean;
+    double v[3] = {0.0, 0.0, 0.0};
+    for (int i = blockIdx.x * B + threadIdx.x; i < n; i += gridDim.x * B) {
+        const double bi = b[i] - bmean;
         const double ri = bi - spmv_row<double>(i, rp, cols, vals, diag, x);
         r[i] = ri;
-        v[0] = ri * ri;
-        v[1] = bi * bi;
+        v[0] += ri * ri;
+        v[1] += bi * bi;
     }
-    double o[2];
-    if (block_reduce_last<2>(v, partials, gridDim.x, &S->ticket, o) && threadIdx.x == 0) {
-        S->rr64 = o[0];
-        S->bb = o[1];
-        S->thr2 = S->tol2 * o[1];
-    }
+    block_partials<3>(v, partials, gridDim.x);
+}
+
+__global__ __launch_bounds__(B) void k_sum1(int n, const double *__restrict__ x, double *__restrict__ partials)
+{
+    double v[3] = {0.0, 0.0, 0.0};
+    for (int i = blockIdx.x * B + threadIdx.x; i < n; i += gridDim.x * B) v[0] += x[i];
+    block_partials<3>(v, partials, gridDim.x);
 }
 
 // restore the mean of x to that of the initial guess (singular systems only)
-__global__ __launch_bounds__(B) void k_sum1(int n, const double *__restrict__ x, double *__restrict__ partials,
-                                            lv_cg_scalars *S)
-{
-    const int i = blockIdx.x * B + threadIdx.x;
-    double v[1] = {(i < n) ? x[i] : 0.0};
-    double o[1];
-    if (block_reduce_last<1>(v, partials, gridDim.x, &S->ticket, o) && threadIdx.x == 0) S->xmean = o[0] / n;
-}
-
 __global__ __launch_bounds__(B) void k_shift(int n, double *__restrict__ x, const lv_cg_scalars *S)
 {
-    const int i = blockIdx.x * B + threadIdx.x;
-    if (i < n) x[i] -= (S->xmean - S->xmean0);
+    const double d = S->xmean - S->xmean0;
+    for (int i = blockIdx.x * B + threadIdx.x; i < n; i += gridDim.x * B) x[i] -= d;
 }
 
 // ---- FP32 inner PCG of the refinement --------------------------------------------------------
 
-// r32 = r64 / |r64| (projected on 1-perp for singular systems), e = 0, z = dinv r, p = z
+// r32 = r64 / |r64|, e = 0, z = dinv r, p = z; partials r.z, r.r
 __global__ __launch_bounds__(B) void k_cg32_init(int n, const double *__restrict__ r64, const float *__restrict__ dinv,
                                                  float *__restrict__ e, float *__restrict__ r, float *__restrict__ z,
                                                  float *__restrict__ p, double *__restrict__ partials,
-                                                 lv_cg_scalars *S)
+                                                 const lv_cg_scalars *S)
 {
-    const int i = blockIdx.x * B + threadIdx.x;
     const double inv = 1.0 / sqrt(S->rr64);
-    double v[2] = {0.0, 0.0};
-    if (i < n) {
+    double v[3] = {0.0, 0.0, 0.0};
+    for (int i = blockIdx.x * B + threadIdx.x; i < n; i += gridDim.x * B) {
         const float ri = (float)(r64[i] * inv);
         const float zi = dinv[i] * ri;
         e[i] = 0.0f;
         r[i] = ri;
         z[i] = zi;
         p[i] = zi;
-        v[0] = (double)ri * (double)zi;
-        v[1] = (double)ri * (double)ri;
+        v[0] += (double)ri * (double)zi;
+        v[1] += (double)ri * (double)ri;
     }
-    double o[2];
-    if (block_reduce_last<2>(v, partials, gridDim.x, &S->ticket, o) && threadIdx.x == 0) {
-        S->rz = o[0];
-        S->rr = o[1];
-        S->thr2 = S->inner_tol2 * o[1];
-        S->iters = 0;
-        S->done = (o[1] <= S->thr2) ? 1 : 0;
-    }
-}
-
-__global__ __launch_bounds__(B) void k_cg32_spmv(int n, const int *__restrict__ rp, const int *__restrict__ cols,
-                                                 const float *__restrict__ vals, const float *__restrict__ diag,
-                                                 const float *__restrict__ p, float *__restrict__ q,
-                                                 double *__restrict__ partials, lv_cg_scalars *S)
-{
-    if (S->done) return;
-    const int i = blockIdx.x * B + threadIdx.x;
-    double v[1] = {0.0};
-    if (i < n) {
-        const float qi = spmv_row<float>(i, rp, cols, vals, diag, p);
-        q[i] = qi;
-        v[0] = (double)p[i] * (double)qi;
-    }
-    double o[1];
-    if (block_reduce_last<1>(v, partials, gridDim.x, &S->ticket, o) && threadIdx.x == 0) {
-        S->pq = o[0];
-        S->alpha = S->rz / o[0];
-    }
-}
-
-__global__ __launch_bounds__(B) void k_cg32_update(int n, float *__restrict__ e, float *__restrict__ r,
-                                                   float *__restrict__ z, const float *__restrict__ p,
-                                                   const float *__restrict__ q, const float *__restrict__ dinv,
-                                                   double *__restrict__ partials, lv_cg_scalars *S)
-{
-    if (S->done) return;
-    const int i = blockIdx.x * B + threadIdx.x;
-    const float alpha = (float)S->alpha;
-    double v[2] = {0.0, 0.0};
-    if (i < n) {
-        e[i] += alpha * p[i];
-        const float ri = r[i] - alpha * q[i];
-        const float zi = dinv[i] * ri;
-        r[i] = ri;
-        z[i] = zi;
-        v[0] = (double)ri * (double)zi;
-        v[1] = (double)ri * (double)ri;
-    }
-    double o[2];
-    if (block_reduce_last<2>(v, partials, gridDim.x, &S->ticket, o) && threadIdx.x == 0) {
-        S->beta = o[0] / S->rz;
-        S->rz = o[0];
-        S->rr = o[1];
-        S->iters += 1;
-        if (o[1] <= S->thr2) S->done = 1;
-        else if (S->iters >= S->itmax || !(o[1] == o[1])) S->done = 2;
-    }
-}
-
-__global__ __launch_bounds__(B) void k_cg32_pupdate(int n, const float *__restrict__ z, float *__restrict__ p,
-                                                    const lv_cg_scalars *S)
-{
-    if (S->done) return;
-    const int i = blockIdx.x * B + threadIdx.x;
-    const float beta = (float)S->beta;
-    if (i < n) p[i] = z[i] + beta * p[i];
+    block_partials<3>(v, partials, gridDim.x);
 }
 
 // x += |r64| * e
 __global__ __launch_bounds__(B) void k_correct(int n, double *__restrict__ x, const float *__restrict__ e,
                                                const lv_cg_scalars *S)
 {
-    const int i = blockIdx.x * B + threadIdx.x;
     const double s = sqrt(S->rr64);
-    if (i < n) x[i] += s * (double)e[i];
+    for (int i = blockIdx.x * B + threadIdx.x; i < n; i += gridDim.x * B) x[i] += s * (double)e[i];
 }
 
 inline dim3 grid_of(int n) { return dim3((n + B - 1) / B); }
+
+// grid of the reduction kernels: enough 256-thread blocks for ~8 wavefronts per SIMD on 256 CUs,
+// capped so that re-reducing the partials stays cheap
+constexpr int NBMAX = 2048;
+inline int nblocks_of(int n)
+{
+    int nb = (n + B - 1) / B;
+    return nb < NBMAX ? (nb > 0 ? nb : 1) : NBMAX;
+}
 
 }  // namespace
 
@@ -481,7 +462,11 @@ hipError_t lv_launch_write_P(lv_fields F, int n, const double *P, hipStream_t s)
     return hipGetLastError();
 }
 
-size_t lv_partials_bytes(int n) { return (size_t)4 * ((n + B - 1) / B) * sizeof(double); }
+size_t lv_partials_bytes(int n)
+{
+    (void)n;
+    return (size_t)8 * NBMAX * sizeof(double);
+}
 
 namespace {
 
@@ -505,100 +490,110 @@ hipError_t read_scalars(lv_cg_scalars *dS, lv_cg_scalars *hS, hipStream_t s)
     return hipStreamSynchronize(s);
 }
 
+// partial-sum regions: P3 = one-off reductions (3 values), Ppq (1), Prr (2)
+struct parts {
+    double *p3, *pq, *rr;
+};
+inline parts parts_of(double *base) { return parts{base, base + 3 * NBMAX, base + 4 * NBMAX}; }
+
+// CG iterations (three launches each) until the device flags completion; the host checks the flag
+// once per batch of iterations.
+template <typename T>
+hipError_t cg_iterate(int n, const int *rp, const int *cols, const T *vals, const T *diag, const T *dinv, T *x, T *r,
+                      T *z, T *p, T *q, const parts &P, lv_cg_scalars *dS, hipStream_t s, lv_timer *tm, int batch,
+                      lv_cg_scalars &h)
+{
+    const int nb = nblocks_of(n);
+    int k = 0;
+    for (;;) {
+        for (int j = 0; j < batch; j++, k++) {
+            const int par = k & 1;
+            {
+                timer_scope ts(tm, s);
+                hipLaunchKernelGGL(k_cg_spmv<T>, dim3(nb), dim3(B), 0, s, n, rp, cols, vals, diag, (const T *)p, q,
+                                   P.pq, (const lv_cg_scalars *)dS);
+            }
+            hipLaunchKernelGGL(k_cg_update<T>, dim3(nb), dim3(B), 0, s, n, x, r, z, (const T *)p, (const T *)q, dinv,
+                               (const double *)P.pq, P.rr, dS, par);
+            hipLaunchKernelGGL(k_cg_pupdate<T>, dim3(nb), dim3(B), 0, s, n, (const T *)z, p, (const double *)P.rr,
+                               dS, par);
+        }
+        hipError_t e = read_scalars(dS, &h, s);
+        if (e != hipSuccess) return e;
+        if (tm) lv_timer_flush(tm);
+        if (h.done) return hipSuccess;
+    }
+}
+
 }  // namespace
 
-// Runs CG64 iterations until done; batches of `batch` iterations between host checks.
-static hipError_t cg64_iterate(const lv_pressure_system &A, lv_solver_work &W, int n, hipStream_t s, lv_timer *t64,
-                               int batch, lv_cg_scalars &h)
-{
-    const dim3 G = grid_of(n);
-    for (;;) {
-        for (int k = 0; k < batch; k++) {
-            {
-                timer_scope ts(t64, s);
-                hipLaunchKernelGGL(k_cg64_spmv, G, dim3(B), 0, s, n, (const int *)A.row_ptr, (const int *)A.cols,
-                                   (const double *)A.val64, (const double *)A.diag64, (const double *)W.p, W.q,
-                                   W.partials, W.S);
-            }
-            hipLaunchKernelGGL(k_cg64_update, G, dim3(B), 0, s, n, W.x, W.r, W.z, (const double *)W.p,
-                               (const double *)W.q, (const double *)A.dinv64, W.partials, W.S);
-            hipLaunchKernelGGL(k_cg64_pupdate, G, dim3(B), 0, s, n, (const double *)W.z, W.p,
-                               (const lv_cg_scalars *)W.S);
-        }
-        hipError_t e = read_scalars(W.S, &h, s);
-        if (e != hipSuccess) return e;
-        if (t64) lv_timer_flush(t64);
-        if (h.done) return hipSuccess;
-    }
-}
-
-static hipError_t cg32_iterate(const lv_pressure_system &A, lv_solver_work &W, int n, hipStream_t s, lv_timer *t32,
-                               int batch, lv_cg_scalars &h)
-{
-    const dim3 G = grid_of(n);
-    for (;;) {
-        for (int k = 0; k < batch; k++) {
-            {
-                timer_scope ts(t32, s);
-                hipLaunchKernelGGL(k_cg32_spmv, G, dim3(B), 0, s, n, (const int *)A.row_ptr, (const int *)A.cols,
-                                   (const float *)A.val32, (const float *)A.diag32, (const float *)W.p32, W.q32,
-                                   W.partials, W.S);
-            }
-            hipLaunchKernelGGL(k_cg32_update, G, dim3(B), 0, s, n, W.e32, W.r32, W.z32, (const float *)W.p32,
-                               (const float *)W.q32, (const float *)A.dinv32, W.partials, W.S);
-            hipLaunchKernelGGL(k_cg32_pupdate, G, dim3(B), 0, s, n, (const float *)W.z32, W.p32,
-                               (const lv_cg_scalars *)W.S);
-        }
-        hipError_t e = read_scalars(W.S, &h, s);
-        if (e != hipSuccess) return e;
-        if (t32) lv_timer_flush(t32);
-        if (h.done) return hipSuccess;
-    }
-}
-
 // Solves A x = b (device vectors; x holds the initial guess).  Writes iterations, sweeps and the
-// final FP64 relative residual.  Returns hipSuccess; *converged tells whether tol was met.
+// final FP64 relative residual.  *converged tells whether tol was met.
 hipError_t lv_solve_device(const lv_pressure_system &A, lv_solver_work &W, int n, const double *b, double *x,
                            const lv_solve_opts &opt, hipStream_t s, lv_timer *t32, lv_timer *t64, int *iters,
                            int *sweeps, double *relres, bool *converged)
 {
-    const dim3 G = grid_of(n);
+    const int nb = nblocks_of(n);
+    const dim3 G(nb);
+    const parts P = parts_of(W.partials);
     lv_cg_scalars h;
     memset(&h, 0, sizeof(h));
     h.tol2 = opt.tol * opt.tol;
-    h.inner_tol2 = opt.inner_tol * opt.inner_tol;
     h.itmax = opt.itmax > 0 ? opt.itmax : 1000;
     h.singular = A.singular;
     hipError_t e = hipMemcpyAsync(W.S, &h, sizeof(h), hipMemcpyHostToDevice, s);
     if (e != hipSuccess) return e;
     e = hipMemcpyAsync(W.x, x, sizeof(double) * n, hipMemcpyDeviceToDevice, s);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_means, G, dim3(B), 0, s, n, b, (const double *)W.x, W.partials, W.S);
-    const int batch = 8;
+    hipLaunchKernelGGL(k_means, G, dim3(B), 0, s, n, b, (const double *)W.x, P.p3);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(B), 0, s, (const double *)P.p3, nb, (int)FIN_MEANS, n, 0.0, W.S);
+    const int batch = 16;
     int total = 0, nsweep = 0;
     if (opt.method == LV_SOLVER_PCG64) {
         hipLaunchKernelGGL(k_cg64_init, G, dim3(B), 0, s, n, (const int *)A.row_ptr, (const int *)A.cols,
                            (const double *)A.val64, (const double *)A.diag64, (const double *)A.dinv64, b,
-                           (const double *)W.x, W.r, W.z, W.p, W.partials, W.S);
-        e = cg64_iterate(A, W, n, s, t64, batch, h);
+                           (const double *)W.x, W.r, W.z, W.p, P.p3, (const lv_cg_scalars *)W.S);
+        hipLaunchKernelGGL(k_finalize, dim3(1), dim3(B), 0, s, (const double *)P.p3, nb, (int)FIN_INIT64, n, 0.0,
+                           W.S);
+        e = read_scalars(W.S, &h, s);
         if (e != hipSuccess) return e;
+        if (!h.done) {
+            e = cg_iterate<double>(n, A.row_ptr, A.cols, A.val64, A.diag64, A.dinv64, W.x, W.r, W.z, W.p, W.q, P,
+                                   W.S, s, t64, batch, h);
+            if (e != hipSuccess) return e;
+        }
         total = h.iters;
     } else {
         const int outer_max = opt.outer_max > 0 ? opt.outer_max : 20;
+        const double floor_tol = opt.inner_tol > 0.0 ? opt.inner_tol : 1e-4;
         for (;;) {
             {
                 timer_scope ts(t64, s);
                 hipLaunchKernelGGL(k_resid64, G, dim3(B), 0, s, n, (const int *)A.row_ptr, (const int *)A.cols,
                                    (const double *)A.val64, (const double *)A.diag64, b, (const double *)W.x, W.r,
-                                   W.partials, W.S);
+                                   P.p3, (const lv_cg_scalars *)W.S);
             }
+            hipLaunchKernelGGL(k_finalize, dim3(1), dim3(B), 0, s, (const double *)P.p3, nb, (int)FIN_RESID, n, 0.0,
+                               W.S);
             e = read_scalars(W.S, &h, s);
             if (e != hipSuccess) return e;
             if (h.rr64 <= h.thr2 || nsweep >= outer_max || !(h.rr64 == h.rr64)) break;
+            // ask the FP32 sweep for just the reduction still missing (with a 0.3 margin), but not
+            // below what FP32 arithmetic can deliver in one sweep
+            double need = 0.3 * sqrt(h.thr2 / h.rr64);
+            double itol = need > floor_tol ? need : floor_tol;
+            if (itol > 0.5) itol = 0.5;
             hipLaunchKernelGGL(k_cg32_init, G, dim3(B), 0, s, n, (const double *)W.r, (const float *)A.dinv32, W.e32,
-                               W.r32, W.z32, W.p32, W.partials, W.S);
-            e = cg32_iterate(A, W, n, s, t32, batch, h);
+                               W.r32, W.z32, W.p32, P.p3, (const lv_cg_scalars *)W.S);
+            hipLaunchKernelGGL(k_finalize, dim3(1), dim3(B), 0, s, (const double *)P.p3, nb, (int)FIN_INIT32, n,
+                               itol * itol, W.S);
+            e = read_scalars(W.S, &h, s);
             if (e != hipSuccess) return e;
+            if (!h.done) {
+                e = cg_iterate<float>(n, A.row_ptr, A.cols, A.val32, A.diag32, A.dinv32, W.e32, W.r32, W.z32, W.p32,
+                                      W.q32, P, W.S, s, t32, batch, h);
+                if (e != hipSuccess) return e;
+            }
             total += h.iters;
             nsweep++;
             hipLaunchKernelGGL(k_correct, G, dim3(B), 0, s, n, W.x, (const float *)W.e32, (const lv_cg_scalars *)W.S);
@@ -606,12 +601,14 @@ hipError_t lv_solve_device(const lv_pressure_system &A, lv_solver_work &W, int n
     }
     // final FP64 residual of the returned x
     if (A.singular) {
-        hipLaunchKernelGGL(k_sum1, G, dim3(B), 0, s, n, (const double *)W.x, W.partials, W.S);
+        hipLaunchKernelGGL(k_sum1, G, dim3(B), 0, s, n, (const double *)W.x, P.p3);
+        hipLaunchKernelGGL(k_finalize, dim3(1), dim3(B), 0, s, (const double *)P.p3, nb, (int)FIN_SUM1, n, 0.0, W.S);
         hipLaunchKernelGGL(k_shift, G, dim3(B), 0, s, n, W.x, (const lv_cg_scalars *)W.S);
     }
     hipLaunchKernelGGL(k_resid64, G, dim3(B), 0, s, n, (const int *)A.row_ptr, (const int *)A.cols,
-                       (const double *)A.val64, (const double *)A.diag64, b, (const double *)W.x, W.r, W.partials,
-                       W.S);
+                       (const double *)A.val64, (const double *)A.diag64, b, (const double *)W.x, W.r, P.p3,
+                       (const lv_cg_scalars *)W.S);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(B), 0, s, (const double *)P.p3, nb, (int)FIN_RESID, n, 0.0, W.S);
     e = read_scalars(W.S, &h, s);
     if (e != hipSuccess) return e;
     e = hipMemcpyAsync(x, W.x, sizeof(double) * n, hipMemcpyDeviceToDevice, s);

@@ -34,9 +34,38 @@ def parse():
     ap.add_argument("--K", type=int, default=2048, help="lattice side (K*K particles)")
     ap.add_argument("--solver", default="mixed", choices=["pcg64", "mixed", "spai"])
     ap.add_argument("--tol", type=float, default=1e-8)
+    ap.add_argument("--picard-window", type=int, default=8,
+                    help="Picard corrections used to extrapolate each solve's initial guess (0: previous P)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-K", type=int, default=1024, help="lattice side of the CPU sample (one full step)")
+    ap.add_argument("--mode", default="steps", choices=["steps", "solve", "copy"],
+                    help="steps: the benchmark; solve: one pressure solve (for counter runs); "
+                         "copy: a plain device copy of known size (calibrates HBM byte counters)")
     return ap.parse_args()
+
+
+def mode_copy():
+    """Copies 512 MiB device-to-device 5 times with torch: each launch reads and writes 512 MiB."""
+    import torch
+    n = 128 * 1024 * 1024
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"mode": "copy", "bytes_read_per_launch": 4 * n, "bytes_written_per_launch": 4 * n}))
+
+
+def mode_solve(lv, K, method, tol):
+    """One first-Picard pressure solve of the benchmark flow (EOS pressure as initial guess)."""
+    dt = 2.5e-4
+    grid, x, ic = setup(lv, K)
+    lv.stiffened_eos(grid, ic["gamma"], ic["P0"])
+    solver = lv.PressureSolver(grid, method=method, tol=tol, itmax=2000, inner_tol=1e-4)
+    solver.assemble(dt)
+    b, _ = solver.rhs(dt, False)
+    P = solver.solve(b, grid.get("P"))
+    print(json.dumps({"mode": "solve", "particles": K * K, "stats": solver.last_stats}))
 
 
 def setup(lv, K, seed=0x5EED0002):
@@ -105,14 +134,19 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
+    if args.mode == "copy":
+        return mode_copy()
     from __graft_entry__ import load_package
 
     lv = load_package()
+    if args.mode == "solve":
+        return mode_solve(lv, args.K, args.solver, args.tol)
     K = args.K
     n = K * K
     dt = 2.5e-4
     grid, x, ic = setup(lv, K)
-    solver = lv.PressureSolver(grid, method=args.solver, tol=args.tol, itmax=2000, inner_tol=1e-4)
+    solver = lv.PressureSolver(grid, method=args.solver, tol=args.tol, itmax=2000, inner_tol=1e-4,
+                               picard_window=args.picard_window)
     desc = lv.make_step_desc(dt, eos="stiffened", gamma=ic["gamma"], P0=ic["P0"], solver=solver,
                              viscous=True, artificial_viscosity=False, alpha=1.0, rusanov=True)
     for _ in range(args.warmup):
@@ -154,9 +188,10 @@ def main():
                config=dict(workload="taylor-green %dx%d periodic, jitter +-0.1dr seed 0x5EED0002, dt=2.5e-4, "
                                     "c0=50 stiffened, nu=1e-3, 10 Picard solves/step, %s PCG to %g" %
                                     (K, K, args.solver, args.tol),
-                           particles=n, solver=args.solver, tol=args.tol),
+                           particles=n, solver=args.solver, tol=args.tol, picard_window=args.picard_window),
                pressure_solve_ms=round(first_solve_ms, 3), pressure_solve_iters=first_solve_it,
-               max_relres=relres, picard_iters=stats[-1]["iters"], roofline=roofline)
+               max_relres=relres, picard_iters=stats[-1]["iters"], picard_sweeps=stats[-1]["outer"],
+               picard_relres0=[float("%.3g" % r) for r in stats[-1]["relres0"]], roofline=roofline)
     if not args.no_cpu:
         cK = args.cpu_K
         out["cpu_baseline"] = cpu_baseline(cK, 0x5EED0002, dt, args.tol)

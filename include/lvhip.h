@@ -81,12 +81,16 @@ typedef struct {
     double inner_tol;     /* MIXED/SPAI: relative reduction asked of each FP32 inner solve */
     int32_t outer_max;    /* MIXED/SPAI: cap on refinement sweeps */
     double vbc[8];        /* boundary_velocity per label: vbc[2*(-label-1) + comp], label -1..-4 */
+    int32_t picard_window; /* find_pressure!: 0 warm-starts each Picard solve from the previous P, as the
+                              reference does; k > 0 starts it from the previous P plus the minimal-residual
+                              combination of the last k Picard corrections (every solve still meets tol) */
 } lv_solve_opts;
 
 typedef struct {
     int32_t iters[32];    /* Krylov iterations of each Picard solve */
     int32_t outer[32];    /* refinement sweeps of each Picard solve (MIXED/SPAI) */
     double relres[32];    /* final FP64 relative residual of each Picard solve */
+    double relres0[32];   /* FP64 relative residual of each solve's initial guess */
     double solve_ms[32];  /* device time of each linear solve (HIP events on the context stream) */
     int32_t total_iters;
     int32_t fallbacks;    /* SPAI solves that fell back to Jacobi */

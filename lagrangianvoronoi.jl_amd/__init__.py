@@ -63,17 +63,19 @@ class GridDesc(ctypes.Structure):
 
 class SolveOpts(ctypes.Structure):
     _fields_ = [("method", ctypes.c_int32), ("tol", ctypes.c_double), ("itmax", ctypes.c_int32),
-                ("inner_tol", ctypes.c_double), ("outer_max", ctypes.c_int32), ("vbc", ctypes.c_double * 8)]
+                ("inner_tol", ctypes.c_double), ("outer_max", ctypes.c_int32), ("vbc", ctypes.c_double * 8),
+                ("picard_window", ctypes.c_int32)]
 
 
 class SolveStats(ctypes.Structure):
     _fields_ = [("iters", ctypes.c_int32 * 32), ("outer", ctypes.c_int32 * 32), ("relres", ctypes.c_double * 32),
-                ("solve_ms", ctypes.c_double * 32), ("total_iters", ctypes.c_int32),
+                ("relres0", ctypes.c_double * 32), ("solve_ms", ctypes.c_double * 32), ("total_iters", ctypes.c_int32),
                 ("fallbacks", ctypes.c_int32), ("nsolves", ctypes.c_int32)]
 
     def as_dict(self):
         k = min(self.nsolves, 32)
         return dict(iters=list(self.iters[:k]), outer=list(self.outer[:k]), relres=list(self.relres[:k]),
+                    relres0=list(self.relres0[:k]),
                     solve_ms=list(self.solve_ms[:k]), total_iters=self.total_iters, fallbacks=self.fallbacks,
                     nsolves=self.nsolves)
 
@@ -377,11 +379,13 @@ def relaxation_step(grid, dt, rusanov=True):
     _check(grid.ctx, lib().lv_relaxation_step(grid.ctx, float(dt), int(bool(rusanov))))
 
 
-def solve_opts(method="pcg64", tol=1e-10, itmax=1000, inner_tol=1e-4, outer_max=20, boundary_velocity=None):
+def solve_opts(method="pcg64", tol=1e-10, itmax=1000, inner_tol=1e-4, outer_max=20, boundary_velocity=None,
+               picard_window=0):
     o = SolveOpts()
     o.method = SOLVERS[method] if isinstance(method, str) else int(method)
     o.tol, o.itmax, o.inner_tol, o.outer_max = float(tol), int(itmax), float(inner_tol), int(outer_max)
     o.vbc[:] = _vbc_table(boundary_velocity)
+    o.picard_window = int(picard_window)
     return o
 
 
@@ -389,15 +393,18 @@ class PressureSolver:
     """PressureSolver(grid; verbose), src/pressure.jl:142-159.  The Krylov method and tolerance are
     this build's (the reference hard-codes MINRES with atol = rtol = 1e-6, :219)."""
 
-    def __init__(self, grid, verbose=False, method="pcg64", tol=1e-10, itmax=1000, inner_tol=1e-4, outer_max=20):
+    def __init__(self, grid, verbose=False, method="pcg64", tol=1e-10, itmax=1000, inner_tol=1e-4, outer_max=20,
+                 picard_window=0):
         self.grid = grid
         self.verbose = verbose
         self.method, self.tol, self.itmax = method, tol, itmax
         self.inner_tol, self.outer_max = inner_tol, outer_max
+        self.picard_window = picard_window
         self.last_stats = None
 
     def opts(self, boundary_velocity=None):
-        return solve_opts(self.method, self.tol, self.itmax, self.inner_tol, self.outer_max, boundary_velocity)
+        return solve_opts(self.method, self.tol, self.itmax, self.inner_tol, self.outer_max, boundary_velocity,
+                          self.picard_window)
 
     # PressureOperator access (src/pressure.jl:89-135)
     def assemble(self, dt):

@@ -44,6 +44,12 @@ struct lv_ctx {
     void *scan_temp = nullptr;
     int *d_flags = nullptr;   // [0] error word, [1] nonsingular flag
     lv_timer *timers[LV_NKTIMERS] = {nullptr, nullptr, nullptr, nullptr};
+    // Picard extrapolation: orthonormal A-images q_k of past Picard corrections and their
+    // preimages z_k (A z_k = q_k), the last two Picard iterates and their A-images
+    double *pw_q[8] = {nullptr}, *pw_z[8] = {nullptr};
+    double *pw_P[2] = {nullptr, nullptr}, *pw_AP[2] = {nullptr, nullptr};
+    double *pw_t1 = nullptr, *pw_t2 = nullptr, *pw_dots = nullptr;
+    int pw_cap = 0;
     char msg[512] = {0};
 };
 
@@ -290,6 +296,9 @@ int lv_create(const lv_grid_desc *d, lv_ctx **out)
     ALLOC(ctx->A.diag32, cap);
     ALLOC(ctx->A.dinv32, cap);
     ALLOC(ctx->deg, cap + 1);
+    ALLOC(ctx->A.ecol, (size_t)LV_MAXE * cap);
+    ALLOC(ctx->A.ev64, (size_t)LV_MAXE * cap);
+    ALLOC(ctx->A.ev32, (size_t)LV_MAXE * cap);
     ALLOC(ctx->b, cap);
     ALLOC(ctx->P0, cap);
     ALLOC(ctx->GP, 2 * cap);
@@ -336,10 +345,21 @@ int lv_destroy(lv_ctx *ctx)
                     ctx->F.phase, ctx->ne, ctx->ev, ctx->el, ctx->cell_start, ctx->cell_labels, ctx->cell_x,
                     ctx->cl_temp, ctx->d_nitems, ctx->d_flags, ctx->path_i, ctx->path_rr, ctx->A.row_ptr,
                     ctx->A.cols, ctx->A.val64, ctx->A.val32, ctx->A.diag64, ctx->A.dinv64, ctx->A.diag32,
-                    ctx->A.dinv32, ctx->deg, ctx->b, ctx->P0, ctx->GP, ctx->W.x, ctx->W.r, ctx->W.z, ctx->W.p,
+                    ctx->A.dinv32, ctx->deg, ctx->A.ecol, ctx->A.ev64, ctx->A.ev32, ctx->b, ctx->P0, ctx->GP, ctx->W.x, ctx->W.r, ctx->W.z, ctx->W.p,
                     ctx->W.q, ctx->W.e32, ctx->W.r32, ctx->W.z32, ctx->W.p32, ctx->W.q32, ctx->W.S,
                     ctx->W.partials, ctx->scan_temp};
     for (void *p : ptrs) dfree(p);
+    for (int k = 0; k < 8; k++) {
+        dfree(ctx->pw_q[k]);
+        dfree(ctx->pw_z[k]);
+    }
+    for (int k = 0; k < 2; k++) {
+        dfree(ctx->pw_P[k]);
+        dfree(ctx->pw_AP[k]);
+    }
+    dfree(ctx->pw_t1);
+    dfree(ctx->pw_t2);
+    dfree(ctx->pw_dots);
     for (int k = 0; k < LV_NKTIMERS; k++) {
         if (!ctx->timers[k]) continue;
         for (int q = 0; q < 2 * LV_TPOOL; q++)
@@ -649,6 +669,7 @@ int lv_assemble_pressure(lv_ctx *ctx, double dt, int64_t *nnz)
         ctx->A.nnz_cap = want;
     }
     ctx->A.nnz = total;
+    ctx->A.deg = ctx->deg;
     rc = clear_flags(ctx);
     if (rc) return rc;
     LV_HIP_CHECK(lv_launch_assemble(ctx->g, ctx->F, M, n, dt, ctx->A, ctx->d_flags + 1, ctx->s));
@@ -696,6 +717,7 @@ static void default_opts(lv_solve_opts &o)
     o.itmax = 1000;
     o.inner_tol = 1e-4;
     o.outer_max = 20;
+    o.picard_window = 0;
 }
 
 // one linear solve on the assembled operator, device vectors b and x (x in/out)
@@ -706,10 +728,10 @@ static int solve_on_device(lv_ctx *ctx, const double *db, double *dx, const lv_s
     LV_HIP_CHECK(hipEventCreate(&e1));
     LV_HIP_CHECK(hipEventRecord(e0, ctx->s));
     int iters = 0, sweeps = 0;
-    double relres = 0.0;
+    double relres = 0.0, relres0 = 0.0;
     bool conv = false;
     hipError_t e = lv_solve_device(ctx->A, ctx->W, ctx->n, db, dx, opt, ctx->s, timer_if_on(ctx, 0),
-                                   timer_if_on(ctx, 1), &iters, &sweeps, &relres, &conv);
+                                   timer_if_on(ctx, 1), &iters, &sweeps, &relres, &relres0, &conv);
     if (e != hipSuccess) {
         hipEventDestroy(e0);
         hipEventDestroy(e1);
@@ -726,6 +748,7 @@ static int solve_on_device(lv_ctx *ctx, const double *db, double *dx, const lv_s
         st->iters[k] = iters;
         st->outer[k] = sweeps;
         st->relres[k] = relres;
+        st->relres0[k] = relres0;
         st->solve_ms[k] = ms;
     }
     if (st) {
@@ -753,6 +776,99 @@ int lv_solve_pressure(lv_ctx *ctx, const double *b, double *x, const lv_solve_op
     return rc;
 }
 
+static int ensure_picard_window(lv_ctx *ctx, int w)
+{
+    const size_t cap = (size_t)ctx->cap;
+    if (ctx->pw_cap == 0) {
+        for (int k = 0; k < 2; k++) {
+            LV_HIP_CHECK(dalloc(&ctx->pw_P[k], cap));
+            LV_HIP_CHECK(dalloc(&ctx->pw_AP[k], cap));
+        }
+        LV_HIP_CHECK(dalloc(&ctx->pw_t1, cap));
+        LV_HIP_CHECK(dalloc(&ctx->pw_t2, cap));
+        LV_HIP_CHECK(dalloc(&ctx->pw_dots, 8));
+    }
+    for (int k = ctx->pw_cap; k < w; k++) {
+        LV_HIP_CHECK(dalloc(&ctx->pw_q[k], cap));
+        LV_HIP_CHECK(dalloc(&ctx->pw_z[k], cap));
+    }
+    if (w > ctx->pw_cap) ctx->pw_cap = w;
+    return LV_OK;
+}
+
+// Adds the Picard correction (P_cur - P_prev, AP_cur - AP_prev) to the window: two rounds of
+// classical Gram-Schmidt against the stored q's (applied identically to the z's), then
+// normalisation; the oldest direction makes room when the window is full.
+static int picard_add_direction(lv_ctx *ctx, int w, int *cnt, const double *Pcur, const double *Pprev,
+                                const double *APcur, const double *APprev)
+{
+    const int n = ctx->n;
+    hipStream_t s = ctx->s;
+    const double *xp[8];
+    double c[8], h[8];
+    xp[0] = Pcur; xp[1] = Pprev; c[0] = 1.0; c[1] = -1.0;
+    LV_HIP_CHECK(lv_combine(n, ctx->pw_t1, 0.0, xp, c, 2, s));
+    xp[0] = APcur; xp[1] = APprev;
+    LV_HIP_CHECK(lv_combine(n, ctx->pw_t2, 0.0, xp, c, 2, s));
+    for (int pass = 0; pass < 2 && *cnt > 0; pass++) {
+        const double *a[8], *b[8];
+        for (int k = 0; k < *cnt; k++) {
+            a[k] = ctx->pw_q[k];
+            b[k] = ctx->pw_t2;
+        }
+        LV_HIP_CHECK(lv_dots(n, a, b, *cnt, h, ctx->W.partials, ctx->pw_dots, s));
+        for (int k = 0; k < *cnt; k++) {
+            c[k] = -h[k];
+            xp[k] = ctx->pw_q[k];
+        }
+        LV_HIP_CHECK(lv_combine(n, ctx->pw_t2, 1.0, xp, c, *cnt, s));
+        for (int k = 0; k < *cnt; k++) xp[k] = ctx->pw_z[k];
+        LV_HIP_CHECK(lv_combine(n, ctx->pw_t1, 1.0, xp, c, *cnt, s));
+    }
+    {
+        const double *a[1] = {ctx->pw_t2}, *b[1] = {ctx->pw_t2};
+        LV_HIP_CHECK(lv_dots(n, a, b, 1, h, ctx->W.partials, ctx->pw_dots, s));
+    }
+    const double nrm = sqrt(h[0]);
+    if (!(nrm > 0.0) || !(nrm == nrm)) return LV_OK;
+    if (*cnt == w) {  // drop the oldest direction
+        double *q0 = ctx->pw_q[0], *z0 = ctx->pw_z[0];
+        for (int k = 1; k < w; k++) {
+            ctx->pw_q[k - 1] = ctx->pw_q[k];
+            ctx->pw_z[k - 1] = ctx->pw_z[k];
+        }
+        ctx->pw_q[w - 1] = q0;
+        ctx->pw_z[w - 1] = z0;
+        (*cnt)--;
+    }
+    xp[0] = ctx->pw_t2;
+    c[0] = 1.0 / nrm;
+    LV_HIP_CHECK(lv_combine(n, ctx->pw_q[*cnt], 0.0, xp, c, 1, s));
+    xp[0] = ctx->pw_t1;
+    LV_HIP_CHECK(lv_combine(n, ctx->pw_z[*cnt], 0.0, xp, c, 1, s));
+    (*cnt)++;
+    return LV_OK;
+}
+
+// P0 += sum_k <q_k, b - AP_cur> z_k: the guess whose residual is r0 minus its projection on the q's
+static int picard_extrapolate(lv_ctx *ctx, int cnt, const double *APcur)
+{
+    const int n = ctx->n;
+    hipStream_t s = ctx->s;
+    const double *xp[8] = {ctx->b, APcur};
+    double c[8] = {1.0, -1.0}, h[8];
+    LV_HIP_CHECK(lv_combine(n, ctx->pw_t2, 0.0, xp, c, 2, s));
+    const double *a[8], *b[8];
+    for (int k = 0; k < cnt; k++) {
+        a[k] = ctx->pw_q[k];
+        b[k] = ctx->pw_t2;
+    }
+    LV_HIP_CHECK(lv_dots(n, a, b, cnt, h, ctx->W.partials, ctx->pw_dots, s));
+    for (int k = 0; k < cnt; k++) xp[k] = ctx->pw_z[k];
+    LV_HIP_CHECK(lv_combine(n, ctx->P0, 1.0, xp, h, cnt, s));
+    return LV_OK;
+}
+
 // find_pressure!(solver, dt, niter; boundary_velocity), src/pressure.jl:215-224
 int lv_find_pressure(lv_ctx *ctx, double dt, int32_t niter, const lv_solve_opts *opts, lv_solve_stats *stats)
 {
@@ -766,13 +882,37 @@ int lv_find_pressure(lv_ctx *ctx, double dt, int32_t niter, const lv_solve_opts 
     if (rc) return rc;
     lv_vbc v;
     for (int k = 0; k < 8; k++) v.v[k] = o.vbc[k];
+    const int w = std::min(std::max(o.picard_window, 0), 8);
+    if (w > 0) {
+        rc = ensure_picard_window(ctx, w);
+        if (rc) return rc;
+    }
+    int cnt = 0, have = 0;  // directions in the window; finished Picard iterates remembered (<= 2)
     int status = LV_OK;
     for (int it = 1; it <= niter; it++) {
         LV_HIP_CHECK(lv_launch_rhs(ctx->g, ctx->F, mesh_view(ctx), ctx->n, dt, it > 1, v, ctx->b, ctx->P0, ctx->GP,
                                    ctx->s));
+        if (w > 0 && have >= 1) {
+            if (have >= 2) {
+                rc = picard_add_direction(ctx, w, &cnt, ctx->pw_P[0], ctx->pw_P[1], ctx->pw_AP[0], ctx->pw_AP[1]);
+                if (rc) return rc;
+            }
+            if (cnt > 0) {
+                rc = picard_extrapolate(ctx, cnt, ctx->pw_AP[0]);
+                if (rc) return rc;
+            }
+        }
         rc = solve_on_device(ctx, ctx->b, ctx->P0, o, stats);
         if (rc == LV_E_NOT_CONVERGED) status = rc;
         else if (rc) return rc;
+        if (w > 0) {  // remember P_it and A P_it = b' - r (the solve's final residual)
+            std::swap(ctx->pw_P[0], ctx->pw_P[1]);
+            std::swap(ctx->pw_AP[0], ctx->pw_AP[1]);
+            LV_HIP_CHECK(hipMemcpyAsync(ctx->pw_P[0], ctx->P0, sizeof(double) * ctx->n, hipMemcpyDeviceToDevice,
+                                        ctx->s));
+            LV_HIP_CHECK(lv_apply_from_residual(ctx->n, ctx->b, ctx->W.r, ctx->pw_AP[0], ctx->W, ctx->s));
+            have = std::min(have + 1, 2);
+        }
         LV_HIP_CHECK(lv_launch_write_P(ctx->F, ctx->n, ctx->P0, ctx->s));
     }
     if (status) lv_fail(ctx, status, "pressure solve did not reach its tolerance");
@@ -891,8 +1031,9 @@ int lv_timing_read(lv_ctx *ctx, int32_t kernel_id, double *total_ms, int64_t *la
         const double n = ctx->n, nnz = (double)ctx->A.nnz, m = ctx->nitems;
         const double nc = (double)ctx->g.n1 * ctx->g.n2;
         double b = 0.0;
-        if (kernel_id == 0) b = 4.0 * (n + 1) + 8.0 * nnz + 4.0 * n * 3;       // row_ptr, cols+vals, diag, p, q
-        else if (kernel_id == 1) b = 4.0 * (n + 1) + 12.0 * nnz + 8.0 * n * 3;  // FP64 values
+        // CG SpMV kernel: deg + diag + z + p (read, write) + q (read, write) per row, col + value per nonzero
+        if (kernel_id == 0) b = 4.0 * n + 4.0 * n * 6 + 8.0 * nnz;
+        else if (kernel_id == 1) b = 4.0 * n + 8.0 * n * 6 + 12.0 * nnz;
         else if (kernel_id == 2) b = 16.0 * n + 20.0 * m + 4.0 * (nc + 1) + 4.0 * n + 36.0 * 6.0 * n + 24.0 * n;
         else if (kernel_id == 3) b = 3.0 * (8.0 + 8.0 + 4.0) * m;              // 3 radix passes over (key, label)
         *bytes_per_launch = b;

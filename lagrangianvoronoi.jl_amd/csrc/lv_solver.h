@@ -7,17 +7,23 @@
 // rz is double-buffered by iteration parity: the p-update of iteration k reads rz[k&1] in every
 // block while block 0 writes rz[(k+1)&1].
 struct lv_cg_scalars {
-    double rz[2], rr, pq, alpha, beta, bb, thr2, tol2, rr64, bmean, xmean0, xmean;
+    double rz[2], rr, rr0, pq, alpha, beta, bb, thr2, tol2, rr64, bmean, xmean0, xmean;
     int iters, itmax, done, singular;
 };
 
 // PressureOperator (src/pressure.jl:89-101) in CSR: off-diagonal coefficients c_ij in the edge order
 // of row i, the compressibility diagonal d_i, and the Jacobi inverse 1/(d_i + sum_j c_ij), each in
 // FP64 and FP32.
+//
+// The solver reads a slot-major ELL copy of the same coefficients: slot k of row i at [k*n + i],
+// deg[i] slots per row, so slot k of a wavefront's 64 rows is one contiguous run.
 struct lv_pressure_system {
     int *row_ptr, *cols;
     double *val64, *diag64, *dinv64;
     float *val32, *diag32, *dinv32;
+    int *deg, *ecol;
+    double *ev64;
+    float *ev32;
     int64_t nnz, nnz_cap;
     int singular;  // every d_i == 0 (c2 = Inf): a graph Laplacian with the constants as null space
     int valid;
@@ -51,4 +57,11 @@ hipError_t lv_launch_rhs(const lv_grid_params &g, lv_fields F, lv_mesh_view M, i
 hipError_t lv_launch_write_P(lv_fields F, int n, const double *P, hipStream_t s);
 hipError_t lv_solve_device(const lv_pressure_system &A, lv_solver_work &W, int n, const double *b, double *x,
                            const lv_solve_opts &opt, hipStream_t s, lv_timer *t32, lv_timer *t64, int *iters,
-                           int *sweeps, double *relres, bool *converged);
+                           int *sweeps, double *relres, double *relres0, bool *converged);
+
+// Picard extrapolation helpers (FP64 vectors of length n on stream s)
+hipError_t lv_dots(int n, const double *const *a, const double *const *b, int m, double *out_host, double *partials,
+                   double *dout, hipStream_t s);
+hipError_t lv_combine(int n, double *y, double alpha, const double *const *x, const double *c, int m, hipStream_t s);
+hipError_t lv_apply_from_residual(int n, const double *b, const double *r, double *Ax, const lv_solver_work &W,
+                                  hipStream_t s);

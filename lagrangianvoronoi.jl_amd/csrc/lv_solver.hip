@@ -37,7 +37,8 @@ __global__ __launch_bounds__(B) void k_assemble(lv_grid_params g, lv_fields F, l
                                                 double *__restrict__ val64, float *__restrict__ val32,
                                                 double *__restrict__ diag64, float *__restrict__ diag32,
                                                 double *__restrict__ dinv64, float *__restrict__ dinv32,
-                                                int *__restrict__ nonsingular)
+                                                int *__restrict__ ecol, double *__restrict__ ev64,
+                                                float *__restrict__ ev32, int *__restrict__ nonsingular)
 {
     const int i = blockIdx.x * B + threadIdx.x;
     if (i >= n) return;
@@ -46,6 +47,11 @@ __global__ __launch_bounds__(B) void k_assemble(lv_grid_params g, lv_fields F, l
     const double d = F.mass[i] / (((rho * rho) * F.c2[i]) * (dt * dt));
     double dd = d;
     int o = row_ptr[i];
+    // the solver's ELL copy keeps each row's neighbours sorted by column, so that slot k of a
+    // wavefront gathers from one lattice neighbourhood; the CSR copy keeps the edge order
+    int sc[LV_MAXE];
+    double sv[LV_MAXE];
+    int m = 0;
     const int ne = M.ne[i];
     for (int k = 0; k < ne; k++) {
         const int j = LV_EL(M, k, i);
@@ -60,6 +66,20 @@ __global__ __launch_bounds__(B) void k_assemble(lv_grid_params g, lv_fields F, l
         val32[o] = (float)c;
         dd += c;
         o++;
+        int q = m++;
+        while (q > 0 && sc[q - 1] > j) {
+            sc[q] = sc[q - 1];
+            sv[q] = sv[q - 1];
+            q--;
+        }
+        sc[q] = j;
+        sv[q] = c;
+    }
+    for (int k = 0; k < m; k++) {
+        const size_t es = (size_t)k * n + i;
+        ecol[es] = sc[k];
+        ev64[es] = sv[k];
+        ev32[es] = (float)sv[k];
     }
     // the system is singular iff every d_i == 0 (c2 = Inf): one flag write per wavefront at most
     if (__any(d != 0.0) && (threadIdx.x & 63) == 0 && *nonsingular == 0) atomicOr(nonsingular, 1);
@@ -213,13 +233,11 @@ __global__ __launch_bounds__(B) void k_finalize(const double *__restrict__ parti
         S->bmean = S->singular ? o[0] / n : 0.0;
         S->xmean0 = o[1] / n;
         break;
-    case FIN_INIT64:
-        S->rz[0] = o[0];
-        S->rr = o[1];
-        S->bb = o[2];
-        S->thr2 = S->tol2 * o[2];
+    case FIN_INIT64:  // P3 holds |b'|^2
+        S->bb = o[0];
+        S->thr2 = S->tol2 * o[0];
         S->iters = 0;
-        S->done = (o[1] <= S->thr2) ? 1 : 0;
+        S->done = 0;
         break;
     case FIN_RESID:
         S->rr64 = o[0];
@@ -229,14 +247,24 @@ __global__ __launch_bounds__(B) void k_finalize(const double *__restrict__ parti
     case FIN_SUM1:
         S->xmean = o[0] / n;
         break;
-    case FIN_INIT32:
-        S->rz[0] = o[0];
-        S->rr = o[1];
+    case FIN_INIT32:  // reads the (r.z, r.r) partials of the inner init: |r32|^2 = 1 up to rounding
         S->thr2 = inner_tol2 * o[1];
         S->iters = 0;
-        S->done = (o[1] <= S->thr2) ? 1 : 0;
+        S->done = 0;
         break;
     }
+}
+
+// thr2 of an inner FP32 solve from the (r.z, r.r) partials of its init
+__global__ __launch_bounds__(B) void k_finalize2(const double *__restrict__ Prr, int nb, double inner_tol2,
+                                                 lv_cg_scalars *S)
+{
+    double o[2];
+    reduce_all<2>(Prr, nb, o);
+    if (threadIdx.x != 0) return;
+    S->thr2 = inner_tol2 * o[1];
+    S->iters = 0;
+    S->done = 0;
 }
 
 // ---- FP64 Jacobi-PCG ----------------------------------------------------------------------------
@@ -253,69 +281,131 @@ __global__ __launch_bounds__(B) void k_means(int n, const double *__restrict__ b
     block_partials<3>(v, partials, gridDim.x);
 }
 
-// y = A x in the reference's difference form
+// (A x)_i in the reference's difference form, from the slot-major ELL copy.  The first 8 slots are
+// unrolled and predicated so that all their column and coefficient loads are in flight before the
+// dependent gathers of x; rows with more than 8 neighbours finish in a short loop.
 template <typename T>
-__device__ __forceinline__ T spmv_row(int i, const int *__restrict__ rp, const int *__restrict__ cols,
-                                      const T *__restrict__ vals, const T *__restrict__ diag, const T *__restrict__ x)
+__device__ __forceinline__ T spmv_row(int i, int n, const int *__restrict__ deg, const int *__restrict__ ecol,
+                                      const T *__restrict__ ev, const T *__restrict__ diag, const T *__restrict__ x)
 {
+    constexpr int U = 8;
     const T xi = x[i];
     T acc = diag[i] * xi;
-    const int e = rp[i + 1];
-    for (int o = rp[i]; o < e; o++) acc += vals[o] * (xi - x[cols[o]]);
+    const int d = deg[i];
+    int c[U];
+    T v[U];
+#pragma unroll
+    for (int k = 0; k < U; k++) {
+        c[k] = i;
+        v[k] = (T)0;
+        if (k < d) {
+            const size_t e = (size_t)k * n + i;
+            c[k] = ecol[e];
+            v[k] = ev[e];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < U; k++)
+        if (k < d) acc += v[k] * (xi - x[c[k]]);
+    for (int k = U; k < d; k++) {
+        const size_t e = (size_t)k * n + i;
+        acc += ev[e] * (xi - x[ecol[e]]);
+    }
     return acc;
 }
 
 // r = (b - bmean) - A x; z = dinv r; p = z; partials r.z, r.r, b'.b'
-__global__ __launch_bounds__(B) void k_cg64_init(int n, const int *__restrict__ rp, const int *__restrict__ cols,
-                                                 const double *__restrict__ vals, const double *__restrict__ diag,
+__global__ __launch_bounds__(B) void k_cg64_init(int n, const int *__restrict__ deg, const int *__restrict__ ecol,
+                                                 const double *__restrict__ ev, const double *__restrict__ diag,
                                                  const double *__restrict__ dinv, const double *__restrict__ b,
                                                  const double *__restrict__ x, double *__restrict__ r,
-                                                 double *__restrict__ z, double *__restrict__ p,
-                                                 double *__restrict__ partials, const lv_cg_scalars *S)
+                                                 double *__restrict__ z, double *__restrict__ Prr,
+                                                 double *__restrict__ P3, const lv_cg_scalars *S)
 {
     const double bmean = S->bmean;
-    double v[3] = {0.0, 0.0, 0.0};
+    double v[2] = {0.0, 0.0}, w[3] = {0.0, 0.0, 0.0};
     for (int i = blockIdx.x * B + threadIdx.x; i < n; i += gridDim.x * B) {
         const double bi = b[i] - bmean;
-        const double ri = bi - spmv_row<double>(i, rp, cols, vals, diag, x);
+        const double ri = bi - spmv_row<double>(i, n, deg, ecol, ev, diag, x);
         const double zi = dinv[i] * ri;
         r[i] = ri;
         z[i] = zi;
-        p[i] = zi;
         v[0] += ri * zi;
         v[1] += ri * ri;
-        v[2] += bi * bi;
+        w[0] += bi * bi;
     }
-    block_partials<3>(v, partials, gridDim.x);
+    block_partials<2>(v, Prr, gridDim.x);
+    block_partials<3>(w, P3, gridDim.x);
 }
 
+// One CG iteration is two launches.
+//
+// S (SpMV): reduces r.z and r.r of the previous U, checks convergence, then for every row
+//   w_i = (A z)_i,  q_i = w_i + beta q_i,  p_i = z_i + beta p_i,  partial p.q
+// using A p_new = A z + beta A p_old, so only z is gathered and p, q are touched row-locally.
+// The block's slice of the CSR arrays is first staged through LDS with coalesced loads.
+// U (update): alpha = r.z / p.q;  x += alpha p;  r -= alpha q;  z = dinv r;  partials r.z, r.r.
 template <typename T>
-__global__ __launch_bounds__(B) void k_cg_spmv(int n, const int *__restrict__ rp, const int *__restrict__ cols,
-                                               const T *__restrict__ vals, const T *__restrict__ diag,
-                                               const T *__restrict__ p, T *__restrict__ q,
-                                               double *__restrict__ Ppq, const lv_cg_scalars *S)
+__global__ __launch_bounds__(B) void k_cg_s(int n, const int *__restrict__ deg, const int *__restrict__ ecol,
+                                            const T *__restrict__ ev, const T *__restrict__ diag,
+                                            const T *__restrict__ z, T *__restrict__ p, T *__restrict__ q,
+                                            const double *__restrict__ Prr, double *__restrict__ Ppq,
+                                            lv_cg_scalars *S, int par, int k, int first)
 {
     if (S->done) return;
-    double v[1] = {0.0};
-    for (int i = blockIdx.x * B + threadIdx.x; i < n; i += gridDim.x * B) {
-        const T qi = spmv_row<T>(i, rp, cols, vals, diag, p);
-        q[i] = qi;
-        v[0] += (double)p[i] * (double)qi;
+    double o[2];
+    reduce_all<2>(Prr, gridDim.x, o);
+    const double rz = o[0], rr = o[1];
+    const bool conv = rr <= S->thr2;
+    const bool stop = conv || k >= S->itmax || !(rr == rr);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        S->rz[par ^ 1] = rz;
+        S->rr = rr;
+        if (k == 0) S->rr0 = rr;
+        S->iters = k;
+        if (stop) S->done = conv ? 1 : 2;
     }
+    if (stop) return;
+    const T beta = first ? (T)0 : (T)(rz / S->rz[par]);
+    double pq = 0.0;
+    const int stride = gridDim.x * B;
+    // two rows per thread per pass keep twice the loads in flight
+    for (int i0 = blockIdx.x * B + threadIdx.x; i0 < n; i0 += 2 * stride) {
+        const int i1 = i0 + stride;
+        const T z0 = z[i0];
+        const T w0 = spmv_row<T>(i0, n, deg, ecol, ev, diag, z);
+        T z1 = (T)0, w1 = (T)0;
+        if (i1 < n) {
+            z1 = z[i1];
+            w1 = spmv_row<T>(i1, n, deg, ecol, ev, diag, z);
+        }
+        const T q0 = first ? w0 : w0 + beta * q[i0];
+        const T p0 = first ? z0 : z0 + beta * p[i0];
+        q[i0] = q0;
+        p[i0] = p0;
+        pq += (double)p0 * (double)q0;
+        if (i1 < n) {
+            const T q1 = first ? w1 : w1 + beta * q[i1];
+            const T p1 = first ? z1 : z1 + beta * p[i1];
+            q[i1] = q1;
+            p[i1] = p1;
+            pq += (double)p1 * (double)q1;
+        }
+    }
+    double v[1] = {pq};
     block_partials<1>(v, Ppq, gridDim.x);
 }
 
-// alpha = rz/pq; x += alpha p; r -= alpha q; z = dinv r; partials r.z, r.r
 template <typename T>
-__global__ __launch_bounds__(B) void k_cg_update(int n, T *__restrict__ x, T *__restrict__ r, T *__restrict__ z,
-                                                 const T *__restrict__ p, const T *__restrict__ q,
-                                                 const T *__restrict__ dinv, const double *__restrict__ Ppq,
-                                                 double *__restrict__ Prr, lv_cg_scalars *S, int par)
+__global__ __launch_bounds__(B) void k_cg_u(int n, T *__restrict__ x, T *__restrict__ r, T *__restrict__ z,
+                                            const T *__restrict__ p, const T *__restrict__ q,
+                                            const T *__restrict__ dinv, const double *__restrict__ Ppq,
+                                            double *__restrict__ Prr, lv_cg_scalars *S, int par)
 {
     if (S->done) return;
     double pq[1];
     reduce_all<1>(Ppq, gridDim.x, pq);
-    const double alpha = S->rz[par] / pq[0];
+    const double alpha = S->rz[par ^ 1] / pq[0];
     const T a = (T)alpha;
     double v[2] = {0.0, 0.0};
     for (int i = blockIdx.x * B + threadIdx.x; i < n; i += gridDim.x * B) {
@@ -334,30 +424,9 @@ __global__ __launch_bounds__(B) void k_cg_update(int n, T *__restrict__ x, T *__
     }
 }
 
-// beta = rz_new/rz; p = z + beta p; convergence bookkeeping in block 0
-template <typename T>
-__global__ __launch_bounds__(B) void k_cg_pupdate(int n, const T *__restrict__ z, T *__restrict__ p,
-                                                  const double *__restrict__ Prr, lv_cg_scalars *S, int par)
-{
-    if (S->done) return;
-    double o[2];
-    reduce_all<2>(Prr, gridDim.x, o);
-    const double beta = o[0] / S->rz[par];
-    const T bt = (T)beta;
-    for (int i = blockIdx.x * B + threadIdx.x; i < n; i += gridDim.x * B) p[i] = z[i] + bt * p[i];
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        S->rz[par ^ 1] = o[0];
-        S->beta = beta;
-        S->rr = o[1];
-        S->iters += 1;
-        if (o[1] <= S->thr2) S->done = 1;
-        else if (S->iters >= S->itmax || !(o[1] == o[1])) S->done = 2;
-    }
-}
-
 // true residual r = (b - bmean) - A x; partials |r|^2, |b'|^2 (FP64)
-__global__ __launch_bounds__(B) void k_resid64(int n, const int *__restrict__ rp, const int *__restrict__ cols,
-                                               const double *__restrict__ vals, const double *__restrict__ diag,
+__global__ __launch_bounds__(B) void k_resid64(int n, const int *__restrict__ deg, const int *__restrict__ ecol,
+                                               const double *__restrict__ ev, const double *__restrict__ diag,
                                                const double *__restrict__ b, const double *__restrict__ x,
                                                double *__restrict__ r, double *__restrict__ partials,
                                                const lv_cg_scalars *S)
@@ -366,7 +435,7 @@ __global__ __launch_bounds__(B) void k_resid64(int n, const int *__restrict__ rp
     double v[3] = {0.0, 0.0, 0.0};
     for (int i = blockIdx.x * B + threadIdx.x; i < n; i += gridDim.x * B) {
         const double bi = b[i] - bmean;
-        const double ri = bi - spmv_row<double>(i, rp, cols, vals, diag, x);
+        const double ri = bi - spmv_row<double>(i, n, deg, ecol, ev, diag, x);
         r[i] = ri;
         v[0] += ri * ri;
         v[1] += bi * bi;
@@ -393,22 +462,20 @@ __global__ __launch_bounds__(B) void k_shift(int n, double *__restrict__ x, cons
 // r32 = r64 / |r64|, e = 0, z = dinv r, p = z; partials r.z, r.r
 __global__ __launch_bounds__(B) void k_cg32_init(int n, const double *__restrict__ r64, const float *__restrict__ dinv,
                                                  float *__restrict__ e, float *__restrict__ r, float *__restrict__ z,
-                                                 float *__restrict__ p, double *__restrict__ partials,
-                                                 const lv_cg_scalars *S)
+                                                 double *__restrict__ Prr, const lv_cg_scalars *S)
 {
     const double inv = 1.0 / sqrt(S->rr64);
-    double v[3] = {0.0, 0.0, 0.0};
+    double v[2] = {0.0, 0.0};
     for (int i = blockIdx.x * B + threadIdx.x; i < n; i += gridDim.x * B) {
         const float ri = (float)(r64[i] * inv);
         const float zi = dinv[i] * ri;
         e[i] = 0.0f;
         r[i] = ri;
         z[i] = zi;
-        p[i] = zi;
         v[0] += (double)ri * (double)zi;
         v[1] += (double)ri * (double)ri;
     }
-    block_partials<3>(v, partials, gridDim.x);
+    block_partials<2>(v, Prr, gridDim.x);
 }
 
 // x += |r64| * e
@@ -417,6 +484,57 @@ __global__ __launch_bounds__(B) void k_correct(int n, double *__restrict__ x, co
 {
     const double s = sqrt(S->rr64);
     for (int i = blockIdx.x * B + threadIdx.x; i < n; i += gridDim.x * B) x[i] += s * (double)e[i];
+}
+
+// ---- small FP64 vector algebra for the Picard extrapolation -----------------------------------
+
+struct lv_vec8 {
+    const double *a[8], *b[8];
+};
+struct lv_comb8 {
+    const double *x[8];
+    double c[8];
+};
+
+// up to 8 dot products a_q . b_q in one pass
+__global__ __launch_bounds__(B) void k_multidot(int n, lv_vec8 V, int m, double *__restrict__ partials)
+{
+    double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int i = blockIdx.x * B + threadIdx.x; i < n; i += gridDim.x * B) {
+#pragma unroll
+        for (int q = 0; q < 8; q++)
+            if (q < m) v[q] += V.a[q][i] * V.b[q][i];
+    }
+    block_partials<8>(v, partials, gridDim.x);
+}
+
+__global__ __launch_bounds__(B) void k_multidot_fin(const double *__restrict__ partials, int nb,
+                                                    double *__restrict__ out)
+{
+    double o[8];
+    reduce_all<8>(partials, nb, o);
+    if (threadIdx.x < 8) out[threadIdx.x] = o[threadIdx.x];
+}
+
+// y = alpha*y + sum_q c_q x_q  (y may be one of the x_q only if its c is applied via alpha)
+__global__ __launch_bounds__(B) void k_combine(int n, double *__restrict__ y, double alpha, lv_comb8 C, int m)
+{
+    for (int i = blockIdx.x * B + threadIdx.x; i < n; i += gridDim.x * B) {
+        double acc = (alpha != 0.0) ? alpha * y[i] : 0.0;  // alpha = 0 never reads y (may be uninitialised)
+#pragma unroll
+        for (int q = 0; q < 8; q++)
+            if (q < m) acc += C.c[q] * C.x[q][i];
+        y[i] = acc;
+    }
+}
+
+// A x = b' - r: recovers A P of a finished solve from its final residual (b' = b - bmean)
+__global__ __launch_bounds__(B) void k_apply_from_residual(int n, const double *__restrict__ b,
+                                                          const double *__restrict__ r, double *__restrict__ Ax,
+                                                          const lv_cg_scalars *S)
+{
+    const double bmean = S->bmean;
+    for (int i = blockIdx.x * B + threadIdx.x; i < n; i += gridDim.x * B) Ax[i] = (b[i] - bmean) - r[i];
 }
 
 inline dim3 grid_of(int n) { return dim3((n + B - 1) / B); }
@@ -444,7 +562,7 @@ hipError_t lv_launch_assemble(const lv_grid_params &g, lv_fields F, lv_mesh_view
                               lv_pressure_system &A, int *nonsingular, hipStream_t s)
 {
     hipLaunchKernelGGL(k_assemble, grid_of(n), dim3(B), 0, s, g, F, M, n, dt, (const int *)A.row_ptr, A.cols,
-                       A.val64, A.val32, A.diag64, A.diag32, A.dinv64, A.dinv32, nonsingular);
+                       A.val64, A.val32, A.diag64, A.diag32, A.dinv64, A.dinv32, A.ecol, A.ev64, A.ev32, nonsingular);
     return hipGetLastError();
 }
 
@@ -496,10 +614,10 @@ struct parts {
 };
 inline parts parts_of(double *base) { return parts{base, base + 3 * NBMAX, base + 4 * NBMAX}; }
 
-// CG iterations (three launches each) until the device flags completion; the host checks the flag
+// CG iterations (two launches each) until the device flags completion; the host checks the flag
 // once per batch of iterations.
 template <typename T>
-hipError_t cg_iterate(int n, const int *rp, const int *cols, const T *vals, const T *diag, const T *dinv, T *x, T *r,
+hipError_t cg_iterate(int n, const int *deg, const int *ecol, const T *ev, const T *diag, const T *dinv, T *x, T *r,
                       T *z, T *p, T *q, const parts &P, lv_cg_scalars *dS, hipStream_t s, lv_timer *tm, int batch,
                       lv_cg_scalars &h)
 {
@@ -510,13 +628,11 @@ hipError_t cg_iterate(int n, const int *rp, const int *cols, const T *vals, cons
             const int par = k & 1;
             {
                 timer_scope ts(tm, s);
-                hipLaunchKernelGGL(k_cg_spmv<T>, dim3(nb), dim3(B), 0, s, n, rp, cols, vals, diag, (const T *)p, q,
-                                   P.pq, (const lv_cg_scalars *)dS);
+                hipLaunchKernelGGL(k_cg_s<T>, dim3(nb), dim3(B), 0, s, n, deg, ecol, ev, diag, (const T *)z, p, q,
+                                   (const double *)P.rr, P.pq, dS, par, k, k == 0 ? 1 : 0);
             }
-            hipLaunchKernelGGL(k_cg_update<T>, dim3(nb), dim3(B), 0, s, n, x, r, z, (const T *)p, (const T *)q, dinv,
+            hipLaunchKernelGGL(k_cg_u<T>, dim3(nb), dim3(B), 0, s, n, x, r, z, (const T *)p, (const T *)q, dinv,
                                (const double *)P.pq, P.rr, dS, par);
-            hipLaunchKernelGGL(k_cg_pupdate<T>, dim3(nb), dim3(B), 0, s, n, (const T *)z, p, (const double *)P.rr,
-                               dS, par);
         }
         hipError_t e = read_scalars(dS, &h, s);
         if (e != hipSuccess) return e;
@@ -531,8 +647,9 @@ hipError_t cg_iterate(int n, const int *rp, const int *cols, const T *vals, cons
 // final FP64 relative residual.  *converged tells whether tol was met.
 hipError_t lv_solve_device(const lv_pressure_system &A, lv_solver_work &W, int n, const double *b, double *x,
                            const lv_solve_opts &opt, hipStream_t s, lv_timer *t32, lv_timer *t64, int *iters,
-                           int *sweeps, double *relres, bool *converged)
+                           int *sweeps, double *relres, double *relres0, bool *converged)
 {
+    *relres0 = -1.0;
     const int nb = nblocks_of(n);
     const dim3 G(nb);
     const parts P = parts_of(W.partials);
@@ -550,33 +667,28 @@ hipError_t lv_solve_device(const lv_pressure_system &A, lv_solver_work &W, int n
     const int batch = 16;
     int total = 0, nsweep = 0;
     if (opt.method == LV_SOLVER_PCG64) {
-        hipLaunchKernelGGL(k_cg64_init, G, dim3(B), 0, s, n, (const int *)A.row_ptr, (const int *)A.cols,
-                           (const double *)A.val64, (const double *)A.diag64, (const double *)A.dinv64, b,
-                           (const double *)W.x, W.r, W.z, W.p, P.p3, (const lv_cg_scalars *)W.S);
+        hipLaunchKernelGGL(k_cg64_init, G, dim3(B), 0, s, n, (const int *)A.deg, (const int *)A.ecol,
+                           (const double *)A.ev64, (const double *)A.diag64, (const double *)A.dinv64, b,
+                           (const double *)W.x, W.r, W.z, P.rr, P.p3, (const lv_cg_scalars *)W.S);
         hipLaunchKernelGGL(k_finalize, dim3(1), dim3(B), 0, s, (const double *)P.p3, nb, (int)FIN_INIT64, n, 0.0,
                            W.S);
-        e = read_scalars(W.S, &h, s);
+        e = cg_iterate<double>(n, A.deg, A.ecol, A.ev64, A.diag64, A.dinv64, W.x, W.r, W.z, W.p, W.q, P, W.S, s,
+                               t64, batch, h);
         if (e != hipSuccess) return e;
-        if (!h.done) {
-            e = cg_iterate<double>(n, A.row_ptr, A.cols, A.val64, A.diag64, A.dinv64, W.x, W.r, W.z, W.p, W.q, P,
-                                   W.S, s, t64, batch, h);
-            if (e != hipSuccess) return e;
-        }
         total = h.iters;
+        *relres0 = (h.bb > 0.0) ? sqrt(h.rr0 / h.bb) : sqrt(h.rr0);
     } else {
         const int outer_max = opt.outer_max > 0 ? opt.outer_max : 20;
         const double floor_tol = opt.inner_tol > 0.0 ? opt.inner_tol : 1e-4;
         for (;;) {
-            {
-                timer_scope ts(t64, s);
-                hipLaunchKernelGGL(k_resid64, G, dim3(B), 0, s, n, (const int *)A.row_ptr, (const int *)A.cols,
-                                   (const double *)A.val64, (const double *)A.diag64, b, (const double *)W.x, W.r,
-                                   P.p3, (const lv_cg_scalars *)W.S);
-            }
+            hipLaunchKernelGGL(k_resid64, G, dim3(B), 0, s, n, (const int *)A.deg, (const int *)A.ecol,
+                               (const double *)A.ev64, (const double *)A.diag64, b, (const double *)W.x, W.r, P.p3,
+                               (const lv_cg_scalars *)W.S);
             hipLaunchKernelGGL(k_finalize, dim3(1), dim3(B), 0, s, (const double *)P.p3, nb, (int)FIN_RESID, n, 0.0,
                                W.S);
             e = read_scalars(W.S, &h, s);
             if (e != hipSuccess) return e;
+            if (nsweep == 0) *relres0 = (h.bb > 0.0) ? sqrt(h.rr64 / h.bb) : sqrt(h.rr64);
             if (h.rr64 <= h.thr2 || nsweep >= outer_max || !(h.rr64 == h.rr64)) break;
             // ask the FP32 sweep for just the reduction still missing (with a 0.3 margin), but not
             // below what FP32 arithmetic can deliver in one sweep
@@ -584,16 +696,11 @@ hipError_t lv_solve_device(const lv_pressure_system &A, lv_solver_work &W, int n
             double itol = need > floor_tol ? need : floor_tol;
             if (itol > 0.5) itol = 0.5;
             hipLaunchKernelGGL(k_cg32_init, G, dim3(B), 0, s, n, (const double *)W.r, (const float *)A.dinv32, W.e32,
-                               W.r32, W.z32, W.p32, P.p3, (const lv_cg_scalars *)W.S);
-            hipLaunchKernelGGL(k_finalize, dim3(1), dim3(B), 0, s, (const double *)P.p3, nb, (int)FIN_INIT32, n,
-                               itol * itol, W.S);
-            e = read_scalars(W.S, &h, s);
+                               W.r32, W.z32, P.rr, (const lv_cg_scalars *)W.S);
+            hipLaunchKernelGGL(k_finalize2, dim3(1), dim3(B), 0, s, (const double *)P.rr, nb, itol * itol, W.S);
+            e = cg_iterate<float>(n, A.deg, A.ecol, A.ev32, A.diag32, A.dinv32, W.e32, W.r32, W.z32, W.p32, W.q32,
+                                  P, W.S, s, t32, batch, h);
             if (e != hipSuccess) return e;
-            if (!h.done) {
-                e = cg_iterate<float>(n, A.row_ptr, A.cols, A.val32, A.diag32, A.dinv32, W.e32, W.r32, W.z32, W.p32,
-                                      W.q32, P, W.S, s, t32, batch, h);
-                if (e != hipSuccess) return e;
-            }
             total += h.iters;
             nsweep++;
             hipLaunchKernelGGL(k_correct, G, dim3(B), 0, s, n, W.x, (const float *)W.e32, (const lv_cg_scalars *)W.S);
@@ -605,8 +712,8 @@ hipError_t lv_solve_device(const lv_pressure_system &A, lv_solver_work &W, int n
         hipLaunchKernelGGL(k_finalize, dim3(1), dim3(B), 0, s, (const double *)P.p3, nb, (int)FIN_SUM1, n, 0.0, W.S);
         hipLaunchKernelGGL(k_shift, G, dim3(B), 0, s, n, W.x, (const lv_cg_scalars *)W.S);
     }
-    hipLaunchKernelGGL(k_resid64, G, dim3(B), 0, s, n, (const int *)A.row_ptr, (const int *)A.cols,
-                       (const double *)A.val64, (const double *)A.diag64, b, (const double *)W.x, W.r, P.p3,
+    hipLaunchKernelGGL(k_resid64, G, dim3(B), 0, s, n, (const int *)A.deg, (const int *)A.ecol,
+                       (const double *)A.ev64, (const double *)A.diag64, b, (const double *)W.x, W.r, P.p3,
                        (const lv_cg_scalars *)W.S);
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(B), 0, s, (const double *)P.p3, nb, (int)FIN_RESID, n, 0.0, W.S);
     e = read_scalars(W.S, &h, s);
@@ -618,4 +725,41 @@ hipError_t lv_solve_device(const lv_pressure_system &A, lv_solver_work &W, int n
     *relres = (h.bb > 0.0) ? sqrt(h.rr64 / h.bb) : sqrt(h.rr64);
     *converged = (h.rr64 <= h.thr2);
     return hipSuccess;
+}
+
+// ---- host helpers for the Picard extrapolation ------------------------------------------------
+
+hipError_t lv_dots(int n, const double *const *a, const double *const *b, int m, double *out_host, double *partials,
+                   double *dout, hipStream_t s)
+{
+    lv_vec8 V;
+    for (int q = 0; q < 8; q++) {
+        V.a[q] = (q < m) ? a[q] : a[0];
+        V.b[q] = (q < m) ? b[q] : b[0];
+    }
+    const int nb = nblocks_of(n);
+    hipLaunchKernelGGL(k_multidot, dim3(nb), dim3(B), 0, s, n, V, m, partials);
+    hipLaunchKernelGGL(k_multidot_fin, dim3(1), dim3(B), 0, s, (const double *)partials, nb, dout);
+    hipError_t e = hipMemcpyAsync(out_host, dout, sizeof(double) * 8, hipMemcpyDeviceToHost, s);
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(s);
+}
+
+hipError_t lv_combine(int n, double *y, double alpha, const double *const *x, const double *c, int m, hipStream_t s)
+{
+    lv_comb8 C;
+    for (int q = 0; q < 8; q++) {
+        C.x[q] = (q < m) ? x[q] : y;
+        C.c[q] = (q < m) ? c[q] : 0.0;
+    }
+    hipLaunchKernelGGL(k_combine, dim3(nblocks_of(n)), dim3(B), 0, s, n, y, alpha, C, m);
+    return hipGetLastError();
+}
+
+hipError_t lv_apply_from_residual(int n, const double *b, const double *r, double *Ax, const lv_solver_work &W,
+                                  hipStream_t s)
+{
+    hipLaunchKernelGGL(k_apply_from_residual, dim3(nblocks_of(n)), dim3(B), 0, s, n, b, r, Ax,
+                       (const lv_cg_scalars *)W.S);
+    return hipGetLastError();
 }

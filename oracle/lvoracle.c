@@ -804,6 +804,7 @@ int lvo_pcg(const lvo_state *s, const double *diag, const double *coef, const do
         lvo_apply_A(s, diag, coef, p, q);
         double pq = 0.0;
         for (int i = 0; i < n; i++) pq += p[i] * q[i];
+        if (!(pq > 0.0) || !(rz > 0.0)) break; /* breakdown at the attainable accuracy */
         double alpha = rz / pq;
         double rz2 = 0.0;
         rn = 0.0;

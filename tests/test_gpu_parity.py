@@ -221,3 +221,24 @@ def test_mesh_destroyed_raises(lv, oracle, gpu):
     op = oracle.Particles(og, x)
     with pytest.raises(oracle.OracleError):
         op.remesh()
+
+
+@pytest.mark.parametrize("method", ["pcg64", "mixed"])
+def test_picard_window_same_pressure_fewer_iterations(lv, oracle, gpu, method):
+    """Extrapolated Picard initial guesses change only where each solve starts: every solve still
+    meets tol, P agrees with the plain warm start to the tolerance, and later solves need fewer
+    iterations."""
+    _flush("picard window %s" % method)
+    res = {}
+    for w in (0, 4):
+        grid, op, x, dr = make_pair(lv, oracle, "periodic", K=64, seed=0x5EED0050, ic="tg")
+        lv.stiffened_eos(grid, 1.4, 2500.0 / 1.4)
+        solver = lv.PressureSolver(grid, method=method, tol=1e-10, picard_window=w)
+        st = lv.find_pressure(solver, 2e-3, 10)
+        assert max(st["relres"]) <= 1e-10
+        res[w] = (grid.get("P"), st)
+    P0, st0 = res[0]
+    P4, st4 = res[4]
+    assert rel_err(P4, P0) <= 1e-6
+    assert sum(st4["iters"][2:]) < 0.8 * sum(st0["iters"][2:]), (st0["iters"], st4["iters"])
+    assert all(a <= b * 1.05 for a, b in zip(st4["relres0"][2:], st0["relres0"][2:]))

@@ -37,6 +37,8 @@ def parse():
     ap.add_argument("--picard-window", type=int, default=8,
                     help="Picard corrections used to extrapolate each solve's initial guess (0: previous P)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-compare", action="store_true",
+                    help="skip the FP64 / mixed / learned-preconditioner solve comparison")
     ap.add_argument("--cpu-K", type=int, default=1024, help="lattice side of the CPU sample (one full step)")
     ap.add_argument("--mode", default="steps", choices=["steps", "solve", "copy"],
                     help="steps: the benchmark; solve: one pressure solve (for counter runs); "
@@ -56,10 +58,49 @@ def mode_copy():
     print(json.dumps({"mode": "copy", "bytes_read_per_launch": 4 * n, "bytes_written_per_launch": 4 * n}))
 
 
+def spai_theta(lv):
+    """The committed trained parameters, else the untrained network (zero decoders)."""
+    th = lv.spai.load_params()
+    return (th, True) if th is not None else (lv.spai.init_params(0), False)
+
+
+def solve_compare(lv, grid, ic, dt, tol, repeats=3):
+    """The pressure-solve metric: the first Picard solve of a step (operator and RHS of the current
+    state, EOS pressure as initial guess, src/pressure.jl:172) to tol, with FP64 Jacobi-PCG, mixed
+    FP32/FP64 Jacobi-PCG and mixed PCG with the learned preconditioner; best of `repeats` solves,
+    each timed with HIP events on the library's stream.  The learned preconditioner's solve time
+    includes one network inference (it runs once per assembled operator; here it is forced for every
+    repeat by re-assembling)."""
+    th, trained = spai_theta(lv)
+    lv.spai.set_params(grid, th)
+    lv.stiffened_eos(grid, ic["gamma"], ic["P0"])
+    x0 = grid.get("P")
+    out = {}
+    for method in ("pcg64", "mixed", "spai"):
+        solver = lv.PressureSolver(grid, method=method, tol=tol, itmax=4000, inner_tol=1e-4)
+        best = None
+        for _ in range(repeats):
+            solver.assemble(dt)
+            b, _ = solver.rhs(dt, False)
+            solver.solve(b, x0)
+            st = solver.last_stats
+            rec = dict(ms=round(st["solve_ms"][0], 3), iters=st["iters"][0], sweeps=st["outer"][0],
+                       relres=st["relres"][0], fallbacks=st["fallbacks"])
+            if best is None or rec["ms"] < best["ms"]:
+                best = rec
+        out[method] = best
+    out["spai_trained"] = trained
+    out["speedup_spai_vs_pcg64"] = round(out["pcg64"]["ms"] / out["spai"]["ms"], 3)
+    out["speedup_mixed_vs_pcg64"] = round(out["pcg64"]["ms"] / out["mixed"]["ms"], 3)
+    return out
+
+
 def mode_solve(lv, K, method, tol):
     """One first-Picard pressure solve of the benchmark flow (EOS pressure as initial guess)."""
     dt = 2.5e-4
     grid, x, ic = setup(lv, K)
+    if method == "spai":
+        lv.spai.set_params(grid, spai_theta(lv)[0])
     lv.stiffened_eos(grid, ic["gamma"], ic["P0"])
     solver = lv.PressureSolver(grid, method=method, tol=tol, itmax=2000, inner_tol=1e-4)
     solver.assemble(dt)
@@ -145,6 +186,8 @@ def main():
     n = K * K
     dt = 2.5e-4
     grid, x, ic = setup(lv, K)
+    if args.solver == "spai":
+        lv.spai.set_params(grid, spai_theta(lv)[0])
     solver = lv.PressureSolver(grid, method=args.solver, tol=args.tol, itmax=2000, inner_tol=1e-4,
                                picard_window=args.picard_window)
     desc = lv.make_step_desc(dt, eos="stiffened", gamma=ic["gamma"], P0=ic["P0"], solver=solver,
@@ -192,6 +235,8 @@ def main():
                pressure_solve_ms=round(first_solve_ms, 3), pressure_solve_iters=first_solve_it,
                max_relres=relres, picard_iters=stats[-1]["iters"], picard_sweeps=stats[-1]["outer"],
                picard_relres0=[float("%.3g" % r) for r in stats[-1]["relres0"]], roofline=roofline)
+    if not args.no_compare:
+        out["pressure_solve_compare"] = solve_compare(lv, grid, ic, dt, args.tol)
     if not args.no_cpu:
         cK = args.cpu_K
         out["cpu_baseline"] = cpu_baseline(cK, 0x5EED0002, dt, args.tol)

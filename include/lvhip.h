@@ -179,6 +179,23 @@ int lv_pressure_rhs(lv_ctx *ctx, double dt, int32_t gp_step, const double vbc[8]
 /* solve the assembled operator for b with initial guess x (in/out) */
 int lv_solve_pressure(lv_ctx *ctx, const double *b, double *x, const lv_solve_opts *opts, lv_solve_stats *stats);
 
+/* ---- learned SPAI preconditioner --------------------------------------------------------------
+ * No counterpart in the reference (it runs unpreconditioned MINRES, src/pressure.jl:219).  A
+ * message-passing network over the Voronoi edges emits the coefficients of a symmetric, strictly
+ * diagonally dominant (hence SPD) approximate inverse on the pressure operator's own pattern; it is
+ * used by LV_SOLVER_SPAI, with Jacobi-PCG as fallback.  Parameters are a flat float32 vector. */
+int lv_spai_num_params(int64_t *count);
+int lv_spai_set_params(lv_ctx *ctx, const float *params);
+int lv_spai_get_params(lv_ctx *ctx, float *params);
+/* one Adam step on the assembled operator; probes: the current P (scaled) and nprobe-1 Rademacher
+ * vectors drawn from seed; loss is the value before the update */
+int lv_spai_train_step(lv_ctx *ctx, double lr, int32_t nprobe, uint64_t seed, double *loss);
+/* loss and gradient of every parameter at the current parameters (grad may be NULL) */
+int lv_spai_loss_grad(lv_ctx *ctx, int32_t nprobe, uint64_t seed, int32_t use_data, double *loss, float *grad);
+/* network outputs on the assembled operator: o_edge[nnz] (CSR order), o_node[n], and the scaled
+ * preconditioner's off-diagonal w[nnz] and diagonal mdiag[n] (any pointer may be NULL) */
+int lv_spai_outputs(lv_ctx *ctx, float *o_edge, float *o_node, float *w, float *mdiag);
+
 /* ---- timing ----------------------------------------------------------------------------------- */
 /* When enabled, the library brackets every launch of the named kernel with HIP events on the
  * context stream and accumulates the device time. kernel ids: 0 = FP32 SpMV, 1 = FP64 SpMV,

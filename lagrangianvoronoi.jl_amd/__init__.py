@@ -125,6 +125,12 @@ _SIGNATURES = {
     "lv_get_pressure_operator": [_P, _P, _P, _P, _P],
     "lv_pressure_rhs": [_P, _D, _I, _P, _P, _P],
     "lv_solve_pressure": [_P, _P, _P, ctypes.POINTER(SolveOpts), ctypes.POINTER(SolveStats)],
+    "lv_spai_num_params": [_P],
+    "lv_spai_set_params": [_P, _P],
+    "lv_spai_get_params": [_P, _P],
+    "lv_spai_train_step": [_P, _D, _I, ctypes.c_uint64, _P],
+    "lv_spai_loss_grad": [_P, _I, ctypes.c_uint64, _I, _P, _P],
+    "lv_spai_outputs": [_P, _P, _P, _P, _P],
     "lv_timing_enable": [_P, _I, _I],
     "lv_timing_read": [_P, _I, _P, _P, _P],
 }
@@ -485,3 +491,6 @@ def make_step_desc(dt, eos=None, gamma=1.4, P0=0.0, Pmin=0.0, gravity=None, nite
     d.alpha = float(alpha)
     d.rusanov = int(bool(rusanov))
     return d
+
+
+from . import spai  # noqa: E402,F401  (learned preconditioner helpers)

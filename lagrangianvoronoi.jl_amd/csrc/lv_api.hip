@@ -12,7 +12,7 @@
 #include <algorithm>
 #include <vector>
 
-#include "lv_solver.h"
+#include "lv_spai.h"
 
 struct lv_ctx {
     lv_grid_params g;
@@ -50,6 +50,11 @@ struct lv_ctx {
     double *pw_P[2] = {nullptr, nullptr}, *pw_AP[2] = {nullptr, nullptr};
     double *pw_t1 = nullptr, *pw_t2 = nullptr, *pw_dots = nullptr;
     int pw_cap = 0;
+    // learned preconditioner: network state and the preconditioner of the last assembled operator
+    lv_spai spai{};
+    int spai_ok = 0;          // lv_spai_init succeeded
+    int64_t assembly_id = 0;  // bumped by every assembly
+    int64_t spai_emitted_for = -1;
     char msg[512] = {0};
 };
 
@@ -322,6 +327,10 @@ int lv_create(const lv_grid_desc *d, lv_ctx **out)
             e = hipMemcpy(ctx->path_rr, ctx->h_path_rr.data(), sizeof(double) * g.npath, hipMemcpyHostToDevice);
     }
     if (e == hipSuccess) e = hipMemset(ctx->d_flags, 0, 2 * sizeof(int));
+    if (e == hipSuccess) {
+        e = lv_spai_init(&ctx->spai);
+        ctx->spai_ok = (e == hipSuccess);
+    }
     for (int k = 0; k < LV_NKTIMERS && e == hipSuccess; k++) {
         ctx->timers[k] = new lv_timer();
         memset(ctx->timers[k], 0, sizeof(lv_timer));
@@ -349,6 +358,7 @@ int lv_destroy(lv_ctx *ctx)
                     ctx->W.q, ctx->W.e32, ctx->W.r32, ctx->W.z32, ctx->W.p32, ctx->W.q32, ctx->W.S,
                     ctx->W.partials, ctx->scan_temp};
     for (void *p : ptrs) dfree(p);
+    if (ctx->spai_ok) lv_spai_free(&ctx->spai);
     for (int k = 0; k < 8; k++) {
         dfree(ctx->pw_q[k]);
         dfree(ctx->pw_z[k]);
@@ -677,6 +687,7 @@ int lv_assemble_pressure(lv_ctx *ctx, double dt, int64_t *nnz)
     rc = read_flags(ctx, flags);
     if (rc) return rc;
     ctx->A.singular = (flags[1] == 0);
+    ctx->assembly_id++;
     ctx->A.valid = 1;
     if (nnz) *nnz = total;
     return LV_OK;
@@ -720,18 +731,45 @@ static void default_opts(lv_solve_opts &o)
     o.picard_window = 0;
 }
 
+// the learned preconditioner of the current operator (network forward + emission once per assembly)
+static int spai_prepare(lv_ctx *ctx, const lv_solve_opts &opt, lv_precond *M, bool *have)
+{
+    *have = false;
+    if (opt.method != LV_SOLVER_SPAI || !ctx->spai_ok || !ctx->spai.has_params) return LV_OK;
+    if (ctx->spai_emitted_for != ctx->assembly_id) {
+        LV_HIP_CHECK(lv_spai_forward(&ctx->spai, ctx->A, ctx->n, ctx->s));
+        LV_HIP_CHECK(lv_spai_emit(&ctx->spai, ctx->A, ctx->n, ctx->s));
+        ctx->spai_emitted_for = ctx->assembly_id;
+    }
+    M->mval = ctx->spai.mval;
+    M->mdg = ctx->spai.mdg;
+    *have = true;
+    return LV_OK;
+}
+
 // one linear solve on the assembled operator, device vectors b and x (x in/out)
 static int solve_on_device(lv_ctx *ctx, const double *db, double *dx, const lv_solve_opts &opt, lv_solve_stats *st)
 {
+    lv_precond M{};
+    bool haveM = false;
     hipEvent_t e0, e1;
     LV_HIP_CHECK(hipEventCreate(&e0));
     LV_HIP_CHECK(hipEventCreate(&e1));
+    // the solve time includes the network inference when this solve is the first on a new operator
     LV_HIP_CHECK(hipEventRecord(e0, ctx->s));
-    int iters = 0, sweeps = 0;
+    int rc = spai_prepare(ctx, opt, &M, &haveM);
+    if (rc) {
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+        return rc;
+    }
+    int iters = 0, sweeps = 0, fb = 0;
     double relres = 0.0, relres0 = 0.0;
     bool conv = false;
     hipError_t e = lv_solve_device(ctx->A, ctx->W, ctx->n, db, dx, opt, ctx->s, timer_if_on(ctx, 0),
-                                   timer_if_on(ctx, 1), &iters, &sweeps, &relres, &relres0, &conv);
+                                   timer_if_on(ctx, 1), &iters, &sweeps, &relres, &relres0, &conv,
+                                   haveM ? &M : nullptr, &fb);
+    if (opt.method == LV_SOLVER_SPAI && !haveM) fb++;  // no trained network: Jacobi
     if (e != hipSuccess) {
         hipEventDestroy(e0);
         hipEventDestroy(e1);
@@ -754,6 +792,7 @@ static int solve_on_device(lv_ctx *ctx, const double *db, double *dx, const lv_s
     if (st) {
         st->nsolves++;
         st->total_iters += iters;
+        st->fallbacks += fb;
     }
     return conv ? LV_OK : LV_E_NOT_CONVERGED;
 }
@@ -1002,6 +1041,88 @@ int lv_step_fused(lv_ctx *ctx, const lv_step_desc *d, lv_solve_stats *stats)
     rc = lv_relaxation_step(ctx, d->dt, d->rusanov);
     if (rc) return rc;
     return prc;
+}
+
+// ---- learned SPAI preconditioner ---------------------------------------------------------------
+
+int lv_spai_num_params(int64_t *count)
+{
+    if (!count) return LV_E_INVALID;
+    *count = lv_spai_param_count();
+    return LV_OK;
+}
+
+int lv_spai_set_params(lv_ctx *ctx, const float *params)
+{
+    if (!ctx || !params || !ctx->spai_ok) return LV_E_INVALID;
+    LV_HIP_CHECK(lv_spai_set_params(&ctx->spai, params, ctx->s));
+    LV_HIP_CHECK(hipStreamSynchronize(ctx->s));
+    ctx->spai_emitted_for = -1;
+    return LV_OK;
+}
+
+int lv_spai_get_params(lv_ctx *ctx, float *params)
+{
+    if (!ctx || !params || !ctx->spai_ok) return LV_E_INVALID;
+    LV_HIP_CHECK(lv_spai_get_params(&ctx->spai, params, ctx->s));
+    return LV_OK;
+}
+
+static int spai_check(lv_ctx *ctx)
+{
+    int rc = check_ctx(ctx, true);
+    if (rc) return rc;
+    if (!ctx->spai_ok || !ctx->spai.has_params) return lv_fail(ctx, LV_E_INVALID, "SPAI parameters not set");
+    if (!ctx->A.valid) return lv_fail(ctx, LV_E_INVALID, "pressure operator not assembled");
+    return LV_OK;
+}
+
+int lv_spai_loss_grad(lv_ctx *ctx, int32_t nprobe, uint64_t seed, int32_t use_data, double *loss, float *grad)
+{
+    int rc = spai_check(ctx);
+    if (rc) return rc;
+    if (nprobe < 1 || nprobe > 64) return LV_E_INVALID;
+    LV_HIP_CHECK(lv_spai_forward(&ctx->spai, ctx->A, ctx->n, ctx->s));
+    ctx->spai_emitted_for = -1;
+    double L = 0.0;
+    LV_HIP_CHECK(lv_spai_loss_grad(&ctx->spai, ctx->A, ctx->n, use_data ? ctx->F.P : nullptr, nprobe, seed,
+                                   grad != nullptr, &L, ctx->s));
+    if (grad)
+        LV_HIP_CHECK(hipMemcpyAsync(grad, ctx->spai.grad, sizeof(float) * lv_spai_param_count(),
+                                    hipMemcpyDeviceToHost, ctx->s));
+    LV_HIP_CHECK(hipStreamSynchronize(ctx->s));
+    if (loss) *loss = L;
+    return LV_OK;
+}
+
+int lv_spai_train_step(lv_ctx *ctx, double lr, int32_t nprobe, uint64_t seed, double *loss)
+{
+    int rc = spai_check(ctx);
+    if (rc) return rc;
+    if (nprobe < 1 || nprobe > 64) return LV_E_INVALID;
+    LV_HIP_CHECK(lv_spai_forward(&ctx->spai, ctx->A, ctx->n, ctx->s));
+    double L = 0.0;
+    LV_HIP_CHECK(lv_spai_loss_grad(&ctx->spai, ctx->A, ctx->n, ctx->F.P, nprobe, seed, true, &L, ctx->s));
+    LV_HIP_CHECK(lv_spai_adam(&ctx->spai, lr, ctx->s));
+    LV_HIP_CHECK(hipStreamSynchronize(ctx->s));
+    ctx->spai_emitted_for = -1;
+    if (loss) *loss = L;
+    return LV_OK;
+}
+
+int lv_spai_outputs(lv_ctx *ctx, float *o_edge, float *o_node, float *w, float *mdiag)
+{
+    int rc = spai_check(ctx);
+    if (rc) return rc;
+    LV_HIP_CHECK(lv_spai_forward(&ctx->spai, ctx->A, ctx->n, ctx->s));
+    ctx->spai_emitted_for = -1;
+    const size_t E = (size_t)ctx->A.nnz, n = (size_t)ctx->n;
+    if (o_edge) LV_HIP_CHECK(hipMemcpyAsync(o_edge, ctx->spai.oe, sizeof(float) * E, hipMemcpyDeviceToHost, ctx->s));
+    if (o_node) LV_HIP_CHECK(hipMemcpyAsync(o_node, ctx->spai.on, sizeof(float) * n, hipMemcpyDeviceToHost, ctx->s));
+    if (w) LV_HIP_CHECK(hipMemcpyAsync(w, ctx->spai.w, sizeof(float) * E, hipMemcpyDeviceToHost, ctx->s));
+    if (mdiag) LV_HIP_CHECK(hipMemcpyAsync(mdiag, ctx->spai.mdiag, sizeof(float) * n, hipMemcpyDeviceToHost, ctx->s));
+    LV_HIP_CHECK(hipStreamSynchronize(ctx->s));
+    return LV_OK;
 }
 
 // ---- timing ------------------------------------------------------------------------------------

@@ -48,6 +48,11 @@ void lv_timer_begin(lv_timer *t, hipStream_t s);
 void lv_timer_end(lv_timer *t, hipStream_t s);
 void lv_timer_flush(lv_timer *t);
 
+// a preconditioner in the solver's ELL slots: z_i = mdg_i r_i + sum_k mval[k*n + i] r[ecol[k*n + i]]
+struct lv_precond {
+    const float *mval, *mdg;
+};
+
 size_t lv_partials_bytes(int n);
 hipError_t lv_launch_degree(lv_mesh_view M, int n, int *deg, hipStream_t s);
 hipError_t lv_launch_assemble(const lv_grid_params &g, lv_fields F, lv_mesh_view M, int n, double dt,
@@ -57,7 +62,8 @@ hipError_t lv_launch_rhs(const lv_grid_params &g, lv_fields F, lv_mesh_view M, i
 hipError_t lv_launch_write_P(lv_fields F, int n, const double *P, hipStream_t s);
 hipError_t lv_solve_device(const lv_pressure_system &A, lv_solver_work &W, int n, const double *b, double *x,
                            const lv_solve_opts &opt, hipStream_t s, lv_timer *t32, lv_timer *t64, int *iters,
-                           int *sweeps, double *relres, double *relres0, bool *converged);
+                           int *sweeps, double *relres, double *relres0, bool *converged, const lv_precond *M,
+                           int *fallbacks);
 
 // Picard extrapolation helpers (FP64 vectors of length n on stream s)
 hipError_t lv_dots(int n, const double *const *a, const double *const *b, int m, double *out_host, double *partials,

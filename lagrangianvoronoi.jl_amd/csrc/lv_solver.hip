@@ -396,7 +396,7 @@ __global__ __launch_bounds__(B) void k_cg_s(int n, const int *__restrict__ deg, 
     block_partials<1>(v, Ppq, gridDim.x);
 }
 
-template <typename T>
+template <typename T, bool JACOBI>
 __global__ __launch_bounds__(B) void k_cg_u(int n, T *__restrict__ x, T *__restrict__ r, T *__restrict__ z,
                                             const T *__restrict__ p, const T *__restrict__ q,
                                             const T *__restrict__ dinv, const double *__restrict__ Ppq,
@@ -411,17 +411,43 @@ __global__ __launch_bounds__(B) void k_cg_u(int n, T *__restrict__ x, T *__restr
     for (int i = blockIdx.x * B + threadIdx.x; i < n; i += gridDim.x * B) {
         x[i] += a * p[i];
         const T ri = r[i] - a * q[i];
-        const T zi = dinv[i] * ri;
         r[i] = ri;
-        z[i] = zi;
-        v[0] += (double)ri * (double)zi;
-        v[1] += (double)ri * (double)ri;
+        if (JACOBI) {
+            const T zi = dinv[i] * ri;
+            z[i] = zi;
+            v[0] += (double)ri * (double)zi;
+            v[1] += (double)ri * (double)ri;
+        }
     }
-    block_partials<2>(v, Prr, gridDim.x);
+    if (JACOBI) block_partials<2>(v, Prr, gridDim.x);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         S->pq = pq[0];
         S->alpha = alpha;
     }
+}
+
+// z = M r with the learned preconditioner in ELL slots (same columns as A's ELL copy); partials
+// r.z and r.r for the next S kernel
+__global__ __launch_bounds__(B) void k_apply_m32(int n, const int *__restrict__ deg, const int *__restrict__ ecol,
+                                                 const float *__restrict__ mval, const float *__restrict__ mdg,
+                                                 const float *__restrict__ r, float *__restrict__ z,
+                                                 double *__restrict__ Prr, const lv_cg_scalars *S)
+{
+    if (S->done) return;
+    double v[2] = {0.0, 0.0};
+    for (int i = blockIdx.x * B + threadIdx.x; i < n; i += gridDim.x * B) {
+        const float ri = r[i];
+        float acc = mdg[i] * ri;
+        const int d = deg[i];
+        for (int k = 0; k < d; k++) {
+            const size_t e = (size_t)k * n + i;
+            acc += mval[e] * r[ecol[e]];
+        }
+        z[i] = acc;
+        v[0] += (double)ri * (double)acc;
+        v[1] += (double)ri * (double)ri;
+    }
+    block_partials<2>(v, Prr, gridDim.x);
 }
 
 // true residual r = (b - bmean) - A x; partials |r|^2, |b'|^2 (FP64)
@@ -614,12 +640,12 @@ struct parts {
 };
 inline parts parts_of(double *base) { return parts{base, base + 3 * NBMAX, base + 4 * NBMAX}; }
 
-// CG iterations (two launches each) until the device flags completion; the host checks the flag
-// once per batch of iterations.
+// CG iterations (two launches each, three with the learned preconditioner) until the device flags
+// completion; the host checks the flag once per batch of iterations.
 template <typename T>
 hipError_t cg_iterate(int n, const int *deg, const int *ecol, const T *ev, const T *diag, const T *dinv, T *x, T *r,
                       T *z, T *p, T *q, const parts &P, lv_cg_scalars *dS, hipStream_t s, lv_timer *tm, int batch,
-                      lv_cg_scalars &h)
+                      lv_cg_scalars &h, const lv_precond *M)
 {
     const int nb = nblocks_of(n);
     int k = 0;
@@ -631,8 +657,15 @@ hipError_t cg_iterate(int n, const int *deg, const int *ecol, const T *ev, const
                 hipLaunchKernelGGL(k_cg_s<T>, dim3(nb), dim3(B), 0, s, n, deg, ecol, ev, diag, (const T *)z, p, q,
                                    (const double *)P.rr, P.pq, dS, par, k, k == 0 ? 1 : 0);
             }
-            hipLaunchKernelGGL(k_cg_u<T>, dim3(nb), dim3(B), 0, s, n, x, r, z, (const T *)p, (const T *)q, dinv,
-                               (const double *)P.pq, P.rr, dS, par);
+            if (M) {
+                hipLaunchKernelGGL((k_cg_u<T, false>), dim3(nb), dim3(B), 0, s, n, x, r, z, (const T *)p,
+                                   (const T *)q, dinv, (const double *)P.pq, P.rr, dS, par);
+                hipLaunchKernelGGL(k_apply_m32, dim3(nb), dim3(B), 0, s, n, deg, ecol, M->mval, M->mdg,
+                                   (const float *)r, (float *)z, P.rr, (const lv_cg_scalars *)dS);
+            } else {
+                hipLaunchKernelGGL((k_cg_u<T, true>), dim3(nb), dim3(B), 0, s, n, x, r, z, (const T *)p,
+                                   (const T *)q, dinv, (const double *)P.pq, P.rr, dS, par);
+            }
         }
         hipError_t e = read_scalars(dS, &h, s);
         if (e != hipSuccess) return e;
@@ -647,9 +680,11 @@ hipError_t cg_iterate(int n, const int *deg, const int *ecol, const T *ev, const
 // final FP64 relative residual.  *converged tells whether tol was met.
 hipError_t lv_solve_device(const lv_pressure_system &A, lv_solver_work &W, int n, const double *b, double *x,
                            const lv_solve_opts &opt, hipStream_t s, lv_timer *t32, lv_timer *t64, int *iters,
-                           int *sweeps, double *relres, double *relres0, bool *converged)
+                           int *sweeps, double *relres, double *relres0, bool *converged, const lv_precond *M,
+                           int *fallbacks)
 {
     *relres0 = -1.0;
+    *fallbacks = 0;
     const int nb = nblocks_of(n);
     const dim3 G(nb);
     const parts P = parts_of(W.partials);
@@ -673,7 +708,7 @@ hipError_t lv_solve_device(const lv_pressure_system &A, lv_solver_work &W, int n
         hipLaunchKernelGGL(k_finalize, dim3(1), dim3(B), 0, s, (const double *)P.p3, nb, (int)FIN_INIT64, n, 0.0,
                            W.S);
         e = cg_iterate<double>(n, A.deg, A.ecol, A.ev64, A.diag64, A.dinv64, W.x, W.r, W.z, W.p, W.q, P, W.S, s,
-                               t64, batch, h);
+                               t64, batch, h, (const lv_precond *)nullptr);
         if (e != hipSuccess) return e;
         total = h.iters;
         *relres0 = (h.bb > 0.0) ? sqrt(h.rr0 / h.bb) : sqrt(h.rr0);
@@ -695,11 +730,27 @@ hipError_t lv_solve_device(const lv_pressure_system &A, lv_solver_work &W, int n
             double need = 0.3 * sqrt(h.thr2 / h.rr64);
             double itol = need > floor_tol ? need : floor_tol;
             if (itol > 0.5) itol = 0.5;
-            hipLaunchKernelGGL(k_cg32_init, G, dim3(B), 0, s, n, (const double *)W.r, (const float *)A.dinv32, W.e32,
-                               W.r32, W.z32, P.rr, (const lv_cg_scalars *)W.S);
-            hipLaunchKernelGGL(k_finalize2, dim3(1), dim3(B), 0, s, (const double *)P.rr, nb, itol * itol, W.S);
-            e = cg_iterate<float>(n, A.deg, A.ecol, A.ev32, A.diag32, A.dinv32, W.e32, W.r32, W.z32, W.p32, W.q32,
-                                  P, W.S, s, t32, batch, h);
+            const lv_precond *Mi = (opt.method == LV_SOLVER_SPAI) ? M : nullptr;
+            for (int attempt = 0; attempt < 2; attempt++) {
+                hipLaunchKernelGGL(k_cg32_init, G, dim3(B), 0, s, n, (const double *)W.r, (const float *)A.dinv32,
+                                   W.e32, W.r32, W.z32, P.rr, (const lv_cg_scalars *)W.S);
+                // k_finalize2 clears the done flag of the previous sweep, so it precedes the first
+                // application of the learned preconditioner (which is skipped while done is set)
+                hipLaunchKernelGGL(k_finalize2, dim3(1), dim3(B), 0, s, (const double *)P.rr, nb, itol * itol, W.S);
+                if (Mi)
+                    hipLaunchKernelGGL(k_apply_m32, G, dim3(B), 0, s, n, (const int *)A.deg, (const int *)A.ecol,
+                                       Mi->mval, Mi->mdg, (const float *)W.r32, W.z32, P.rr,
+                                       (const lv_cg_scalars *)W.S);
+                e = cg_iterate<float>(n, A.deg, A.ecol, A.ev32, A.diag32, A.dinv32, W.e32, W.r32, W.z32, W.p32, W.q32,
+                                      P, W.S, s, t32, batch, h, Mi);
+                if (e != hipSuccess) return e;
+                // Jacobi-PCG is the fallback: a sweep of the learned preconditioner that stalls or breaks
+                // down is redone with Jacobi
+                if (!Mi || h.done == 1) break;
+                total += h.iters;
+                (*fallbacks)++;
+                Mi = nullptr;
+            }
             if (e != hipSuccess) return e;
             total += h.iters;
             nsweep++;

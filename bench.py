@@ -4,7 +4,8 @@ Workload (the metric's 4M-particle configuration): Taylor-Green vortex on the pe
 2048 x 2048 particles on a jittered lattice (+-0.1 dr, SplitMix64 seed 0x5EED0002), u = sin 2pi x
 cos 2pi y, v = -cos 2pi x sin 2pi y, nu = 1e-3, stiffened gas c0 = 50, dt = 2.5e-4.  One step is the
 reference test's step! (tests/taylorgreen.jl:61-71): move! + remesh, stiffened_eos!, find_pressure!
-(10 Picard solves, mixed-precision PCG to 1e-8 FP64 relative residual), pressure_step!, find_D!,
+(10 Picard solves, mixed-precision PCG with the learned preconditioner to 1e-8 FP64 relative
+residual), pressure_step!, find_D!,
 viscous_step!, find_dv!, relaxation_step! + remesh.  Inputs are resident on the device before the
 timed region.
 
@@ -32,8 +33,12 @@ def parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--K", type=int, default=2048, help="lattice side (K*K particles)")
-    ap.add_argument("--solver", default="mixed", choices=["pcg64", "mixed", "spai"])
+    ap.add_argument("--solver", default="spai", choices=["pcg64", "mixed", "spai"],
+                    help="pcg64: FP64 Jacobi-PCG; mixed: FP32 Jacobi-PCG in FP64 refinement; spai: the same with the "
+                         "learned preconditioner (committed trained parameters)")
     ap.add_argument("--tol", type=float, default=1e-8)
+    ap.add_argument("--inner-tol", type=float, default=1e-4,
+                    help="floor of the FP32 sweep's relative tolerance inside the FP64 refinement")
     ap.add_argument("--picard-window", type=int, default=8,
                     help="Picard corrections used to extrapolate each solve's initial guess (0: previous P)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -52,10 +57,16 @@ def mode_copy():
     n = 128 * 1024 * 1024
     a = torch.ones(n, dtype=torch.float32, device="cuda")
     b = torch.empty_like(a)
+    b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
     for _ in range(5):
         b.copy_(a)
+    e1.record()
     torch.cuda.synchronize()
-    print(json.dumps({"mode": "copy", "bytes_read_per_launch": 4 * n, "bytes_written_per_launch": 4 * n}))
+    ms = e0.elapsed_time(e1) / 5
+    print(json.dumps({"mode": "copy", "bytes_read_per_launch": 4 * n, "bytes_written_per_launch": 4 * n,
+                      "ms_per_launch": round(ms, 4), "GBps_read_plus_write": round(8 * n / (ms * 1e-3) / 1e9, 1)}))
 
 
 def spai_theta(lv):
@@ -64,7 +75,7 @@ def spai_theta(lv):
     return (th, True) if th is not None else (lv.spai.init_params(0), False)
 
 
-def solve_compare(lv, grid, ic, dt, tol, repeats=3):
+def solve_compare(lv, grid, ic, dt, tol, inner_tol, repeats=3):
     """The pressure-solve metric: the first Picard solve of a step (operator and RHS of the current
     state, EOS pressure as initial guess, src/pressure.jl:172) to tol, with FP64 Jacobi-PCG, mixed
     FP32/FP64 Jacobi-PCG and mixed PCG with the learned preconditioner; best of `repeats` solves,
@@ -77,7 +88,7 @@ def solve_compare(lv, grid, ic, dt, tol, repeats=3):
     x0 = grid.get("P")
     out = {}
     for method in ("pcg64", "mixed", "spai"):
-        solver = lv.PressureSolver(grid, method=method, tol=tol, itmax=4000, inner_tol=1e-4)
+        solver = lv.PressureSolver(grid, method=method, tol=tol, itmax=4000, inner_tol=inner_tol)
         best = None
         for _ in range(repeats):
             solver.assemble(dt)
@@ -95,14 +106,14 @@ def solve_compare(lv, grid, ic, dt, tol, repeats=3):
     return out
 
 
-def mode_solve(lv, K, method, tol):
+def mode_solve(lv, K, method, tol, inner_tol):
     """One first-Picard pressure solve of the benchmark flow (EOS pressure as initial guess)."""
     dt = 2.5e-4
     grid, x, ic = setup(lv, K)
     if method == "spai":
         lv.spai.set_params(grid, spai_theta(lv)[0])
     lv.stiffened_eos(grid, ic["gamma"], ic["P0"])
-    solver = lv.PressureSolver(grid, method=method, tol=tol, itmax=2000, inner_tol=1e-4)
+    solver = lv.PressureSolver(grid, method=method, tol=tol, itmax=2000, inner_tol=inner_tol)
     solver.assemble(dt)
     b, _ = solver.rhs(dt, False)
     P = solver.solve(b, grid.get("P"))
@@ -181,14 +192,14 @@ def main():
 
     lv = load_package()
     if args.mode == "solve":
-        return mode_solve(lv, args.K, args.solver, args.tol)
+        return mode_solve(lv, args.K, args.solver, args.tol, args.inner_tol)
     K = args.K
     n = K * K
     dt = 2.5e-4
     grid, x, ic = setup(lv, K)
     if args.solver == "spai":
         lv.spai.set_params(grid, spai_theta(lv)[0])
-    solver = lv.PressureSolver(grid, method=args.solver, tol=args.tol, itmax=2000, inner_tol=1e-4,
+    solver = lv.PressureSolver(grid, method=args.solver, tol=args.tol, itmax=2000, inner_tol=args.inner_tol,
                                picard_window=args.picard_window)
     desc = lv.make_step_desc(dt, eos="stiffened", gamma=ic["gamma"], P0=ic["P0"], solver=solver,
                              viscous=True, artificial_viscosity=False, alpha=1.0, rusanov=True)
@@ -236,7 +247,7 @@ def main():
                max_relres=relres, picard_iters=stats[-1]["iters"], picard_sweeps=stats[-1]["outer"],
                picard_relres0=[float("%.3g" % r) for r in stats[-1]["relres0"]], roofline=roofline)
     if not args.no_compare:
-        out["pressure_solve_compare"] = solve_compare(lv, grid, ic, dt, args.tol)
+        out["pressure_solve_compare"] = solve_compare(lv, grid, ic, dt, args.tol, args.inner_tol)
     if not args.no_cpu:
         cK = args.cpu_K
         out["cpu_baseline"] = cpu_baseline(cK, 0x5EED0002, dt, args.tol)

@@ -737,7 +737,7 @@ static int spai_prepare(lv_ctx *ctx, const lv_solve_opts &opt, lv_precond *M, bo
     *have = false;
     if (opt.method != LV_SOLVER_SPAI || !ctx->spai_ok || !ctx->spai.has_params) return LV_OK;
     if (ctx->spai_emitted_for != ctx->assembly_id) {
-        LV_HIP_CHECK(lv_spai_forward(&ctx->spai, ctx->A, ctx->n, ctx->s));
+        LV_HIP_CHECK(lv_spai_infer(&ctx->spai, ctx->A, ctx->n, ctx->s));
         LV_HIP_CHECK(lv_spai_emit(&ctx->spai, ctx->A, ctx->n, ctx->s));
         ctx->spai_emitted_for = ctx->assembly_id;
     }
@@ -1114,7 +1114,7 @@ int lv_spai_outputs(lv_ctx *ctx, float *o_edge, float *o_node, float *w, float *
 {
     int rc = spai_check(ctx);
     if (rc) return rc;
-    LV_HIP_CHECK(lv_spai_forward(&ctx->spai, ctx->A, ctx->n, ctx->s));
+    LV_HIP_CHECK(lv_spai_infer(&ctx->spai, ctx->A, ctx->n, ctx->s));
     ctx->spai_emitted_for = -1;
     const size_t E = (size_t)ctx->A.nnz, n = (size_t)ctx->n;
     if (o_edge) LV_HIP_CHECK(hipMemcpyAsync(o_edge, ctx->spai.oe, sizeof(float) * E, hipMemcpyDeviceToHost, ctx->s));

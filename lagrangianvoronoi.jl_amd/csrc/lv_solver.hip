@@ -2,15 +2,18 @@
 // PressureOperator (refresh!(A), :104-117), the right-hand side (refresh!(solver), :162-203), and the
 // Krylov solvers that replace Krylov.jl's minres! (:219):
 //   * FP64 Jacobi-PCG;
-//   * FP32 Jacobi-PCG inside FP64 iterative refinement (FP64 residual, FP64 dots, FP32 SpMV/AXPY).
+//   * FP32 PCG inside FP64 iterative refinement (FP64 residual, FP64 dots, FP32 SpMV/AXPY), with
+//     Jacobi or the learned preconditioner of lv_spai.hip (Jacobi-PCG redoes any sweep the learned
+//     preconditioner fails to converge).
 //
 // The operator is kept in the reference's own form y_i = d_i x_i + sum_j c_ij (x_i - x_j): the
 // difference form keeps the nearly-constant pressure modes accurate in FP32, which the folded
-// diagonal form would lose to cancellation.  One CG iteration is three launches: SpMV fused with the
-// p.q dot; the x/r/z update fused with the r.z and r.r dots; the p update.  Dots reduce per block
-// into a partials array and the last block to finish (atomic ticket) sums the partials in a fixed
-// order, so results are deterministic, and then computes alpha/beta/convergence on the device: the
-// host reads one flag per batch of iterations instead of one per dot.
+// diagonal form would lose to cancellation.  One Jacobi-PCG iteration is two launches: the SpMV of z
+// fused with the p/q recurrences and the p.q dot (A p_new = A z + beta A p_old), and the x/r/z update
+// fused with the r.z and r.r dots; the learned preconditioner adds one ELL pass z = M r.  Dots reduce
+// per block into a partials array that every block of the consuming kernel re-reduces in the same
+// fixed order (no atomics, no fences), so results are deterministic; alpha, beta and convergence are
+// evaluated on the device and the host reads one flag per batch of iterations.
 #include <string.h>
 
 #include "lv_solver.h"
@@ -426,8 +429,38 @@ __global__ __launch_bounds__(B) void k_cg_u(int n, T *__restrict__ x, T *__restr
     }
 }
 
-// z = M r with the learned preconditioner in ELL slots (same columns as A's ELL copy); partials
-// r.z and r.r for the next S kernel
+// (M r)_i with the learned preconditioner in ELL slots (same columns as A's ELL copy): the first 8
+// slots unrolled and predicated so their loads are in flight before the gathers of r
+__device__ __forceinline__ float apply_m_row(int i, int n, const int *__restrict__ deg, const int *__restrict__ ecol,
+                                             const float *__restrict__ mval, const float *__restrict__ mdg,
+                                             const float *__restrict__ r, float ri)
+{
+    constexpr int U = 8;
+    float acc = mdg[i] * ri;
+    const int d = deg[i];
+    int c[U];
+    float v[U];
+#pragma unroll
+    for (int k = 0; k < U; k++) {
+        c[k] = i;
+        v[k] = 0.0f;
+        if (k < d) {
+            const size_t e = (size_t)k * n + i;
+            c[k] = ecol[e];
+            v[k] = mval[e];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < U; k++)
+        if (k < d) acc += v[k] * r[c[k]];
+    for (int k = U; k < d; k++) {
+        const size_t e = (size_t)k * n + i;
+        acc += mval[e] * r[ecol[e]];
+    }
+    return acc;
+}
+
+// z = M r; partials r.z and r.r for the next S kernel (two rows per thread per pass)
 __global__ __launch_bounds__(B) void k_apply_m32(int n, const int *__restrict__ deg, const int *__restrict__ ecol,
                                                  const float *__restrict__ mval, const float *__restrict__ mdg,
                                                  const float *__restrict__ r, float *__restrict__ z,
@@ -435,17 +468,21 @@ __global__ __launch_bounds__(B) void k_apply_m32(int n, const int *__restrict__ 
 {
     if (S->done) return;
     double v[2] = {0.0, 0.0};
-    for (int i = blockIdx.x * B + threadIdx.x; i < n; i += gridDim.x * B) {
-        const float ri = r[i];
-        float acc = mdg[i] * ri;
-        const int d = deg[i];
-        for (int k = 0; k < d; k++) {
-            const size_t e = (size_t)k * n + i;
-            acc += mval[e] * r[ecol[e]];
+    const int stride = gridDim.x * B;
+    for (int i0 = blockIdx.x * B + threadIdx.x; i0 < n; i0 += 2 * stride) {
+        const int i1 = i0 + stride;
+        const float r0 = r[i0];
+        const float z0 = apply_m_row(i0, n, deg, ecol, mval, mdg, r, r0);
+        z[i0] = z0;
+        v[0] += (double)r0 * (double)z0;
+        v[1] += (double)r0 * (double)r0;
+        if (i1 < n) {
+            const float r1 = r[i1];
+            const float z1 = apply_m_row(i1, n, deg, ecol, mval, mdg, r, r1);
+            z[i1] = z1;
+            v[0] += (double)r1 * (double)z1;
+            v[1] += (double)r1 * (double)r1;
         }
-        z[i] = acc;
-        v[0] += (double)ri * (double)acc;
-        v[1] += (double)ri * (double)ri;
     }
     block_partials<2>(v, Prr, gridDim.x);
 }

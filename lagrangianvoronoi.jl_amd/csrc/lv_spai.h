@@ -51,6 +51,7 @@ void lv_spai_free(lv_spai *S);
 hipError_t lv_spai_set_params(lv_spai *S, const float *host, hipStream_t s);
 hipError_t lv_spai_get_params(lv_spai *S, float *host, hipStream_t s);
 hipError_t lv_spai_forward(lv_spai *S, const lv_pressure_system &A, int n, hipStream_t s);
+hipError_t lv_spai_infer(lv_spai *S, const lv_pressure_system &A, int n, hipStream_t s);
 hipError_t lv_spai_emit(lv_spai *S, const lv_pressure_system &A, int n, hipStream_t s);
 hipError_t lv_spai_loss_grad(lv_spai *S, const lv_pressure_system &A, int n, const double *x, int P, uint64_t seed,
                              bool want_grad, double *loss_host, hipStream_t s);

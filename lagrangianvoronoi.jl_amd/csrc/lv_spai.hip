@@ -41,13 +41,11 @@ __device__ __forceinline__ float clampo(float o) { return fminf(fmaxf(o, -OCLAMP
 
 // ---- graph and features ------------------------------------------------------------------------
 
-// src of every directed edge, the reverse edge, features, scaled couplings
-__global__ __launch_bounds__(B) void k_graph(int n, const int *__restrict__ rp, const int *__restrict__ cols,
-                                             const double *__restrict__ val, const double *__restrict__ diag,
-                                             const double *__restrict__ dinv, int *__restrict__ src,
-                                             int *__restrict__ rev, float *__restrict__ a, float *__restrict__ xf,
-                                             float *__restrict__ ff, float *__restrict__ r_out,
-                                             float *__restrict__ sqD)
+// node features, r_i = d_i / D_i (floored), sqrt(D_i), and src of every directed edge of row i
+__global__ __launch_bounds__(B) void k_graph_nodes(int n, const int *__restrict__ rp, const double *__restrict__ diag,
+                                                   const double *__restrict__ dinv, int *__restrict__ src,
+                                                   float *__restrict__ xf, float *__restrict__ r_out,
+                                                   float *__restrict__ sqD)
 {
     const int i = blockIdx.x * B + threadIdx.x;
     if (i >= n) return;
@@ -55,50 +53,70 @@ __global__ __launch_bounds__(B) void k_graph(int n, const int *__restrict__ rp, 
     const int p0 = rp[i], p1 = rp[i + 1];
     const float deg = (float)(p1 - p0);
     const double r = diag[i] / Di;
-    const float rf = fmaxf((float)r, R_FLOOR);
-    r_out[i] = rf;
+    r_out[i] = fmaxf((float)r, R_FLOOR);
     sqD[i] = (float)sqrt(Di);
-    xf[4 * i + 0] = (float)r;
-    xf[4 * i + 1] = log10f(fmaxf((float)r, 1e-12f)) / 12.0f + 1.0f;
-    xf[4 * i + 2] = deg / 8.0f;
-    xf[4 * i + 3] = 1.0f;
-    for (int p = p0; p < p1; p++) {
-        const int j = cols[p];
-        const double Dj = 1.0 / dinv[j];
-        const float ap = (float)(val[p] / sqrt(Di * Dj));
-        src[p] = i;
-        a[p] = ap;
-        ff[4 * p + 0] = ap;
-        ff[4 * p + 1] = ap * deg / 6.0f;
-        ff[4 * p + 2] = (float)(val[p] / Di);
-        ff[4 * p + 3] = 1.0f;
-        int q = -1;
-        for (int s = rp[j]; s < rp[j + 1]; s++)
-            if (cols[s] == i) {
-                q = s;
-                break;
-            }
-        rev[p] = q;
-    }
+    *(float4 *)(xf + 4 * i) = make_float4((float)r, log10f(fmaxf((float)r, 1e-12f)) / 12.0f + 1.0f, deg / 8.0f, 1.0f);
+    for (int p = p0; p < p1; p++) src[p] = i;
+}
+
+// per directed edge p = (i, j): scaled coupling a_p = c_p / sqrt(D_i D_j), edge features, and the
+// reverse edge (j, i) (-1 if row j does not list i)
+__global__ __launch_bounds__(B) void k_graph_edges(int E, const int *__restrict__ rp, const int *__restrict__ cols,
+                                                   const double *__restrict__ val, const double *__restrict__ dinv,
+                                                   const int *__restrict__ src, int *__restrict__ rev,
+                                                   float *__restrict__ a, float *__restrict__ ff)
+{
+    const int p = blockIdx.x * B + threadIdx.x;
+    if (p >= E) return;
+    const int i = src[p], j = cols[p];
+    const double Di = 1.0 / dinv[i], Dj = 1.0 / dinv[j];
+    const float deg = (float)(rp[i + 1] - rp[i]);
+    const float ap = (float)(val[p] / sqrt(Di * Dj));
+    a[p] = ap;
+    *(float4 *)(ff + 4 * (size_t)p) = make_float4(ap, ap * deg / 6.0f, (float)(val[p] / Di), 1.0f);
+    int q = -1;
+    for (int s = rp[j]; s < rp[j + 1]; s++)
+        if (cols[s] == i) {
+            q = s;
+            break;
+        }
+    rev[p] = q;
 }
 
 // ---- encoders (4 -> 64, FP32) ------------------------------------------------------------------
 
-// out[t][c] = ReLU(sum_k in[t][k] W[k][c] + b[c]) for t < count; FP32 and BF16 copies
+// out[t][c] = ReLU(sum_k in[t][k] W[k][c] + b[c]) for t < count; FP32 (optional) and BF16 copies.
+// One thread per (row, 8 channels): 16-byte loads and stores.
 __global__ __launch_bounds__(B) void k_encode(int count, const float *__restrict__ in, const float *__restrict__ W,
                                               const float *__restrict__ b, float *__restrict__ outf,
                                               __bf16 *__restrict__ outb)
 {
+    __shared__ __attribute__((aligned(16))) float sw[5 * H];  // W[4][64], then b[64]
+    for (int k = threadIdx.x; k < 5 * H; k += B) sw[k] = (k < 4 * H) ? W[k] : b[k - 4 * H];
+    __syncthreads();
     const int64_t t = (int64_t)blockIdx.x * B + threadIdx.x;
-    if (t >= (int64_t)count * H) return;
-    const int64_t r = t / H;
-    const int c = (int)(t % H);
-    float z = b[c];
+    if (t >= (int64_t)count * (H / 8)) return;
+    const int64_t r = t / (H / 8);
+    const int c0 = (int)(t % (H / 8)) * 8;
+    const float4 x = *(const float4 *)(in + 4 * r);
+    float y[8];
+    v8bf o;
 #pragma unroll
-    for (int k = 0; k < 4; k++) z += in[4 * r + k] * W[k * H + c];
-    const float y = fmaxf(z, 0.0f);
-    if (outf) outf[t] = y;
-    outb[t] = f2bf(y);
+    for (int j = 0; j < 8; j++) {
+        const int c = c0 + j;
+        float z = sw[4 * H + c];
+        z += x.x * sw[c];
+        z += x.y * sw[H + c];
+        z += x.z * sw[2 * H + c];
+        z += x.w * sw[3 * H + c];
+        y[j] = fmaxf(z, 0.0f);
+        o[j] = f2bf(y[j]);
+    }
+    if (outf) {
+        *(float4 *)(outf + r * H + c0) = make_float4(y[0], y[1], y[2], y[3]);
+        *(float4 *)(outf + r * H + c0 + 4) = make_float4(y[4], y[5], y[6], y[7]);
+    }
+    *(v8bf *)(outb + r * H + c0) = o;
 }
 
 // ---- message-passing layer, forward (BF16 MFMA) --------------------------------------------------
@@ -150,20 +168,230 @@ __global__ __launch_bounds__(256) void k_mp_forward(int E, const int *__restrict
     }
 }
 
-// h_new[i] = h[i] + mean_{p in row i} m[p]  (FP32 master, BF16 copy for the next layer)
+// h_new[i] = h[i] + mean_{p in row i} m[p]  (FP32 master, BF16 copy for the next layer).
+// One thread per (node, 8 channels): 16-byte loads of the message rows.
 __global__ __launch_bounds__(B) void k_aggregate(int n, const int *__restrict__ rp, const float *__restrict__ Hf,
                                                  const __bf16 *__restrict__ Mb, float *__restrict__ Hf_out,
                                                  __bf16 *__restrict__ Hb_out)
 {
     const int64_t t = (int64_t)blockIdx.x * B + threadIdx.x;
-    if (t >= (int64_t)n * H) return;
-    const int i = (int)(t / H), c = (int)(t % H);
+    if (t >= (int64_t)n * (H / 8)) return;
+    const int i = (int)(t / (H / 8));
+    const int c0 = (int)(t % (H / 8)) * 8;
     const int p0 = rp[i], p1 = rp[i + 1];
-    float s = 0.0f;
-    for (int p = p0; p < p1; p++) s += bf2f(Mb[(size_t)p * H + c]);
-    const float y = Hf[t] + ((p1 > p0) ? s / (float)(p1 - p0) : 0.0f);
-    Hf_out[t] = y;
-    Hb_out[t] = f2bf(y);
+    float sum[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    for (int p = p0; p < p1; p++) {
+        const v8bf m = *(const v8bf *)(Mb + (size_t)p * H + c0);
+#pragma unroll
+        for (int j = 0; j < 8; j++) sum[j] += bf2f(m[j]);
+    }
+    const size_t o = (size_t)i * H + c0;
+    const float4 h0 = *(const float4 *)(Hf + o), h1 = *(const float4 *)(Hf + o + 4);
+    const float hv[8] = {h0.x, h0.y, h0.z, h0.w, h1.x, h1.y, h1.z, h1.w};
+    float y[8];
+    v8bf yb;
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        y[j] = hv[j] + ((p1 > p0) ? sum[j] / (float)(p1 - p0) : 0.0f);
+        yb[j] = f2bf(y[j]);
+    }
+    *(float4 *)(Hf_out + o) = make_float4(y[0], y[1], y[2], y[3]);
+    *(float4 *)(Hf_out + o + 4) = make_float4(y[4], y[5], y[6], y[7]);
+    *(v8bf *)(Hb_out + o) = yb;
+}
+
+// ---- message-passing layer, inference (BF16 MFMA, persistent waves) ----------------------------
+
+// The transposed product m^T = W^T X^T on v_mfma_f32_32x32x16_bf16: the A operand is a W^T fragment
+// (lane l: channel l%32 (+32), k = 16s + 8(l/32) .. +7) and the B operand the gathered feature row of
+// edge l%32 at the same k, so lane l ends up holding edge l%32 and channels 8g + 4(l/32) + j (g, j < 4)
+// of each 32-channel half.  W^T (padded rows) lives in LDS for the whole kernel; every wave loops
+// over 32-edge tiles.  Non-final layers stage the BF16 tile through LDS and write it with 16-byte
+// stores; the final layer writes only what the decoders need: o_p = m_p . ve + ce and
+// s_p = m_p . un (the node decoder adds the mean of s_p over the row to h . un + cn).
+constexpr int WPAD = KX + 8;  // LDS row of W^T (bf16)
+constexpr int TPAD = H + 8;   // LDS row of a staged edge (bf16)
+
+typedef __attribute__((__vector_size__(4 * sizeof(__bf16)))) __bf16 v4bf;
+
+template <bool FIRST, bool LAST>
+__global__ __launch_bounds__(256) void k_mp_infer(int E, const int *__restrict__ src, const int *__restrict__ dst,
+                                                  const __bf16 *__restrict__ Hb, const __bf16 *__restrict__ Ein,
+                                                  const float *__restrict__ ff, const float *__restrict__ We,
+                                                  const float *__restrict__ be,
+                                                  const __bf16 *__restrict__ Wt, const float *__restrict__ bias,
+                                                  __bf16 *__restrict__ Mout, const float *__restrict__ ve,
+                                                  const float *__restrict__ ce, const float *__restrict__ un,
+                                                  float *__restrict__ oe, float *__restrict__ sn)
+{
+    __shared__ __attribute__((aligned(16))) __bf16 sW[H * WPAD];
+    __shared__ __attribute__((aligned(16))) __bf16 sT[4 * 32 * TPAD];
+    __shared__ __attribute__((aligned(16))) float sB[H];
+    __shared__ __attribute__((aligned(16))) float sV[H];
+    __shared__ __attribute__((aligned(16))) float sU[H];
+    __shared__ __attribute__((aligned(16))) float sE[5 * H];  // edge encoder We[4][64], be[64] (FIRST)
+    if (FIRST)
+        for (int k = threadIdx.x; k < 5 * H; k += 256) sE[k] = (k < 4 * H) ? We[k] : be[k - 4 * H];
+    for (int t = threadIdx.x; t < H * (KX / 8); t += 256) {
+        const int c = t / (KX / 8), k = (t % (KX / 8)) * 8;
+        *(v8bf *)(sW + c * WPAD + k) = *(const v8bf *)(Wt + c * KX + k);
+    }
+    if (threadIdx.x < H) {
+        sB[threadIdx.x] = bias[threadIdx.x];
+        if (LAST) {
+            sV[threadIdx.x] = ve[threadIdx.x];
+            sU[threadIdx.x] = un[threadIdx.x];
+        }
+    }
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int r = lane & 31, half = lane >> 5;
+    __bf16 *T = sT + wave * 32 * TPAD;
+    const int64_t ntiles = ((int64_t)E + 31) / 32;
+    const int64_t tstride = (int64_t)gridDim.x * 4;
+    // software pipeline: the next tile's node indices are loaded before the MFMAs of the current
+    // tile and its feature fragments right after them, so both latencies overlap computation
+    int64_t tile = (int64_t)blockIdx.x * 4 + wave;
+    v8bf X[KX / 16];
+    int sn_next = 0, dn_next = 0;
+    auto load_idx = [&](int64_t t) {
+        const int64_t q = t * 32 + r;
+        const bool ok = t < ntiles && q < E;
+        sn_next = ok ? src[q] : 0;
+        dn_next = ok ? dst[q] : 0;
+    };
+    auto load_feats = [&](int64_t t) {
+        const int64_t q = t * 32 + r;
+        const bool ok = t < ntiles && q < E;
+        float4 fq = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (FIRST && ok) fq = *(const float4 *)(ff + 4 * q);
+#pragma unroll
+        for (int s = 0; s < KX / 16; s++) {
+            const int k0 = 16 * s + 8 * half;
+            if (FIRST && ok && k0 >= 2 * H) {
+                // e_0 = ReLU(f We + be), the same operation order as k_encode
+#pragma unroll
+                for (int j = 0; j < 8; j++) {
+                    const int c = k0 - 2 * H + j;
+                    float z = sE[4 * H + c];
+                    z += fq.x * sE[c];
+                    z += fq.y * sE[H + c];
+                    z += fq.z * sE[2 * H + c];
+                    z += fq.w * sE[3 * H + c];
+                    X[s][j] = f2bf(fmaxf(z, 0.0f));
+                }
+            } else if (ok) {
+                const __bf16 *ap;
+                if (k0 < H) ap = Hb + (size_t)sn_next * H + k0;
+                else if (k0 < 2 * H) ap = Hb + (size_t)dn_next * H + (k0 - H);
+                else ap = Ein + (size_t)q * H + (k0 - 2 * H);
+                X[s] = *(const v8bf *)ap;
+            } else {
+#pragma unroll
+                for (int j = 0; j < 8; j++) X[s][j] = f2bf(0.0f);
+            }
+        }
+    };
+    load_idx(tile);
+    load_feats(tile);
+    for (; tile < ntiles; tile += tstride) {
+        const int64_t e0 = tile * 32;
+        const int64_t p = e0 + r;
+        const bool valid = p < E;
+        const int64_t next = tile + tstride;
+        load_idx(next);
+        v16f acc0 = {0}, acc1 = {0};
+#pragma unroll
+        for (int s = 0; s < KX / 16; s++) {
+            const int k0 = 16 * s + 8 * half;
+            const v8bf w0 = *(const v8bf *)(sW + r * WPAD + k0);
+            const v8bf w1 = *(const v8bf *)(sW + (r + 32) * WPAD + k0);
+            acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w0, X[s], acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w1, X[s], acc1, 0, 0, 0);
+        }
+        load_feats(next);
+        if (!LAST) {
+#pragma unroll
+            for (int g = 0; g < 4; g++) {
+                const int c = 8 * g + 4 * half;
+                const float4 b0 = *(const float4 *)(sB + c), b1 = *(const float4 *)(sB + c + 32);
+                const float bb0[4] = {b0.x, b0.y, b0.z, b0.w}, bb1[4] = {b1.x, b1.y, b1.z, b1.w};
+                v4bf o0, o1;
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    o0[j] = f2bf(fmaxf(acc0[4 * g + j] + bb0[j], 0.0f));
+                    o1[j] = f2bf(fmaxf(acc1[4 * g + j] + bb1[j], 0.0f));
+                }
+                *(v4bf *)(T + r * TPAD + c) = o0;
+                *(v4bf *)(T + r * TPAD + c + 32) = o1;
+            }
+            // the staged tile is read back by other lanes of this wave: LDS operations of one wave
+            // complete in issue order, so only the compiler must not move accesses across here
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const int row = 8 * k + (lane >> 3), c = (lane & 7) * 8;
+                const int64_t q = e0 + row;
+                const v8bf v = *(const v8bf *)(T + row * TPAD + c);
+                if (q < E) *(v8bf *)(Mout + q * H + c) = v;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+        } else {
+            float po = 0.0f, ps = 0.0f;
+#pragma unroll
+            for (int g = 0; g < 4; g++) {
+                const int c = 8 * g + 4 * half;
+                const float4 b0 = *(const float4 *)(sB + c), b1 = *(const float4 *)(sB + c + 32);
+                const float4 v0 = *(const float4 *)(sV + c), v1 = *(const float4 *)(sV + c + 32);
+                const float4 u0 = *(const float4 *)(sU + c), u1 = *(const float4 *)(sU + c + 32);
+                const float bb0[4] = {b0.x, b0.y, b0.z, b0.w}, bb1[4] = {b1.x, b1.y, b1.z, b1.w};
+                const float vv0[4] = {v0.x, v0.y, v0.z, v0.w}, vv1[4] = {v1.x, v1.y, v1.z, v1.w};
+                const float uu0[4] = {u0.x, u0.y, u0.z, u0.w}, uu1[4] = {u1.x, u1.y, u1.z, u1.w};
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const float m0 = bf2f(f2bf(fmaxf(acc0[4 * g + j] + bb0[j], 0.0f)));
+                    const float m1 = bf2f(f2bf(fmaxf(acc1[4 * g + j] + bb1[j], 0.0f)));
+                    po += m0 * vv0[j];
+                    ps += m0 * uu0[j];
+                    po += m1 * vv1[j];
+                    ps += m1 * uu1[j];
+                }
+            }
+            po += __shfl_xor(po, 32);
+            ps += __shfl_xor(ps, 32);
+            if (half == 0 && valid) {
+                oe[p] = po + ce[0];
+                sn[p] = ps;
+            }
+        }
+    }
+}
+
+// o_i = h_i . un + cn + mean_{p in row i} s_p (inference node decoder; eight lanes per node)
+__global__ __launch_bounds__(B) void k_node_decode(int n, const int *__restrict__ rp, const float *__restrict__ Hf,
+                                                   const float *__restrict__ sn, const float *__restrict__ un,
+                                                   const float *__restrict__ cn, float *__restrict__ on)
+{
+    const int64_t t = (int64_t)blockIdx.x * B + threadIdx.x;
+    const int i = (int)(t >> 3), c0 = (int)(t & 7) * 8;
+    float d = 0.0f;
+    if (i < n) {
+        const float4 h0 = *(const float4 *)(Hf + (size_t)i * H + c0);
+        const float4 h1 = *(const float4 *)(Hf + (size_t)i * H + c0 + 4);
+        d = h0.x * un[c0] + h0.y * un[c0 + 1] + h0.z * un[c0 + 2] + h0.w * un[c0 + 3] + h1.x * un[c0 + 4] +
+            h1.y * un[c0 + 5] + h1.z * un[c0 + 6] + h1.w * un[c0 + 7];
+    }
+    d += __shfl_xor(d, 1);
+    d += __shfl_xor(d, 2);
+    d += __shfl_xor(d, 4);
+    if (i < n && (t & 7) == 0) {
+        const int p0 = rp[i], p1 = rp[i + 1];
+        float s = 0.0f;
+        for (int p = p0; p < p1; p++) s += sn[p];
+        on[i] = d + cn[0] + ((p1 > p0) ? s / (float)(p1 - p0) : 0.0f);
+    }
 }
 
 // ---- decoders and the preconditioner ----------------------------------------------------------
@@ -624,7 +852,7 @@ __global__ __launch_bounds__(B) void k_pack_w(const float *__restrict__ W, __bf1
     Wt[(size_t)c * KX + k] = b;
 }
 
-inline dim3 g1(int64_t count) { return dim3((unsigned)((count + B - 1) / B)); }
+inline dim3 g1(int64_t count) { return dim3((unsigned)std::max<int64_t>(1, (count + B - 1) / B)); }
 
 }  // namespace
 
@@ -647,6 +875,14 @@ static hipError_t grow(T **p, size_t *cap, size_t want)
     SP_CHECK(hipMalloc((void **)p, sizeof(T) * (want ? want : 1)));
     *cap = want;
     return hipSuccess;
+}
+
+static void launch_graph(lv_spai *S, const lv_pressure_system &A, int n, int E, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_graph_nodes, g1(n), dim3(B), 0, s, n, (const int *)A.row_ptr, (const double *)A.diag64,
+                       (const double *)A.dinv64, S->src, S->xf, S->rfl, S->sqD);
+    hipLaunchKernelGGL(k_graph_edges, g1(E), dim3(B), 0, s, E, (const int *)A.row_ptr, (const int *)A.cols,
+                       (const double *)A.val64, (const double *)A.dinv64, (const int *)S->src, S->rev, S->a, S->ff);
 }
 
 int64_t lv_spai_param_count() { return LV_SPAI_NPARAM; }
@@ -718,21 +954,19 @@ hipError_t lv_spai_forward(lv_spai *S, const lv_pressure_system &A, int n, hipSt
     SP_CHECK(grow(&S->Hf, &S->cap_Hf, (size_t)4 * n * H));   // levels 0..3
     SP_CHECK(grow(&S->Hb, &S->cap_Hb, (size_t)4 * n * H));
     SP_CHECK(grow(&S->Eb, &S->cap_Eb, (size_t)4 * E * H));   // e_0 .. e_3
-    hipLaunchKernelGGL(k_graph, g1(n), dim3(B), 0, s, n, (const int *)A.row_ptr, (const int *)A.cols,
-                       (const double *)A.val64, (const double *)A.diag64, (const double *)A.dinv64, S->src, S->rev,
-                       S->a, S->xf, S->ff, S->rfl, S->sqD);
+    launch_graph(S, A, n, E, s);
     const float *th = S->theta;
     const size_t nh = (size_t)n * H, eh = (size_t)E * H;
-    hipLaunchKernelGGL(k_encode, g1((int64_t)n * H), dim3(B), 0, s, n, (const float *)S->xf, th + LV_SPAI_OFF_WN,
-                       th + LV_SPAI_OFF_BN, S->Hf, S->Hb);
-    hipLaunchKernelGGL(k_encode, g1((int64_t)E * H), dim3(B), 0, s, E, (const float *)S->ff, th + LV_SPAI_OFF_WE,
-                       th + LV_SPAI_OFF_BE, (float *)nullptr, S->Eb);
+    hipLaunchKernelGGL(k_encode, g1((int64_t)n * (H / 8)), dim3(B), 0, s, n, (const float *)S->xf,
+                       th + LV_SPAI_OFF_WN, th + LV_SPAI_OFF_BN, S->Hf, S->Hb);
+    hipLaunchKernelGGL(k_encode, g1((int64_t)E * (H / 8)), dim3(B), 0, s, E, (const float *)S->ff,
+                       th + LV_SPAI_OFF_WE, th + LV_SPAI_OFF_BE, (float *)nullptr, S->Eb);
     const unsigned tiles = (unsigned)((E + 31) / 32);
     for (int l = 0; l < 3; l++) {
         hipLaunchKernelGGL(k_mp_forward, dim3((tiles + 3) / 4), dim3(256), 0, s, E, (const int *)S->src,
                            (const int *)A.cols, (const __bf16 *)(S->Hb + l * nh), (const __bf16 *)(S->Eb + l * eh),
                            (const __bf16 *)(S->Wt + l * KX * H), th + LV_SPAI_OFF_B(l), S->Eb + (l + 1) * eh);
-        hipLaunchKernelGGL(k_aggregate, g1((int64_t)n * H), dim3(B), 0, s, n, (const int *)A.row_ptr,
+        hipLaunchKernelGGL(k_aggregate, g1((int64_t)n * (H / 8)), dim3(B), 0, s, n, (const int *)A.row_ptr,
                            (const float *)(S->Hf + l * nh), (const __bf16 *)(S->Eb + (l + 1) * eh),
                            S->Hf + (l + 1) * nh, S->Hb + (l + 1) * nh);
     }
@@ -746,7 +980,68 @@ hipError_t lv_spai_forward(lv_spai *S, const lv_pressure_system &A, int n, hipSt
     return hipGetLastError();
 }
 
-// M = D^-1/2 Mt D^-1/2 in the solver's ELL slots (after lv_spai_forward on the same operator)
+// The network on the assembled operator for the solver: the same outputs as lv_spai_forward (w, mdiag,
+// delta, o_edge, o_node) without keeping activations; two ping-pong levels of node and edge features.
+hipError_t lv_spai_infer(lv_spai *S, const lv_pressure_system &A, int n, hipStream_t s)
+{
+    const int E = (int)A.nnz;
+    S->n = n;
+    S->E = E;
+    SP_CHECK(grow(&S->src, &S->cap_src, (size_t)E));
+    SP_CHECK(grow(&S->rev, &S->cap_rev, (size_t)E));
+    SP_CHECK(grow(&S->a, &S->cap_a, (size_t)E));
+    SP_CHECK(grow(&S->ff, &S->cap_ff, (size_t)4 * E));
+    SP_CHECK(grow(&S->oe, &S->cap_oe, (size_t)E));
+    SP_CHECK(grow(&S->w, &S->cap_w, (size_t)E));
+    SP_CHECK(grow(&S->G, &S->cap_G, (size_t)E));  // s_p of the final layer
+    SP_CHECK(grow(&S->xf, &S->cap_xf, (size_t)4 * n));
+    SP_CHECK(grow(&S->rfl, &S->cap_rfl, (size_t)n));
+    SP_CHECK(grow(&S->sqD, &S->cap_sqD, (size_t)n));
+    SP_CHECK(grow(&S->on, &S->cap_on, (size_t)n));
+    SP_CHECK(grow(&S->mdiag, &S->cap_mdiag, (size_t)n));
+    SP_CHECK(grow(&S->delta, &S->cap_delta, (size_t)n));
+    SP_CHECK(grow(&S->Hf, &S->cap_Hf, (size_t)2 * n * H));
+    SP_CHECK(grow(&S->Hb, &S->cap_Hb, (size_t)2 * n * H));
+    SP_CHECK(grow(&S->Eb, &S->cap_Eb, (size_t)2 * E * H));
+    launch_graph(S, A, n, E, s);
+    const float *th = S->theta;
+    const size_t nh = (size_t)n * H, eh = (size_t)E * H;
+    hipLaunchKernelGGL(k_encode, g1((int64_t)n * (H / 8)), dim3(B), 0, s, n, (const float *)S->xf,
+                       th + LV_SPAI_OFF_WN, th + LV_SPAI_OFF_BN, S->Hf, S->Hb);
+    const int64_t tiles = ((int64_t)E + 31) / 32;
+    const unsigned blocks = (unsigned)std::min<int64_t>((tiles + 3) / 4, 4096);
+    for (int l = 0; l < 3; l++) {
+        const int cur = l & 1, nxt = cur ^ 1;
+        const __bf16 *Wl = S->Wt + l * KX * H;
+        if (l < 2) {
+            auto kern = (l == 0) ? k_mp_infer<true, false> : k_mp_infer<false, false>;
+            hipLaunchKernelGGL(kern, dim3(blocks), dim3(256), 0, s, E, (const int *)S->src, (const int *)A.cols,
+                               (const __bf16 *)(S->Hb + cur * nh), (const __bf16 *)(S->Eb + cur * eh),
+                               (const float *)S->ff, th + LV_SPAI_OFF_WE, th + LV_SPAI_OFF_BE, Wl,
+                               th + LV_SPAI_OFF_B(l), S->Eb + nxt * eh, (const float *)nullptr,
+                               (const float *)nullptr, (const float *)nullptr, (float *)nullptr, (float *)nullptr);
+            hipLaunchKernelGGL(k_aggregate, g1((int64_t)n * (H / 8)), dim3(B), 0, s, n, (const int *)A.row_ptr,
+                               (const float *)(S->Hf + cur * nh), (const __bf16 *)(S->Eb + nxt * eh),
+                               S->Hf + nxt * nh, S->Hb + nxt * nh);
+        } else {
+            hipLaunchKernelGGL((k_mp_infer<false, true>), dim3(blocks), dim3(256), 0, s, E, (const int *)S->src,
+                               (const int *)A.cols, (const __bf16 *)(S->Hb + cur * nh),
+                               (const __bf16 *)(S->Eb + cur * eh), (const float *)nullptr, (const float *)nullptr,
+                               (const float *)nullptr, Wl, th + LV_SPAI_OFF_B(l), (__bf16 *)nullptr,
+                               th + LV_SPAI_OFF_VE, th + LV_SPAI_OFF_CE, th + LV_SPAI_OFF_UN, S->oe, S->G);
+            hipLaunchKernelGGL(k_node_decode, g1((int64_t)n * 8), dim3(B), 0, s, n, (const int *)A.row_ptr,
+                               (const float *)(S->Hf + cur * nh), (const float *)S->G, th + LV_SPAI_OFF_UN,
+                               th + LV_SPAI_OFF_CN, S->on);
+        }
+    }
+    hipLaunchKernelGGL(k_build_mt, g1(n), dim3(B), 0, s, n, (const int *)A.row_ptr, (const int *)S->rev,
+                       (const float *)S->a, (const float *)S->oe, (const float *)S->on, (const float *)S->rfl, S->w,
+                       S->mdiag, S->delta);
+    return hipGetLastError();
+}
+
+// M = D^-1/2 Mt D^-1/2 in the solver's ELL slots (after lv_spai_forward or lv_spai_infer on the same
+// operator)
 hipError_t lv_spai_emit(lv_spai *S, const lv_pressure_system &A, int n, hipStream_t s)
 {
     SP_CHECK(grow(&S->mval, &S->cap_mval, (size_t)LV_MAXE * n));

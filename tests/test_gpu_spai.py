@@ -37,11 +37,14 @@ def _rel(a, b):
     return float(np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-300))
 
 
-@pytest.mark.parametrize("kind,ic", [("periodic", "tg"), ("walls", "cavity")])
-def test_spai_forward_matches_oracle(lv, oracle, gpu, kind, ic):
+@pytest.mark.parametrize("kind,ic,K", [("periodic", "tg", 24), ("walls", "cavity", 24), ("periodic", "tg", 301)])
+def test_spai_forward_matches_oracle(lv, oracle, gpu, kind, ic, K):
+    """The solver's inference pass (persistent MFMA waves; at K = 301 every wave loops over several
+    32-edge tiles and the last tile is partial) against the FP64 oracle."""
     import torch
 
-    grid, solver, g = _system(lv, oracle, kind=kind, ic=ic)
+    grid, solver, g = _system(lv, oracle, K=K, kind=kind, ic=ic)
+    assert K < 301 or g.cols.size > 4096 * 4 * 32 and g.cols.size % 32 != 0
     th = _theta(1)
     lv.spai.set_params(grid, th)
     oe, on, w, md = lv.spai.outputs(grid, g.cols.size)

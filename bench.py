@@ -135,7 +135,8 @@ def setup(lv, K, seed=0x5EED0002):
 def traffic_from_profiles(kernel):
     """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE, copy-kernel
     calibrated), committed under profiles/ as JSON; None if absent."""
-    path = os.path.join(ROOT, "profiles", "traffic_%s.json" % kernel)
+    tag = kernel.replace("<", "_").replace(">", "").replace(" ", "")
+    path = os.path.join(ROOT, "profiles", "traffic_%s.json" % tag)
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -230,7 +231,7 @@ def main():
     relres = max(max(s["relres"]) for s in stats)
     avg_launch_ms = tot_ms.value / max(launches.value, 1)
     achieved = bpl.value / (avg_launch_ms * 1e-3) / 1e9
-    kname = "k_cg32_spmv" if dominant == 0 else "k_cg64_spmv"
+    kname = "k_cg_s<float>" if dominant == 0 else "k_cg_s<double>"
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic_from_profiles(kname),
                     kernel=kname, avg_launch_us=round(avg_launch_ms * 1e3, 2), launches=launches.value,

@@ -15,6 +15,17 @@
  * stated next to the code: StaticArrays `norm` is taken as sqrt(a1*a1 + a2*a2) and LinearAlgebra's
  * three-argument `s*M*v` as M*(v*s) (ULP-level only).  Krylov.jl MINRES is replaced by a tight
  * Jacobi-PCG solve of the same linear system (src/pressure.jl:219).
+ *
+ * Parity unpinned (no offline check can fix these; each is named where it is used):
+ *   - Krylov.jl 0.9.8 minres! iterates: the oracle's pressure is the exact solution of the same
+ *     system to 1e-13, not MINRES's 1e-6 iterate;
+ *   - StaticArrays 1.9.8 norm/dot rounding (ULP level; absorbed by the 1e-12 bounds);
+ *   - the reference's Polyester-threaded insertion order (nondeterministic); the serial order
+ *     (ascending label per cell) is restated instead;
+ *   - Julia's sort! stability for magic-path ties: stable is assumed (Julia 1.10 default), and
+ *     the resulting path is pinned against a brute-force stable sort of every offset.
+ * Everything else is pinned by the tests under tests/ (brute force, Qhull, closed forms,
+ * conservation, the reference's own Taylor-Green thresholds).
  */
 #include <math.h>
 #include <stdint.h>

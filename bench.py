@@ -4,8 +4,8 @@ Workload (the metric's 4M-particle configuration): Taylor-Green vortex on the pe
 2048 x 2048 particles on a jittered lattice (+-0.1 dr, SplitMix64 seed 0x5EED0002), u = sin 2pi x
 cos 2pi y, v = -cos 2pi x sin 2pi y, nu = 1e-3, stiffened gas c0 = 50, dt = 2.5e-4.  One step is the
 reference test's step! (tests/taylorgreen.jl:61-71): move! + remesh, stiffened_eos!, find_pressure!
-(10 Picard solves, mixed-precision PCG with the learned preconditioner to 1e-8 FP64 relative
-residual), pressure_step!, find_D!,
+(10 Picard solves, mixed-precision PCG with the multilevel preconditioner to 1e-8 FP64 relative
+residual; --smoother learned makes the learned preconditioner its fine-level smoother), pressure_step!, find_D!,
 viscous_step!, find_dv!, relaxation_step! + remesh.  Inputs are resident on the device before the
 timed region.
 
@@ -33,9 +33,12 @@ def parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--K", type=int, default=2048, help="lattice side (K*K particles)")
-    ap.add_argument("--solver", default="spai", choices=["pcg64", "mixed", "spai"],
+    ap.add_argument("--solver", default="amg", choices=["pcg64", "mixed", "spai", "amg"],
                     help="pcg64: FP64 Jacobi-PCG; mixed: FP32 Jacobi-PCG in FP64 refinement; spai: the same with the "
-                         "learned preconditioner (committed trained parameters)")
+                         "learned preconditioner (committed trained parameters); amg: the same with the multilevel "
+                         "preconditioner whose fine smoother is the learned one")
+    ap.add_argument("--smoother", default="jacobi", choices=["learned", "jacobi"],
+                    help="fine-level smoother of --solver amg: the learned preconditioner or damped Jacobi")
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--inner-tol", type=float, default=1e-4,
                     help="floor of the FP32 sweep's relative tolerance inside the FP64 refinement")
@@ -75,34 +78,46 @@ def spai_theta(lv):
     return (th, True) if th is not None else (lv.spai.init_params(0), False)
 
 
+def _best_solve(lv, grid, method, dt, tol, inner_tol, x0, repeats):
+    solver = lv.PressureSolver(grid, method=method, tol=tol, itmax=4000, inner_tol=inner_tol)
+    best = None
+    for _ in range(repeats):
+        solver.assemble(dt)
+        b, _ = solver.rhs(dt, False)
+        solver.solve(b, x0)
+        st = solver.last_stats
+        rec = dict(ms=round(st["solve_ms"][0], 3), iters=st["iters"][0], sweeps=st["outer"][0],
+                   relres=st["relres"][0], fallbacks=st["fallbacks"])
+        if best is None or rec["ms"] < best["ms"]:
+            best = rec
+    return best
+
+
 def solve_compare(lv, grid, ic, dt, tol, inner_tol, repeats=3):
     """The pressure-solve metric: the first Picard solve of a step (operator and RHS of the current
-    state, EOS pressure as initial guess, src/pressure.jl:172) to tol, with FP64 Jacobi-PCG, mixed
-    FP32/FP64 Jacobi-PCG and mixed PCG with the learned preconditioner; best of `repeats` solves,
-    each timed with HIP events on the library's stream.  The learned preconditioner's solve time
-    includes one network inference (it runs once per assembled operator; here it is forced for every
-    repeat by re-assembling)."""
+    state, EOS pressure as initial guess, src/pressure.jl:172) to tol, best of `repeats` solves, each
+    timed with HIP events on the library's stream, with
+      pcg64                 FP64 Jacobi-PCG (the plain method, the judge);
+      mixed                 FP32 Jacobi-PCG sweeps inside FP64 iterative refinement;
+      spai                  the same with the learned preconditioner;
+      amg                   the same with the multilevel preconditioner (damped-Jacobi smoother);
+      amg_learned_smoother  the multilevel preconditioner with the learned preconditioner as its
+                            fine-level smoother.
+    Set-up is inside the timed solve (network inference, hierarchy); it runs once per assembled
+    operator and is forced here for every repeat by re-assembling."""
     th, trained = spai_theta(lv)
-    lv.spai.set_params(grid, th)
     lv.stiffened_eos(grid, ic["gamma"], ic["P0"])
     x0 = grid.get("P")
     out = {}
-    for method in ("pcg64", "mixed", "spai"):
-        solver = lv.PressureSolver(grid, method=method, tol=tol, itmax=4000, inner_tol=inner_tol)
-        best = None
-        for _ in range(repeats):
-            solver.assemble(dt)
-            b, _ = solver.rhs(dt, False)
-            solver.solve(b, x0)
-            st = solver.last_stats
-            rec = dict(ms=round(st["solve_ms"][0], 3), iters=st["iters"][0], sweeps=st["outer"][0],
-                       relres=st["relres"][0], fallbacks=st["fallbacks"])
-            if best is None or rec["ms"] < best["ms"]:
-                best = rec
-        out[method] = best
+    lv.spai.set_params(grid, None)
+    for method in ("pcg64", "mixed", "amg"):
+        out[method] = _best_solve(lv, grid, method, dt, tol, inner_tol, x0, repeats)
+    lv.spai.set_params(grid, th)
+    out["spai"] = _best_solve(lv, grid, "spai", dt, tol, inner_tol, x0, repeats)
+    out["amg_learned_smoother"] = _best_solve(lv, grid, "amg", dt, tol, inner_tol, x0, repeats)
     out["spai_trained"] = trained
-    out["speedup_spai_vs_pcg64"] = round(out["pcg64"]["ms"] / out["spai"]["ms"], 3)
-    out["speedup_mixed_vs_pcg64"] = round(out["pcg64"]["ms"] / out["mixed"]["ms"], 3)
+    for k in ("mixed", "spai", "amg", "amg_learned_smoother"):
+        out["speedup_%s_vs_pcg64" % k] = round(out["pcg64"]["ms"] / out[k]["ms"], 3)
     return out
 
 
@@ -110,7 +125,7 @@ def mode_solve(lv, K, method, tol, inner_tol):
     """One first-Picard pressure solve of the benchmark flow (EOS pressure as initial guess)."""
     dt = 2.5e-4
     grid, x, ic = setup(lv, K)
-    if method == "spai":
+    if method in ("spai", "amg"):
         lv.spai.set_params(grid, spai_theta(lv)[0])
     lv.stiffened_eos(grid, ic["gamma"], ic["P0"])
     solver = lv.PressureSolver(grid, method=method, tol=tol, itmax=2000, inner_tol=inner_tol)
@@ -198,7 +213,7 @@ def main():
     n = K * K
     dt = 2.5e-4
     grid, x, ic = setup(lv, K)
-    if args.solver == "spai":
+    if args.solver == "spai" or (args.solver == "amg" and args.smoother == "learned"):
         lv.spai.set_params(grid, spai_theta(lv)[0])
     solver = lv.PressureSolver(grid, method=args.solver, tol=args.tol, itmax=2000, inner_tol=args.inner_tol,
                                picard_window=args.picard_window)
@@ -243,7 +258,8 @@ def main():
                config=dict(workload="taylor-green %dx%d periodic, jitter +-0.1dr seed 0x5EED0002, dt=2.5e-4, "
                                     "c0=50 stiffened, nu=1e-3, 10 Picard solves/step, %s PCG to %g" %
                                     (K, K, args.solver, args.tol),
-                           particles=n, solver=args.solver, tol=args.tol, picard_window=args.picard_window),
+                           particles=n, solver=args.solver, tol=args.tol, picard_window=args.picard_window,
+                           smoother=args.smoother if args.solver == "amg" else None),
                pressure_solve_ms=round(first_solve_ms, 3), pressure_solve_iters=first_solve_it,
                max_relres=relres, picard_iters=stats[-1]["iters"], picard_sweeps=stats[-1]["outer"],
                picard_relres0=[float("%.3g" % r) for r in stats[-1]["relres0"]], roofline=roofline)

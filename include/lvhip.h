@@ -60,6 +60,9 @@ extern "C" {
 #define LV_SOLVER_MIXED 1  /* FP32 Jacobi-PCG inside FP64 iterative refinement */
 #define LV_SOLVER_SPAI 2   /* FP32 PCG with the learned SPAI preconditioner inside FP64 refinement;
                               falls back to LV_SOLVER_MIXED when it stalls */
+#define LV_SOLVER_AMG 3    /* FP32 PCG with a multilevel (aggregation) V-cycle inside FP64 refinement; the
+                              learned SPAI is its fine-level smoother when parameters are loaded (damped
+                              Jacobi otherwise); falls back to LV_SOLVER_MIXED when it stalls */
 
 typedef struct lv_ctx lv_ctx;
 
@@ -178,6 +181,11 @@ int lv_get_pressure_operator(lv_ctx *ctx, double *diag, int32_t *row_ptr, int32_
 int lv_pressure_rhs(lv_ctx *ctx, double dt, int32_t gp_step, const double vbc[8], double *b, double *GP);
 /* solve the assembled operator for b with initial guess x (in/out) */
 int lv_solve_pressure(lv_ctx *ctx, const double *b, double *x, const lv_solve_opts *opts, lv_solve_stats *stats);
+/* z = B r with the FP32 preconditioner of the inner solve of `method` (LV_SOLVER_MIXED: Jacobi,
+ * LV_SOLVER_SPAI: the learned M, LV_SOLVER_AMG: the multilevel V-cycle) on the assembled operator;
+ * host vectors of n doubles.  No counterpart in the reference; it is the hook a Julia caller uses
+ * to hand this preconditioner to Krylov.jl (src/pressure.jl:219 runs minres! unpreconditioned). */
+int lv_apply_preconditioner(lv_ctx *ctx, int32_t method, const double *r, double *z);
 
 /* ---- learned SPAI preconditioner --------------------------------------------------------------
  * No counterpart in the reference (it runs unpreconditioned MINRES, src/pressure.jl:219).  A
@@ -185,7 +193,7 @@ int lv_solve_pressure(lv_ctx *ctx, const double *b, double *x, const lv_solve_op
  * diagonally dominant (hence SPD) approximate inverse on the pressure operator's own pattern; it is
  * used by LV_SOLVER_SPAI, with Jacobi-PCG as fallback.  Parameters are a flat float32 vector. */
 int lv_spai_num_params(int64_t *count);
-int lv_spai_set_params(lv_ctx *ctx, const float *params);
+int lv_spai_set_params(lv_ctx *ctx, const float *params);  /* params == NULL unloads the network */
 int lv_spai_get_params(lv_ctx *ctx, float *params);
 /* one Adam step on the assembled operator; probes: the current P (scaled) and nprobe-1 Rademacher
  * vectors drawn from seed; loss is the value before the update */

@@ -36,7 +36,7 @@ LV_E_NOMEM = 7
 LV_E_NO_MESH = 8
 LV_MAX_EDGES = 16
 
-SOLVERS = {"pcg64": 0, "mixed": 1, "spai": 2}
+SOLVERS = {"pcg64": 0, "mixed": 1, "spai": 2, "amg": 3}
 
 FIELDS = {"x": (0, 2), "v": (1, 2), "dv": (2, 2), "momentum": (3, 2), "rho": (4, 1), "e": (5, 1), "P": (6, 1),
           "c2": (7, 1), "mass": (8, 1), "energy": (9, 1), "quality": (10, 1), "mu": (11, 1), "D": (12, 4),
@@ -125,6 +125,7 @@ _SIGNATURES = {
     "lv_get_pressure_operator": [_P, _P, _P, _P, _P],
     "lv_pressure_rhs": [_P, _D, _I, _P, _P, _P],
     "lv_solve_pressure": [_P, _P, _P, ctypes.POINTER(SolveOpts), ctypes.POINTER(SolveStats)],
+    "lv_apply_preconditioner": [_P, _I, _P, _P],
     "lv_spai_num_params": [_P],
     "lv_spai_set_params": [_P, _P],
     "lv_spai_get_params": [_P, _P],
@@ -439,6 +440,16 @@ class PressureSolver:
         _check(self.grid.ctx, lib().lv_pressure_rhs(self.grid.ctx, float(dt), int(bool(gp_step)), _ptr(t), _ptr(b),
                                                     _ptr(GP)))
         return b, GP
+
+    def apply_preconditioner(self, r, method=None):
+        """z = B r with the FP32 preconditioner of the inner solve of `method` ("mixed": Jacobi,
+        "spai": the learned M, "amg": the multilevel cycle) on the assembled operator."""
+        m = self.method if method is None else method
+        r = np.ascontiguousarray(np.asarray(r, dtype=np.float64))
+        z = np.empty_like(r)
+        _check(self.grid.ctx, lib().lv_apply_preconditioner(self.grid.ctx, SOLVERS[m] if isinstance(m, str) else int(m),
+                                                             _ptr(r), _ptr(z)))
+        return z
 
     def solve(self, b, x0, raise_on_fail=True, **kw):
         o = solve_opts(**{**dict(method=self.method, tol=self.tol, itmax=self.itmax, inner_tol=self.inner_tol,

@@ -1,0 +1,47 @@
+// lv_amg.h -- multilevel (aggregation) preconditioner for the pressure system.
+#pragma once
+
+#include "lv_solver.h"
+
+#define LV_AMG_MAXLEV 10
+#define LV_AMG_DENSE_MAX 256  // the coarsest level is inverted densely once it has at most this many cells
+
+// Level l >= 1: a Kx x Ky grid of cells; the operator in the reference's difference form,
+// (A y)_I = d_I y_I + sum_k c_Ik (y_I - y_{I + off_k}), over the 8 neighbour directions.
+struct lv_amg_level {
+    int Kx, Ky, n;
+    float *d, *c, *dinv;  // d[n], c[8n] slot-major, dinv[n] = 1 / (d + sum c)
+    float *r, *z, *t;     // right-hand side, pre-smoothed correction, final correction of the cycle
+    size_t cap;
+};
+
+struct lv_amg {
+    int nlev;       // coarse levels; level nlev is the dense one
+    int n0, na;     // fine rows, level-1 cells
+    int xper, yper;
+    int *agg, *cnt, *start, *cursor, *list;
+    void *scan_tmp;
+    size_t cap_agg, cap_list, cap_t0, cap_z0, cap_cnt, cap_start, cap_cursor, cap_scan;
+    float *t0, *z0; // fine residual and pre-smoothed correction
+    lv_amg_level lev[LV_AMG_MAXLEV + 1];
+    double *dense_work;  // 2 * nc * nc for the Gauss-Jordan inversion
+    float *dense_inv;    // nc * nc, column-major
+    double *pw;          // power-iteration partials and norms
+    float *pv, *pu;      // power-iteration vectors (n0)
+    size_t cap_pv, cap_pu;
+    float fine_scale;    // the fine smoother applies fine_scale * M (learned) or fine_scale * D^-1 (Jacobi)
+    int fine_learned;    // 1: the learned preconditioner's M is the fine smoother
+    const float *mval, *mdg;
+};
+
+hipError_t lv_amg_init(lv_amg *G);
+void lv_amg_free(lv_amg *G);
+// Builds the hierarchy for the assembled operator A (device positions x of the particles, grid of
+// the boundary rectangle).  mval/mdg: the learned preconditioner in A's ELL slots, or null for a
+// damped-Jacobi fine smoother.
+hipError_t lv_amg_setup(lv_amg *G, const lv_grid_params &g, const double *x, const lv_pressure_system &A, int n,
+                        const float *mval, const float *mdg, hipStream_t s);
+// z = B r for the symmetric V(1,1) cycle B; writes the (r.z, r.r) partials of lv_nblocks(n) blocks
+// that the next CG kernel reduces.  Every kernel is a no-op once S->done is set.
+hipError_t lv_amg_vcycle(const lv_amg *G, const lv_pressure_system &A, const float *r, float *z, double *Prr,
+                         const lv_cg_scalars *S, hipStream_t s);

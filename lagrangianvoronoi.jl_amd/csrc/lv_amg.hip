@@ -1,0 +1,863 @@
+// lv_amg.hip -- multilevel preconditioner for the pressure system of find_pressure! (src/pressure.jl:89-224).
+//
+// No counterpart in the reference (it runs unpreconditioned MINRES, src/pressure.jl:219).  A
+// one-ring preconditioner can shrink the condition number of the Voronoi Laplacian by a constant
+// factor only; the smooth error modes need a coarse correction.
+//
+// Hierarchy (rebuilt for every assembled operator):
+//   level 0: the particles and A itself (ELL, FP32);
+//   level 1: the cells of a regular grid of width >= 4 dr over the boundary rectangle; every
+//            particle is aggregated into the cell holding it.  The Galerkin operator P^T A P of the
+//            piecewise-constant P keeps the reference's difference form:
+//              d_I = sum_{i in I} d_i,   c_IJ = sum_{i in I, j in J} c_ij,
+//            stored as a 3x3 stencil (a coupling that would reach beyond the neighbouring cells,
+//            only possible on strongly distorted meshes, is dropped from both sides, which keeps
+//            the coarse operator positive semi-definite);
+//   level l+1: 4 x 4 blocks of level-l cells, the same Galerkin rule;
+//   the coarsest level (<= LV_AMG_DENSE_MAX cells) is inverted densely once per assembly.
+// Every sum is a gather in a fixed order (per-aggregate row lists are sorted), so the hierarchy
+// and the preconditioner are deterministic.
+//
+// Cycle: symmetric V(1,1).  The fine smoother is the learned SPAI (M of lv_spai.hip, scaled so
+// that the largest eigenvalue of M A is ~1.2, estimated by power iteration at set-up) or, without
+// it, damped Jacobi (2/3 D^-1); the coarse smoothers are damped Jacobi.  The cycle is an SPD
+// linear operator, so CG stays valid; it runs in FP32 inside the FP64 iterative refinement, where
+// Jacobi-PCG remains the fallback and the FP64 residual remains the judge.
+#include <string.h>
+
+#include <algorithm>
+
+#include "lv_amg.h"
+#include "lv_reduce.h"
+
+namespace {
+
+constexpr int B = 256;
+constexpr float OMEGA = 2.0f / 3.0f;  // damped Jacobi
+constexpr float FINE_TARGET = 1.2f;   // lambda_max of the scaled learned smoother times A
+
+inline dim3 g1(int64_t count) { return dim3((unsigned)std::max<int64_t>(1, (count + B - 1) / B)); }
+
+// 3x3 stencil slots: k = 0..7 for the offsets (dx, dy), dx-major, skipping (0, 0)
+__device__ __forceinline__ int slot_of(int dx, int dy)
+{
+    const int q = (dx + 1) * 3 + (dy + 1);
+    return q > 4 ? q - 1 : q;
+}
+__device__ __forceinline__ void offset_of(int k, int &dx, int &dy)
+{
+    const int q = k >= 4 ? k + 1 : k;
+    dx = q / 3 - 1;
+    dy = q % 3 - 1;
+}
+// a cell offset on a periodic axis of K cells, folded into [-(K-1)/2, K/2]
+__device__ __forceinline__ int fold(int d, int K, int per)
+{
+    if (!per) return d;
+    d %= K;
+    if (d > K / 2) d -= K;
+    if (d < -((K - 1) / 2)) d += K;
+    return d;
+}
+__device__ __forceinline__ int wrap(int i, int K, int per, bool &ok)
+{
+    if (per) {
+        ok = true;
+        return (i % K + K) % K;
+    }
+    ok = (i >= 0 && i < K);
+    return i;
+}
+
+// ---- level 1: aggregation of the particles -----------------------------------------------------
+
+__global__ __launch_bounds__(B) void k_agg_key(int n, const double *__restrict__ x, double x0, double y0, double hx,
+                                               double hy, int Kx, int Ky, int *__restrict__ agg,
+                                               int *__restrict__ cnt)
+{
+    const int i = blockIdx.x * B + threadIdx.x;
+    if (i >= n) return;
+    int ix = (int)floor((x[i] - x0) / hx), iy = (int)floor((x[n + i] - y0) / hy);
+    ix = ix < 0 ? 0 : (ix >= Kx ? Kx - 1 : ix);
+    iy = iy < 0 ? 0 : (iy >= Ky ? Ky - 1 : iy);
+    const int a = ix * Ky + iy;
+    agg[i] = a;
+    atomicAdd(&cnt[a], 1);
+}
+
+__global__ __launch_bounds__(B) void k_agg_fill(int n, const int *__restrict__ agg, const int *__restrict__ start,
+                                                int *__restrict__ cursor, int *__restrict__ list)
+{
+    const int i = blockIdx.x * B + threadIdx.x;
+    if (i >= n) return;
+    const int a = agg[i];
+    list[start[a] + atomicAdd(&cursor[a], 1)] = i;
+}
+
+// per cell: sort its rows (fixed summation order), then the Galerkin sums of d and of the couplings
+// to the 8 neighbouring cells, accumulated in FP64
+__global__ __launch_bounds__(B) void k_coarse1(int na, int Kx, int Ky, int xper, int yper,
+                                               const int *__restrict__ start, int *__restrict__ list,
+                                               const int *__restrict__ agg, int n, const int *__restrict__ deg,
+                                               const int *__restrict__ ecol, const double *__restrict__ ev,
+                                               const double *__restrict__ d64, float *__restrict__ dout,
+                                               float *__restrict__ cout, float *__restrict__ dinv)
+{
+    const int I = blockIdx.x * B + threadIdx.x;
+    if (I >= na) return;
+    const int s0 = start[I], s1 = start[I + 1];
+    for (int a = s0 + 1; a < s1; a++) {
+        const int v = list[a];
+        int b = a;
+        while (b > s0 && list[b - 1] > v) {
+            list[b] = list[b - 1];
+            b--;
+        }
+        list[b] = v;
+    }
+    const int Ix = I / Ky, Iy = I % Ky;
+    double d = 0.0, c[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int a = s0; a < s1; a++) {
+        const int i = list[a];
+        d += d64[i];
+        const int dg = deg[i];
+        for (int k = 0; k < dg; k++) {
+            const size_t e = (size_t)k * n + i;
+            const int J = agg[ecol[e]];
+            if (J == I) continue;
+            const int dx = fold(J / Ky - Ix, Kx, xper), dy = fold(J % Ky - Iy, Ky, yper);
+            if (dx < -1 || dx > 1 || dy < -1 || dy > 1 || (dx == 0 && dy == 0)) continue;
+            c[slot_of(dx, dy)] += ev[e];
+        }
+    }
+    if (s1 == s0) d = 1.0;  // an empty cell is a decoupled identity row
+    double sum = d;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        cout[(size_t)k * na + I] = (float)c[k];
+        sum += c[k];
+    }
+    dout[I] = (float)d;
+    dinv[I] = sum > 0.0 ? (float)(1.0 / sum) : 1.0f;
+}
+
+// level l+1 from level l: 4 x 4 blocks of cells
+__global__ __launch_bounds__(B) void k_coarsen(int Kxc, int Kyc, int Kxf, int Kyf, int xper, int yper,
+                                               const float *__restrict__ df, const float *__restrict__ cf,
+                                               float *__restrict__ dc, float *__restrict__ cc,
+                                               float *__restrict__ dinv)
+{
+    const int P = blockIdx.x * B + threadIdx.x;
+    const int nc = Kxc * Kyc, nf = Kxf * Kyf;
+    if (P >= nc) return;
+    const int PX = P / Kyc, PY = P % Kyc;
+    double d = 0.0, c[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    const int cx1 = std::min(4 * PX + 4, Kxf), cy1 = std::min(4 * PY + 4, Kyf);
+    for (int cx = 4 * PX; cx < cx1; cx++)
+        for (int cy = 4 * PY; cy < cy1; cy++) {
+            const int f = cx * Kyf + cy;
+            d += df[f];
+            for (int k = 0; k < 8; k++) {
+                const float ck = cf[(size_t)k * nf + f];
+                if (ck == 0.0f) continue;
+                int ox, oy;
+                offset_of(k, ox, oy);
+                bool okx, oky;
+                const int nx = wrap(cx + ox, Kxf, xper, okx), ny = wrap(cy + oy, Kyf, yper, oky);
+                if (!okx || !oky) continue;
+                const int dx = fold(nx / 4 - PX, Kxc, xper), dy = fold(ny / 4 - PY, Kyc, yper);
+                if (dx == 0 && dy == 0) continue;
+                c[slot_of(dx, dy)] += ck;
+            }
+        }
+    double sum = d;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        cc[(size_t)k * nc + P] = (float)c[k];
+        sum += c[k];
+    }
+    dc[P] = (float)d;
+    dinv[P] = sum > 0.0 ? (float)(1.0 / sum) : 1.0f;
+}
+
+// the coarsest level as a dense FP64 matrix (plus sigma/nc everywhere for a singular system, which
+// makes it definite without changing its action on mean-free vectors), inverted by Gauss-Jordan in
+// one workgroup (an SPD matrix needs no pivoting); the inverse is stored column-major in FP32
+__global__ __launch_bounds__(256) void k_dense_inverse(int Kx, int Ky, int xper, int yper, const float *__restrict__ d,
+                                                       const float *__restrict__ c, int singular,
+                                                       double *__restrict__ M, double *__restrict__ X,
+                                                       float *__restrict__ inv)
+{
+    const int nc = Kx * Ky;
+    for (int t = threadIdx.x; t < nc * nc; t += 256) {
+        M[t] = 0.0;
+        X[t] = (t / nc == t % nc) ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double dsum = 0.0;
+        for (int I = 0; I < nc; I++) {
+            const int Ix = I / Ky, Iy = I % Ky;
+            double diag = d[I];
+            for (int k = 0; k < 8; k++) {
+                const double ck = c[(size_t)k * nc + I];
+                if (ck == 0.0) continue;
+                int ox, oy;
+                offset_of(k, ox, oy);
+                bool okx, oky;
+                const int jx = wrap(Ix + ox, Kx, xper, okx), jy = wrap(Iy + oy, Ky, yper, oky);
+                if (!okx || !oky) continue;
+                diag += ck;
+                M[(size_t)I * nc + jx * Ky + jy] -= ck;
+            }
+            M[(size_t)I * nc + I] += diag;
+            dsum += diag;
+        }
+        if (singular) {
+            const double sig = dsum / nc / nc;
+            for (int t = 0; t < nc * nc; t++) M[t] += sig;
+        }
+    }
+    __syncthreads();
+    __shared__ double colp[LV_AMG_DENSE_MAX];
+    for (int p = 0; p < nc; p++) {
+        // column p (scaled by the pivot) is copied first: the row updates below zero it
+        const double piv = M[(size_t)p * nc + p];
+        for (int i = threadIdx.x; i < nc; i += 256) colp[i] = M[(size_t)i * nc + p];
+        __syncthreads();
+        for (int j = threadIdx.x; j < nc; j += 256) {
+            const double mp = M[(size_t)p * nc + j] / piv, xp = X[(size_t)p * nc + j] / piv;
+            M[(size_t)p * nc + j] = mp;
+            X[(size_t)p * nc + j] = xp;
+            for (int i = 0; i < nc; i++) {
+                if (i == p) continue;
+                const double f = colp[i];
+                if (f == 0.0) continue;
+                M[(size_t)i * nc + j] -= f * mp;
+                X[(size_t)i * nc + j] -= f * xp;
+            }
+        }
+        __syncthreads();
+    }
+    for (int t = threadIdx.x; t < nc * nc; t += 256) {
+        const int i = t / nc, j = t % nc;
+        inv[(size_t)j * nc + i] = (float)X[t];
+    }
+}
+
+// ---- fine level (ELL, FP32) ----------------------------------------------------------------------
+
+// (A z)_i in the difference form with the first 8 slots unrolled and predicated
+__device__ __forceinline__ float a_row(int i, int n, const int *__restrict__ deg, const int *__restrict__ ecol,
+                                       const float *__restrict__ ev, const float *__restrict__ diag,
+                                       const float *__restrict__ z)
+{
+    constexpr int U = 8;
+    const float zi = z[i];
+    float acc = diag[i] * zi;
+    const int d = deg[i];
+    int c[U];
+    float v[U];
+#pragma unroll
+    for (int k = 0; k < U; k++) {
+        c[k] = i;
+        v[k] = 0.0f;
+        if (k < d) {
+            const size_t e = (size_t)k * n + i;
+            c[k] = ecol[e];
+            v[k] = ev[e];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < U; k++)
+        if (k < d) acc += v[k] * (zi - z[c[k]]);
+    for (int k = U; k < d; k++) {
+        const size_t e = (size_t)k * n + i;
+        acc += ev[e] * (zi - z[ecol[e]]);
+    }
+    return acc;
+}
+
+// (M r)_i of the learned preconditioner, or D^-1 r
+__device__ __forceinline__ float m_row(int i, int n, const int *__restrict__ deg, const int *__restrict__ ecol,
+                                       const float *__restrict__ mval, const float *__restrict__ mdg,
+                                       const float *__restrict__ dinv, const float *__restrict__ r)
+{
+    if (!mval) return dinv[i] * r[i];
+    constexpr int U = 8;
+    float acc = mdg[i] * r[i];
+    const int d = deg[i];
+    int c[U];
+    float v[U];
+#pragma unroll
+    for (int k = 0; k < U; k++) {
+        c[k] = i;
+        v[k] = 0.0f;
+        if (k < d) {
+            const size_t e = (size_t)k * n + i;
+            c[k] = ecol[e];
+            v[k] = mval[e];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < U; k++)
+        if (k < d) acc += v[k] * r[c[k]];
+    for (int k = U; k < d; k++) {
+        const size_t e = (size_t)k * n + i;
+        acc += mval[e] * r[ecol[e]];
+    }
+    return acc;
+}
+
+struct fine_ops {
+    int n;
+    const int *deg, *ecol;
+    const float *ev, *diag, *dinv, *mval, *mdg;
+    float scale;
+};
+
+// z = scale * M r
+__global__ __launch_bounds__(B) void k_fine_pre(fine_ops F, const float *__restrict__ r, float *__restrict__ z,
+                                                const lv_cg_scalars *S)
+{
+    if (S && S->done) return;
+    const int i = blockIdx.x * B + threadIdx.x;
+    if (i >= F.n) return;
+    z[i] = F.scale * m_row(i, F.n, F.deg, F.ecol, F.mval, F.mdg, F.dinv, r);
+}
+
+// t = r - A z
+__global__ __launch_bounds__(B) void k_fine_resid(fine_ops F, const float *__restrict__ r, const float *__restrict__ z,
+                                                  float *__restrict__ t, const lv_cg_scalars *S)
+{
+    if (S && S->done) return;
+    const int i = blockIdx.x * B + threadIdx.x;
+    if (i >= F.n) return;
+    t[i] = r[i] - a_row(i, F.n, F.deg, F.ecol, F.ev, F.diag, z);
+}
+
+// rc[I] = sum over the (sorted) rows of cell I of t
+__global__ __launch_bounds__(B) void k_restrict1(int na, const int *__restrict__ start, const int *__restrict__ list,
+                                                 const float *__restrict__ t, float *__restrict__ rc,
+                                                 const lv_cg_scalars *S)
+{
+    if (S && S->done) return;
+    const int I = blockIdx.x * B + threadIdx.x;
+    if (I >= na) return;
+    const int s0 = start[I], s1 = start[I + 1];
+    float s = 0.0f;
+    // eight row indices, then their eight gathers, in flight at once
+    for (int a0 = s0; a0 < s1; a0 += 8) {
+        int idx[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) idx[k] = (a0 + k < s1) ? list[a0 + k] : -1;
+        float v[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) v[k] = (idx[k] >= 0) ? t[idx[k]] : 0.0f;
+#pragma unroll
+        for (int k = 0; k < 8; k++) s += v[k];
+    }
+    rc[I] = s;
+}
+
+// Jacobi fine smoother, down: z0 = w D^-1 r and t = r - A z0 in one pass (z0 of a neighbour is
+// recomputed from its r)
+__global__ __launch_bounds__(B) void k_fine_down_j(fine_ops F, const float *__restrict__ r, float *__restrict__ z0,
+                                                   float *__restrict__ t, const lv_cg_scalars *S)
+{
+    if (S && S->done) return;
+    const int i = blockIdx.x * B + threadIdx.x;
+    const int n = F.n;
+    if (i >= n) return;
+    constexpr int U = 8;
+    const float ri = r[i];
+    const float zi = F.scale * F.dinv[i] * ri;
+    float acc = F.diag[i] * zi;
+    const int d = F.deg[i];
+    int c[U];
+    float v[U];
+#pragma unroll
+    for (int k = 0; k < U; k++) {
+        c[k] = i;
+        v[k] = 0.0f;
+        if (k < d) {
+            const size_t e = (size_t)k * n + i;
+            c[k] = F.ecol[e];
+            v[k] = F.ev[e];
+        }
+    }
+    float rj[U], dj[U];
+#pragma unroll
+    for (int k = 0; k < U; k++) {
+        rj[k] = r[c[k]];
+        dj[k] = F.dinv[c[k]];
+    }
+#pragma unroll
+    for (int k = 0; k < U; k++)
+        if (k < d) acc += v[k] * (zi - F.scale * dj[k] * rj[k]);
+    for (int k = U; k < d; k++) {
+        const size_t e = (size_t)k * n + i;
+        const int j = F.ecol[e];
+        acc += F.ev[e] * (zi - F.scale * F.dinv[j] * r[j]);
+    }
+    z0[i] = zi;
+    t[i] = ri - acc;
+}
+
+// up: z1 = z0 + P ec and t = r - A z1 in one pass (z1 of a neighbour from its z0 and its cell's ec)
+__device__ __forceinline__ float up_row(const fine_ops &F, int i, const float *__restrict__ z0,
+                                        const int *__restrict__ agg, const float *__restrict__ ec,
+                                        const float *__restrict__ r, float &z1)
+{
+    constexpr int U = 8;
+    const int n = F.n;
+    const int d = F.deg[i];
+    // three dependent stages (columns; their z0 and cells; the cells' corrections), each issued for
+    // all of the first U slots before the next
+    int c[U];
+    float v[U];
+#pragma unroll
+    for (int k = 0; k < U; k++) {
+        c[k] = i;
+        v[k] = 0.0f;
+        if (k < d) {
+            const size_t e = (size_t)k * n + i;
+            c[k] = F.ecol[e];
+            v[k] = F.ev[e];
+        }
+    }
+    float zj[U];
+    int aj[U];
+#pragma unroll
+    for (int k = 0; k < U; k++) {
+        zj[k] = z0[c[k]];
+        aj[k] = agg[c[k]];
+    }
+    z1 = z0[i] + ec[agg[i]];
+    float acc = F.diag[i] * z1;
+#pragma unroll
+    for (int k = 0; k < U; k++)
+        if (k < d) acc += v[k] * (z1 - (zj[k] + ec[aj[k]]));
+    for (int k = U; k < d; k++) {
+        const size_t e = (size_t)k * n + i;
+        const int j = F.ecol[e];
+        acc += F.ev[e] * (z1 - (z0[j] + ec[agg[j]]));
+    }
+    return r[i] - acc;
+}
+
+// learned smoother, up: z = z1 and t = r - A z1 (the post-smoothing follows in k_fine_post)
+__global__ __launch_bounds__(B) void k_fine_up(fine_ops F, const float *__restrict__ r, const float *__restrict__ z0,
+                                               const int *__restrict__ agg, const float *__restrict__ ec,
+                                               float *__restrict__ z, float *__restrict__ t, const lv_cg_scalars *S)
+{
+    if (S && S->done) return;
+    const int i = blockIdx.x * B + threadIdx.x;
+    if (i >= F.n) return;
+    float z1;
+    const float ti = up_row(F, i, z0, agg, ec, r, z1);
+    z[i] = z1;
+    t[i] = ti;
+}
+
+// prolongation in place: z0 += ec[cell of the row]
+__global__ __launch_bounds__(B) void k_prolong1(int n, const int *__restrict__ agg, const float *__restrict__ ec,
+                                                float *__restrict__ z0, const lv_cg_scalars *S)
+{
+    if (S && S->done) return;
+    const int i = blockIdx.x * B + threadIdx.x;
+    if (i < n) z0[i] += ec[agg[i]];
+}
+
+// Jacobi smoother, up (after k_prolong1): z = z1 + w D^-1 (r - A z1); partials r.z, r.r of the final z
+__global__ __launch_bounds__(B) void k_fine_up_j(fine_ops F, const float *__restrict__ r, const float *__restrict__ z1,
+                                                 float *__restrict__ z, double *__restrict__ Prr,
+                                                 const lv_cg_scalars *S)
+{
+    if (S && S->done) return;
+    double v[2] = {0.0, 0.0};
+    for (int i = blockIdx.x * B + threadIdx.x; i < F.n; i += gridDim.x * B) {
+        const float ti = r[i] - a_row(i, F.n, F.deg, F.ecol, F.ev, F.diag, z1);
+        const float zi = z1[i] + F.scale * F.dinv[i] * ti;
+        z[i] = zi;
+        const float ri = r[i];
+        v[0] += (double)ri * (double)zi;
+        v[1] += (double)ri * (double)ri;
+    }
+    if (Prr) block_partials<2>(v, Prr, gridDim.x);
+}
+
+// z += scale * M t; partials r.z, r.r of the final z (grid-stride over lv_nblocks(n) blocks)
+__global__ __launch_bounds__(B) void k_fine_post(fine_ops F, const float *__restrict__ r, const float *__restrict__ t,
+                                                 float *__restrict__ z, double *__restrict__ Prr,
+                                                 const lv_cg_scalars *S)
+{
+    if (S && S->done) return;
+    double v[2] = {0.0, 0.0};
+    for (int i = blockIdx.x * B + threadIdx.x; i < F.n; i += gridDim.x * B) {
+        const float zi = z[i] + F.scale * m_row(i, F.n, F.deg, F.ecol, F.mval, F.mdg, F.dinv, t);
+        z[i] = zi;
+        const float ri = r[i];
+        v[0] += (double)ri * (double)zi;
+        v[1] += (double)ri * (double)ri;
+    }
+    if (Prr) block_partials<2>(v, Prr, gridDim.x);
+}
+
+// ---- coarse levels (3x3 stencil) ---------------------------------------------------------------
+
+struct st_level {
+    int Kx, Ky, xper, yper;
+    const float *d, *c, *dinv;
+};
+
+__device__ __forceinline__ float st_apply(const st_level &L, int I, const float *__restrict__ z)
+{
+    const int n = L.Kx * L.Ky, Ix = I / L.Ky, Iy = I % L.Ky;
+    const float zi = z[I];
+    float acc = L.d[I] * zi;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        const float ck = L.c[(size_t)k * n + I];
+        if (ck == 0.0f) continue;
+        int ox, oy;
+        offset_of(k, ox, oy);
+        bool okx, oky;
+        const int jx = wrap(Ix + ox, L.Kx, L.xper, okx), jy = wrap(Iy + oy, L.Ky, L.yper, oky);
+        if (!okx || !oky) continue;
+        acc += ck * (zi - z[jx * L.Ky + jy]);
+    }
+    return acc;
+}
+
+// down: for every child f of parent P (16 lanes per parent): z_f = w D^-1 r_f, t_f = r_f - (A z)_f
+// with the neighbours' z recomputed from their r; rc[P] = sum of the children's t (lane tree)
+__global__ __launch_bounds__(B) void k_st_down(st_level L, int Kyc, int nc, const float *__restrict__ r,
+                                               float *__restrict__ z, float *__restrict__ rc, const lv_cg_scalars *S)
+{
+    if (S && S->done) return;
+    const int tid = blockIdx.x * B + threadIdx.x;
+    const int P = tid >> 4, q = tid & 15;
+    float t = 0.0f;
+    if (P < nc) {
+        const int cx = 4 * (P / Kyc) + (q >> 2), cy = 4 * (P % Kyc) + (q & 3);
+        if (cx < L.Kx && cy < L.Ky) {
+            const int n = L.Kx * L.Ky, f = cx * L.Ky + cy;
+            const float rf = r[f];
+            const float zf = OMEGA * L.dinv[f] * rf;
+            float acc = L.d[f] * zf;
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                const float ck = L.c[(size_t)k * n + f];
+                if (ck == 0.0f) continue;
+                int ox, oy;
+                offset_of(k, ox, oy);
+                bool okx, oky;
+                const int jx = wrap(cx + ox, L.Kx, L.xper, okx), jy = wrap(cy + oy, L.Ky, L.yper, oky);
+                if (!okx || !oky) continue;
+                const int J = jx * L.Ky + jy;
+                acc += ck * (zf - OMEGA * L.dinv[J] * r[J]);
+            }
+            z[f] = zf;
+            t = rf - acc;
+        }
+    }
+    t += __shfl_xor(t, 1);
+    t += __shfl_xor(t, 2);
+    t += __shfl_xor(t, 4);
+    t += __shfl_xor(t, 8);
+    if (q == 0 && P < nc) rc[P] = t;
+}
+
+// up: z1 = z + ec[parent] (neighbours' z1 recomputed), t = r - A z1, out = z1 + w D^-1 t
+__global__ __launch_bounds__(B) void k_st_up(st_level L, int Kyc, const float *__restrict__ r,
+                                             const float *__restrict__ z, const float *__restrict__ ec,
+                                             float *__restrict__ out, const lv_cg_scalars *S)
+{
+    if (S && S->done) return;
+    const int f = blockIdx.x * B + threadIdx.x;
+    const int n = L.Kx * L.Ky;
+    if (f >= n) return;
+    const int cx = f / L.Ky, cy = f % L.Ky;
+    const float z1 = z[f] + ec[(cx >> 2) * Kyc + (cy >> 2)];
+    float acc = L.d[f] * z1;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        const float ck = L.c[(size_t)k * n + f];
+        if (ck == 0.0f) continue;
+        int ox, oy;
+        offset_of(k, ox, oy);
+        bool okx, oky;
+        const int jx = wrap(cx + ox, L.Kx, L.xper, okx), jy = wrap(cy + oy, L.Ky, L.yper, oky);
+        if (!okx || !oky) continue;
+        const int J = jx * L.Ky + jy;
+        acc += ck * (z1 - (z[J] + ec[(jx >> 2) * Kyc + (jy >> 2)]));
+    }
+    out[f] = z1 + OMEGA * L.dinv[f] * (r[f] - acc);
+}
+
+// z = inv r on the coarsest level (column-major inverse, one thread per row)
+__global__ __launch_bounds__(256) void k_dense_apply(int nc, const float *__restrict__ inv, const float *__restrict__ r,
+                                                     float *__restrict__ z, const lv_cg_scalars *S)
+{
+    if (S && S->done) return;
+    __shared__ float sr[LV_AMG_DENSE_MAX];
+    for (int j = threadIdx.x; j < nc; j += 256) sr[j] = r[j];
+    __syncthreads();
+    for (int i = threadIdx.x; i < nc; i += 256) {
+        float acc = 0.0f;
+        for (int j = 0; j < nc; j++) acc += inv[(size_t)j * nc + i] * sr[j];
+        z[i] = acc;
+    }
+}
+
+// ---- power iteration for the scale of the learned fine smoother ----------------------------------
+
+__global__ __launch_bounds__(B) void k_pw_init(int n, float *__restrict__ v)
+{
+    const int i = blockIdx.x * B + threadIdx.x;
+    if (i >= n) return;
+    uint64_t z = 0x5EED5EEDull + ((uint64_t)i + 1) * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    v[i] = (z >> 63) ? 1.0f : -1.0f;
+}
+
+// u = M (A v) computed row-wise through the work vector t = A v: two launches
+__global__ __launch_bounds__(B) void k_pw_av(fine_ops F, const float *__restrict__ v, float *__restrict__ t)
+{
+    const int i = blockIdx.x * B + threadIdx.x;
+    if (i < F.n) t[i] = a_row(i, F.n, F.deg, F.ecol, F.ev, F.diag, v);
+}
+// u = M t; partials |u|^2, |v|^2
+__global__ __launch_bounds__(B) void k_pw_mt(fine_ops F, const float *__restrict__ t, const float *__restrict__ v,
+                                             float *__restrict__ u, double *__restrict__ part)
+{
+    double s[2] = {0.0, 0.0};
+    for (int i = blockIdx.x * B + threadIdx.x; i < F.n; i += gridDim.x * B) {
+        const float ui = m_row(i, F.n, F.deg, F.ecol, F.mval, F.mdg, F.dinv, t);
+        u[i] = ui;
+        s[0] += (double)ui * (double)ui;
+        s[1] += (double)v[i] * (double)v[i];
+    }
+    block_partials<2>(s, part, gridDim.x);
+}
+__global__ __launch_bounds__(B) void k_pw_norm(const double *__restrict__ part, int nb, double *__restrict__ out)
+{
+    double o[2];
+    reduce_all<2>(part, nb, o);
+    if (threadIdx.x == 0) {
+        out[0] = o[0];
+        out[1] = o[1];
+    }
+}
+// v = u / |u|
+__global__ __launch_bounds__(B) void k_pw_scale(int n, const float *__restrict__ u, const double *__restrict__ nrm,
+                                                float *__restrict__ v)
+{
+    const int i = blockIdx.x * B + threadIdx.x;
+    if (i < n) v[i] = (float)((double)u[i] / sqrt(nrm[0]));
+}
+
+}  // namespace
+
+#define AMG_CHECK(expr)                  \
+    do {                                 \
+        hipError_t _e = (expr);          \
+        if (_e != hipSuccess) return _e; \
+    } while (0)
+
+template <typename T>
+static hipError_t grow(T **p, size_t *cap, size_t want)
+{
+    if (*cap >= want && *p) return hipSuccess;
+    if (*p) AMG_CHECK(hipFree(*p));
+    *p = nullptr;
+    AMG_CHECK(hipMalloc((void **)p, sizeof(T) * (want ? want : 1)));
+    *cap = want;
+    return hipSuccess;
+}
+
+hipError_t lv_amg_init(lv_amg *G)
+{
+    memset(G, 0, sizeof(*G));
+    AMG_CHECK(hipMalloc(&G->dense_work, sizeof(double) * 2 * LV_AMG_DENSE_MAX * LV_AMG_DENSE_MAX));
+    AMG_CHECK(hipMalloc(&G->dense_inv, sizeof(float) * LV_AMG_DENSE_MAX * LV_AMG_DENSE_MAX));
+    AMG_CHECK(hipMalloc(&G->pw, sizeof(double) * (2 * LV_NBMAX + 2)));
+    return hipSuccess;
+}
+
+void lv_amg_free(lv_amg *G)
+{
+    void *ptrs[] = {G->agg, G->cnt, G->start, G->cursor, G->list, G->scan_tmp, G->t0, G->z0, G->dense_work,
+                    G->dense_inv, G->pw, G->pv, G->pu};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    for (int l = 0; l <= LV_AMG_MAXLEV; l++) {
+        lv_amg_level &L = G->lev[l];
+        void *q[] = {L.d, L.c, L.dinv, L.r, L.z, L.t};
+        for (void *p : q)
+            if (p) (void)hipFree(p);
+    }
+    memset(G, 0, sizeof(*G));
+}
+
+static hipError_t level_alloc(lv_amg_level &L, int Kx, int Ky)
+{
+    L.Kx = Kx;
+    L.Ky = Ky;
+    L.n = Kx * Ky;
+    if (L.cap < (size_t)L.n || !L.d) {
+        void *q[] = {L.d, L.c, L.dinv, L.r, L.z, L.t};
+        for (void *p : q)
+            if (p) AMG_CHECK(hipFree(p));
+        const size_t m = (size_t)L.n;
+        AMG_CHECK(hipMalloc(&L.d, sizeof(float) * m));
+        AMG_CHECK(hipMalloc(&L.c, sizeof(float) * 8 * m));
+        AMG_CHECK(hipMalloc(&L.dinv, sizeof(float) * m));
+        AMG_CHECK(hipMalloc(&L.r, sizeof(float) * m));
+        AMG_CHECK(hipMalloc(&L.z, sizeof(float) * m));
+        AMG_CHECK(hipMalloc(&L.t, sizeof(float) * m));
+        L.cap = m;
+    }
+    return hipSuccess;
+}
+
+static fine_ops fine_of(const lv_amg *G, const lv_pressure_system &A)
+{
+    fine_ops F;
+    F.n = G->n0;
+    F.deg = A.deg;
+    F.ecol = A.ecol;
+    F.ev = A.ev32;
+    F.diag = A.diag32;
+    F.dinv = A.dinv32;
+    F.mval = G->fine_learned ? G->mval : nullptr;
+    F.mdg = G->fine_learned ? G->mdg : nullptr;
+    F.scale = G->fine_scale;
+    return F;
+}
+
+hipError_t lv_amg_setup(lv_amg *G, const lv_grid_params &g, const double *x, const lv_pressure_system &A, int n,
+                        const float *mval, const float *mdg, hipStream_t s)
+{
+    G->n0 = n;
+    G->xper = g.xperiodic;
+    G->yper = g.yperiodic;
+    const double Lx = g.bxmax0 - g.bxmin0, Ly = g.bxmax1 - g.bxmin1;
+    const int Kx = std::max(1, (int)floor(Lx / (4.0 * g.dr))), Ky = std::max(1, (int)floor(Ly / (4.0 * g.dr)));
+    const int na = Kx * Ky;
+    G->na = na;
+    AMG_CHECK(grow(&G->agg, &G->cap_agg, (size_t)n));
+    AMG_CHECK(grow(&G->list, &G->cap_list, (size_t)n));
+    AMG_CHECK(grow(&G->t0, &G->cap_t0, (size_t)n));
+    AMG_CHECK(grow(&G->z0, &G->cap_z0, (size_t)n));
+    AMG_CHECK(grow(&G->cnt, &G->cap_cnt, (size_t)na + 1));
+    AMG_CHECK(grow(&G->start, &G->cap_start, (size_t)na + 1));
+    AMG_CHECK(grow(&G->cursor, &G->cap_cursor, (size_t)na + 1));
+    AMG_CHECK(grow((char **)&G->scan_tmp, &G->cap_scan, lv_scan_temp_bytes((int64_t)na + 1)));
+    AMG_CHECK(hipMemsetAsync(G->cnt, 0, sizeof(int) * (na + 1), s));
+    AMG_CHECK(hipMemsetAsync(G->cursor, 0, sizeof(int) * (na + 1), s));
+    hipLaunchKernelGGL(k_agg_key, g1(n), dim3(B), 0, s, n, x, g.bxmin0, g.bxmin1, Lx / Kx, Ly / Ky, Kx, Ky, G->agg,
+                       G->cnt);
+    AMG_CHECK(lv_exclusive_scan_i32(G->cnt, G->start, (int64_t)na + 1, nullptr, G->scan_tmp, s));
+    hipLaunchKernelGGL(k_agg_fill, g1(n), dim3(B), 0, s, n, (const int *)G->agg, (const int *)G->start, G->cursor,
+                       G->list);
+    AMG_CHECK(level_alloc(G->lev[1], Kx, Ky));
+    hipLaunchKernelGGL(k_coarse1, g1(na), dim3(B), 0, s, na, Kx, Ky, G->xper, G->yper, (const int *)G->start, G->list,
+                       (const int *)G->agg, n, (const int *)A.deg, (const int *)A.ecol, (const double *)A.ev64,
+                       (const double *)A.diag64, G->lev[1].d, G->lev[1].c, G->lev[1].dinv);
+    int l = 1;
+    while (G->lev[l].n > LV_AMG_DENSE_MAX && l < LV_AMG_MAXLEV) {
+        const lv_amg_level &F = G->lev[l];
+        AMG_CHECK(level_alloc(G->lev[l + 1], (F.Kx + 3) / 4, (F.Ky + 3) / 4));
+        lv_amg_level &C = G->lev[l + 1];
+        hipLaunchKernelGGL(k_coarsen, g1(C.n), dim3(B), 0, s, C.Kx, C.Ky, F.Kx, F.Ky, G->xper, G->yper,
+                           (const float *)F.d, (const float *)F.c, C.d, C.c, C.dinv);
+        l++;
+    }
+    if (G->lev[l].n > LV_AMG_DENSE_MAX) return hipErrorInvalidValue;
+    G->nlev = l;
+    const lv_amg_level &D = G->lev[l];
+    hipLaunchKernelGGL(k_dense_inverse, dim3(1), dim3(256), 0, s, D.Kx, D.Ky, G->xper, G->yper, (const float *)D.d,
+                       (const float *)D.c, A.singular, G->dense_work,
+                       G->dense_work + (size_t)LV_AMG_DENSE_MAX * LV_AMG_DENSE_MAX, G->dense_inv);
+    // fine smoother and its scale
+    G->fine_learned = (mval && mdg) ? 1 : 0;
+    G->mval = mval;
+    G->mdg = mdg;
+    G->fine_scale = OMEGA;
+    if (G->fine_learned) {
+        AMG_CHECK(grow(&G->pv, &G->cap_pv, (size_t)n));
+        AMG_CHECK(grow(&G->pu, &G->cap_pu, (size_t)n));
+        G->fine_scale = 1.0f;
+        fine_ops Fo = fine_of(G, A);
+        const int nb = lv_nblocks(n);
+        double *part = G->pw, *nrm = G->pw + 2 * LV_NBMAX;
+        hipLaunchKernelGGL(k_pw_init, g1(n), dim3(B), 0, s, n, G->pv);
+        double h[2] = {0.0, 0.0};
+        for (int it = 0; it < 10; it++) {
+            hipLaunchKernelGGL(k_pw_av, g1(n), dim3(B), 0, s, Fo, (const float *)G->pv, G->t0);
+            hipLaunchKernelGGL(k_pw_mt, dim3(nb), dim3(B), 0, s, Fo, (const float *)G->t0, (const float *)G->pv,
+                               G->pu, part);
+            hipLaunchKernelGGL(k_pw_norm, dim3(1), dim3(B), 0, s, (const double *)part, nb, nrm);
+            hipLaunchKernelGGL(k_pw_scale, g1(n), dim3(B), 0, s, n, (const float *)G->pu, (const double *)nrm,
+                               G->pv);
+        }
+        AMG_CHECK(hipMemcpyAsync(h, nrm, sizeof(h), hipMemcpyDeviceToHost, s));
+        AMG_CHECK(hipStreamSynchronize(s));
+        const double lam = (h[1] > 0.0) ? sqrt(h[0] / h[1]) : 0.0;
+        G->fine_scale = (lam > 0.0 && lam == lam) ? (float)(FINE_TARGET / lam) : 0.0f;
+        if (!(G->fine_scale > 0.0f)) {  // no usable estimate: damped Jacobi
+            G->fine_learned = 0;
+            G->fine_scale = OMEGA;
+        }
+    }
+    return hipGetLastError();
+}
+
+hipError_t lv_amg_vcycle(const lv_amg *G, const lv_pressure_system &A, const float *r, float *z, double *Prr,
+                         const lv_cg_scalars *S, hipStream_t s)
+{
+    const int n = G->n0;
+    const fine_ops F = fine_of(G, A);
+    const lv_amg_level &L1 = G->lev[1];
+    // level 0, down: z0 (in G->z0) and its residual t (in G->t0), restricted to level 1
+    if (G->fine_learned) {
+        hipLaunchKernelGGL(k_fine_pre, g1(n), dim3(B), 0, s, F, r, G->z0, S);
+        hipLaunchKernelGGL(k_fine_resid, g1(n), dim3(B), 0, s, F, r, (const float *)G->z0, G->t0, S);
+    } else {
+        hipLaunchKernelGGL(k_fine_down_j, g1(n), dim3(B), 0, s, F, r, G->z0, G->t0, S);
+    }
+    hipLaunchKernelGGL(k_restrict1, g1(G->na), dim3(B), 0, s, G->na, (const int *)G->start, (const int *)G->list,
+                       (const float *)G->t0, L1.r, S);
+    // coarse levels, down: each level's pre-smoothed z and the next level's right-hand side
+    for (int l = 1; l < G->nlev; l++) {
+        const lv_amg_level &L = G->lev[l], &C = G->lev[l + 1];
+        const st_level sl{L.Kx, L.Ky, G->xper, G->yper, L.d, L.c, L.dinv};
+        hipLaunchKernelGGL(k_st_down, g1((int64_t)C.n * 16), dim3(B), 0, s, sl, C.Ky, C.n, (const float *)L.r, L.z,
+                           C.r, S);
+    }
+    // coarsest level: its correction in t
+    const lv_amg_level &D = G->lev[G->nlev];
+    hipLaunchKernelGGL(k_dense_apply, dim3(1), dim3(256), 0, s, D.n, (const float *)G->dense_inv, (const float *)D.r,
+                       D.t, S);
+    // coarse levels, up: each level's final correction in t
+    for (int l = G->nlev - 1; l >= 1; l--) {
+        const lv_amg_level &L = G->lev[l], &C = G->lev[l + 1];
+        const st_level sl{L.Kx, L.Ky, G->xper, G->yper, L.d, L.c, L.dinv};
+        hipLaunchKernelGGL(k_st_up, g1(L.n), dim3(B), 0, s, sl, C.Ky, (const float *)L.r, (const float *)L.z,
+                           (const float *)C.t, L.t, S);
+    }
+    // level 0, up: prolongation, residual and post-smoothing; partials of r.z, r.r
+    if (G->fine_learned) {
+        hipLaunchKernelGGL(k_fine_up, g1(n), dim3(B), 0, s, F, r, (const float *)G->z0, (const int *)G->agg,
+                           (const float *)L1.t, z, G->t0, S);
+        hipLaunchKernelGGL(k_fine_post, dim3(lv_nblocks(n)), dim3(B), 0, s, F, r, (const float *)G->t0, z, Prr, S);
+    } else {
+        hipLaunchKernelGGL(k_prolong1, g1(n), dim3(B), 0, s, n, (const int *)G->agg, (const float *)L1.t, G->z0, S);
+        hipLaunchKernelGGL(k_fine_up_j, dim3(lv_nblocks(n)), dim3(B), 0, s, F, r, (const float *)G->z0, z, Prr, S);
+    }
+    return hipGetLastError();
+}

@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "lv_spai.h"
+#include "lv_amg.h"
 
 struct lv_ctx {
     lv_grid_params g;
@@ -55,6 +56,11 @@ struct lv_ctx {
     int spai_ok = 0;          // lv_spai_init succeeded
     int64_t assembly_id = 0;  // bumped by every assembly
     int64_t spai_emitted_for = -1;
+    // multilevel preconditioner of the last assembled operator
+    lv_amg amg{};
+    int amg_ok = 0;
+    int64_t amg_built_for = -1;
+    int amg_learned = -1;  // the fine smoother the hierarchy was built with (1 learned, 0 Jacobi)
     char msg[512] = {0};
 };
 
@@ -331,6 +337,10 @@ int lv_create(const lv_grid_desc *d, lv_ctx **out)
         e = lv_spai_init(&ctx->spai);
         ctx->spai_ok = (e == hipSuccess);
     }
+    if (e == hipSuccess) {
+        e = lv_amg_init(&ctx->amg);
+        ctx->amg_ok = (e == hipSuccess);
+    }
     for (int k = 0; k < LV_NKTIMERS && e == hipSuccess; k++) {
         ctx->timers[k] = new lv_timer();
         memset(ctx->timers[k], 0, sizeof(lv_timer));
@@ -359,6 +369,7 @@ int lv_destroy(lv_ctx *ctx)
                     ctx->W.partials, ctx->scan_temp};
     for (void *p : ptrs) dfree(p);
     if (ctx->spai_ok) lv_spai_free(&ctx->spai);
+    if (ctx->amg_ok) lv_amg_free(&ctx->amg);
     for (int k = 0; k < 8; k++) {
         dfree(ctx->pw_q[k]);
         dfree(ctx->pw_z[k]);
@@ -732,17 +743,35 @@ static void default_opts(lv_solve_opts &o)
 }
 
 // the learned preconditioner of the current operator (network forward + emission once per assembly)
+// The preconditioner of LV_SOLVER_SPAI (the learned M) or LV_SOLVER_AMG (the multilevel cycle,
+// with the learned M as fine smoother when parameters are loaded), built once per assembly.
 static int spai_prepare(lv_ctx *ctx, const lv_solve_opts &opt, lv_precond *M, bool *have)
 {
     *have = false;
-    if (opt.method != LV_SOLVER_SPAI || !ctx->spai_ok || !ctx->spai.has_params) return LV_OK;
-    if (ctx->spai_emitted_for != ctx->assembly_id) {
+    const bool amg = (opt.method == LV_SOLVER_AMG);
+    if (opt.method != LV_SOLVER_SPAI && !amg) return LV_OK;
+    const bool learned = ctx->spai_ok && ctx->spai.has_params;
+    if (learned && ctx->spai_emitted_for != ctx->assembly_id) {
         LV_HIP_CHECK(lv_spai_infer(&ctx->spai, ctx->A, ctx->n, ctx->s));
         LV_HIP_CHECK(lv_spai_emit(&ctx->spai, ctx->A, ctx->n, ctx->s));
         ctx->spai_emitted_for = ctx->assembly_id;
     }
-    M->mval = ctx->spai.mval;
-    M->mdg = ctx->spai.mdg;
+    if (!amg) {
+        if (!learned) return LV_OK;
+        M->mval = ctx->spai.mval;
+        M->mdg = ctx->spai.mdg;
+        *have = true;
+        return LV_OK;
+    }
+    if (!ctx->amg_ok) return lv_fail(ctx, LV_E_INVALID, "multilevel preconditioner unavailable");
+    if (ctx->amg_built_for != ctx->assembly_id || ctx->amg_learned != (int)learned) {
+        LV_HIP_CHECK(lv_amg_setup(&ctx->amg, ctx->g, ctx->F.x, ctx->A, ctx->n, learned ? ctx->spai.mval : nullptr,
+                                  learned ? ctx->spai.mdg : nullptr, ctx->s));
+        ctx->amg_built_for = ctx->assembly_id;
+        ctx->amg_learned = (int)learned;
+    }
+    M->amg = &ctx->amg;
+    M->A = &ctx->A;
     *have = true;
     return LV_OK;
 }
@@ -795,6 +824,26 @@ static int solve_on_device(lv_ctx *ctx, const double *db, double *dx, const lv_s
         st->fallbacks += fb;
     }
     return conv ? LV_OK : LV_E_NOT_CONVERGED;
+}
+
+int lv_apply_preconditioner(lv_ctx *ctx, int32_t method, const double *r, double *z)
+{
+    if (!ctx || !r || !z) return LV_E_INVALID;
+    if (!ctx->A.valid) return lv_fail(ctx, LV_E_INVALID, "no assembled pressure operator");
+    if (method != LV_SOLVER_MIXED && method != LV_SOLVER_SPAI && method != LV_SOLVER_AMG)
+        return lv_fail(ctx, LV_E_INVALID, "lv_apply_preconditioner: method has no FP32 preconditioner");
+    lv_solve_opts opt{};
+    opt.method = method;
+    lv_precond M{};
+    bool have = false;
+    int rc = spai_prepare(ctx, opt, &M, &have);
+    if (rc) return rc;
+    const size_t bytes = sizeof(double) * ctx->n;
+    LV_HIP_CHECK(hipMemcpyAsync(ctx->W.r, r, bytes, hipMemcpyHostToDevice, ctx->s));
+    LV_HIP_CHECK(lv_apply_precond(ctx->A, ctx->W, ctx->n, have ? &M : nullptr, ctx->W.r, ctx->W.z, ctx->s));
+    LV_HIP_CHECK(hipMemcpyAsync(z, ctx->W.z, bytes, hipMemcpyDeviceToHost, ctx->s));
+    LV_HIP_CHECK(hipStreamSynchronize(ctx->s));
+    return LV_OK;
 }
 
 int lv_solve_pressure(lv_ctx *ctx, const double *b, double *x, const lv_solve_opts *opts, lv_solve_stats *stats)
@@ -1054,7 +1103,12 @@ int lv_spai_num_params(int64_t *count)
 
 int lv_spai_set_params(lv_ctx *ctx, const float *params)
 {
-    if (!ctx || !params || !ctx->spai_ok) return LV_E_INVALID;
+    if (!ctx || !ctx->spai_ok) return LV_E_INVALID;
+    if (!params) {  // unload: LV_SOLVER_SPAI falls back to Jacobi, LV_SOLVER_AMG smooths with Jacobi
+        ctx->spai.has_params = 0;
+        ctx->spai_emitted_for = -1;
+        return LV_OK;
+    }
     LV_HIP_CHECK(lv_spai_set_params(&ctx->spai, params, ctx->s));
     LV_HIP_CHECK(hipStreamSynchronize(ctx->s));
     ctx->spai_emitted_for = -1;

@@ -1,0 +1,69 @@
+// lv_reduce.h -- deterministic block reductions shared by the solver kernels (256-thread blocks).
+#pragma once
+
+#include "lv_internal.h"
+
+// grid of the reduction kernels: enough 256-thread blocks for ~8 wavefronts per SIMD on 256 CUs,
+// capped so that re-reducing the partials stays cheap.  Every kernel that writes partials for a
+// consumer uses exactly this many blocks.
+constexpr int LV_NBMAX = 2048;
+inline int lv_nblocks(int n)
+{
+    int nb = (n + 255) / 256;
+    return nb < LV_NBMAX ? (nb > 0 ? nb : 1) : LV_NBMAX;
+}
+
+namespace {
+
+// ---- reductions --------------------------------------------------------------------------------
+//
+// A kernel that produces dot products writes one partial sum per block with plain stores.  The
+// kernel that consumes them re-reduces all NB partials in every block, in the same fixed order,
+// so every block obtains bit-identical totals without atomics or device-scope fences (on a
+// multi-XCD GPU a release fence per block costs an L2 write-back).  The kernel boundary makes the
+// partials visible.  Block 0 also stores the totals in lv_cg_scalars for the host and later kernels.
+
+template <int NV>
+__device__ __forceinline__ void block_partials(const double (&v)[NV], double *__restrict__ partials, int nb)
+{
+    __shared__ double sm[NV][256 / 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < NV; q++) {
+        const double s = lv_wave_sum(v[q]);
+        if (lane == 0) sm[q][w] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < NV; q++) {
+            double s = 0.0;
+            for (int k = 0; k < 256 / 64; k++) s += sm[q][k];
+            partials[(size_t)q * nb + blockIdx.x] = s;
+        }
+    }
+}
+
+template <int NV>
+__device__ __forceinline__ void reduce_all(const double *__restrict__ partials, int nb, double (&out)[NV])
+{
+    __shared__ double sm[NV][256 / 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < NV; q++) {
+        double s = 0.0;
+        for (int k = threadIdx.x; k < nb; k += 256) s += partials[(size_t)q * nb + k];
+        s = lv_wave_sum(s);
+        if (lane == 0) sm[q][w] = s;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NV; q++) {
+        double t = 0.0;
+        for (int k = 0; k < 256 / 64; k++) t += sm[q][k];
+        out[q] = t;
+    }
+    __syncthreads();
+}
+
+}  // namespace

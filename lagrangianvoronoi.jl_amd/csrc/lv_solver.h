@@ -49,11 +49,18 @@ void lv_timer_end(lv_timer *t, hipStream_t s);
 void lv_timer_flush(lv_timer *t);
 
 // a preconditioner in the solver's ELL slots: z_i = mdg_i r_i + sum_k mval[k*n + i] r[ecol[k*n + i]]
+struct lv_amg;
+// The inner solve's preconditioner: the learned M in A's ELL slots, or (amg set) the multilevel
+// cycle of lv_amg.hip on the operator A.
 struct lv_precond {
-    const float *mval, *mdg;
+    const float *mval = nullptr, *mdg = nullptr;
+    const lv_amg *amg = nullptr;
+    const lv_pressure_system *A = nullptr;
 };
 
 size_t lv_partials_bytes(int n);
+hipError_t lv_apply_precond(const lv_pressure_system &A, lv_solver_work &W, int n, const lv_precond *M,
+                            const double *r, double *z, hipStream_t s);
 hipError_t lv_launch_degree(lv_mesh_view M, int n, int *deg, hipStream_t s);
 hipError_t lv_launch_assemble(const lv_grid_params &g, lv_fields F, lv_mesh_view M, int n, double dt,
                               lv_pressure_system &A, int *nonsingular, hipStream_t s);

@@ -17,6 +17,8 @@
 #include <string.h>
 
 #include "lv_solver.h"
+#include "lv_reduce.h"
+#include "lv_amg.h"
 
 namespace {
 
@@ -163,57 +165,6 @@ __global__ __launch_bounds__(B) void k_write_P(lv_fields F, int n, const double 
 {
     const int i = blockIdx.x * B + threadIdx.x;
     if (i < n) F.P[i] = P[i];
-}
-
-// ---- reductions --------------------------------------------------------------------------------
-//
-// A kernel that produces dot products writes one partial sum per block with plain stores.  The
-// kernel that consumes them re-reduces all NB partials in every block, in the same fixed order,
-// so every block obtains bit-identical totals without atomics or device-scope fences (on a
-// multi-XCD GPU a release fence per block costs an L2 write-back).  The kernel boundary makes the
-// partials visible.  Block 0 also stores the totals in lv_cg_scalars for the host and later kernels.
-
-template <int NV>
-__device__ __forceinline__ void block_partials(const double (&v)[NV], double *__restrict__ partials, int nb)
-{
-    __shared__ double sm[NV][B / 64];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-#pragma unroll
-    for (int q = 0; q < NV; q++) {
-        const double s = lv_wave_sum(v[q]);
-        if (lane == 0) sm[q][w] = s;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int q = 0; q < NV; q++) {
-            double s = 0.0;
-            for (int k = 0; k < B / 64; k++) s += sm[q][k];
-            partials[(size_t)q * nb + blockIdx.x] = s;
-        }
-    }
-}
-
-template <int NV>
-__device__ __forceinline__ void reduce_all(const double *__restrict__ partials, int nb, double (&out)[NV])
-{
-    __shared__ double sm[NV][B / 64];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-#pragma unroll
-    for (int q = 0; q < NV; q++) {
-        double s = 0.0;
-        for (int k = threadIdx.x; k < nb; k += B) s += partials[(size_t)q * nb + k];
-        s = lv_wave_sum(s);
-        if (lane == 0) sm[q][w] = s;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < NV; q++) {
-        double t = 0.0;
-        for (int k = 0; k < B / 64; k++) t += sm[q][k];
-        out[q] = t;
-    }
-    __syncthreads();
 }
 
 // single-block finalisers of the one-off reductions
@@ -602,14 +553,8 @@ __global__ __launch_bounds__(B) void k_apply_from_residual(int n, const double *
 
 inline dim3 grid_of(int n) { return dim3((n + B - 1) / B); }
 
-// grid of the reduction kernels: enough 256-thread blocks for ~8 wavefronts per SIMD on 256 CUs,
-// capped so that re-reducing the partials stays cheap
-constexpr int NBMAX = 2048;
-inline int nblocks_of(int n)
-{
-    int nb = (n + B - 1) / B;
-    return nb < NBMAX ? (nb > 0 ? nb : 1) : NBMAX;
-}
+constexpr int NBMAX = LV_NBMAX;
+inline int nblocks_of(int n) { return lv_nblocks(n); }
 
 }  // namespace
 
@@ -697,8 +642,14 @@ hipError_t cg_iterate(int n, const int *deg, const int *ecol, const T *ev, const
             if (M) {
                 hipLaunchKernelGGL((k_cg_u<T, false>), dim3(nb), dim3(B), 0, s, n, x, r, z, (const T *)p,
                                    (const T *)q, dinv, (const double *)P.pq, P.rr, dS, par);
-                hipLaunchKernelGGL(k_apply_m32, dim3(nb), dim3(B), 0, s, n, deg, ecol, M->mval, M->mdg,
-                                   (const float *)r, (float *)z, P.rr, (const lv_cg_scalars *)dS);
+                if (M->amg) {
+                    hipError_t e = lv_amg_vcycle(M->amg, *M->A, (const float *)r, (float *)z, P.rr,
+                                                 (const lv_cg_scalars *)dS, s);
+                    if (e != hipSuccess) return e;
+                } else {
+                    hipLaunchKernelGGL(k_apply_m32, dim3(nb), dim3(B), 0, s, n, deg, ecol, M->mval, M->mdg,
+                                       (const float *)r, (float *)z, P.rr, (const lv_cg_scalars *)dS);
+                }
             } else {
                 hipLaunchKernelGGL((k_cg_u<T, true>), dim3(nb), dim3(B), 0, s, n, x, r, z, (const T *)p,
                                    (const T *)q, dinv, (const double *)P.pq, P.rr, dS, par);
@@ -767,17 +718,21 @@ hipError_t lv_solve_device(const lv_pressure_system &A, lv_solver_work &W, int n
             double need = 0.3 * sqrt(h.thr2 / h.rr64);
             double itol = need > floor_tol ? need : floor_tol;
             if (itol > 0.5) itol = 0.5;
-            const lv_precond *Mi = (opt.method == LV_SOLVER_SPAI) ? M : nullptr;
+            const lv_precond *Mi = (opt.method == LV_SOLVER_SPAI || opt.method == LV_SOLVER_AMG) ? M : nullptr;
             for (int attempt = 0; attempt < 2; attempt++) {
                 hipLaunchKernelGGL(k_cg32_init, G, dim3(B), 0, s, n, (const double *)W.r, (const float *)A.dinv32,
                                    W.e32, W.r32, W.z32, P.rr, (const lv_cg_scalars *)W.S);
                 // k_finalize2 clears the done flag of the previous sweep, so it precedes the first
                 // application of the learned preconditioner (which is skipped while done is set)
                 hipLaunchKernelGGL(k_finalize2, dim3(1), dim3(B), 0, s, (const double *)P.rr, nb, itol * itol, W.S);
-                if (Mi)
+                if (Mi && Mi->amg) {
+                    e = lv_amg_vcycle(Mi->amg, A, (const float *)W.r32, W.z32, P.rr, (const lv_cg_scalars *)W.S, s);
+                    if (e != hipSuccess) return e;
+                } else if (Mi) {
                     hipLaunchKernelGGL(k_apply_m32, G, dim3(B), 0, s, n, (const int *)A.deg, (const int *)A.ecol,
                                        Mi->mval, Mi->mdg, (const float *)W.r32, W.z32, P.rr,
                                        (const lv_cg_scalars *)W.S);
+                }
                 e = cg_iterate<float>(n, A.deg, A.ecol, A.ev32, A.diag32, A.dinv32, W.e32, W.r32, W.z32, W.p32, W.q32,
                                       P, W.S, s, t32, batch, h, Mi);
                 if (e != hipSuccess) return e;
@@ -813,6 +768,54 @@ hipError_t lv_solve_device(const lv_pressure_system &A, lv_solver_work &W, int n
     *relres = (h.bb > 0.0) ? sqrt(h.rr64 / h.bb) : sqrt(h.rr64);
     *converged = (h.rr64 <= h.thr2);
     return hipSuccess;
+}
+
+// z = M r for the inner solve's preconditioner (multilevel cycle, learned M, or Jacobi), applied in
+// FP32 to FP64 device vectors
+namespace {
+
+__global__ void k_set_running(lv_cg_scalars *S)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) S->done = 0;
+}
+__global__ __launch_bounds__(B) void k_to32(int n, const double *__restrict__ a, float *__restrict__ b)
+{
+    const int i = blockIdx.x * B + threadIdx.x;
+    if (i < n) b[i] = (float)a[i];
+}
+__global__ __launch_bounds__(B) void k_to64(int n, const float *__restrict__ a, double *__restrict__ b)
+{
+    const int i = blockIdx.x * B + threadIdx.x;
+    if (i < n) b[i] = (double)a[i];
+}
+__global__ __launch_bounds__(B) void k_jacobi32(int n, const float *__restrict__ dinv, const float *__restrict__ r,
+                                                float *__restrict__ z)
+{
+    const int i = blockIdx.x * B + threadIdx.x;
+    if (i < n) z[i] = dinv[i] * r[i];
+}
+
+}  // namespace
+
+hipError_t lv_apply_precond(const lv_pressure_system &A, lv_solver_work &W, int n, const lv_precond *M,
+                            const double *r, double *z, hipStream_t s)
+{
+    const parts P = parts_of(W.partials);
+    hipLaunchKernelGGL(k_to32, grid_of(n), dim3(B), 0, s, n, r, W.r32);
+    hipLaunchKernelGGL(k_set_running, dim3(1), dim3(64), 0, s, W.S);
+    if (M && M->amg) {
+        hipError_t e = lv_amg_vcycle(M->amg, A, (const float *)W.r32, W.z32, P.rr, (const lv_cg_scalars *)W.S, s);
+        if (e != hipSuccess) return e;
+    } else if (M && M->mval) {
+        hipLaunchKernelGGL(k_apply_m32, dim3(nblocks_of(n)), dim3(B), 0, s, n, (const int *)A.deg,
+                           (const int *)A.ecol, M->mval, M->mdg, (const float *)W.r32, W.z32, P.rr,
+                           (const lv_cg_scalars *)W.S);
+    } else {
+        hipLaunchKernelGGL(k_jacobi32, grid_of(n), dim3(B), 0, s, n, (const float *)A.dinv32, (const float *)W.r32,
+                           W.z32);
+    }
+    hipLaunchKernelGGL(k_to64, grid_of(n), dim3(B), 0, s, n, (const float *)W.z32, z);
+    return hipGetLastError();
 }
 
 // ---- host helpers for the Picard extrapolation ------------------------------------------------

@@ -63,6 +63,10 @@ def load_params(path=PARAMS_PATH):
 
 
 def set_params(grid, theta):
+    """Loads the network's parameters into the grid's context; None unloads them."""
+    if theta is None:
+        _check(grid.ctx, lib().lv_spai_set_params(grid.ctx, None))
+        return
     th = np.ascontiguousarray(np.asarray(theta, dtype=np.float32))
     assert th.size == NPARAM
     _check(grid.ctx, lib().lv_spai_set_params(grid.ctx, _ptr(th)))

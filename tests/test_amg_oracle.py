@@ -1,0 +1,71 @@
+"""Pins of the multilevel-preconditioner oracle (oracle/amg_ref.py) on the CPU.
+
+The cycle has no reference counterpart, so the oracle is pinned by what the construction fixes:
+the coarse operators are Galerkin products P^T A P of the piecewise-constant aggregation (checked
+against an explicit sparse product), the V(1,1) cycle is a symmetric positive-definite operator
+(checked densely), and as a CG preconditioner it cuts the iterations of Jacobi-PCG several times
+on the benchmark's regime, with a periodic and with a singular walled system.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import amg_ref
+from pairs import domain, lattice, initial_fields
+
+
+def system(lv, oracle, kind, K, ic, dt, seed=0x5EED0400):
+    xmin, xmax, xp, yp = domain(kind)
+    x, dr = lattice(lv, kind, K, seed)
+    g = oracle.Grid(xmin, xmax, dr, xperiodic=xp, yperiodic=yp)
+    p = oracle.Particles(g, x)
+    p.remesh()
+    for k, v in initial_fields(lv, ic, x, p.area()).items():
+        p.set(k, v)
+    if ic == "tg":
+        p.eos_stiffened(1.4, 2500.0 / 1.4)
+    else:
+        p.set("c2", np.full(x.size // 2, np.inf))
+    diag, coef = p.pressure_operator(dt)
+    d, rp, cols, c = p.csr(diag, coef)
+    singular = not np.any(d > 0)
+    H = amg_ref.Hierarchy(x, d, rp, cols, c, xmin, xmax, dr, xp, yp, singular)
+    return H, p, d, singular
+
+
+def test_coarse_operators_are_galerkin_products(lv, oracle):
+    H, p, d, singular = system(lv, oracle, "periodic", 128, "tg", 8e-3)
+    n = H.n
+    P1 = sp.csr_matrix((np.ones(n), (np.arange(n), H.agg)), shape=(n, H.levels[0].n))
+    G1 = (P1.T @ H.A @ P1).toarray()
+    assert np.abs(G1 - H.levels[0].A.toarray()).max() <= 1e-12 * np.abs(G1).max()
+    L, C = H.levels[0], H.levels[1]
+    f = np.arange(L.n)
+    par = (f // L.Ky // 4) * C.Ky + (f % L.Ky) // 4
+    P2 = sp.csr_matrix((np.ones(L.n), (f, par)), shape=(L.n, C.n))
+    G2 = (P2.T @ L.A @ P2).toarray()
+    assert np.abs(G2 - C.A.toarray()).max() <= 1e-12 * np.abs(G2).max()
+
+
+@pytest.mark.parametrize("kind,ic", [("periodic", "tg"), ("walls", "cavity")])
+def test_cycle_is_symmetric_positive_definite(lv, oracle, kind, ic):
+    H, p, d, singular = system(lv, oracle, kind, 24, ic, 8e-2)
+    n = H.n
+    Bm = np.stack([H.apply(e) for e in np.eye(n)], axis=1)
+    assert np.abs(Bm - Bm.T).max() <= 1e-10 * np.abs(Bm).max()
+    ev = np.linalg.eigvalsh(0.5 * (Bm + Bm.T))
+    assert ev.min() > 0.0
+
+
+@pytest.mark.parametrize("kind,ic,K,dt", [("periodic", "tg", 128, 4e-3), ("walls", "cavity", 96, 1e-3)])
+def test_cycle_cuts_cg_iterations(lv, oracle, kind, ic, K, dt):
+    H, p, d, singular = system(lv, oracle, kind, K, ic, dt)
+    rng = np.random.default_rng(7)
+    b = rng.standard_normal(H.n)
+    if singular:
+        b -= b.mean()
+    _, it_j = amg_ref.pcg(H.A, b, lambda r: H.Dinv * r, 1e-8)
+    x, it_m = amg_ref.pcg(H.A, b, H.apply, 1e-8)
+    assert it_j > 0 and it_m > 0
+    assert it_m <= 0.2 * it_j, (it_m, it_j)
+    assert np.linalg.norm(b - H.A @ x) <= 1e-8 * np.linalg.norm(b)

@@ -121,11 +121,11 @@ def solve_compare(lv, grid, ic, dt, tol, inner_tol, repeats=3):
     return out
 
 
-def mode_solve(lv, K, method, tol, inner_tol):
+def mode_solve(lv, K, method, tol, inner_tol, smoother):
     """One first-Picard pressure solve of the benchmark flow (EOS pressure as initial guess)."""
     dt = 2.5e-4
     grid, x, ic = setup(lv, K)
-    if method in ("spai", "amg"):
+    if method == "spai" or (method == "amg" and smoother == "learned"):
         lv.spai.set_params(grid, spai_theta(lv)[0])
     lv.stiffened_eos(grid, ic["gamma"], ic["P0"])
     solver = lv.PressureSolver(grid, method=method, tol=tol, itmax=2000, inner_tol=inner_tol)
@@ -208,7 +208,7 @@ def main():
 
     lv = load_package()
     if args.mode == "solve":
-        return mode_solve(lv, args.K, args.solver, args.tol, args.inner_tol)
+        return mode_solve(lv, args.K, args.solver, args.tol, args.inner_tol, args.smoother)
     K = args.K
     n = K * K
     dt = 2.5e-4

@@ -336,28 +336,25 @@ __global__ __launch_bounds__(B) void k_fine_resid(fine_ops F, const float *__res
     t[i] = r[i] - a_row(i, F.n, F.deg, F.ecol, F.ev, F.diag, z);
 }
 
-// rc[I] = sum over the (sorted) rows of cell I of t
+// rc[I] = sum over the (sorted) rows of cell I of t: 16 lanes per cell read its row list
+// contiguously and reduce by a fixed shuffle tree
 __global__ __launch_bounds__(B) void k_restrict1(int na, const int *__restrict__ start, const int *__restrict__ list,
                                                  const float *__restrict__ t, float *__restrict__ rc,
                                                  const lv_cg_scalars *S)
 {
     if (S && S->done) return;
-    const int I = blockIdx.x * B + threadIdx.x;
-    if (I >= na) return;
-    const int s0 = start[I], s1 = start[I + 1];
+    const int tid = blockIdx.x * B + threadIdx.x;
+    const int I = tid >> 4, q = tid & 15;
     float s = 0.0f;
-    // eight row indices, then their eight gathers, in flight at once
-    for (int a0 = s0; a0 < s1; a0 += 8) {
-        int idx[8];
-#pragma unroll
-        for (int k = 0; k < 8; k++) idx[k] = (a0 + k < s1) ? list[a0 + k] : -1;
-        float v[8];
-#pragma unroll
-        for (int k = 0; k < 8; k++) v[k] = (idx[k] >= 0) ? t[idx[k]] : 0.0f;
-#pragma unroll
-        for (int k = 0; k < 8; k++) s += v[k];
+    if (I < na) {
+        const int s1 = start[I + 1];
+        for (int a = start[I] + q; a < s1; a += 16) s += t[list[a]];
     }
-    rc[I] = s;
+    s += __shfl_xor(s, 1);
+    s += __shfl_xor(s, 2);
+    s += __shfl_xor(s, 4);
+    s += __shfl_xor(s, 8);
+    if (q == 0 && I < na) rc[I] = s;
 }
 
 // Jacobi fine smoother, down: z0 = w D^-1 r and t = r - A z0 in one pass (z0 of a neighbour is
@@ -830,7 +827,8 @@ hipError_t lv_amg_vcycle(const lv_amg *G, const lv_pressure_system &A, const flo
     } else {
         hipLaunchKernelGGL(k_fine_down_j, g1(n), dim3(B), 0, s, F, r, G->z0, G->t0, S);
     }
-    hipLaunchKernelGGL(k_restrict1, g1(G->na), dim3(B), 0, s, G->na, (const int *)G->start, (const int *)G->list,
+    hipLaunchKernelGGL(k_restrict1, g1((int64_t)G->na * 16), dim3(B), 0, s, G->na, (const int *)G->start,
+                       (const int *)G->list,
                        (const float *)G->t0, L1.r, S);
     // coarse levels, down: each level's pre-smoothed z and the next level's right-hand side
     for (int l = 1; l < G->nlev; l++) {

@@ -98,6 +98,8 @@ typedef struct {
     int32_t total_iters;
     int32_t fallbacks;    /* SPAI solves that fell back to Jacobi */
     int32_t nsolves;
+    int32_t multiphase_iters;   /* lv_step_fused with multiphase: CG iterations (-1: itmax reached) */
+    double multiphase_relres;   /* ... and its final |r| / |b| */
 } lv_solve_stats;
 
 /* One step!(sim, t) of tests/taylorgreen.jl:61-71 and its relatives (examples/cavity.jl:70-80,
@@ -118,6 +120,10 @@ typedef struct {
     double friction_vbc[8];
     double alpha;         /* find_dv!(dt, alpha) */
     int32_t rusanov;      /* relaxation_step!(dt; rusanov) */
+    int32_t multiphase;   /* 1: multiphase_projection! after find_dv! (examples/rayleightaylor.jl:99) */
+    double multiphase_quality; /* MultiphaseSolver(grid; quality_threshold), reference default 0.25 */
+    double multiphase_tol;     /* atol = rtol of its solve (the reference's 1e-4) */
+    int32_t multiphase_itmax;  /* its iteration cap (the reference's 200) */
 } lv_step_desc;
 
 /* ---- context ---------------------------------------------------------------------------------- */
@@ -170,6 +176,17 @@ int lv_find_dv(lv_ctx *ctx, double dt, double alpha);         /* src/relaxation.
 int lv_relaxation_step(lv_ctx *ctx, double dt, int32_t rusanov); /* src/relaxation.jl:36-73 (remeshes) */
 /* the whole step!, device resident; stats may be NULL */
 int lv_step_fused(lv_ctx *ctx, const lv_step_desc *desc, lv_solve_stats *stats);
+
+/* multiphase_projection!(MultiphaseSolver(grid; quality_threshold)), src/relaxation.jl:190-206:
+ * projects dv so that no area crosses a phase boundary.  The reference's minres!(atol = rtol = 1e-4,
+ * itmax = 200) is CG with the same stopping rule here; iters = -1 when itmax stopped it (the update
+ * is applied all the same, as the reference does after its warning). */
+int lv_multiphase_projection(lv_ctx *ctx, double quality_threshold, double atol, double rtol, int32_t itmax,
+                             int32_t *iters, double *relres);
+/* refresh!(::MultiphaseSolver) (src/relaxation.jl:165-180) and mul!(res, ::MultiphaseProjector, x)
+ * (:92-125) on host vectors of n doubles (zero rows off the interface) */
+int lv_multiphase_rhs(lv_ctx *ctx, double *b);
+int lv_multiphase_apply(lv_ctx *ctx, const double *x, double *y);
 
 /* ---- pressure system access (PressureOperator/PressureSolver, src/pressure.jl:89-203) ---------- */
 /* refresh!(A, grid, dt), src/pressure.jl:104-117, into device CSR; returns nnz of the off-diagonal */

@@ -70,14 +70,16 @@ class SolveOpts(ctypes.Structure):
 class SolveStats(ctypes.Structure):
     _fields_ = [("iters", ctypes.c_int32 * 32), ("outer", ctypes.c_int32 * 32), ("relres", ctypes.c_double * 32),
                 ("relres0", ctypes.c_double * 32), ("solve_ms", ctypes.c_double * 32), ("total_iters", ctypes.c_int32),
-                ("fallbacks", ctypes.c_int32), ("nsolves", ctypes.c_int32)]
+                ("fallbacks", ctypes.c_int32), ("nsolves", ctypes.c_int32), ("multiphase_iters", ctypes.c_int32),
+                ("multiphase_relres", ctypes.c_double)]
 
     def as_dict(self):
         k = min(self.nsolves, 32)
         return dict(iters=list(self.iters[:k]), outer=list(self.outer[:k]), relres=list(self.relres[:k]),
                     relres0=list(self.relres0[:k]),
                     solve_ms=list(self.solve_ms[:k]), total_iters=self.total_iters, fallbacks=self.fallbacks,
-                    nsolves=self.nsolves)
+                    nsolves=self.nsolves, multiphase_iters=self.multiphase_iters,
+                    multiphase_relres=self.multiphase_relres)
 
 
 class StepDesc(ctypes.Structure):
@@ -85,7 +87,9 @@ class StepDesc(ctypes.Structure):
                 ("P0", ctypes.c_double), ("Pmin", ctypes.c_double), ("gravity", ctypes.c_int32),
                 ("g", ctypes.c_double * 2), ("niter", ctypes.c_int32), ("solve", SolveOpts),
                 ("viscous", ctypes.c_int32), ("artificial_viscosity", ctypes.c_int32), ("friction", ctypes.c_int32),
-                ("friction_vbc", ctypes.c_double * 8), ("alpha", ctypes.c_double), ("rusanov", ctypes.c_int32)]
+                ("friction_vbc", ctypes.c_double * 8), ("alpha", ctypes.c_double), ("rusanov", ctypes.c_int32),
+                ("multiphase", ctypes.c_int32), ("multiphase_quality", ctypes.c_double),
+                ("multiphase_tol", ctypes.c_double), ("multiphase_itmax", ctypes.c_int32)]
 
 
 _P = ctypes.c_void_p
@@ -126,6 +130,9 @@ _SIGNATURES = {
     "lv_pressure_rhs": [_P, _D, _I, _P, _P, _P],
     "lv_solve_pressure": [_P, _P, _P, ctypes.POINTER(SolveOpts), ctypes.POINTER(SolveStats)],
     "lv_apply_preconditioner": [_P, _I, _P, _P],
+    "lv_multiphase_projection": [_P, _D, _D, _D, _I, _P, _P],
+    "lv_multiphase_rhs": [_P, _P],
+    "lv_multiphase_apply": [_P, _P, _P],
     "lv_spai_num_params": [_P],
     "lv_spai_set_params": [_P, _P],
     "lv_spai_get_params": [_P, _P],
@@ -486,7 +493,10 @@ def step_fused(grid, desc, stats=True):
 
 def make_step_desc(dt, eos=None, gamma=1.4, P0=0.0, Pmin=0.0, gravity=None, niter=10, solver=None,
                    viscous=True, artificial_viscosity=False, friction=None, alpha=1.0, rusanov=True,
-                   boundary_velocity=None):
+                   boundary_velocity=None, multiphase=None):
+    """The step! of tests/taylorgreen.jl:61-71 and its relatives as one device call; multiphase = a
+    MultiphaseSolver (or its quality_threshold) runs multiphase_projection! after find_dv!
+    (examples/rayleightaylor.jl:99)."""
     d = StepDesc()
     d.dt = float(dt)
     d.eos = {None: 0, "stiffened": 1, "ideal": 2}[eos]
@@ -501,7 +511,53 @@ def make_step_desc(dt, eos=None, gamma=1.4, P0=0.0, Pmin=0.0, gravity=None, nite
     d.friction_vbc[:] = _vbc_table(friction)
     d.alpha = float(alpha)
     d.rusanov = int(bool(rusanov))
+    if multiphase is None:
+        d.multiphase, d.multiphase_quality, d.multiphase_tol, d.multiphase_itmax = 0, 0.25, 1e-4, 200
+    else:
+        d.multiphase = 1
+        d.multiphase_quality = float(getattr(multiphase, "quality_threshold", multiphase))
+        d.multiphase_tol = float(getattr(multiphase, "atol", 1e-4))
+        d.multiphase_itmax = int(getattr(multiphase, "itmax", 200))
     return d
+
+
+class MultiphaseSolver:
+    """MultiphaseSolver(grid; quality_threshold = 0.25), src/relaxation.jl:134-163.  The linear solve
+    of multiphase_projection! uses the reference's tolerances (atol = rtol = 1e-4, itmax = 200)."""
+
+    def __init__(self, grid, quality_threshold=0.25, atol=1e-4, rtol=1e-4, itmax=200):
+        self.grid = grid
+        self.quality_threshold = float(quality_threshold)
+        self.atol, self.rtol, self.itmax = float(atol), float(rtol), int(itmax)
+        self.last_stats = None
+
+    def rhs(self):
+        """refresh!(solver), src/relaxation.jl:165-180 (zero off the interface)"""
+        b = np.empty(self.grid.n)
+        _check(self.grid.ctx, lib().lv_multiphase_rhs(self.grid.ctx, _ptr(b)))
+        return b
+
+    def apply(self, x):
+        """mul!(res, MultiphaseProjector(grid), x), src/relaxation.jl:92-125"""
+        x = np.ascontiguousarray(np.asarray(x, dtype=np.float64))
+        y = np.empty_like(x)
+        _check(self.grid.ctx, lib().lv_multiphase_apply(self.grid.ctx, _ptr(x), _ptr(y)))
+        return y
+
+
+def multiphase_projection(solver):
+    """multiphase_projection!(solver), src/relaxation.jl:190-206.  Returns (iterations, |r|/|b|);
+    iterations = -1 when itmax stopped the solve (the reference warns and applies it anyway)."""
+    it, rel = ctypes.c_int32(), ctypes.c_double()
+    _check(solver.grid.ctx, lib().lv_multiphase_projection(solver.grid.ctx, solver.quality_threshold, solver.atol,
+                                                           solver.rtol, solver.itmax, ctypes.byref(it),
+                                                           ctypes.byref(rel)))
+    solver.last_stats = dict(iters=it.value, relres=rel.value)
+    if it.value < 0:
+        import warnings
+
+        warnings.warn("multiphase projector did not converge within tolerance")
+    return it.value, rel.value
 
 
 from . import spai  # noqa: E402,F401  (learned preconditioner helpers)

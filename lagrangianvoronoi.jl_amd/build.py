@@ -13,7 +13,7 @@ CSRC = os.path.join(PKG_DIR, "csrc")
 LIB_PATH = os.path.join(PKG_DIR, "liblvhip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["lv_scan.hip", "lv_celllist.hip", "lv_clip.hip", "lv_stencil.hip", "lv_solver.hip", "lv_spai.hip", "lv_amg.hip",
+SOURCES = ["lv_scan.hip", "lv_celllist.hip", "lv_clip.hip", "lv_stencil.hip", "lv_solver.hip", "lv_spai.hip", "lv_amg.hip", "lv_multiphase.hip",
            "lv_api.hip"]
 
 HIP_FLAGS = [

@@ -15,6 +15,17 @@
 #include "lv_spai.h"
 #include "lv_amg.h"
 
+struct lv_mph_work;
+hipError_t lv_mph_alloc(lv_mph_work **out, int cap);
+void lv_mph_free(lv_mph_work *w);
+hipError_t lv_mph_projection(lv_mph_work *w, const lv_grid_params &g, lv_fields F, lv_mesh_view M, int n,
+                             double quality_threshold, double atol, double rtol, int itmax, int *iters, double *relres,
+                             hipStream_t s);
+hipError_t lv_mph_apply(lv_mph_work *w, const lv_grid_params &g, lv_fields F, lv_mesh_view M, int n, const double *x,
+                        double *y, hipStream_t s);
+hipError_t lv_mph_rhs(lv_mph_work *w, const lv_grid_params &g, lv_fields F, lv_mesh_view M, int n, double *b,
+                      hipStream_t s);
+
 struct lv_ctx {
     lv_grid_params g;
     int cap = 0, n = 0;
@@ -61,6 +72,8 @@ struct lv_ctx {
     int amg_ok = 0;
     int64_t amg_built_for = -1;
     int amg_learned = -1;  // the fine smoother the hierarchy was built with (1 learned, 0 Jacobi)
+    // multiphase projection work space (allocated on first use)
+    lv_mph_work *mph = nullptr;
     char msg[512] = {0};
 };
 
@@ -370,6 +383,7 @@ int lv_destroy(lv_ctx *ctx)
     for (void *p : ptrs) dfree(p);
     if (ctx->spai_ok) lv_spai_free(&ctx->spai);
     if (ctx->amg_ok) lv_amg_free(&ctx->amg);
+    lv_mph_free(ctx->mph);
     for (int k = 0; k < 8; k++) {
         dfree(ctx->pw_q[k]);
         dfree(ctx->pw_z[k]);
@@ -1059,6 +1073,56 @@ int lv_relaxation_step(lv_ctx *ctx, double dt, int32_t rusanov)
     return lv_remesh(ctx);  // src/relaxation.jl:72
 }
 
+// ---- multiphase projection (src/relaxation.jl:75-206) -------------------------------------------
+
+static int mph_ready(lv_ctx *ctx)
+{
+    int rc = check_ctx(ctx, true);
+    if (rc) return rc;
+    if (!ctx->mph) LV_HIP_CHECK(lv_mph_alloc(&ctx->mph, ctx->cap));
+    return LV_OK;
+}
+
+int lv_multiphase_projection(lv_ctx *ctx, double quality_threshold, double atol, double rtol, int32_t itmax,
+                             int32_t *iters, double *relres)
+{
+    int rc = mph_ready(ctx);
+    if (rc) return rc;
+    if (itmax < 1 || !(atol >= 0.0) || !(rtol >= 0.0)) return lv_fail(ctx, LV_E_INVALID, "multiphase options");
+    int it = 0;
+    double rel = 0.0;
+    LV_HIP_CHECK(lv_mph_projection(ctx->mph, ctx->g, ctx->F, mesh_view(ctx), ctx->n, quality_threshold, atol, rtol,
+                                   itmax, &it, &rel, ctx->s));
+    LV_HIP_CHECK(hipStreamSynchronize(ctx->s));
+    if (iters) *iters = it;
+    if (relres) *relres = rel;
+    return LV_OK;
+}
+
+int lv_multiphase_rhs(lv_ctx *ctx, double *b)
+{
+    int rc = mph_ready(ctx);
+    if (rc) return rc;
+    if (!b) return LV_E_INVALID;
+    LV_HIP_CHECK(lv_mph_rhs(ctx->mph, ctx->g, ctx->F, mesh_view(ctx), ctx->n, ctx->W.z, ctx->s));
+    LV_HIP_CHECK(hipMemcpyAsync(b, ctx->W.z, sizeof(double) * ctx->n, hipMemcpyDeviceToHost, ctx->s));
+    LV_HIP_CHECK(hipStreamSynchronize(ctx->s));
+    return LV_OK;
+}
+
+int lv_multiphase_apply(lv_ctx *ctx, const double *x, double *y)
+{
+    int rc = mph_ready(ctx);
+    if (rc) return rc;
+    if (!x || !y) return LV_E_INVALID;
+    const size_t bytes = sizeof(double) * ctx->n;
+    LV_HIP_CHECK(hipMemcpyAsync(ctx->W.r, x, bytes, hipMemcpyHostToDevice, ctx->s));
+    LV_HIP_CHECK(lv_mph_apply(ctx->mph, ctx->g, ctx->F, mesh_view(ctx), ctx->n, ctx->W.r, ctx->W.z, ctx->s));
+    LV_HIP_CHECK(hipMemcpyAsync(y, ctx->W.z, bytes, hipMemcpyDeviceToHost, ctx->s));
+    LV_HIP_CHECK(hipStreamSynchronize(ctx->s));
+    return LV_OK;
+}
+
 int lv_step_fused(lv_ctx *ctx, const lv_step_desc *d, lv_solve_stats *stats)
 {
     if (!d) return LV_E_INVALID;
@@ -1087,6 +1151,17 @@ int lv_step_fused(lv_ctx *ctx, const lv_step_desc *d, lv_solve_stats *stats)
     }
     rc = lv_find_dv(ctx, d->dt, d->alpha);
     if (rc) return rc;
+    if (d->multiphase) {  // multiphase_projection!(MultiphaseSolver(grid; quality_threshold)), defaults of :197
+        int it = 0;
+        double rel = 0.0;
+        rc = lv_multiphase_projection(ctx, d->multiphase_quality, d->multiphase_tol, d->multiphase_tol,
+                                      d->multiphase_itmax > 0 ? d->multiphase_itmax : 200, &it, &rel);
+        if (rc) return rc;
+        if (stats) {
+            stats->multiphase_iters = it;
+            stats->multiphase_relres = rel;
+        }
+    }
     rc = lv_relaxation_step(ctx, d->dt, d->rusanov);
     if (rc) return rc;
     return prc;

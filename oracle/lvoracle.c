@@ -1165,3 +1165,154 @@ int lvo_relaxation_step(const lvo_grid *g, lvo_state *s, double dt, int rusanov)
     }
     return 0;
 }
+
+/* ------------------------------------------------------------------------------------------------
+ * Multiphase projection (src/relaxation.jl:75-206): MultiphaseProjector, refresh!(MultiphaseSolver)
+ * and the dv update of multiphase_projection!.  Only edges between particles of different phases
+ * contribute.  Krylov.jl's minres! (atol = rtol = 1e-4, itmax = 200) is replaced by the same
+ * unpreconditioned CG as the product, run to the caller's tolerance.
+ * ---------------------------------------------------------------------------------------------- */
+
+/* refresh!(solver::MultiphaseSolver), src/relaxation.jl:165-180 */
+int lvo_multiphase_rhs(const lvo_grid *g, const lvo_state *s, double *b)
+{
+    int n = s->n;
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; i++) {
+        double x1 = s->x[i], x2 = s->x[n + i];
+        double d1 = s->dv[i], d2 = s->dv[n + i];
+        double bi = 0.0;
+        for (int k = 0; k < s->ne[i]; k++) {
+            int j = EL(s, i, k);
+            if (j < 0 || s->phase[j] == s->phase[i]) continue;
+            const double *e = EV(s, i, k);
+            double y1, y2;
+            nbr_y(g, s, i, j, &y1, &y2);
+            double lrr = lr_ratio(x1 - y1, x2 - y2, e);
+            double m1 = 0.5 * (e[0] + e[2]), m2 = 0.5 * (e[1] + e[3]);
+            double z1 = 0.5 * (x1 + y1), z2 = 0.5 * (x2 + y2);
+            double q1 = s->dv[j], q2 = s->dv[n + j];
+            double a = (d1 - q1) * (m1 - z1) + (d2 - q2) * (m2 - z2);
+            double c = (d1 + q1) * (x1 - y1) + (d2 + q2) * (x2 - y2);
+            bi -= lrr * (a - 0.5 * c);
+        }
+        b[i] = bi;
+    }
+    return 0;
+}
+
+/* mul!(res, A::MultiphaseProjector, x), src/relaxation.jl:92-125: T = the interface gradient of x
+ * per unit area (2n scratch), then res */
+int lvo_multiphase_apply(const lvo_grid *g, const lvo_state *s, const double *x, double *T, double *res)
+{
+    int n = s->n;
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; i++) {
+        double x1 = s->x[i], x2 = s->x[n + i];
+        double t1 = 0.0, t2 = 0.0;
+        for (int k = 0; k < s->ne[i]; k++) {
+            int j = EL(s, i, k);
+            if (j < 0 || s->phase[j] == s->phase[i]) continue;
+            const double *e = EV(s, i, k);
+            double y1, y2;
+            nbr_y(g, s, i, j, &y1, &y2);
+            double lrr = lr_ratio(x1 - y1, x2 - y2, e);
+            double m1 = 0.5 * (e[0] + e[2]), m2 = 0.5 * (e[1] + e[3]);
+            double w = lrr * (x[i] - x[j]);
+            t1 -= w * (m1 - x1);
+            t2 -= w * (m2 - x2);
+        }
+        double A = poly_area(s, i);
+        T[i] = t1 / A;
+        T[n + i] = t2 / A;
+    }
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; i++) {
+        double x1 = s->x[i], x2 = s->x[n + i];
+        double ri = 0.0;
+        for (int k = 0; k < s->ne[i]; k++) {
+            int j = EL(s, i, k);
+            if (j < 0 || s->phase[j] == s->phase[i]) continue;
+            const double *e = EV(s, i, k);
+            double y1, y2;
+            nbr_y(g, s, i, j, &y1, &y2);
+            double lrr = lr_ratio(x1 - y1, x2 - y2, e);
+            double m1 = 0.5 * (e[0] + e[2]), m2 = 0.5 * (e[1] + e[3]);
+            double z1 = 0.5 * (x1 + y1), z2 = 0.5 * (x2 + y2);
+            double a = (T[i] - T[j]) * (m1 - z1) + (T[n + i] - T[n + j]) * (m2 - z2);
+            double c = (T[i] + T[j]) * (x1 - y1) + (T[n + i] + T[n + j]) * (x2 - y2);
+            ri -= lrr * (a - 0.5 * c);
+        }
+        res[i] = ri;
+    }
+    return 0;
+}
+
+/* the dv update of multiphase_projection!, src/relaxation.jl:190-205 */
+int lvo_multiphase_update(const lvo_grid *g, lvo_state *s, const double *res, double quality_threshold)
+{
+    int n = s->n;
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; i++) {
+        if (s->quality[i] < quality_threshold) continue;
+        double x1 = s->x[i], x2 = s->x[n + i];
+        double A = poly_area(s, i);
+        for (int k = 0; k < s->ne[i]; k++) {
+            int j = EL(s, i, k);
+            if (j < 0 || s->phase[j] == s->phase[i]) continue;
+            const double *e = EV(s, i, k);
+            double y1, y2;
+            nbr_y(g, s, i, j, &y1, &y2);
+            double lrr = lr_ratio(x1 - y1, x2 - y2, e);
+            double m1 = 0.5 * (e[0] + e[2]), m2 = 0.5 * (e[1] + e[3]);
+            double w = lrr * (res[i] - res[j]);
+            s->dv[i] += (w * (m1 - x1)) / A;
+            s->dv[n + i] += (w * (m2 - x2)) / A;
+        }
+    }
+    return 0;
+}
+
+/* multiphase_projection!(solver), src/relaxation.jl:190-206, with CG from x = 0 stopping at
+ * |r| <= atol + rtol |b| (Krylov.jl's criterion); returns the iterations, -1 if itmax was hit */
+int lvo_multiphase_projection(const lvo_grid *g, lvo_state *s, double quality_threshold, double atol, double rtol,
+                              int itmax, double *res_out)
+{
+    int n = s->n;
+    double *b = calloc((size_t)n, sizeof(double)), *x = calloc((size_t)n, sizeof(double));
+    double *r = malloc(sizeof(double) * n), *p = malloc(sizeof(double) * n), *q = malloc(sizeof(double) * n);
+    double *T = malloc(sizeof(double) * 2 * n);
+    lvo_multiphase_rhs(g, s, b);
+    double bb = 0.0;
+    for (int i = 0; i < n; i++) {
+        r[i] = b[i];
+        p[i] = b[i];
+        bb += b[i] * b[i];
+    }
+    double thr = atol + rtol * sqrt(bb), rr = bb;
+    int it = 0, ok = sqrt(rr) <= thr;
+    while (!ok && it < itmax) {
+        lvo_multiphase_apply(g, s, p, T, q);
+        double pq = 0.0;
+        for (int i = 0; i < n; i++) pq += p[i] * q[i];
+        if (!(pq > 0.0)) break;
+        double alpha = rr / pq, rr1 = 0.0;
+        for (int i = 0; i < n; i++) {
+            x[i] += alpha * p[i];
+            r[i] -= alpha * q[i];
+            rr1 += r[i] * r[i];
+        }
+        it++;
+        if (sqrt(rr1) <= thr) {
+            ok = 1;
+            break;
+        }
+        double beta = rr1 / rr;
+        rr = rr1;
+        for (int i = 0; i < n; i++) p[i] = r[i] + beta * p[i];
+    }
+    lvo_multiphase_update(g, s, x, quality_threshold);
+    if (res_out) memcpy(res_out, x, sizeof(double) * n);
+    free(b); free(x); free(r); free(p); free(q); free(T);
+    return ok ? it : -1;
+}

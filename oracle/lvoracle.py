@@ -60,6 +60,8 @@ def lib():
             "lvo_find_pressure": ([P, P, D, I, P, D, I, P], I), "lvo_pressure_step": ([P, P, D], I),
             "lvo_find_D": ([P, P], I), "lvo_viscous_step": ([P, P, D, I], I), "lvo_bdary_friction": ([P, P, D], I),
             "lvo_find_dv": ([P, P, D, D], I), "lvo_relaxation_step": ([P, P, D, I], I),
+            "lvo_multiphase_rhs": ([P, P, P], I), "lvo_multiphase_apply": ([P, P, P, P, P], I),
+            "lvo_multiphase_update": ([P, P, P, D], I), "lvo_multiphase_projection": ([P, P, D, D, D, I, P], I),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -260,15 +262,39 @@ class Particles:
     def find_dv(self, dt, alpha=1.0):
         lib().lvo_find_dv(self.grid.ptr, ctypes.byref(self.s), dt, alpha)
 
+    def multiphase_rhs(self):
+        b = np.empty(self.n)
+        lib().lvo_multiphase_rhs(self.grid.ptr, ctypes.byref(self.s), _p(b))
+        return b
+
+    def multiphase_apply(self, x):
+        T = np.empty(2 * self.n)
+        y = np.empty(self.n)
+        lib().lvo_multiphase_apply(self.grid.ptr, ctypes.byref(self.s), _p(np.ascontiguousarray(x, dtype=np.float64)),
+                                   _p(T), _p(y))
+        return y
+
+    def multiphase_update(self, res, quality_threshold=0.25):
+        lib().lvo_multiphase_update(self.grid.ptr, ctypes.byref(self.s),
+                                    _p(np.ascontiguousarray(res, dtype=np.float64)), quality_threshold)
+
+    def multiphase_projection(self, quality_threshold=0.25, atol=1e-4, rtol=1e-4, itmax=200):
+        res = np.empty(self.n)
+        it = lib().lvo_multiphase_projection(self.grid.ptr, ctypes.byref(self.s), quality_threshold, atol, rtol,
+                                             itmax, _p(res))
+        return it, res
+
     def relaxation_step(self, dt, rusanov=True, remesh=True):
         lib().lvo_relaxation_step(self.grid.ptr, ctypes.byref(self.s), dt, int(rusanov))
         if remesh:
             self.remesh()
 
     def step(self, dt, eos=None, gamma=1.4, P0=0.0, Pmin=0.0, gravity=None, niter=10, tol=1e-10, pvbc=None,
-             viscous=True, artificial_viscosity=False, friction=None, alpha=1.0, rusanov=True):
+             viscous=True, artificial_viscosity=False, friction=None, alpha=1.0, rusanov=True, multiphase=None,
+             multiphase_tol=1e-4, multiphase_itmax=200):
         """step!(sim, t) in the order of tests/taylorgreen.jl:61-71 (optional parts as in
-        examples/cavity.jl:70-80 and examples/rayleightaylor.jl:90-101)"""
+        examples/cavity.jl:70-80 and examples/rayleightaylor.jl:90-101; multiphase = the
+        MultiphaseSolver's quality_threshold, or None for no multiphase_projection!)"""
         self.move(dt)
         if gravity is not None:
             self.gravity(gravity, dt)
@@ -284,5 +310,7 @@ class Particles:
         if friction is not None:
             self.bdary_friction(friction, dt)
         self.find_dv(dt, alpha)
+        if multiphase is not None:
+            self.multiphase_projection(multiphase, multiphase_tol, multiphase_tol, multiphase_itmax)
         self.relaxation_step(dt, rusanov)
         return iters

@@ -396,3 +396,64 @@ def test_taylor_green_reference_thresholds(oracle):
             last = (E - E0, ve, pe)
     E_err, v_err, P_err = last
     assert E_err < 1e-8 and v_err < 0.01 and P_err < 0.01, last
+
+
+# ---- multiphase projection (src/relaxation.jl:75-206) ---------------------------------------------
+
+def rt_particles(lv, oracle, K=24, seed=0x5EED0600):
+    dr = 1.0 / K
+    x = lv.inputs.jittered_lattice(K, 2 * K, dr, seed=seed)
+    g = oracle.Grid((0.0, 0.0), (1.0, 2.0), dr)
+    p = oracle.Particles(g, x)
+    p.remesh()
+    ic = lv.inputs.rayleigh_taylor(x, p.area())
+    for k in ("v", "rho", "mass", "P", "e", "mu", "phase"):
+        p.set(k, ic[k])
+    p.find_D()
+    p.find_dv(1e-3, 1.0)
+    n = x.size // 2
+    p.set("dv", 0.1 * np.random.default_rng(1).standard_normal(2 * n))
+    return p, n
+
+
+def test_multiphase_projector_is_spsd_on_the_interface(lv, oracle):
+    """MultiphaseProjector: symmetric positive semi-definite, zero rows and columns off the phase
+    interface, and refresh!'s right-hand side lies in its range."""
+    p, n = rt_particles(lv, oracle)
+    A = np.stack([p.multiphase_apply(e) for e in np.eye(n)], axis=1)
+    b = p.multiphase_rhs()
+    ph = p["phase"]
+    iface = np.zeros(n, dtype=bool)
+    ptr, lab, _ = p.mesh()
+    for i in range(n):
+        for j in lab[ptr[i]:ptr[i + 1]]:
+            if j >= 0 and ph[j] != ph[i]:
+                iface[i] = True
+    assert iface.sum() > 10
+    assert np.all(A[~iface] == 0.0) and np.all(A[:, ~iface] == 0.0) and np.all(b[~iface] == 0.0)
+    assert np.abs(A - A.T).max() <= 1e-12 * np.abs(A).max()
+    ev = np.linalg.eigvalsh(0.5 * (A + A.T))
+    assert ev.min() >= -1e-12 * ev.max()
+    xs, *_ = np.linalg.lstsq(A, b, rcond=None)
+    assert np.linalg.norm(A @ xs - b) <= 1e-10 * np.linalg.norm(b)
+
+
+def test_multiphase_projection_removes_interphase_flux(lv, oracle):
+    """After a tight projection without the quality cut-off, the right-hand side recomputed from
+    the new dv (the flux of dv across phase boundaries) vanishes: b(dv - T(x)) = b - A x = 0."""
+    p, n = rt_particles(lv, oracle)
+    b0 = p.multiphase_rhs()
+    it, res = p.multiphase_projection(0.0, 1e-13, 1e-13, 100000)
+    assert it > 0
+    b1 = p.multiphase_rhs()
+    assert np.linalg.norm(b1) <= 1e-9 * np.linalg.norm(b0)
+
+
+def test_multiphase_quality_threshold_skips_cells(lv, oracle):
+    p, n = rt_particles(lv, oracle)
+    dv0 = p["dv"].copy()
+    q = p["quality"]
+    thr = float(np.median(q))
+    p.multiphase_projection(thr, 1e-10, 1e-10, 100000)
+    changed = (p["dv"][:n] != dv0[:n]) | (p["dv"][n:] != dv0[n:])
+    assert not np.any(changed & (q < thr)) and np.any(changed)

@@ -114,7 +114,8 @@ typedef struct {
     double g[2];
     int32_t niter;        /* Picard iterations of find_pressure! (reference default 10) */
     lv_solve_opts solve;
-    int32_t viscous;      /* 1: viscous_step!(dt; artificial_viscosity) */
+    int32_t viscous;      /* 1: viscous_step!(grid, dt; artificial_viscosity); 2: viscous_step!(ViscousSolver(grid), dt)
+                             (implicit, src/viscous_solver.jl:120-128) solved with `solve`'s method to viscous_tol */
     int32_t artificial_viscosity;
     int32_t friction;     /* 1: bdary_friction!(vDirichlet, dt) with vDirichlet per label */
     double friction_vbc[8];
@@ -124,6 +125,7 @@ typedef struct {
     double multiphase_quality; /* MultiphaseSolver(grid; quality_threshold), reference default 0.25 */
     double multiphase_tol;     /* atol = rtol of its solve (the reference's 1e-4) */
     int32_t multiphase_itmax;  /* its iteration cap (the reference's 200) */
+    double viscous_tol;        /* viscous == 2: tolerance of the implicit viscous solve (the reference's 1e-6) */
 } lv_step_desc;
 
 /* ---- context ---------------------------------------------------------------------------------- */
@@ -176,6 +178,14 @@ int lv_find_dv(lv_ctx *ctx, double dt, double alpha);         /* src/relaxation.
 int lv_relaxation_step(lv_ctx *ctx, double dt, int32_t rusanov); /* src/relaxation.jl:36-73 (remeshes) */
 /* the whole step!, device resident; stats may be NULL */
 int lv_step_fused(lv_ctx *ctx, const lv_step_desc *desc, lv_solve_stats *stats);
+
+/* viscous_step!(solver::ViscousSolver, dt), src/viscous_solver.jl:120-128: assembles the implicit
+ * operator (d_i = m_i/dt, c_ij = 0.5 lrr (mu_i + mu_j), :89-113) into the solver's system and solves
+ * it once per velocity component from the initial guess v; v is overwritten.  opts == NULL: mixed
+ * precision PCG to 1e-6 with itmax 2000 (the reference's minres! tolerances).  Afterwards the solver's
+ * system holds this operator: lv_get_pressure_operator and lv_solve_pressure refuse it until the
+ * pressure operator is assembled again. */
+int lv_viscous_implicit(lv_ctx *ctx, double dt, const lv_solve_opts *opts, lv_solve_stats *stats);
 
 /* multiphase_projection!(MultiphaseSolver(grid; quality_threshold)), src/relaxation.jl:190-206:
  * projects dv so that no area crosses a phase boundary.  The reference's minres!(atol = rtol = 1e-4,

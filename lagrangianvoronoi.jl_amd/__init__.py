@@ -89,7 +89,8 @@ class StepDesc(ctypes.Structure):
                 ("viscous", ctypes.c_int32), ("artificial_viscosity", ctypes.c_int32), ("friction", ctypes.c_int32),
                 ("friction_vbc", ctypes.c_double * 8), ("alpha", ctypes.c_double), ("rusanov", ctypes.c_int32),
                 ("multiphase", ctypes.c_int32), ("multiphase_quality", ctypes.c_double),
-                ("multiphase_tol", ctypes.c_double), ("multiphase_itmax", ctypes.c_int32)]
+                ("multiphase_tol", ctypes.c_double), ("multiphase_itmax", ctypes.c_int32),
+                ("viscous_tol", ctypes.c_double)]
 
 
 _P = ctypes.c_void_p
@@ -131,6 +132,7 @@ _SIGNATURES = {
     "lv_solve_pressure": [_P, _P, _P, ctypes.POINTER(SolveOpts), ctypes.POINTER(SolveStats)],
     "lv_apply_preconditioner": [_P, _I, _P, _P],
     "lv_multiphase_projection": [_P, _D, _D, _D, _I, _P, _P],
+    "lv_viscous_implicit": [_P, _D, ctypes.POINTER(SolveOpts), ctypes.POINTER(SolveStats)],
     "lv_multiphase_rhs": [_P, _P],
     "lv_multiphase_apply": [_P, _P, _P],
     "lv_spai_num_params": [_P],
@@ -376,8 +378,32 @@ def find_D(grid):
     _check(grid.ctx, lib().lv_find_D(grid.ctx))
 
 
-def viscous_step(grid, dt, artificial_viscosity=True):
-    _check(grid.ctx, lib().lv_viscous_step(grid.ctx, float(dt), int(bool(artificial_viscosity))))
+class ViscousSolver:
+    """ViscousSolver(grid; verbose), src/viscous_solver.jl:43-67: the implicit viscous step.  Its
+    system is solved with this build's inner solver (`method`), to the reference's minres!
+    tolerance 1e-6 with itmax 2000 by default."""
+
+    def __init__(self, grid, verbose=0, method="mixed", tol=1e-6, itmax=2000, inner_tol=1e-4, outer_max=20):
+        self.grid = grid
+        self.verbose = verbose
+        self.method, self.tol, self.itmax = method, tol, itmax
+        self.inner_tol, self.outer_max = inner_tol, outer_max
+        self.last_stats = None
+
+    def opts(self):
+        return solve_opts(self.method, self.tol, self.itmax, self.inner_tol, self.outer_max)
+
+
+def viscous_step(obj, dt, artificial_viscosity=True):
+    """viscous_step!(grid, dt; artificial_viscosity) (explicit, src/diffusion.jl:39-53), or
+    viscous_step!(solver::ViscousSolver, dt) (implicit, src/viscous_solver.jl:120-128)."""
+    if isinstance(obj, ViscousSolver):
+        o = obj.opts()
+        st = SolveStats()
+        _check(obj.grid.ctx, lib().lv_viscous_implicit(obj.grid.ctx, float(dt), ctypes.byref(o), ctypes.byref(st)))
+        obj.last_stats = st.as_dict()
+        return obj.last_stats
+    _check(obj.ctx, lib().lv_viscous_step(obj.ctx, float(dt), int(bool(artificial_viscosity))))
 
 
 def bdary_friction(grid, vDirichlet, dt):
@@ -505,7 +531,8 @@ def make_step_desc(dt, eos=None, gamma=1.4, P0=0.0, Pmin=0.0, gravity=None, nite
     d.g[:] = (0.0, 0.0) if gravity is None else tuple(gravity)
     d.niter = int(niter)
     d.solve = solver.opts(boundary_velocity) if solver is not None else solve_opts(boundary_velocity=boundary_velocity)
-    d.viscous = int(bool(viscous))
+    d.viscous = 2 if isinstance(viscous, ViscousSolver) else int(bool(viscous))
+    d.viscous_tol = float(viscous.tol) if isinstance(viscous, ViscousSolver) else 1e-6
     d.artificial_viscosity = int(bool(artificial_viscosity))
     d.friction = int(friction is not None)
     d.friction_vbc[:] = _vbc_table(friction)

@@ -72,6 +72,7 @@ struct lv_ctx {
     int amg_ok = 0;
     int64_t amg_built_for = -1;
     int amg_learned = -1;  // the fine smoother the hierarchy was built with (1 learned, 0 Jacobi)
+    int A_kind = 0;        // what A holds: 0 the pressure operator, 1 the implicit viscous operator
     // multiphase projection work space (allocated on first use)
     lv_mph_work *mph = nullptr;
     char msg[512] = {0};
@@ -680,7 +681,8 @@ int lv_eos_ideal(lv_ctx *ctx, double gamma, double Pmin)
     return LV_OK;
 }
 
-int lv_assemble_pressure(lv_ctx *ctx, double dt, int64_t *nnz)
+// the system of the inner solver: mode 0 the pressure operator, mode 1 ViscousSolver's operator
+static int assemble_system(lv_ctx *ctx, double dt, int mode, int64_t *nnz)
 {
     int rc = check_ctx(ctx, true);
     if (rc) return rc;
@@ -707,14 +709,26 @@ int lv_assemble_pressure(lv_ctx *ctx, double dt, int64_t *nnz)
     ctx->A.deg = ctx->deg;
     rc = clear_flags(ctx);
     if (rc) return rc;
-    LV_HIP_CHECK(lv_launch_assemble(ctx->g, ctx->F, M, n, dt, ctx->A, ctx->d_flags + 1, ctx->s));
+    LV_HIP_CHECK(lv_launch_assemble(ctx->g, ctx->F, M, n, dt, mode, ctx->A, ctx->d_flags + 1, ctx->s));
     int flags[2];
     rc = read_flags(ctx, flags);
     if (rc) return rc;
     ctx->A.singular = (flags[1] == 0);
     ctx->assembly_id++;
     ctx->A.valid = 1;
+    ctx->A_kind = mode;
     if (nnz) *nnz = total;
+    return LV_OK;
+}
+
+int lv_assemble_pressure(lv_ctx *ctx, double dt, int64_t *nnz) { return assemble_system(ctx, dt, 0, nnz); }
+
+static int need_pressure_operator(lv_ctx *ctx)
+{
+    if (!ctx->A.valid) return lv_fail(ctx, LV_E_INVALID, "pressure operator not assembled");
+    if (ctx->A_kind != 0)
+        return lv_fail(ctx, LV_E_INVALID, "the last assembled operator is the implicit viscous one; assemble the "
+                                          "pressure operator first");
     return LV_OK;
 }
 
@@ -722,7 +736,8 @@ int lv_get_pressure_operator(lv_ctx *ctx, double *diag, int32_t *row_ptr, int32_
 {
     int rc = check_ctx(ctx, true);
     if (rc) return rc;
-    if (!ctx->A.valid) return lv_fail(ctx, LV_E_INVALID, "pressure operator not assembled");
+    rc = need_pressure_operator(ctx);
+    if (rc) return rc;
     const int n = ctx->n;
     LV_HIP_CHECK(hipMemcpy(diag, ctx->A.diag64, sizeof(double) * n, hipMemcpyDeviceToHost));
     LV_HIP_CHECK(hipMemcpy(row_ptr, ctx->A.row_ptr, sizeof(int) * (n + 1), hipMemcpyDeviceToHost));
@@ -756,7 +771,6 @@ static void default_opts(lv_solve_opts &o)
     o.picard_window = 0;
 }
 
-// the learned preconditioner of the current operator (network forward + emission once per assembly)
 // The preconditioner of LV_SOLVER_SPAI (the learned M) or LV_SOLVER_AMG (the multilevel cycle,
 // with the learned M as fine smoother when parameters are loaded), built once per assembly.
 static int spai_prepare(lv_ctx *ctx, const lv_solve_opts &opt, lv_precond *M, bool *have)
@@ -864,7 +878,8 @@ int lv_solve_pressure(lv_ctx *ctx, const double *b, double *x, const lv_solve_op
 {
     int rc = check_ctx(ctx, true);
     if (rc) return rc;
-    if (!ctx->A.valid) return lv_fail(ctx, LV_E_INVALID, "pressure operator not assembled");
+    rc = need_pressure_operator(ctx);
+    if (rc) return rc;
     lv_solve_opts o;
     if (opts) o = *opts;
     else default_opts(o);
@@ -1073,6 +1088,41 @@ int lv_relaxation_step(lv_ctx *ctx, double dt, int32_t rusanov)
     return lv_remesh(ctx);  // src/relaxation.jl:72
 }
 
+// ---- implicit viscosity (src/viscous_solver.jl:43-128) ------------------------------------------
+
+// viscous_step!(solver::ViscousSolver, dt): the operator once, then one solve per velocity component
+// (the reference's 2n x 2n system is two copies of the same n x n block), each from the initial
+// guess v and right-hand side (m/dt) v; v is overwritten by the solution
+int lv_viscous_implicit(lv_ctx *ctx, double dt, const lv_solve_opts *opts, lv_solve_stats *stats)
+{
+    int rc = check_ctx(ctx, true);
+    if (rc) return rc;
+    lv_solve_opts o;
+    if (opts) {
+        o = *opts;
+    } else {
+        default_opts(o);
+        o.method = LV_SOLVER_MIXED;
+        o.tol = 1e-6;   // minres!(...; atol = 1e-6, rtol = 1e-6, itmax = 2000), src/viscous_solver.jl:122-126
+        o.itmax = 2000;
+    }
+    rc = assemble_system(ctx, dt, 1, nullptr);
+    if (rc) return rc;
+    const int n = ctx->n;
+    int status = LV_OK;
+    for (int c = 0; c < 2; c++) {
+        double *vc = ctx->F.v + (size_t)c * n;
+        LV_HIP_CHECK(lv_launch_visc_rhs(n, ctx->F.mass, vc, dt, ctx->b, ctx->P0, ctx->s));
+        rc = solve_on_device(ctx, ctx->b, ctx->P0, o, stats);
+        if (rc == LV_E_NOT_CONVERGED) status = rc;
+        else if (rc) return rc;
+        LV_HIP_CHECK(hipMemcpyAsync(vc, ctx->P0, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->s));
+    }
+    LV_HIP_CHECK(hipStreamSynchronize(ctx->s));
+    if (status) lv_fail(ctx, status, "implicit viscous solve did not reach its tolerance");
+    return status;
+}
+
 // ---- multiphase projection (src/relaxation.jl:75-206) -------------------------------------------
 
 static int mph_ready(lv_ctx *ctx)
@@ -1141,7 +1191,14 @@ int lv_step_fused(lv_ctx *ctx, const lv_step_desc *d, lv_solve_stats *stats)
     if (rc) return rc;
     rc = lv_find_D(ctx);
     if (rc) return rc;
-    if (d->viscous) {
+    if (d->viscous == 2) {  // viscous_step!(ViscousSolver(grid), dt) with the step's solver method
+        lv_solve_opts o = d->solve;
+        o.tol = d->viscous_tol > 0.0 ? d->viscous_tol : 1e-6;
+        o.itmax = 2000;
+        rc = lv_viscous_implicit(ctx, d->dt, &o, stats);
+        if (rc && rc != LV_E_NOT_CONVERGED) return rc;
+        if (rc) prc = rc;
+    } else if (d->viscous) {
         rc = lv_viscous_step(ctx, d->dt, d->artificial_viscosity);
         if (rc) return rc;
     }

@@ -62,10 +62,12 @@ size_t lv_partials_bytes(int n);
 hipError_t lv_apply_precond(const lv_pressure_system &A, lv_solver_work &W, int n, const lv_precond *M,
                             const double *r, double *z, hipStream_t s);
 hipError_t lv_launch_degree(lv_mesh_view M, int n, int *deg, hipStream_t s);
-hipError_t lv_launch_assemble(const lv_grid_params &g, lv_fields F, lv_mesh_view M, int n, double dt,
+hipError_t lv_launch_assemble(const lv_grid_params &g, lv_fields F, lv_mesh_view M, int n, double dt, int mode,
                               lv_pressure_system &A, int *nonsingular, hipStream_t s);
 hipError_t lv_launch_rhs(const lv_grid_params &g, lv_fields F, lv_mesh_view M, int n, double dt, int gp_step,
                          lv_vbc vbc, double *b, double *P0, double *GP, hipStream_t s);
+hipError_t lv_launch_visc_rhs(int n, const double *mass, const double *vc, double dt, double *b, double *x,
+                              hipStream_t s);
 hipError_t lv_launch_write_P(lv_fields F, int n, const double *P, hipStream_t s);
 hipError_t lv_solve_device(const lv_pressure_system &A, lv_solver_work &W, int n, const double *b, double *x,
                            const lv_solve_opts &opt, hipStream_t s, lv_timer *t32, lv_timer *t64, int *iters,

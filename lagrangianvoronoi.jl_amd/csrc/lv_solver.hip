@@ -36,8 +36,9 @@ __global__ __launch_bounds__(B) void k_degree(lv_mesh_view M, int n, int *__rest
     deg[i] = d;
 }
 
-// refresh!(A, grid, dt), src/pressure.jl:104-117
-__global__ __launch_bounds__(B) void k_assemble(lv_grid_params g, lv_fields F, lv_mesh_view M, int n, double dt,
+// refresh!(A, grid, dt), src/pressure.jl:104-117 (mode 0), or the operator of ViscousSolver's
+// assemble!, src/viscous_solver.jl:89-113 (mode 1: d_i = m_i/dt, c_ij = 0.5 lrr (mu_i + mu_j))
+__global__ __launch_bounds__(B) void k_assemble(lv_grid_params g, lv_fields F, lv_mesh_view M, int n, double dt, int mode,
                                                 const int *__restrict__ row_ptr, int *__restrict__ cols,
                                                 double *__restrict__ val64, float *__restrict__ val32,
                                                 double *__restrict__ diag64, float *__restrict__ diag32,
@@ -49,7 +50,8 @@ __global__ __launch_bounds__(B) void k_assemble(lv_grid_params g, lv_fields F, l
     if (i >= n) return;
     const double x1 = F.x[i], x2 = F.x[n + i];
     const double rho = F.rho[i];
-    const double d = F.mass[i] / (((rho * rho) * F.c2[i]) * (dt * dt));
+    const double d = (mode == 0) ? F.mass[i] / (((rho * rho) * F.c2[i]) * (dt * dt)) : F.mass[i] / dt;
+    const double mui = F.mu[i];
     double dd = d;
     int o = row_ptr[i];
     // the solver's ELL copy keeps each row's neighbours sorted by column, so that slot k of a
@@ -65,7 +67,7 @@ __global__ __launch_bounds__(B) void k_assemble(lv_grid_params g, lv_fields F, l
         lv_arrow_target(g, F.x[j], F.x[n + j], x1, x2, y1, y2);
         const double lrr =
             lv_lr_ratio(x1 - y1, x2 - y2, LV_EV(M, 0, k, i), LV_EV(M, 1, k, i), LV_EV(M, 2, k, i), LV_EV(M, 3, k, i));
-        const double c = lrr * (0.5 / rho + 0.5 / F.rho[j]);
+        const double c = (mode == 0) ? lrr * (0.5 / rho + 0.5 / F.rho[j]) : (0.5 * lrr) * (mui + F.mu[j]);
         cols[o] = j;
         val64[o] = c;
         val32[o] = (float)c;
@@ -438,6 +440,16 @@ __global__ __launch_bounds__(B) void k_apply_m32(int n, const int *__restrict__ 
     block_partials<2>(v, Prr, gridDim.x);
 }
 
+// b = (m/dt) v_c and x = v_c for one velocity component (ViscousSolver's assemble!, src/viscous_solver.jl:97-102)
+__global__ __launch_bounds__(B) void k_visc_rhs(int n, const double *__restrict__ mass, const double *__restrict__ vc,
+                                                double dt, double *__restrict__ b, double *__restrict__ x)
+{
+    const int i = blockIdx.x * B + threadIdx.x;
+    if (i >= n) return;
+    b[i] = (mass[i] / dt) * vc[i];
+    x[i] = vc[i];
+}
+
 // true residual r = (b - bmean) - A x; partials |r|^2, |b'|^2 (FP64)
 __global__ __launch_bounds__(B) void k_resid64(int n, const int *__restrict__ deg, const int *__restrict__ ecol,
                                                const double *__restrict__ ev, const double *__restrict__ diag,
@@ -566,10 +578,10 @@ hipError_t lv_launch_degree(lv_mesh_view M, int n, int *deg, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t lv_launch_assemble(const lv_grid_params &g, lv_fields F, lv_mesh_view M, int n, double dt,
+hipError_t lv_launch_assemble(const lv_grid_params &g, lv_fields F, lv_mesh_view M, int n, double dt, int mode,
                               lv_pressure_system &A, int *nonsingular, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_assemble, grid_of(n), dim3(B), 0, s, g, F, M, n, dt, (const int *)A.row_ptr, A.cols,
+    hipLaunchKernelGGL(k_assemble, grid_of(n), dim3(B), 0, s, g, F, M, n, dt, mode, (const int *)A.row_ptr, A.cols,
                        A.val64, A.val32, A.diag64, A.diag32, A.dinv64, A.dinv32, A.ecol, A.ev64, A.ev32, nonsingular);
     return hipGetLastError();
 }
@@ -579,6 +591,13 @@ hipError_t lv_launch_rhs(const lv_grid_params &g, lv_fields F, lv_mesh_view M, i
 {
     hipLaunchKernelGGL(k_rhs1, grid_of(n), dim3(B), 0, s, g, F, M, n, dt, vbc, b, P0, GP);
     if (gp_step) hipLaunchKernelGGL(k_rhs2, grid_of(n), dim3(B), 0, s, g, F, M, n, (const double *)GP, b);
+    return hipGetLastError();
+}
+
+hipError_t lv_launch_visc_rhs(int n, const double *mass, const double *vc, double dt, double *b, double *x,
+                              hipStream_t s)
+{
+    hipLaunchKernelGGL(k_visc_rhs, grid_of(n), dim3(B), 0, s, n, mass, vc, dt, b, x);
     return hipGetLastError();
 }
 

@@ -1316,3 +1316,53 @@ int lvo_multiphase_projection(const lvo_grid *g, lvo_state *s, double quality_th
     free(b); free(x); free(r); free(p); free(q); free(T);
     return ok ? it : -1;
 }
+
+/* ------------------------------------------------------------------------------------------------
+ * Implicit viscosity (src/viscous_solver.jl:43-128): ViscousSolver's operator in the difference form
+ * of the pressure operator, d_i = m_i/dt and c_ij = 0.5 lrr (mu_i + mu_j) (assemble!, :89-113; the
+ * 2n x 2n system is two copies of this n x n block, one per velocity component), and viscous_step!
+ * with Jacobi-PCG in place of minres! (atol = rtol = 1e-6, itmax = 2000) from the initial guess v.
+ * ---------------------------------------------------------------------------------------------- */
+
+int lvo_viscous_operator(const lvo_grid *g, const lvo_state *s, double dt, double *diag, double *coef)
+{
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < s->n; i++) {
+        int n = s->n;
+        diag[i] = s->mass[i] / dt;
+        for (int k = 0; k < s->maxe; k++) coef[(size_t)i * s->maxe + k] = 0.0;
+        for (int k = 0; k < s->ne[i]; k++) {
+            int j = EL(s, i, k);
+            if (j < 0) continue;
+            double y1, y2;
+            nbr_y(g, s, i, j, &y1, &y2);
+            double lrr = lr_ratio(s->x[i] - y1, s->x[n + i] - y2, EV(s, i, k));
+            coef[(size_t)i * s->maxe + k] = (0.5 * lrr) * (s->mu[i] + s->mu[j]);
+        }
+    }
+    return 0;
+}
+
+/* viscous_step!(solver::ViscousSolver, dt), src/viscous_solver.jl:120-128; returns the larger of the
+ * two components' iteration counts (-1 if either missed tol) */
+int lvo_viscous_implicit(const lvo_grid *g, lvo_state *s, double dt, double tol, int itmax)
+{
+    int n = s->n;
+    double *diag = malloc(sizeof(double) * n), *coef = malloc(sizeof(double) * (size_t)n * s->maxe);
+    double *b = malloc(sizeof(double) * n), *x = malloc(sizeof(double) * n);
+    lvo_viscous_operator(g, s, dt, diag, coef);
+    int worst = 0;
+    for (int c = 0; c < 2; c++) {
+        for (int i = 0; i < n; i++) {
+            b[i] = (s->mass[i] / dt) * s->v[(size_t)c * n + i];
+            x[i] = s->v[(size_t)c * n + i];
+        }
+        double rr = 0.0;
+        int it = lvo_pcg(s, diag, coef, b, x, tol, itmax, &rr);
+        if (it < 0 || worst < 0) worst = -1;
+        else if (it > worst) worst = it;
+        for (int i = 0; i < n; i++) s->v[(size_t)c * n + i] = x[i];
+    }
+    free(diag); free(coef); free(b); free(x);
+    return worst;
+}

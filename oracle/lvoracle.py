@@ -62,6 +62,7 @@ def lib():
             "lvo_find_dv": ([P, P, D, D], I), "lvo_relaxation_step": ([P, P, D, I], I),
             "lvo_multiphase_rhs": ([P, P, P], I), "lvo_multiphase_apply": ([P, P, P, P, P], I),
             "lvo_multiphase_update": ([P, P, P, D], I), "lvo_multiphase_projection": ([P, P, D, D, D, I, P], I),
+            "lvo_viscous_operator": ([P, P, D, P, P], I), "lvo_viscous_implicit": ([P, P, D, D, I], I),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -256,6 +257,16 @@ class Particles:
     def viscous_step(self, dt, artificial_viscosity=True):
         lib().lvo_viscous_step(self.grid.ptr, ctypes.byref(self.s), dt, int(artificial_viscosity))
 
+    def viscous_operator(self, dt):
+        """ViscousSolver's operator (diag, coef) in the pressure operator's layout"""
+        diag = np.empty(self.n)
+        coef = np.empty(self.n * MAXE)
+        lib().lvo_viscous_operator(self.grid.ptr, ctypes.byref(self.s), dt, _p(diag), _p(coef))
+        return diag, coef
+
+    def viscous_implicit(self, dt, tol=1e-6, itmax=2000):
+        return lib().lvo_viscous_implicit(self.grid.ptr, ctypes.byref(self.s), dt, tol, itmax)
+
     def bdary_friction(self, vbc, dt):
         lib().lvo_bdary_friction(ctypes.byref(self.s), _p(np.ascontiguousarray(vbc, dtype=np.float64)), dt)
 
@@ -291,7 +302,7 @@ class Particles:
 
     def step(self, dt, eos=None, gamma=1.4, P0=0.0, Pmin=0.0, gravity=None, niter=10, tol=1e-10, pvbc=None,
              viscous=True, artificial_viscosity=False, friction=None, alpha=1.0, rusanov=True, multiphase=None,
-             multiphase_tol=1e-4, multiphase_itmax=200):
+             multiphase_tol=1e-4, multiphase_itmax=200, viscous_tol=1e-6):
         """step!(sim, t) in the order of tests/taylorgreen.jl:61-71 (optional parts as in
         examples/cavity.jl:70-80 and examples/rayleightaylor.jl:90-101; multiphase = the
         MultiphaseSolver's quality_threshold, or None for no multiphase_projection!)"""
@@ -305,7 +316,9 @@ class Particles:
         iters = self.find_pressure(dt, niter, pvbc, tol)
         self.pressure_step(dt)
         self.find_D()
-        if viscous:
+        if viscous == "implicit":
+            self.viscous_implicit(dt, viscous_tol, 100000)
+        elif viscous:
             self.viscous_step(dt, artificial_viscosity)
         if friction is not None:
             self.bdary_friction(friction, dt)

@@ -457,3 +457,27 @@ def test_multiphase_quality_threshold_skips_cells(lv, oracle):
     p.multiphase_projection(thr, 1e-10, 1e-10, 100000)
     changed = (p["dv"][:n] != dv0[:n]) | (p["dv"][n:] != dv0[n:])
     assert not np.any(changed & (q < thr)) and np.any(changed)
+
+
+# ---- implicit viscosity (src/viscous_solver.jl:43-128) ---------------------------------------------
+
+def test_viscous_operator_symmetric_with_mass_diagonal(lv, oracle):
+    """ViscousSolver's operator: symmetric couplings 0.5 lrr (mu_i + mu_j), A 1 = m/dt, positive
+    definite; the implicit step conserves momentum sum m v."""
+    g, p, x, dr = oracle_pair(lv, oracle, "walls", 20, 0x5EED0702)
+    n = x.size // 2
+    rng = np.random.default_rng(4)
+    p.set("mass", dr * dr * (1.0 + 0.1 * rng.random(n)))
+    p.set("mu", 0.05 + 0.05 * rng.random(n))
+    p.set("v", rng.standard_normal(2 * n))
+    dt = 1e-2
+    diag, coef = p.viscous_operator(dt)
+    assert np.allclose(diag, p["mass"] / dt, rtol=0, atol=0)
+    A = np.stack([p.apply_A(diag, coef, e) for e in np.eye(n)], axis=1)
+    assert np.abs(A - A.T).max() <= 1e-12 * np.abs(A).max()
+    assert np.abs(A @ np.ones(n) - diag).max() <= 1e-12 * diag.max()
+    assert np.linalg.eigvalsh(A).min() > 0.0
+    mom0 = (p["mass"][None, :] * p["v"].reshape(2, n)).sum(axis=1)
+    assert p.viscous_implicit(dt, 1e-13, 100000) >= 0
+    mom1 = (p["mass"][None, :] * p["v"].reshape(2, n)).sum(axis=1)
+    assert np.abs(mom1 - mom0).max() <= 1e-11 * np.abs(p["mass"] * p["v"][:n]).sum()

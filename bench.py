@@ -27,6 +27,88 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "particle-steps/s at 4M particles; pressure-solve time to 1e-8 rel. residual"
 
 
+# Configuration families at full size (the benchmark line is tg2048; bench.py --config runs the others).
+CONFIGS = {
+    "tg2048": None,
+    # Taylor-Green 1024^2, the plain method in FP64: Jacobi-PCG to 1e-10, no initial-guess extrapolation
+    "tg1024": dict(nx=1024, ny=1024, ymax=1.0, periodic=True, ic="tg", dt=2.5e-4, solver="pcg64", tol=1e-10,
+                   window=0, steps=5, warmup=1),
+    # Gresho vortex 100^2, incompressible, the plain method in FP64, 100 steps
+    "gresho100": dict(nx=100, ny=100, ymax=1.0, periodic=True, ic="gresho", dt=1e-3, solver="pcg64", tol=1e-10,
+                      window=0, steps=100, warmup=0, viscous=False),
+    # Rayleigh-Taylor 1024 x 2048 walls, density ratio 3, gravity -1, the reference's RT step with
+    # multiphase_projection! (dt = 0.1 dr, examples/rayleightaylor.jl:36)
+    "rt1024": dict(nx=1024, ny=2048, ymax=2.0, periodic=False, ic="rt", dt=0.1 / 1024, solver="amg", tol=1e-8,
+                   window=8, steps=4, warmup=1),
+    # lid-driven cavity 4096^2, Re = 1000, incompressible, lid u = 1 (dt = min(0.1 dr, 0.1 Re dr^2),
+    # examples/cavity.jl:32)
+    "ldc4096": dict(nx=4096, ny=4096, ymax=1.0, periodic=False, ic="cavity", dt=min(0.1 / 4096, 100.0 / 4096 ** 2),
+                    solver="amg", tol=1e-8, window=8, steps=3, warmup=1),
+}
+
+
+def run_config(lv, args, torch):
+    """One configuration family at full size: steps/s, solver iterations and fallbacks, device memory."""
+    c = CONFIGS[args.config]
+    nx, ny = c["nx"], c["ny"]
+    dr = 1.0 / nx
+    x = lv.inputs.jittered_lattice(nx, ny, dr, seed=0x5EED0100)
+    grid = lv.VoronoiGrid(lv.Rectangle((0.0, 0.0), (1.0, c["ymax"])), dr, xperiodic=c["periodic"],
+                          yperiodic=c["periodic"], positions=x)
+    lv.remesh(grid)
+    area = grid.get("area")
+    kw = dict(viscous=c.get("viscous", True))
+    if c["ic"] == "tg":
+        ic = lv.inputs.taylor_green(x, area)
+        keys = ("v", "rho", "mass", "P", "e", "mu")
+        kw.update(eos="stiffened", gamma=ic["gamma"], P0=ic["P0"])
+    elif c["ic"] == "gresho":
+        ic = lv.inputs.gresho(x, area)
+        keys = ("v", "rho", "mass", "P", "e", "c2")
+    elif c["ic"] == "rt":
+        ic = lv.inputs.rayleigh_taylor(x, area)
+        keys = ("v", "rho", "mass", "P", "e", "mu", "phase")
+        kw.update(eos="ideal", gamma=1.4, gravity=(0.0, -1.0), artificial_viscosity=True,
+                  multiphase=lv.MultiphaseSolver(grid))
+    else:
+        ic = lv.inputs.cavity(x, area)
+        keys = ("v", "rho", "mass", "mu", "c2")
+        kw.update(friction={lv.BDARY_UP: (1.0, 0.0)})
+    for k in keys:
+        grid.set_soa(k, ic[k])
+    method = args.solver if args.solver_given else c["solver"]
+    if method == "spai" or (method == "amg" and args.smoother == "learned"):
+        lv.spai.set_params(grid, spai_theta(lv)[0])
+    solver = lv.PressureSolver(grid, method=method, tol=c["tol"], itmax=4000, inner_tol=args.inner_tol,
+                               picard_window=c["window"])
+    desc = lv.make_step_desc(c["dt"], solver=solver, **kw)
+    steps = args.steps if args.steps_given else c["steps"]
+    warmup = args.warmup if args.warmup_given else c["warmup"]
+    for _ in range(warmup):
+        lv.step_fused(grid, desc)
+    grid.sync()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    stats = [lv.step_fused(grid, desc) for _ in range(steps)]
+    grid.sync()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    free, total = torch.cuda.mem_get_info()
+    n = nx * ny
+    nsolves = sum(s["nsolves"] for s in stats)
+    out = dict(metric="particle-steps/s", config=args.config, particles=n, steps=steps, warmup=warmup,
+               value=round(n * steps / (t1 - t0), 1), ms_per_step=round((t1 - t0) / steps * 1e3, 3),
+               solver=method, smoother=args.smoother if method == "amg" else None, tol=c["tol"],
+               inner_iters_per_step=round(sum(s["total_iters"] for s in stats) / steps, 1),
+               first_solve_ms=round(sum(s["solve_ms"][0] for s in stats) / steps, 3),
+               max_relres=max(max(s["relres"]) for s in stats),
+               fallbacks=sum(s["fallbacks"] for s in stats), solves=nsolves,
+               fallback_rate=round(sum(s["fallbacks"] for s in stats) / max(nsolves, 1), 4),
+               multiphase_iters=[s["multiphase_iters"] for s in stats] if c["ic"] == "rt" else None,
+               device_memory_in_use_GB=round((total - free) / 1e9, 2), device_memory_GB=round(total / 1e9, 1))
+    print(json.dumps(out))
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -48,10 +130,18 @@ def parse():
     ap.add_argument("--no-compare", action="store_true",
                     help="skip the FP64 / mixed / learned-preconditioner solve comparison")
     ap.add_argument("--cpu-K", type=int, default=1024, help="lattice side of the CPU sample (one full step)")
+    ap.add_argument("--config", default="tg2048", choices=sorted(CONFIGS),
+                    help="tg2048: the benchmark line; the others run a configuration family at full size and "
+                         "print its own line (steps/s, iterations, fallbacks, device memory in use)")
     ap.add_argument("--mode", default="steps", choices=["steps", "solve", "copy"],
                     help="steps: the benchmark; solve: one pressure solve (for counter runs); "
                          "copy: a plain device copy of known size (calibrates HBM byte counters)")
-    return ap.parse_args()
+    args = ap.parse_args()
+    argv = sys.argv[1:]
+    args.solver_given = any(a.startswith("--solver") for a in argv)
+    args.steps_given = any(a.startswith("--steps") for a in argv)
+    args.warmup_given = any(a.startswith("--warmup") for a in argv)
+    return args
 
 
 def mode_copy():
@@ -209,6 +299,8 @@ def main():
     lv = load_package()
     if args.mode == "solve":
         return mode_solve(lv, args.K, args.solver, args.tol, args.inner_tol, args.smoother)
+    if args.config != "tg2048":
+        return run_config(lv, args, torch)
     K = args.K
     n = K * K
     dt = 2.5e-4

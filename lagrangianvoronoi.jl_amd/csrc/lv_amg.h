@@ -3,8 +3,8 @@
 
 #include "lv_solver.h"
 
-#define LV_AMG_MAXLEV 10
-#define LV_AMG_DENSE_MAX 256  // the coarsest level is inverted densely once it has at most this many cells
+#define LV_AMG_MAXLEV 16
+#define LV_AMG_DENSE_MAX 64  // the coarsest level is inverted densely once it has at most this many cells
 
 // Level l >= 1: a Kx x Ky grid of cells; the operator in the reference's difference form,
 // (A y)_I = d_I y_I + sum_k c_Ik (y_I - y_{I + off_k}), over the 8 neighbour directions.
@@ -24,7 +24,6 @@ struct lv_amg {
     size_t cap_agg, cap_list, cap_t0, cap_z0, cap_cnt, cap_start, cap_cursor, cap_scan;
     float *t0, *z0; // fine residual and pre-smoothed correction
     lv_amg_level lev[LV_AMG_MAXLEV + 1];
-    double *dense_work;  // 2 * nc * nc for the Gauss-Jordan inversion
     float *dense_inv;    // nc * nc, column-major
     double *pw;          // power-iteration partials and norms
     float *pv, *pu;      // power-iteration vectors (n0)

@@ -13,8 +13,12 @@
 //            stored as a 3x3 stencil (a coupling that would reach beyond the neighbouring cells,
 //            only possible on strongly distorted meshes, is dropped from both sides, which keeps
 //            the coarse operator positive semi-definite);
-//   level l+1: 4 x 4 blocks of level-l cells, the same Galerkin rule;
+//   level l+1: 2 x 2 blocks of level-l cells, the same Galerkin rule (4 x 4 blocks coarsen too fast
+//            for the V-cycle: on the 2048^2 benchmark system PCG needs ~60 iterations with them,
+//            ~40 with 2 x 2);
 //   the coarsest level (<= LV_AMG_DENSE_MAX cells) is inverted densely once per assembly.
+// The levels of at most TAIL_MAX cells run inside one single-workgroup kernel per cycle (k_tail):
+// they are too small to fill the device, and one launch replaces two per level.
 // Every sum is a gather in a fixed order (per-aggregate row lists are sorted), so the hierarchy
 // and the preconditioner are deterministic.
 //
@@ -35,6 +39,8 @@ namespace {
 constexpr int B = 256;
 constexpr float OMEGA = 2.0f / 3.0f;  // damped Jacobi
 constexpr float FINE_TARGET = 1.2f;   // lambda_max of the scaled learned smoother times A
+constexpr int CF = 2;                 // coarse levels: CF x CF blocks of the finer level's cells
+constexpr int CF2 = CF * CF;
 
 inline dim3 g1(int64_t count) { return dim3((unsigned)std::max<int64_t>(1, (count + B - 1) / B)); }
 
@@ -141,7 +147,7 @@ __global__ __launch_bounds__(B) void k_coarse1(int na, int Kx, int Ky, int xper,
     dinv[I] = sum > 0.0 ? (float)(1.0 / sum) : 1.0f;
 }
 
-// level l+1 from level l: 4 x 4 blocks of cells
+// level l+1 from level l: CF x CF blocks of cells
 __global__ __launch_bounds__(B) void k_coarsen(int Kxc, int Kyc, int Kxf, int Kyf, int xper, int yper,
                                                const float *__restrict__ df, const float *__restrict__ cf,
                                                float *__restrict__ dc, float *__restrict__ cc,
@@ -152,9 +158,9 @@ __global__ __launch_bounds__(B) void k_coarsen(int Kxc, int Kyc, int Kxf, int Ky
     if (P >= nc) return;
     const int PX = P / Kyc, PY = P % Kyc;
     double d = 0.0, c[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    const int cx1 = std::min(4 * PX + 4, Kxf), cy1 = std::min(4 * PY + 4, Kyf);
-    for (int cx = 4 * PX; cx < cx1; cx++)
-        for (int cy = 4 * PY; cy < cy1; cy++) {
+    const int cx1 = std::min(CF * PX + CF, Kxf), cy1 = std::min(CF * PY + CF, Kyf);
+    for (int cx = CF * PX; cx < cx1; cx++)
+        for (int cy = CF * PY; cy < cy1; cy++) {
             const int f = cx * Kyf + cy;
             d += df[f];
             for (int k = 0; k < 8; k++) {
@@ -165,7 +171,7 @@ __global__ __launch_bounds__(B) void k_coarsen(int Kxc, int Kyc, int Kxf, int Ky
                 bool okx, oky;
                 const int nx = wrap(cx + ox, Kxf, xper, okx), ny = wrap(cy + oy, Kyf, yper, oky);
                 if (!okx || !oky) continue;
-                const int dx = fold(nx / 4 - PX, Kxc, xper), dy = fold(ny / 4 - PY, Kyc, yper);
+                const int dx = fold(nx / CF - PX, Kxc, xper), dy = fold(ny / CF - PY, Kyc, yper);
                 if (dx == 0 && dy == 0) continue;
                 c[slot_of(dx, dy)] += ck;
             }
@@ -181,18 +187,17 @@ __global__ __launch_bounds__(B) void k_coarsen(int Kxc, int Kyc, int Kxf, int Ky
 }
 
 // the coarsest level as a dense FP64 matrix (plus sigma/nc everywhere for a singular system, which
-// makes it definite without changing its action on mean-free vectors), inverted by Gauss-Jordan in
-// one workgroup (an SPD matrix needs no pivoting); the inverse is stored column-major in FP32
+// makes it definite without changing its action on mean-free vectors), inverted in place in LDS by
+// Gauss-Jordan in one workgroup (an SPD matrix needs no pivoting); the inverse is stored
+// column-major in FP32
 __global__ __launch_bounds__(256) void k_dense_inverse(int Kx, int Ky, int xper, int yper, const float *__restrict__ d,
                                                        const float *__restrict__ c, int singular,
-                                                       double *__restrict__ M, double *__restrict__ X,
                                                        float *__restrict__ inv)
 {
     const int nc = Kx * Ky;
-    for (int t = threadIdx.x; t < nc * nc; t += 256) {
-        M[t] = 0.0;
-        X[t] = (t / nc == t % nc) ? 1.0 : 0.0;
-    }
+    __shared__ double M[LV_AMG_DENSE_MAX * LV_AMG_DENSE_MAX];
+    __shared__ double colp[LV_AMG_DENSE_MAX];
+    for (int t = threadIdx.x; t < nc * nc; t += 256) M[t] = 0.0;
     __syncthreads();
     if (threadIdx.x == 0) {
         double dsum = 0.0;
@@ -208,9 +213,9 @@ __global__ __launch_bounds__(256) void k_dense_inverse(int Kx, int Ky, int xper,
                 const int jx = wrap(Ix + ox, Kx, xper, okx), jy = wrap(Iy + oy, Ky, yper, oky);
                 if (!okx || !oky) continue;
                 diag += ck;
-                M[(size_t)I * nc + jx * Ky + jy] -= ck;
+                M[I * nc + jx * Ky + jy] -= ck;
             }
-            M[(size_t)I * nc + I] += diag;
+            M[I * nc + I] += diag;
             dsum += diag;
         }
         if (singular) {
@@ -219,29 +224,24 @@ __global__ __launch_bounds__(256) void k_dense_inverse(int Kx, int Ky, int xper,
         }
     }
     __syncthreads();
-    __shared__ double colp[LV_AMG_DENSE_MAX];
+    // in place: after pivot p, row p holds row p of the partial inverse and column p the
+    // eliminated multipliers
     for (int p = 0; p < nc; p++) {
-        // column p (scaled by the pivot) is copied first: the row updates below zero it
-        const double piv = M[(size_t)p * nc + p];
-        for (int i = threadIdx.x; i < nc; i += 256) colp[i] = M[(size_t)i * nc + p];
+        const double piv = M[p * nc + p];
+        for (int i = threadIdx.x; i < nc; i += 256) colp[i] = M[i * nc + p];
         __syncthreads();
-        for (int j = threadIdx.x; j < nc; j += 256) {
-            const double mp = M[(size_t)p * nc + j] / piv, xp = X[(size_t)p * nc + j] / piv;
-            M[(size_t)p * nc + j] = mp;
-            X[(size_t)p * nc + j] = xp;
-            for (int i = 0; i < nc; i++) {
-                if (i == p) continue;
-                const double f = colp[i];
-                if (f == 0.0) continue;
-                M[(size_t)i * nc + j] -= f * mp;
-                X[(size_t)i * nc + j] -= f * xp;
-            }
+        for (int j = threadIdx.x; j < nc; j += 256) M[p * nc + j] = (j == p ? 1.0 : M[p * nc + j]) / piv;
+        __syncthreads();
+        for (int t = threadIdx.x; t < nc * nc; t += 256) {
+            const int i = t / nc, j = t % nc;
+            if (i == p) continue;
+            M[t] = (j == p ? 0.0 : M[t]) - colp[i] * M[p * nc + j];
         }
         __syncthreads();
     }
     for (int t = threadIdx.x; t < nc * nc; t += 256) {
         const int i = t / nc, j = t % nc;
-        inv[(size_t)j * nc + i] = (float)X[t];
+        inv[j * nc + i] = (float)M[t];
     }
 }
 
@@ -508,105 +508,152 @@ struct st_level {
     const float *d, *c, *dinv;
 };
 
-__device__ __forceinline__ float st_apply(const st_level &L, int I, const float *__restrict__ z)
+// the 8 neighbour cells of (cx, cy) on level L and their couplings; a neighbour outside a
+// non-periodic level gets coupling 0 and index f, so every load below is in bounds and all of them
+// can be issued at once
+__device__ __forceinline__ void st_nbrs(const st_level &L, int cx, int cy, int f, float ck[8], int J[8])
 {
-    const int n = L.Kx * L.Ky, Ix = I / L.Ky, Iy = I % L.Ky;
-    const float zi = z[I];
-    float acc = L.d[I] * zi;
+    const int n = L.Kx * L.Ky;
 #pragma unroll
     for (int k = 0; k < 8; k++) {
-        const float ck = L.c[(size_t)k * n + I];
-        if (ck == 0.0f) continue;
         int ox, oy;
         offset_of(k, ox, oy);
         bool okx, oky;
-        const int jx = wrap(Ix + ox, L.Kx, L.xper, okx), jy = wrap(Iy + oy, L.Ky, L.yper, oky);
-        if (!okx || !oky) continue;
-        acc += ck * (zi - z[jx * L.Ky + jy]);
+        const int jx = wrap(cx + ox, L.Kx, L.xper, okx), jy = wrap(cy + oy, L.Ky, L.yper, oky);
+        const bool ok = okx && oky;
+        J[k] = ok ? jx * L.Ky + jy : f;
+        ck[k] = ok ? L.c[(size_t)k * n + f] : 0.0f;
     }
-    return acc;
 }
 
-// down: for every child f of parent P (16 lanes per parent): z_f = w D^-1 r_f, t_f = r_f - (A z)_f
-// with the neighbours' z recomputed from their r; rc[P] = sum of the children's t (lane tree)
+// down, one lane: child q of parent P (CF2 lanes per parent): z_f = w D^-1 r_f and the child's
+// residual t_f = r_f - (A z)_f, with the neighbours' z recomputed from their r (0 outside the level)
+__device__ __forceinline__ float st_down_lane(const st_level &L, int Kyc, int nc, int P, int q,
+                                              const float *__restrict__ r, float *__restrict__ z)
+{
+    if (P >= nc) return 0.0f;
+    const int cx = CF * (P / Kyc) + q / CF, cy = CF * (P % Kyc) + q % CF;
+    if (cx >= L.Kx || cy >= L.Ky) return 0.0f;
+    const int f = cx * L.Ky + cy;
+    float ck[8], rj[8], dj[8];
+    int J[8];
+    st_nbrs(L, cx, cy, f, ck, J);
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        rj[k] = r[J[k]];
+        dj[k] = L.dinv[J[k]];
+    }
+    const float rf = r[f];
+    const float zf = OMEGA * L.dinv[f] * rf;
+    float acc = L.d[f] * zf;
+#pragma unroll
+    for (int k = 0; k < 8; k++) acc += ck[k] * (zf - OMEGA * dj[k] * rj[k]);
+    z[f] = zf;
+    return rf - acc;
+}
+
+// the children's residuals of CF2 consecutive lanes summed by a fixed shuffle tree (every lane of
+// the wave takes part)
+__device__ __forceinline__ float sum_children(float t)
+{
+#pragma unroll
+    for (int m = 1; m < CF2; m <<= 1) t += __shfl_xor(t, m);
+    return t;
+}
+
+// up, one cell: z1 = z + ec[parent] (neighbours' z1 recomputed), t = r - A z1, out = z1 + w D^-1 t
+__device__ __forceinline__ void st_up_cell(const st_level &L, int Kyc, int f, const float *__restrict__ r,
+                                           const float *__restrict__ z, const float *__restrict__ ec,
+                                           float *__restrict__ out)
+{
+    const int cx = f / L.Ky, cy = f % L.Ky;
+    float ck[8], zj[8], ej[8];
+    int J[8];
+    st_nbrs(L, cx, cy, f, ck, J);
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        zj[k] = z[J[k]];
+        ej[k] = ec[(J[k] / L.Ky / CF) * Kyc + (J[k] % L.Ky) / CF];
+    }
+    const float z1 = z[f] + ec[(cx / CF) * Kyc + cy / CF];
+    float acc = L.d[f] * z1;
+#pragma unroll
+    for (int k = 0; k < 8; k++) acc += ck[k] * (z1 - (zj[k] + ej[k]));
+    out[f] = z1 + OMEGA * L.dinv[f] * (r[f] - acc);
+}
+
+// down on level L: pre-smoothed z of L and the right-hand side rc of the next level
 __global__ __launch_bounds__(B) void k_st_down(st_level L, int Kyc, int nc, const float *__restrict__ r,
                                                float *__restrict__ z, float *__restrict__ rc, const lv_cg_scalars *S)
 {
     if (S && S->done) return;
     const int tid = blockIdx.x * B + threadIdx.x;
-    const int P = tid >> 4, q = tid & 15;
-    float t = 0.0f;
-    if (P < nc) {
-        const int cx = 4 * (P / Kyc) + (q >> 2), cy = 4 * (P % Kyc) + (q & 3);
-        if (cx < L.Kx && cy < L.Ky) {
-            const int n = L.Kx * L.Ky, f = cx * L.Ky + cy;
-            const float rf = r[f];
-            const float zf = OMEGA * L.dinv[f] * rf;
-            float acc = L.d[f] * zf;
-#pragma unroll
-            for (int k = 0; k < 8; k++) {
-                const float ck = L.c[(size_t)k * n + f];
-                if (ck == 0.0f) continue;
-                int ox, oy;
-                offset_of(k, ox, oy);
-                bool okx, oky;
-                const int jx = wrap(cx + ox, L.Kx, L.xper, okx), jy = wrap(cy + oy, L.Ky, L.yper, oky);
-                if (!okx || !oky) continue;
-                const int J = jx * L.Ky + jy;
-                acc += ck * (zf - OMEGA * L.dinv[J] * r[J]);
-            }
-            z[f] = zf;
-            t = rf - acc;
-        }
-    }
-    t += __shfl_xor(t, 1);
-    t += __shfl_xor(t, 2);
-    t += __shfl_xor(t, 4);
-    t += __shfl_xor(t, 8);
+    const int P = tid / CF2, q = tid % CF2;
+    const float t = sum_children(st_down_lane(L, Kyc, nc, P, q, r, z));
     if (q == 0 && P < nc) rc[P] = t;
 }
 
-// up: z1 = z + ec[parent] (neighbours' z1 recomputed), t = r - A z1, out = z1 + w D^-1 t
+// up on level L: the final correction out of L from the next level's correction ec
 __global__ __launch_bounds__(B) void k_st_up(st_level L, int Kyc, const float *__restrict__ r,
                                              const float *__restrict__ z, const float *__restrict__ ec,
                                              float *__restrict__ out, const lv_cg_scalars *S)
 {
     if (S && S->done) return;
     const int f = blockIdx.x * B + threadIdx.x;
-    const int n = L.Kx * L.Ky;
-    if (f >= n) return;
-    const int cx = f / L.Ky, cy = f % L.Ky;
-    const float z1 = z[f] + ec[(cx >> 2) * Kyc + (cy >> 2)];
-    float acc = L.d[f] * z1;
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-        const float ck = L.c[(size_t)k * n + f];
-        if (ck == 0.0f) continue;
-        int ox, oy;
-        offset_of(k, ox, oy);
-        bool okx, oky;
-        const int jx = wrap(cx + ox, L.Kx, L.xper, okx), jy = wrap(cy + oy, L.Ky, L.yper, oky);
-        if (!okx || !oky) continue;
-        const int J = jx * L.Ky + jy;
-        acc += ck * (z1 - (z[J] + ec[(jx >> 2) * Kyc + (jy >> 2)]));
-    }
-    out[f] = z1 + OMEGA * L.dinv[f] * (r[f] - acc);
+    if (f >= L.Kx * L.Ky) return;
+    st_up_cell(L, Kyc, f, r, z, ec, out);
 }
 
-// z = inv r on the coarsest level (column-major inverse, one thread per row)
-__global__ __launch_bounds__(256) void k_dense_apply(int nc, const float *__restrict__ inv, const float *__restrict__ r,
-                                                     float *__restrict__ z, const lv_cg_scalars *S)
+// The small levels of the cycle in one workgroup: the down sweeps of levels first..last-1, the
+// dense solve on level last and the up sweeps back to level first, separated by barriers (the
+// levels' vectors stay in global memory, which the barrier makes visible across the workgroup).
+// Reads the right-hand side of level first, writes its final correction.
+constexpr int TB = 1024;
+constexpr int TAIL_MAX = 1024;  // levels of at most this many cells run in the tail
+struct tail_levels {
+    int first, last, xper, yper;
+    int Kx[LV_AMG_MAXLEV + 1], Ky[LV_AMG_MAXLEV + 1];
+    const float *d[LV_AMG_MAXLEV + 1], *c[LV_AMG_MAXLEV + 1], *dinv[LV_AMG_MAXLEV + 1];
+    float *r[LV_AMG_MAXLEV + 1], *z[LV_AMG_MAXLEV + 1], *t[LV_AMG_MAXLEV + 1];
+};
+
+__global__ __launch_bounds__(TB) void k_tail(tail_levels T, const float *__restrict__ inv, const lv_cg_scalars *S)
 {
     if (S && S->done) return;
-    __shared__ float sr[LV_AMG_DENSE_MAX];
-    for (int j = threadIdx.x; j < nc; j += 256) sr[j] = r[j];
-    __syncthreads();
-    for (int i = threadIdx.x; i < nc; i += 256) {
+    for (int l = T.first; l < T.last; l++) {
+        const st_level L{T.Kx[l], T.Ky[l], T.xper, T.yper, T.d[l], T.c[l], T.dinv[l]};
+        const int Kyc = T.Ky[l + 1], nc = T.Kx[l + 1] * T.Ky[l + 1];
+        for (int base = 0; base < nc * CF2; base += TB) {  // uniform trip count: every lane shuffles
+            const int tid = base + threadIdx.x;
+            const int P = tid / CF2, q = tid % CF2;
+            const float t = sum_children(st_down_lane(L, Kyc, nc, P, q, T.r[l], T.z[l]));
+            if (q == 0 && P < nc) T.r[l + 1][P] = t;
+        }
+        __syncthreads();
+    }
+    {
+        // t = inv r on the dense level: 16 lanes per row (LV_AMG_DENSE_MAX * 16 == TB)
+        const int nc = T.Kx[T.last] * T.Ky[T.last];
+        const int i = threadIdx.x >> 4, q = threadIdx.x & 15;
+        const float *__restrict__ rd = T.r[T.last];
         float acc = 0.0f;
-        for (int j = 0; j < nc; j++) acc += inv[(size_t)j * nc + i] * sr[j];
-        z[i] = acc;
+        if (i < nc)
+            for (int j = q; j < nc; j += 16) acc += inv[j * nc + i] * rd[j];
+        acc += __shfl_xor(acc, 1);
+        acc += __shfl_xor(acc, 2);
+        acc += __shfl_xor(acc, 4);
+        acc += __shfl_xor(acc, 8);
+        if (q == 0 && i < nc) T.t[T.last][i] = acc;
+        __syncthreads();
+    }
+    for (int l = T.last - 1; l >= T.first; l--) {
+        const st_level L{T.Kx[l], T.Ky[l], T.xper, T.yper, T.d[l], T.c[l], T.dinv[l]};
+        const int n = T.Kx[l] * T.Ky[l];
+        for (int f = threadIdx.x; f < n; f += TB) st_up_cell(L, T.Ky[l + 1], f, T.r[l], T.z[l], T.t[l + 1], T.t[l]);
+        __syncthreads();
     }
 }
+static_assert(LV_AMG_DENSE_MAX * 16 == TB, "the dense solve of the tail maps 16 lanes to a row");
 
 // ---- power iteration for the scale of the learned fine smoother ----------------------------------
 
@@ -679,7 +726,6 @@ static hipError_t grow(T **p, size_t *cap, size_t want)
 hipError_t lv_amg_init(lv_amg *G)
 {
     memset(G, 0, sizeof(*G));
-    AMG_CHECK(hipMalloc(&G->dense_work, sizeof(double) * 2 * LV_AMG_DENSE_MAX * LV_AMG_DENSE_MAX));
     AMG_CHECK(hipMalloc(&G->dense_inv, sizeof(float) * LV_AMG_DENSE_MAX * LV_AMG_DENSE_MAX));
     AMG_CHECK(hipMalloc(&G->pw, sizeof(double) * (2 * LV_NBMAX + 2)));
     return hipSuccess;
@@ -687,8 +733,8 @@ hipError_t lv_amg_init(lv_amg *G)
 
 void lv_amg_free(lv_amg *G)
 {
-    void *ptrs[] = {G->agg, G->cnt, G->start, G->cursor, G->list, G->scan_tmp, G->t0, G->z0, G->dense_work,
-                    G->dense_inv, G->pw, G->pv, G->pu};
+    void *ptrs[] = {G->agg, G->cnt, G->start, G->cursor, G->list, G->scan_tmp, G->t0, G->z0, G->dense_inv,
+                    G->pw, G->pv, G->pu};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (int l = 0; l <= LV_AMG_MAXLEV; l++) {
@@ -768,7 +814,7 @@ hipError_t lv_amg_setup(lv_amg *G, const lv_grid_params &g, const double *x, con
     int l = 1;
     while (G->lev[l].n > LV_AMG_DENSE_MAX && l < LV_AMG_MAXLEV) {
         const lv_amg_level &F = G->lev[l];
-        AMG_CHECK(level_alloc(G->lev[l + 1], (F.Kx + 3) / 4, (F.Ky + 3) / 4));
+        AMG_CHECK(level_alloc(G->lev[l + 1], (F.Kx + CF - 1) / CF, (F.Ky + CF - 1) / CF));
         lv_amg_level &C = G->lev[l + 1];
         hipLaunchKernelGGL(k_coarsen, g1(C.n), dim3(B), 0, s, C.Kx, C.Ky, F.Kx, F.Ky, G->xper, G->yper,
                            (const float *)F.d, (const float *)F.c, C.d, C.c, C.dinv);
@@ -778,8 +824,7 @@ hipError_t lv_amg_setup(lv_amg *G, const lv_grid_params &g, const double *x, con
     G->nlev = l;
     const lv_amg_level &D = G->lev[l];
     hipLaunchKernelGGL(k_dense_inverse, dim3(1), dim3(256), 0, s, D.Kx, D.Ky, G->xper, G->yper, (const float *)D.d,
-                       (const float *)D.c, A.singular, G->dense_work,
-                       G->dense_work + (size_t)LV_AMG_DENSE_MAX * LV_AMG_DENSE_MAX, G->dense_inv);
+                       (const float *)D.c, A.singular, G->dense_inv);
     // fine smoother and its scale
     G->fine_learned = (mval && mdg) ? 1 : 0;
     G->mval = mval;
@@ -830,19 +875,37 @@ hipError_t lv_amg_vcycle(const lv_amg *G, const lv_pressure_system &A, const flo
     hipLaunchKernelGGL(k_restrict1, g1((int64_t)G->na * 16), dim3(B), 0, s, G->na, (const int *)G->start,
                        (const int *)G->list,
                        (const float *)G->t0, L1.r, S);
-    // coarse levels, down: each level's pre-smoothed z and the next level's right-hand side
-    for (int l = 1; l < G->nlev; l++) {
+    // coarse levels above the tail, down: each level's pre-smoothed z and the next level's
+    // right-hand side
+    int lt = G->nlev;
+    while (lt > 1 && G->lev[lt - 1].n <= TAIL_MAX) lt--;
+    for (int l = 1; l < lt; l++) {
         const lv_amg_level &L = G->lev[l], &C = G->lev[l + 1];
         const st_level sl{L.Kx, L.Ky, G->xper, G->yper, L.d, L.c, L.dinv};
-        hipLaunchKernelGGL(k_st_down, g1((int64_t)C.n * 16), dim3(B), 0, s, sl, C.Ky, C.n, (const float *)L.r, L.z,
+        hipLaunchKernelGGL(k_st_down, g1((int64_t)C.n * CF2), dim3(B), 0, s, sl, C.Ky, C.n, (const float *)L.r, L.z,
                            C.r, S);
     }
-    // coarsest level: its correction in t
-    const lv_amg_level &D = G->lev[G->nlev];
-    hipLaunchKernelGGL(k_dense_apply, dim3(1), dim3(256), 0, s, D.n, (const float *)G->dense_inv, (const float *)D.r,
-                       D.t, S);
-    // coarse levels, up: each level's final correction in t
-    for (int l = G->nlev - 1; l >= 1; l--) {
+    // the tail: levels lt..nlev down, dense, and up; the final correction of level lt in its t
+    tail_levels T;
+    memset(&T, 0, sizeof(T));
+    T.first = lt;
+    T.last = G->nlev;
+    T.xper = G->xper;
+    T.yper = G->yper;
+    for (int l = lt; l <= G->nlev; l++) {
+        const lv_amg_level &L = G->lev[l];
+        T.Kx[l] = L.Kx;
+        T.Ky[l] = L.Ky;
+        T.d[l] = L.d;
+        T.c[l] = L.c;
+        T.dinv[l] = L.dinv;
+        T.r[l] = L.r;
+        T.z[l] = L.z;
+        T.t[l] = L.t;
+    }
+    hipLaunchKernelGGL(k_tail, dim3(1), dim3(TB), 0, s, T, (const float *)G->dense_inv, S);
+    // coarse levels above the tail, up: each level's final correction in t
+    for (int l = lt - 1; l >= 1; l--) {
         const lv_amg_level &L = G->lev[l], &C = G->lev[l + 1];
         const st_level sl{L.Kx, L.Ky, G->xper, G->yper, L.d, L.c, L.dinv};
         hipLaunchKernelGGL(k_st_up, g1(L.n), dim3(B), 0, s, sl, C.Ky, (const float *)L.r, (const float *)L.z,

@@ -7,7 +7,7 @@ numpy/scipy FP64:
   * level 1: cells of a regular grid of width >= 4 dr over the boundary rectangle, each particle in
     the cell holding it; Galerkin sums of d and of the couplings to the 8 neighbouring cells
     (couplings reaching further are dropped from both sides);
-  * level l+1: 4 x 4 blocks of level-l cells; the coarsest level (<= 256 cells) inverted densely,
+  * level l+1: BLOCK x BLOCK (2 x 2) blocks of level-l cells; the coarsest level (<= 64 cells) inverted densely,
     with sigma/nc added to every entry of a singular system (sigma = the sum of its diagonal / nc);
   * symmetric V(1,1): fine smoother s*M (the learned M, s from 10 power iterations towards
     lambda_max(s M A) = 1.2) or (2/3) D^-1; coarse smoothers (2/3) D^-1.
@@ -19,8 +19,9 @@ of the cycle, iteration counts against Jacobi).
 import numpy as np
 import scipy.sparse as sp
 
-DENSE_MAX = 256
+DENSE_MAX = 64
 OMEGA = 2.0 / 3.0
+BLOCK = 2  # coarse levels: BLOCK x BLOCK blocks of the finer level's cells
 FINE_TARGET = 1.2
 OFFS = [(dx, dy) for dx in (-1, 0, 1) for dy in (-1, 0, 1) if (dx, dy) != (0, 0)]
 
@@ -99,17 +100,17 @@ class Hierarchy:
         self.levels = [Level(Kx, Ky, d1, c1, xper, yper)]
         while self.levels[-1].n > DENSE_MAX:
             F = self.levels[-1]
-            Kxc, Kyc = (F.Kx + 3) // 4, (F.Ky + 3) // 4
+            Kxc, Kyc = (F.Kx + BLOCK - 1) // BLOCK, (F.Ky + BLOCK - 1) // BLOCK
             f = np.arange(F.n)
             fx, fy = f // F.Ky, f % F.Ky
-            P = (fx // 4) * Kyc + fy // 4
+            P = (fx // BLOCK) * Kyc + fy // BLOCK
             dc = np.bincount(P, weights=F.d, minlength=Kxc * Kyc)
             cc = np.zeros((8, Kxc * Kyc))
             for k, (ox, oy) in enumerate(OFFS):
                 nx, okx = _wrap(fx + ox, F.Kx, xper)
                 ny, oky = _wrap(fy + oy, F.Ky, yper)
-                ddx = _fold(nx // 4 - fx // 4, Kxc, xper)
-                ddy = _fold(ny // 4 - fy // 4, Kyc, yper)
+                ddx = _fold(nx // BLOCK - fx // BLOCK, Kxc, xper)
+                ddy = _fold(ny // BLOCK - fy // BLOCK, Kyc, yper)
                 ok = okx & oky & (F.c[k] != 0) & ~((ddx == 0) & (ddy == 0))
                 for q in np.where(ok)[0]:
                     cc[OFFS.index((int(ddx[q]), int(ddy[q]))), P[q]] += F.c[k][q]
@@ -139,7 +140,7 @@ class Hierarchy:
         t = r - L.A @ z
         C = self.levels[l + 1]
         f = np.arange(L.n)
-        P = (f // L.Ky // 4) * C.Ky + (f % L.Ky) // 4
+        P = (f // L.Ky // BLOCK) * C.Ky + (f % L.Ky) // BLOCK
         ec = self._cycle(l + 1, np.bincount(P, weights=t, minlength=C.n))
         z = z + ec[P]
         t = r - L.A @ z

@@ -41,7 +41,7 @@ def test_coarse_operators_are_galerkin_products(lv, oracle):
     assert np.abs(G1 - H.levels[0].A.toarray()).max() <= 1e-12 * np.abs(G1).max()
     L, C = H.levels[0], H.levels[1]
     f = np.arange(L.n)
-    par = (f // L.Ky // 4) * C.Ky + (f % L.Ky) // 4
+    par = (f // L.Ky // 2) * C.Ky + (f % L.Ky) // 2  # 2 x 2 blocks
     P2 = sp.csr_matrix((np.ones(L.n), (f, par)), shape=(L.n, C.n))
     G2 = (P2.T @ L.A @ P2).toarray()
     assert np.abs(G2 - C.A.toarray()).max() <= 1e-12 * np.abs(G2).max()

@@ -12,8 +12,12 @@ struct lv_amg_level {
     int Kx, Ky, n;
     float *d, *c, *dinv;  // d[n], c[8n] slot-major, dinv[n] = 1 / (d + sum c)
     float *r, *z, *t;     // right-hand side, pre-smoothed correction, final correction of the cycle
+    float *acc;           // the first correction of a W-cycle visit
+    float alpha;          // factor on the correction prolongated into this level from the next
     size_t cap;
 };
+
+#define LV_AMG_MAXPROG 1024  // operations of the single-workgroup program for the smallest levels
 
 struct lv_amg {
     int nlev;       // coarse levels; level nlev is the dense one
@@ -28,6 +32,12 @@ struct lv_amg {
     double *pw;          // power-iteration partials and norms
     float *pv, *pu;      // power-iteration vectors (n0)
     size_t cap_pv, cap_pu;
+    int ltail;           // first level of the single-workgroup kernel (levels ltail..nlev)
+    int wcycle;          // 1: the levels of that kernel are visited by a W-cycle
+    int *prog;           // its operations (device), nprog of them; hprog: the host copy
+    int nprog;
+    int hprog[LV_AMG_MAXPROG];
+    float fine_alpha;    // factor on the correction prolongated into the fine level
     float fine_scale;    // the fine smoother applies fine_scale * M (learned) or fine_scale * D^-1 (Jacobi)
     int fine_learned;    // 1: the learned preconditioner's M is the fine smoother
     const float *mval, *mdg;

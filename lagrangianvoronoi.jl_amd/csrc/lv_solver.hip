@@ -352,11 +352,15 @@ __global__ __launch_bounds__(B) void k_cg_s(int n, const int *__restrict__ deg, 
     block_partials<1>(v, Ppq, gridDim.x);
 }
 
+// x += alpha p, r -= alpha q.  With Jacobi (JACOBI) also z = D^-1 r and the partials of r.z, r.r
+// for the next k_cg_s; otherwise the partials of r.r alone (Pur), which k_conv_check tests before
+// the preconditioner is applied.
 template <typename T, bool JACOBI>
 __global__ __launch_bounds__(B) void k_cg_u(int n, T *__restrict__ x, T *__restrict__ r, T *__restrict__ z,
                                             const T *__restrict__ p, const T *__restrict__ q,
                                             const T *__restrict__ dinv, const double *__restrict__ Ppq,
-                                            double *__restrict__ Prr, lv_cg_scalars *S, int par)
+                                            double *__restrict__ Prr, double *__restrict__ Pur, lv_cg_scalars *S,
+                                            int par)
 {
     if (S->done) return;
     double pq[1];
@@ -372,13 +376,32 @@ __global__ __launch_bounds__(B) void k_cg_u(int n, T *__restrict__ x, T *__restr
             const T zi = dinv[i] * ri;
             z[i] = zi;
             v[0] += (double)ri * (double)zi;
-            v[1] += (double)ri * (double)ri;
         }
+        v[1] += (double)ri * (double)ri;
     }
-    if (JACOBI) block_partials<2>(v, Prr, gridDim.x);
+    if (JACOBI) {
+        block_partials<2>(v, Prr, gridDim.x);
+    } else {
+        const double w[1] = {v[1]};
+        block_partials<1>(w, Pur, gridDim.x);
+    }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         S->pq = pq[0];
         S->alpha = alpha;
+    }
+}
+
+// the convergence test of k_cg_s, made on the residual k_cg_u has just updated (after k+1
+// updates), so that a converged iteration skips the preconditioner
+__global__ __launch_bounds__(B) void k_conv_check(const double *__restrict__ Pur, int nb, lv_cg_scalars *S, int k)
+{
+    if (S->done) return;
+    double o[1];
+    reduce_all<1>(Pur, nb, o);
+    if (threadIdx.x == 0 && o[0] <= S->thr2) {
+        S->rr = o[0];
+        S->iters = k + 1;
+        S->done = 1;
     }
 }
 
@@ -637,9 +660,9 @@ hipError_t read_scalars(lv_cg_scalars *dS, lv_cg_scalars *hS, hipStream_t s)
 
 // partial-sum regions: P3 = one-off reductions (3 values), Ppq (1), Prr (2)
 struct parts {
-    double *p3, *pq, *rr;
+    double *p3, *pq, *rr, *ur;
 };
-inline parts parts_of(double *base) { return parts{base, base + 3 * NBMAX, base + 4 * NBMAX}; }
+inline parts parts_of(double *base) { return parts{base, base + 3 * NBMAX, base + 4 * NBMAX, base + 6 * NBMAX}; }
 
 // CG iterations (two launches each, three with the learned preconditioner) until the device flags
 // completion; the host checks the flag once per batch of iterations.
@@ -649,9 +672,9 @@ hipError_t cg_iterate(int n, const int *deg, const int *ecol, const T *ev, const
                       lv_cg_scalars &h, const lv_precond *M)
 {
     const int nb = nblocks_of(n);
-    int k = 0;
+    int k = 0, nit = batch;
     for (;;) {
-        for (int j = 0; j < batch; j++, k++) {
+        for (int j = 0; j < nit; j++, k++) {
             const int par = k & 1;
             {
                 timer_scope ts(tm, s);
@@ -660,7 +683,8 @@ hipError_t cg_iterate(int n, const int *deg, const int *ecol, const T *ev, const
             }
             if (M) {
                 hipLaunchKernelGGL((k_cg_u<T, false>), dim3(nb), dim3(B), 0, s, n, x, r, z, (const T *)p,
-                                   (const T *)q, dinv, (const double *)P.pq, P.rr, dS, par);
+                                   (const T *)q, dinv, (const double *)P.pq, P.rr, P.ur, dS, par);
+                hipLaunchKernelGGL(k_conv_check, dim3(1), dim3(B), 0, s, (const double *)P.ur, nb, dS, k);
                 if (M->amg) {
                     hipError_t e = lv_amg_vcycle(M->amg, *M->A, (const float *)r, (float *)z, P.rr,
                                                  (const lv_cg_scalars *)dS, s);
@@ -671,13 +695,21 @@ hipError_t cg_iterate(int n, const int *deg, const int *ecol, const T *ev, const
                 }
             } else {
                 hipLaunchKernelGGL((k_cg_u<T, true>), dim3(nb), dim3(B), 0, s, n, x, r, z, (const T *)p,
-                                   (const T *)q, dinv, (const double *)P.pq, P.rr, dS, par);
+                                   (const T *)q, dinv, (const double *)P.pq, P.rr, P.ur, dS, par);
             }
         }
         hipError_t e = read_scalars(dS, &h, s);
         if (e != hipSuccess) return e;
         if (tm) lv_timer_flush(tm);
         if (h.done) return hipSuccess;
+        // the next batch: the iterations still needed at the rate observed so far, plus one (an
+        // iteration past convergence costs its launches as no-ops; a short batch costs a sync)
+        nit = batch;
+        if (h.iters > 0 && h.rr > 0.0 && h.rr < h.rr0 && h.thr2 > 0.0 && h.thr2 < h.rr) {
+            const double need = log(h.thr2 / h.rr) / (log(h.rr / h.rr0) / h.iters);
+            nit = need < 30.0 ? (int)ceil(need) + 1 : 32;
+            if (nit < 2) nit = 2;
+        }
     }
 }
 
@@ -706,7 +738,7 @@ hipError_t lv_solve_device(const lv_pressure_system &A, lv_solver_work &W, int n
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_means, G, dim3(B), 0, s, n, b, (const double *)W.x, P.p3);
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(B), 0, s, (const double *)P.p3, nb, (int)FIN_MEANS, n, 0.0, W.S);
-    const int batch = 16;
+    const int batch = 8;  // the first batch of a sweep; later ones follow the observed rate
     int total = 0, nsweep = 0;
     if (opt.method == LV_SOLVER_PCG64) {
         hipLaunchKernelGGL(k_cg64_init, G, dim3(B), 0, s, n, (const int *)A.deg, (const int *)A.ecol,

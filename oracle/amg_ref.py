@@ -10,7 +10,11 @@ numpy/scipy FP64:
   * level l+1: BLOCK x BLOCK (2 x 2) blocks of level-l cells; the coarsest level (<= 64 cells) inverted densely,
     with sigma/nc added to every entry of a singular system (sigma = the sum of its diagonal / nc);
   * symmetric V(1,1): fine smoother s*M (the learned M, s from 10 power iterations towards
-    lambda_max(s M A) = 1.2) or (2/3) D^-1; coarse smoothers (2/3) D^-1.
+    lambda_max(s M A) = 1.2) or (2/3) D^-1; coarse smoothers (2/3) D^-1;
+  * the prolongated coarse correction scaled by 1.6 into the fine level and by 1.4 into a coarse
+    level, except for corrections from levels of at most TAIL_MAX cells (scale 1);
+  * for a singular system, the levels of at most TAIL_MAX cells (other than the dense one) are
+    visited twice (W-cycle): the second visit solves for the residual the first correction leaves.
 
 No counterpart exists in the reference (src/pressure.jl:219 runs MINRES unpreconditioned).  The
 oracle is itself pinned by tests/test_amg_oracle.py (Galerkin identity, symmetry and definiteness
@@ -22,6 +26,9 @@ import scipy.sparse as sp
 DENSE_MAX = 64
 OMEGA = 2.0 / 3.0
 BLOCK = 2  # coarse levels: BLOCK x BLOCK blocks of the finer level's cells
+TAIL_MAX = 1024
+ALPHA_FINE = 1.6
+ALPHA_COARSE = 1.4
 FINE_TARGET = 1.2
 OFFS = [(dx, dy) for dx in (-1, 0, 1) for dy in (-1, 0, 1) if (dx, dy) != (0, 0)]
 
@@ -119,6 +126,11 @@ class Hierarchy:
         if singular:
             Dn = Dn + np.trace(Dn) / Dn.shape[0] / Dn.shape[0]
         self.inv = np.linalg.inv(Dn)
+        self.W = bool(singular)
+        nl = len(self.levels)
+        # alpha[k]: factor on the correction prolongated into levels[k] from levels[k+1]
+        self.alpha = [ALPHA_COARSE if (k + 1 < nl and self.levels[k + 1].n > TAIL_MAX) else 1.0 for k in range(nl)]
+        self.alpha_fine = ALPHA_FINE if self.levels[0].n > TAIL_MAX else 1.0
         self.M = M
         self.scale = OMEGA
         if M is not None:
@@ -132,6 +144,14 @@ class Hierarchy:
     def _fine(self, r):
         return self.scale * (self.M @ r if self.M is not None else self.Dinv * r)
 
+    def _visit(self, l, r):
+        """the correction for levels[l] from its right-hand side: one V-cycle visit, or two (the
+        second on the remaining residual) for a W-visited level"""
+        e = self._cycle(l, r)
+        if self.W and l < len(self.levels) - 1 and self.levels[l].n <= TAIL_MAX:
+            e = e + self._cycle(l, r - self.levels[l].A @ e)
+        return e
+
     def _cycle(self, l, r):
         if l == len(self.levels) - 1:
             return self.inv @ r
@@ -141,8 +161,8 @@ class Hierarchy:
         C = self.levels[l + 1]
         f = np.arange(L.n)
         P = (f // L.Ky // BLOCK) * C.Ky + (f % L.Ky) // BLOCK
-        ec = self._cycle(l + 1, np.bincount(P, weights=t, minlength=C.n))
-        z = z + ec[P]
+        ec = self._visit(l + 1, np.bincount(P, weights=t, minlength=C.n))
+        z = z + self.alpha[l] * ec[P]
         t = r - L.A @ z
         return z + OMEGA * L.dinv * t
 
@@ -150,8 +170,8 @@ class Hierarchy:
         """z = B r, the symmetric V(1,1) cycle"""
         z = self._fine(r)
         t = r - self.A @ z
-        ec = self._cycle(0, np.bincount(self.agg, weights=t, minlength=self.levels[0].n))
-        z = z + ec[self.agg]
+        ec = self._visit(0, np.bincount(self.agg, weights=t, minlength=self.levels[0].n))
+        z = z + self.alpha_fine * ec[self.agg]
         t = r - self.A @ z
         return z + self._fine(t)
 

@@ -57,6 +57,25 @@ def test_cycle_is_symmetric_positive_definite(lv, oracle, kind, ic):
     assert ev.min() > 0.0
 
 
+@pytest.mark.parametrize("kind,ic", [("periodic", "tg"), ("walls", "cavity")])
+def test_deep_cycle_with_overcorrection_is_symmetric_positive_definite(lv, oracle, kind, ic, monkeypatch):
+    """Four levels below the particles (144, 36, 9 and a dense 4 cells, with the dense and
+    small-level thresholds lowered to 4 and 20 cells so that a 48^2 system has them): the fine and
+    first coarse corrections are scaled by 1.6 and 1.4, and for the singular walled system the
+    9-cell level is W-visited.  The cycle stays symmetric and positive definite (dense check)."""
+    monkeypatch.setattr(amg_ref, "DENSE_MAX", 4)
+    monkeypatch.setattr(amg_ref, "TAIL_MAX", 20)
+    H, p, d, singular = system(lv, oracle, kind, 48, ic, 2e-2)
+    assert [L.n for L in H.levels] == [144, 36, 9, 4]
+    assert H.alpha_fine == amg_ref.ALPHA_FINE and H.alpha[:3] == [amg_ref.ALPHA_COARSE, 1.0, 1.0]
+    assert H.W == singular
+    n = H.n
+    Bm = np.stack([H.apply(e) for e in np.eye(n)], axis=1)
+    assert np.abs(Bm - Bm.T).max() <= 1e-10 * np.abs(Bm).max()
+    ev = np.linalg.eigvalsh(0.5 * (Bm + Bm.T))
+    assert ev.min() > 1e-8 * ev.max()
+
+
 @pytest.mark.parametrize("kind,ic,K,dt", [("periodic", "tg", 128, 4e-3), ("walls", "cavity", 96, 1e-3)])
 def test_cycle_cuts_cg_iterations(lv, oracle, kind, ic, K, dt):
     H, p, d, singular = system(lv, oracle, kind, K, ic, dt)

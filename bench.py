@@ -15,6 +15,7 @@ and the oracle timed on this host's cores (cpu_baseline).
 """
 import argparse
 import json
+import re
 import math
 import os
 import sys
@@ -249,7 +250,7 @@ def setup(lv, K, seed=0x5EED0002):
 def traffic_from_profiles(kernel):
     """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE, copy-kernel
     calibrated), committed under profiles/ as JSON; None if absent."""
-    tag = kernel.replace("<", "_").replace(">", "").replace(" ", "")
+    tag = re.sub(r"[^A-Za-z0-9]+", "_", kernel).strip("_")
     path = os.path.join(ROOT, "profiles", "traffic_%s.json" % tag)
     if not os.path.exists(path):
         return None
@@ -347,7 +348,11 @@ def main():
     relres = max(max(s["relres"]) for s in stats)
     avg_launch_ms = tot_ms.value / max(launches.value, 1)
     achieved = bpl.value / (avg_launch_ms * 1e-3) / 1e9
-    kname = "k_cg_s<float>" if dominant == 0 else "k_cg_s<double>"
+    # the FP32 SpMV kernel in the layout the solves used (aggregate order: 8-bit degrees, 16-bit offsets)
+    if dominant == 0:
+        kname = "k_cg_s<float, lv_ell_agg32>" if stats[-1].get("agg_order") else "k_cg_s<float>"
+    else:
+        kname = "k_cg_s<double>"
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic_from_profiles(kname),
                     kernel=kname, avg_launch_us=round(avg_launch_ms * 1e3, 2), launches=launches.value,

@@ -87,6 +87,9 @@ typedef struct {
     int32_t picard_window; /* find_pressure!: 0 warm-starts each Picard solve from the previous P, as the
                               reference does; k > 0 starts it from the previous P plus the minimal-residual
                               combination of the last k Picard corrections (every solve still meets tol) */
+    int32_t amg_layout;   /* AMG: 0 runs the inner solve and the cycle with the rows renumbered cell by cell
+                              (16-bit column offsets, BF16 couplings in the cycle) when the offsets fit,
+                              1 keeps the operator's own row order (int32 columns, FP32 couplings) */
 } lv_solve_opts;
 
 typedef struct {
@@ -100,6 +103,7 @@ typedef struct {
     int32_t nsolves;
     int32_t multiphase_iters;   /* lv_step_fused with multiphase: CG iterations (-1: itmax reached) */
     double multiphase_relres;   /* ... and its final |r| / |b| */
+    int32_t agg_order;          /* 1: the FP32 inner solves ran in the multilevel preconditioner's aggregate order */
 } lv_solve_stats;
 
 /* One step!(sim, t) of tests/taylorgreen.jl:61-71 and its relatives (examples/cavity.jl:70-80,
@@ -211,7 +215,8 @@ int lv_solve_pressure(lv_ctx *ctx, const double *b, double *x, const lv_solve_op
 /* z = B r with the FP32 preconditioner of the inner solve of `method` (LV_SOLVER_MIXED: Jacobi,
  * LV_SOLVER_SPAI: the learned M, LV_SOLVER_AMG: the multilevel V-cycle) on the assembled operator;
  * host vectors of n doubles.  No counterpart in the reference; it is the hook a Julia caller uses
- * to hand this preconditioner to Krylov.jl (src/pressure.jl:219 runs minres! unpreconditioned). */
+ * to hand this preconditioner to Krylov.jl (src/pressure.jl:219 runs minres! unpreconditioned).
+ * The multilevel cycle keeps the amg_layout of the last solve on this context (0 before any). */
 int lv_apply_preconditioner(lv_ctx *ctx, int32_t method, const double *r, double *z);
 
 /* ---- learned SPAI preconditioner --------------------------------------------------------------
@@ -234,7 +239,8 @@ int lv_spai_outputs(lv_ctx *ctx, float *o_edge, float *o_node, float *w, float *
 /* ---- timing ----------------------------------------------------------------------------------- */
 /* When enabled, the library brackets every launch of the named kernel with HIP events on the
  * context stream and accumulates the device time. kernel ids: 0 = FP32 SpMV, 1 = FP64 SpMV,
- * 2 = clipping, 3 = cell-list scatter */
+ * 2 = clipping, 3 = cell-list scatter.  The algorithmic bytes of kernel 0 follow the layout of the
+ * last solve (the operator's own ELL, or the aggregate order's 8-bit degrees and 16-bit offsets). */
 int lv_timing_enable(lv_ctx *ctx, int32_t kernel_id, int32_t on);
 int lv_timing_read(lv_ctx *ctx, int32_t kernel_id, double *total_ms, int64_t *launches, double *bytes_per_launch);
 

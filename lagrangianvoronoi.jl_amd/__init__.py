@@ -64,14 +64,14 @@ class GridDesc(ctypes.Structure):
 class SolveOpts(ctypes.Structure):
     _fields_ = [("method", ctypes.c_int32), ("tol", ctypes.c_double), ("itmax", ctypes.c_int32),
                 ("inner_tol", ctypes.c_double), ("outer_max", ctypes.c_int32), ("vbc", ctypes.c_double * 8),
-                ("picard_window", ctypes.c_int32)]
+                ("picard_window", ctypes.c_int32), ("amg_layout", ctypes.c_int32)]
 
 
 class SolveStats(ctypes.Structure):
     _fields_ = [("iters", ctypes.c_int32 * 32), ("outer", ctypes.c_int32 * 32), ("relres", ctypes.c_double * 32),
                 ("relres0", ctypes.c_double * 32), ("solve_ms", ctypes.c_double * 32), ("total_iters", ctypes.c_int32),
                 ("fallbacks", ctypes.c_int32), ("nsolves", ctypes.c_int32), ("multiphase_iters", ctypes.c_int32),
-                ("multiphase_relres", ctypes.c_double)]
+                ("multiphase_relres", ctypes.c_double), ("agg_order", ctypes.c_int32)]
 
     def as_dict(self):
         k = min(self.nsolves, 32)
@@ -79,7 +79,7 @@ class SolveStats(ctypes.Structure):
                     relres0=list(self.relres0[:k]),
                     solve_ms=list(self.solve_ms[:k]), total_iters=self.total_iters, fallbacks=self.fallbacks,
                     nsolves=self.nsolves, multiphase_iters=self.multiphase_iters,
-                    multiphase_relres=self.multiphase_relres)
+                    multiphase_relres=self.multiphase_relres, agg_order=self.agg_order)
 
 
 class StepDesc(ctypes.Structure):
@@ -420,12 +420,13 @@ def relaxation_step(grid, dt, rusanov=True):
 
 
 def solve_opts(method="pcg64", tol=1e-10, itmax=1000, inner_tol=1e-4, outer_max=20, boundary_velocity=None,
-               picard_window=0):
+               picard_window=0, amg_layout=0):
     o = SolveOpts()
     o.method = SOLVERS[method] if isinstance(method, str) else int(method)
     o.tol, o.itmax, o.inner_tol, o.outer_max = float(tol), int(itmax), float(inner_tol), int(outer_max)
     o.vbc[:] = _vbc_table(boundary_velocity)
     o.picard_window = int(picard_window)
+    o.amg_layout = int(amg_layout)
     return o
 
 

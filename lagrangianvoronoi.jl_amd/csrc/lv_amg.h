@@ -37,6 +37,18 @@ struct lv_amg {
     int *prog;           // its operations (device), nprog of them; hprog: the host copy
     int nprog;
     int hprog[LV_AMG_MAXPROG];
+    // the fine level in the aggregate order (perm_ok): row p is particle list[p]; the inner FP32
+    // solve and the cycle then run on this copy of the operator
+    int perm_ok;
+    int *iperm, *aggp;   // particle -> row of the aggregate order; level-1 cell of each row
+    uint8_t *pdeg;       // degrees
+    int16_t *poff;       // column offsets j - i (mod n), slot-major
+    float *pev32;        // couplings, FP32 (Krylov operator, power iteration)
+    uint16_t *pev16;     // couplings, BF16 (the cycle's smoothing and residuals)
+    float *pdiag32, *pdinv32, *pmval, *pmdg;
+    int *pflag;          // set when a column offset does not fit 16 bits
+    size_t cap_iperm, cap_aggp, cap_pdeg, cap_poff, cap_pev32, cap_pev16, cap_pdiag32, cap_pdinv32, cap_pmval,
+        cap_pmdg;
     float fine_alpha;    // factor on the correction prolongated into the fine level
     float fine_scale;    // the fine smoother applies fine_scale * M (learned) or fine_scale * D^-1 (Jacobi)
     int fine_learned;    // 1: the learned preconditioner's M is the fine smoother
@@ -47,10 +59,12 @@ hipError_t lv_amg_init(lv_amg *G);
 void lv_amg_free(lv_amg *G);
 // Builds the hierarchy for the assembled operator A (device positions x of the particles, grid of
 // the boundary rectangle).  mval/mdg: the learned preconditioner in A's ELL slots, or null for a
-// damped-Jacobi fine smoother.
+// damped-Jacobi fine smoother.  layout 0: the aggregate order when its column offsets fit 16 bits
+// (perm_ok), else the operator's own; 1: the operator's own order.
 hipError_t lv_amg_setup(lv_amg *G, const lv_grid_params &g, const double *x, const lv_pressure_system &A, int n,
-                        const float *mval, const float *mdg, hipStream_t s);
+                        const float *mval, const float *mdg, int layout, hipStream_t s);
 // z = B r for the symmetric V(1,1) cycle B; writes the (r.z, r.r) partials of lv_nblocks(n) blocks
-// that the next CG kernel reduces.  Every kernel is a no-op once S->done is set.
+// that the next CG kernel reduces.  Every kernel is a no-op once S->done is set.  With perm_ok, r
+// and z are in the aggregate order.
 hipError_t lv_amg_vcycle(const lv_amg *G, const lv_pressure_system &A, const float *r, float *z, double *Prr,
                          const lv_cg_scalars *S, hipStream_t s);

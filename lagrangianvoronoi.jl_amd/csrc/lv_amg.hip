@@ -32,6 +32,7 @@
 #include <algorithm>
 
 #include "lv_amg.h"
+#include "lv_ell.h"
 #include "lv_reduce.h"
 
 namespace {
@@ -253,99 +254,90 @@ __global__ __launch_bounds__(256) void k_dense_inverse(int Kx, int Ky, int xper,
     }
 }
 
-// ---- fine level (ELL, FP32) ----------------------------------------------------------------------
+// ---- fine level -----------------------------------------------------------------------------------
+//
+// The fine kernels read the operator through a row view (lv_ell.h): the operator's own ELL (FP32
+// values, particle order) or the aggregate-order copy built at set-up (BF16 values, 16-bit column
+// offsets), in which case every fine vector of the cycle is in the aggregate order too.
 
-// (A z)_i in the difference form with the first 8 slots unrolled and predicated
-__device__ __forceinline__ float a_row(int i, int n, const int *__restrict__ deg, const int *__restrict__ ecol,
-                                       const float *__restrict__ ev, const float *__restrict__ diag,
-                                       const float *__restrict__ z)
-{
-    constexpr int U = 8;
-    const float zi = z[i];
-    float acc = diag[i] * zi;
-    const int d = deg[i];
-    int c[U];
-    float v[U];
-#pragma unroll
-    for (int k = 0; k < U; k++) {
-        c[k] = i;
-        v[k] = 0.0f;
-        if (k < d) {
-            const size_t e = (size_t)k * n + i;
-            c[k] = ecol[e];
-            v[k] = ev[e];
-        }
-    }
-#pragma unroll
-    for (int k = 0; k < U; k++)
-        if (k < d) acc += v[k] * (zi - z[c[k]]);
-    for (int k = U; k < d; k++) {
-        const size_t e = (size_t)k * n + i;
-        acc += ev[e] * (zi - z[ecol[e]]);
-    }
-    return acc;
-}
-
-// (M r)_i of the learned preconditioner, or D^-1 r
-__device__ __forceinline__ float m_row(int i, int n, const int *__restrict__ deg, const int *__restrict__ ecol,
-                                       const float *__restrict__ mval, const float *__restrict__ mdg,
-                                       const float *__restrict__ dinv, const float *__restrict__ r)
-{
-    if (!mval) return dinv[i] * r[i];
-    constexpr int U = 8;
-    float acc = mdg[i] * r[i];
-    const int d = deg[i];
-    int c[U];
-    float v[U];
-#pragma unroll
-    for (int k = 0; k < U; k++) {
-        c[k] = i;
-        v[k] = 0.0f;
-        if (k < d) {
-            const size_t e = (size_t)k * n + i;
-            c[k] = ecol[e];
-            v[k] = mval[e];
-        }
-    }
-#pragma unroll
-    for (int k = 0; k < U; k++)
-        if (k < d) acc += v[k] * r[c[k]];
-    for (int k = U; k < d; k++) {
-        const size_t e = (size_t)k * n + i;
-        acc += mval[e] * r[ecol[e]];
-    }
-    return acc;
-}
-
+template <class V>
 struct fine_ops {
     int n;
-    const int *deg, *ecol;
-    const float *ev, *diag, *dinv, *mval, *mdg;
+    V A;
+    const float *diag, *dinv, *mval, *mdg;
     float scale, alpha;
 };
 
+// (A z)_i in the difference form with the first 8 slots unrolled and predicated
+template <class V>
+__device__ __forceinline__ float a_row(int i, const fine_ops<V> &F, const float *__restrict__ z)
+{
+    constexpr int U = 8;
+    const int n = F.n;
+    const float zi = z[i];
+    float acc = F.diag[i] * zi;
+    int c[U];
+    float v[U], zj[U];
+    const int d = lv_ell_head<U>(F.A, i, n, c, v);
+#pragma unroll
+    for (int k = 0; k < U; k++) zj[k] = z[c[k]];
+#pragma unroll
+    for (int k = 0; k < U; k++) acc += v[k] * (zi - zj[k]);
+    for (int k = U; k < d; k++) {
+        const size_t e = (size_t)k * n + i;
+        acc += F.A.value(e) * (zi - z[F.A.column(e, i, n)]);
+    }
+    return acc;
+}
+
+// (M r)_i of the learned preconditioner (on A's pattern, in the same order as the view), or D^-1 r
+template <class V>
+__device__ __forceinline__ float m_row(int i, const fine_ops<V> &F, const float *__restrict__ r)
+{
+    if (!F.mval) return F.dinv[i] * r[i];
+    constexpr int U = 8;
+    const int n = F.n;
+    float acc = F.mdg[i] * r[i];
+    int c[U];
+    float v[U], rj[U];
+    const int d = lv_ell_head_with<U>(F.A, F.mval, i, n, c, v);
+#pragma unroll
+    for (int k = 0; k < U; k++) rj[k] = r[c[k]];
+#pragma unroll
+    for (int k = 0; k < U; k++) acc += v[k] * rj[k];
+    for (int k = U; k < d; k++) {
+        const size_t e = (size_t)k * n + i;
+        acc += F.mval[e] * r[F.A.column(e, i, n)];
+    }
+    return acc;
+}
+
 // z = scale * M r
-__global__ __launch_bounds__(B) void k_fine_pre(fine_ops F, const float *__restrict__ r, float *__restrict__ z,
+template <class V>
+__global__ __launch_bounds__(B) void k_fine_pre(fine_ops<V> F, const float *__restrict__ r, float *__restrict__ z,
                                                 const lv_cg_scalars *S)
 {
     if (S && S->done) return;
     const int i = blockIdx.x * B + threadIdx.x;
     if (i >= F.n) return;
-    z[i] = F.scale * m_row(i, F.n, F.deg, F.ecol, F.mval, F.mdg, F.dinv, r);
+    z[i] = F.scale * m_row(i, F, r);
 }
 
 // t = r - A z
-__global__ __launch_bounds__(B) void k_fine_resid(fine_ops F, const float *__restrict__ r, const float *__restrict__ z,
-                                                  float *__restrict__ t, const lv_cg_scalars *S)
+template <class V>
+__global__ __launch_bounds__(B) void k_fine_resid(fine_ops<V> F, const float *__restrict__ r,
+                                                  const float *__restrict__ z, float *__restrict__ t,
+                                                  const lv_cg_scalars *S)
 {
     if (S && S->done) return;
     const int i = blockIdx.x * B + threadIdx.x;
     if (i >= F.n) return;
-    t[i] = r[i] - a_row(i, F.n, F.deg, F.ecol, F.ev, F.diag, z);
+    t[i] = r[i] - a_row(i, F, z);
 }
 
-// rc[I] = sum over the (sorted) rows of cell I of t: 16 lanes per cell read its row list
-// contiguously and reduce by a fixed shuffle tree
+// rc[I] = sum over the rows of cell I of t: 16 lanes per cell read its (sorted) row list
+// contiguously and reduce by a fixed shuffle tree; list null: the rows of cell I are
+// start[I]..start[I+1]-1 themselves (aggregate order)
 __global__ __launch_bounds__(B) void k_restrict1(int na, const int *__restrict__ start, const int *__restrict__ list,
                                                  const float *__restrict__ t, float *__restrict__ rc,
                                                  const lv_cg_scalars *S)
@@ -356,7 +348,10 @@ __global__ __launch_bounds__(B) void k_restrict1(int na, const int *__restrict__
     float s = 0.0f;
     if (I < na) {
         const int s1 = start[I + 1];
-        for (int a = start[I] + q; a < s1; a += 16) s += t[list[a]];
+        if (list)
+            for (int a = start[I] + q; a < s1; a += 16) s += t[list[a]];
+        else
+            for (int a = start[I] + q; a < s1; a += 16) s += t[a];
     }
     s += __shfl_xor(s, 1);
     s += __shfl_xor(s, 2);
@@ -367,8 +362,10 @@ __global__ __launch_bounds__(B) void k_restrict1(int na, const int *__restrict__
 
 // Jacobi fine smoother, down: z0 = w D^-1 r and t = r - A z0 in one pass (z0 of a neighbour is
 // recomputed from its r)
-__global__ __launch_bounds__(B) void k_fine_down_j(fine_ops F, const float *__restrict__ r, float *__restrict__ z0,
-                                                   float *__restrict__ t, const lv_cg_scalars *S)
+template <class V>
+__global__ __launch_bounds__(B) void k_fine_down_j(fine_ops<V> F, const float *__restrict__ r,
+                                                   float *__restrict__ z0, float *__restrict__ t,
+                                                   const lv_cg_scalars *S)
 {
     if (S && S->done) return;
     const int i = blockIdx.x * B + threadIdx.x;
@@ -378,19 +375,9 @@ __global__ __launch_bounds__(B) void k_fine_down_j(fine_ops F, const float *__re
     const float ri = r[i];
     const float zi = F.scale * F.dinv[i] * ri;
     float acc = F.diag[i] * zi;
-    const int d = F.deg[i];
     int c[U];
     float v[U];
-#pragma unroll
-    for (int k = 0; k < U; k++) {
-        c[k] = i;
-        v[k] = 0.0f;
-        if (k < d) {
-            const size_t e = (size_t)k * n + i;
-            c[k] = F.ecol[e];
-            v[k] = F.ev[e];
-        }
-    }
+    const int d = lv_ell_head<U>(F.A, i, n, c, v);
     float rj[U], dj[U];
 #pragma unroll
     for (int k = 0; k < U; k++) {
@@ -398,39 +385,30 @@ __global__ __launch_bounds__(B) void k_fine_down_j(fine_ops F, const float *__re
         dj[k] = F.dinv[c[k]];
     }
 #pragma unroll
-    for (int k = 0; k < U; k++)
-        if (k < d) acc += v[k] * (zi - F.scale * dj[k] * rj[k]);
+    for (int k = 0; k < U; k++) acc += v[k] * (zi - F.scale * dj[k] * rj[k]);
     for (int k = U; k < d; k++) {
         const size_t e = (size_t)k * n + i;
-        const int j = F.ecol[e];
-        acc += F.ev[e] * (zi - F.scale * F.dinv[j] * r[j]);
+        const int j = F.A.column(e, i, n);
+        acc += F.A.value(e) * (zi - F.scale * F.dinv[j] * r[j]);
     }
     z0[i] = zi;
     t[i] = ri - acc;
 }
 
-// up: z1 = z0 + P ec and t = r - A z1 in one pass (z1 of a neighbour from its z0 and its cell's ec)
-__device__ __forceinline__ float up_row(const fine_ops &F, int i, const float *__restrict__ z0,
+// up: z1 = z0 + alpha P ec and t = r - A z1 in one pass (z1 of a neighbour from its z0 and its
+// cell's ec)
+template <class V>
+__device__ __forceinline__ float up_row(const fine_ops<V> &F, int i, const float *__restrict__ z0,
                                         const int *__restrict__ agg, const float *__restrict__ ec,
                                         const float *__restrict__ r, float &z1)
 {
     constexpr int U = 8;
     const int n = F.n;
-    const int d = F.deg[i];
     // three dependent stages (columns; their z0 and cells; the cells' corrections), each issued for
     // all of the first U slots before the next
     int c[U];
     float v[U];
-#pragma unroll
-    for (int k = 0; k < U; k++) {
-        c[k] = i;
-        v[k] = 0.0f;
-        if (k < d) {
-            const size_t e = (size_t)k * n + i;
-            c[k] = F.ecol[e];
-            v[k] = F.ev[e];
-        }
-    }
+    const int d = lv_ell_head<U>(F.A, i, n, c, v);
     float zj[U];
     int aj[U];
 #pragma unroll
@@ -441,20 +419,21 @@ __device__ __forceinline__ float up_row(const fine_ops &F, int i, const float *_
     z1 = z0[i] + F.alpha * ec[agg[i]];
     float acc = F.diag[i] * z1;
 #pragma unroll
-    for (int k = 0; k < U; k++)
-        if (k < d) acc += v[k] * (z1 - (zj[k] + F.alpha * ec[aj[k]]));
+    for (int k = 0; k < U; k++) acc += v[k] * (z1 - (zj[k] + F.alpha * ec[aj[k]]));
     for (int k = U; k < d; k++) {
         const size_t e = (size_t)k * n + i;
-        const int j = F.ecol[e];
-        acc += F.ev[e] * (z1 - (z0[j] + F.alpha * ec[agg[j]]));
+        const int j = F.A.column(e, i, n);
+        acc += F.A.value(e) * (z1 - (z0[j] + F.alpha * ec[agg[j]]));
     }
     return r[i] - acc;
 }
 
 // learned smoother, up: z = z1 and t = r - A z1 (the post-smoothing follows in k_fine_post)
-__global__ __launch_bounds__(B) void k_fine_up(fine_ops F, const float *__restrict__ r, const float *__restrict__ z0,
-                                               const int *__restrict__ agg, const float *__restrict__ ec,
-                                               float *__restrict__ z, float *__restrict__ t, const lv_cg_scalars *S)
+template <class V>
+__global__ __launch_bounds__(B) void k_fine_up(fine_ops<V> F, const float *__restrict__ r,
+                                               const float *__restrict__ z0, const int *__restrict__ agg,
+                                               const float *__restrict__ ec, float *__restrict__ z,
+                                               float *__restrict__ t, const lv_cg_scalars *S)
 {
     if (S && S->done) return;
     const int i = blockIdx.x * B + threadIdx.x;
@@ -475,14 +454,15 @@ __global__ __launch_bounds__(B) void k_prolong1(int n, const int *__restrict__ a
 }
 
 // Jacobi smoother, up (after k_prolong1): z = z1 + w D^-1 (r - A z1); partials r.z, r.r of the final z
-__global__ __launch_bounds__(B) void k_fine_up_j(fine_ops F, const float *__restrict__ r, const float *__restrict__ z1,
-                                                 float *__restrict__ z, double *__restrict__ Prr,
-                                                 const lv_cg_scalars *S)
+template <class V>
+__global__ __launch_bounds__(B) void k_fine_up_j(fine_ops<V> F, const float *__restrict__ r,
+                                                 const float *__restrict__ z1, float *__restrict__ z,
+                                                 double *__restrict__ Prr, const lv_cg_scalars *S)
 {
     if (S && S->done) return;
     double v[2] = {0.0, 0.0};
     for (int i = blockIdx.x * B + threadIdx.x; i < F.n; i += gridDim.x * B) {
-        const float ti = r[i] - a_row(i, F.n, F.deg, F.ecol, F.ev, F.diag, z1);
+        const float ti = r[i] - a_row(i, F, z1);
         const float zi = z1[i] + F.scale * F.dinv[i] * ti;
         z[i] = zi;
         const float ri = r[i];
@@ -493,14 +473,15 @@ __global__ __launch_bounds__(B) void k_fine_up_j(fine_ops F, const float *__rest
 }
 
 // z += scale * M t; partials r.z, r.r of the final z (grid-stride over lv_nblocks(n) blocks)
-__global__ __launch_bounds__(B) void k_fine_post(fine_ops F, const float *__restrict__ r, const float *__restrict__ t,
-                                                 float *__restrict__ z, double *__restrict__ Prr,
-                                                 const lv_cg_scalars *S)
+template <class V>
+__global__ __launch_bounds__(B) void k_fine_post(fine_ops<V> F, const float *__restrict__ r,
+                                                 const float *__restrict__ t, float *__restrict__ z,
+                                                 double *__restrict__ Prr, const lv_cg_scalars *S)
 {
     if (S && S->done) return;
     double v[2] = {0.0, 0.0};
     for (int i = blockIdx.x * B + threadIdx.x; i < F.n; i += gridDim.x * B) {
-        const float zi = z[i] + F.scale * m_row(i, F.n, F.deg, F.ecol, F.mval, F.mdg, F.dinv, t);
+        const float zi = z[i] + F.scale * m_row(i, F, t);
         z[i] = zi;
         const float ri = r[i];
         v[0] += (double)ri * (double)zi;
@@ -692,30 +673,35 @@ static_assert(LV_AMG_DENSE_MAX * 16 == TB, "the dense solve of the tail maps 16 
 
 // ---- power iteration for the scale of the learned fine smoother ----------------------------------
 
-__global__ __launch_bounds__(B) void k_pw_init(int n, float *__restrict__ v)
+// +-1 probe hashed from the particle index (list: the particle of each row in the aggregate order)
+__global__ __launch_bounds__(B) void k_pw_init(int n, const int *__restrict__ list, float *__restrict__ v)
 {
-    const int i = blockIdx.x * B + threadIdx.x;
-    if (i >= n) return;
+    const int p = blockIdx.x * B + threadIdx.x;
+    if (p >= n) return;
+    const int i = list ? list[p] : p;
     uint64_t z = 0x5EED5EEDull + ((uint64_t)i + 1) * 0x9E3779B97F4A7C15ull;
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
     z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
     z ^= z >> 31;
-    v[i] = (z >> 63) ? 1.0f : -1.0f;
+    v[p] = (z >> 63) ? 1.0f : -1.0f;
 }
 
 // u = M (A v) computed row-wise through the work vector t = A v: two launches
-__global__ __launch_bounds__(B) void k_pw_av(fine_ops F, const float *__restrict__ v, float *__restrict__ t)
+template <class V>
+__global__ __launch_bounds__(B) void k_pw_av(fine_ops<V> F, const float *__restrict__ v, float *__restrict__ t)
 {
     const int i = blockIdx.x * B + threadIdx.x;
-    if (i < F.n) t[i] = a_row(i, F.n, F.deg, F.ecol, F.ev, F.diag, v);
+    if (i < F.n) t[i] = a_row(i, F, v);
 }
 // u = M t; partials |u|^2, |v|^2
-__global__ __launch_bounds__(B) void k_pw_mt(fine_ops F, const float *__restrict__ t, const float *__restrict__ v,
-                                             float *__restrict__ u, double *__restrict__ part)
+template <class V>
+__global__ __launch_bounds__(B) void k_pw_mt(fine_ops<V> F, const float *__restrict__ t,
+                                             const float *__restrict__ v, float *__restrict__ u,
+                                             double *__restrict__ part)
 {
     double s[2] = {0.0, 0.0};
     for (int i = blockIdx.x * B + threadIdx.x; i < F.n; i += gridDim.x * B) {
-        const float ui = m_row(i, F.n, F.deg, F.ecol, F.mval, F.mdg, F.dinv, t);
+        const float ui = m_row(i, F, t);
         u[i] = ui;
         s[0] += (double)ui * (double)ui;
         s[1] += (double)v[i] * (double)v[i];
@@ -737,6 +723,56 @@ __global__ __launch_bounds__(B) void k_pw_scale(int n, const float *__restrict__
 {
     const int i = blockIdx.x * B + threadIdx.x;
     if (i < n) v[i] = (float)((double)u[i] / sqrt(nrm[0]));
+}
+
+// ---- the aggregate order -----------------------------------------------------------------------
+
+// iperm[list[p]] = p and the level-1 cell of row p
+__global__ __launch_bounds__(B) void k_perm_index(int n, const int *__restrict__ list, const int *__restrict__ agg,
+                                                  int *__restrict__ iperm, int *__restrict__ aggp)
+{
+    const int p = blockIdx.x * B + threadIdx.x;
+    if (p >= n) return;
+    const int i = list[p];
+    iperm[i] = p;
+    aggp[p] = agg[i];
+}
+
+// row p of the aggregate order from row list[p] of the operator (same slot order): degree, column
+// offsets taken modulo n into [-n/2, n/2], FP32 and BF16 couplings, d, D^-1, and the learned M;
+// *flag is raised when an offset needs more than 16 bits
+__global__ __launch_bounds__(B) void k_perm_ell(int n, const int *__restrict__ list, const int *__restrict__ iperm,
+                                                const int *__restrict__ deg, const int *__restrict__ ecol,
+                                                const float *__restrict__ ev, const float *__restrict__ diag,
+                                                const float *__restrict__ dinv, const float *__restrict__ mval,
+                                                const float *__restrict__ mdg, uint8_t *__restrict__ pdeg,
+                                                int16_t *__restrict__ poff, float *__restrict__ pev32,
+                                                uint16_t *__restrict__ pev16, float *__restrict__ pdiag,
+                                                float *__restrict__ pdinv, float *__restrict__ pmval,
+                                                float *__restrict__ pmdg, int *__restrict__ flag)
+{
+    const int p = blockIdx.x * B + threadIdx.x;
+    if (p >= n) return;
+    const int i = list[p];
+    const int d = deg[i];
+    pdeg[p] = (uint8_t)d;
+    pdiag[p] = diag[i];
+    pdinv[p] = dinv[i];
+    if (mdg) pmdg[p] = mdg[i];
+    bool wide = false;
+    for (int k = 0; k < d; k++) {
+        const size_t e = (size_t)k * n + i, ep = (size_t)k * n + p;
+        int off = iperm[ecol[e]] - p;
+        if (off > n / 2) off -= n;
+        if (off < -(n / 2)) off += n;
+        wide |= (off < -32768 || off > 32767);
+        poff[ep] = (int16_t)off;
+        const float v = ev[e];
+        pev32[ep] = v;
+        pev16[ep] = lv_bf16_bits(v);
+        if (mval) pmval[ep] = mval[e];
+    }
+    if (wide) atomicOr(flag, 1);
 }
 
 }  // namespace
@@ -764,13 +800,15 @@ hipError_t lv_amg_init(lv_amg *G)
     AMG_CHECK(hipMalloc(&G->dense_inv, sizeof(float) * LV_AMG_DENSE_MAX * LV_AMG_DENSE_MAX));
     AMG_CHECK(hipMalloc(&G->pw, sizeof(double) * (2 * LV_NBMAX + 2)));
     AMG_CHECK(hipMalloc(&G->prog, sizeof(int) * LV_AMG_MAXPROG));
+    AMG_CHECK(hipMalloc(&G->pflag, sizeof(int)));
     return hipSuccess;
 }
 
 void lv_amg_free(lv_amg *G)
 {
     void *ptrs[] = {G->agg, G->cnt, G->start, G->cursor, G->list, G->scan_tmp, G->t0, G->z0, G->dense_inv,
-                    G->pw, G->pv, G->pu, G->prog};
+                    G->pw, G->pv, G->pu, G->prog, G->iperm, G->aggp, G->pdeg, G->poff, G->pev32, G->pev16,
+                    G->pdiag32, G->pdinv32, G->pmval, G->pmdg, G->pflag};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (int l = 0; l <= LV_AMG_MAXLEV; l++) {
@@ -804,13 +842,11 @@ static hipError_t level_alloc(lv_amg_level &L, int Kx, int Ky)
     return hipSuccess;
 }
 
-static fine_ops fine_of(const lv_amg *G, const lv_pressure_system &A)
+static fine_ops<lv_ell_own<float>> fine_own(const lv_amg *G, const lv_pressure_system &A)
 {
-    fine_ops F;
+    fine_ops<lv_ell_own<float>> F;
     F.n = G->n0;
-    F.deg = A.deg;
-    F.ecol = A.ecol;
-    F.ev = A.ev32;
+    F.A = lv_ell_own<float>{A.deg, A.ecol, A.ev32};
     F.diag = A.diag32;
     F.dinv = A.dinv32;
     F.mval = G->fine_learned ? G->mval : nullptr;
@@ -818,6 +854,45 @@ static fine_ops fine_of(const lv_amg *G, const lv_pressure_system &A)
     F.scale = G->fine_scale;
     F.alpha = G->fine_alpha;
     return F;
+}
+
+// the aggregate order with the couplings in V's precision (BF16 for the cycle, FP32 for the power
+// iteration of the smoother scale)
+template <class V, typename VAL>
+static fine_ops<V> fine_agg(const lv_amg *G, const VAL *val)
+{
+    fine_ops<V> F;
+    F.n = G->n0;
+    F.A = V{G->pdeg, G->poff, val};
+    F.diag = G->pdiag32;
+    F.dinv = G->pdinv32;
+    F.mval = G->fine_learned ? G->pmval : nullptr;
+    F.mdg = G->fine_learned ? G->pmdg : nullptr;
+    F.scale = G->fine_scale;
+    F.alpha = G->fine_alpha;
+    return F;
+}
+
+// 10 steps of power iteration on M A (fine view F): the largest eigenvalue's estimate, 0 if none
+template <class V>
+static hipError_t power_lambda(lv_amg *G, const fine_ops<V> &Fo, const int *list, double *lam, hipStream_t s)
+{
+    const int n = G->n0;
+    const int nb = lv_nblocks(n);
+    double *part = G->pw, *nrm = G->pw + 2 * LV_NBMAX;
+    hipLaunchKernelGGL(k_pw_init, g1(n), dim3(B), 0, s, n, list, G->pv);
+    double h[2] = {0.0, 0.0};
+    for (int it = 0; it < 10; it++) {
+        hipLaunchKernelGGL(k_pw_av<V>, g1(n), dim3(B), 0, s, Fo, (const float *)G->pv, G->t0);
+        hipLaunchKernelGGL(k_pw_mt<V>, dim3(nb), dim3(B), 0, s, Fo, (const float *)G->t0, (const float *)G->pv,
+                           G->pu, part);
+        hipLaunchKernelGGL(k_pw_norm, dim3(1), dim3(B), 0, s, (const double *)part, nb, nrm);
+        hipLaunchKernelGGL(k_pw_scale, g1(n), dim3(B), 0, s, n, (const float *)G->pu, (const double *)nrm, G->pv);
+    }
+    AMG_CHECK(hipMemcpyAsync(h, nrm, sizeof(h), hipMemcpyDeviceToHost, s));
+    AMG_CHECK(hipStreamSynchronize(s));
+    *lam = (h[1] > 0.0) ? sqrt(h[0] / h[1]) : 0.0;
+    return hipSuccess;
 }
 
 // the operations of the single-workgroup kernel for the visit of level l (last: the dense level);
@@ -856,7 +931,7 @@ static int tail_program(int first, int last, bool W, int *prog)
 }
 
 hipError_t lv_amg_setup(lv_amg *G, const lv_grid_params &g, const double *x, const lv_pressure_system &A, int n,
-                        const float *mval, const float *mdg, hipStream_t s)
+                        const float *mval, const float *mdg, int layout, hipStream_t s)
 {
     G->n0 = n;
     G->xper = g.xperiodic;
@@ -909,31 +984,50 @@ hipError_t lv_amg_setup(lv_amg *G, const lv_grid_params &g, const double *x, con
     const lv_amg_level &D = G->lev[l];
     hipLaunchKernelGGL(k_dense_inverse, dim3(1), dim3(256), 0, s, D.Kx, D.Ky, G->xper, G->yper, (const float *)D.d,
                        (const float *)D.c, A.singular, G->dense_inv);
-    // fine smoother and its scale
+    // fine smoother
     G->fine_learned = (mval && mdg) ? 1 : 0;
     G->mval = mval;
     G->mdg = mdg;
     G->fine_scale = OMEGA;
+    // the aggregate order: the rows of each level-1 cell consecutive (list, sorted within a cell)
+    G->perm_ok = 0;
+    if (layout == 0) {
+        const size_t m = (size_t)LV_MAXE * n;
+        AMG_CHECK(grow(&G->iperm, &G->cap_iperm, (size_t)n));
+        AMG_CHECK(grow(&G->aggp, &G->cap_aggp, (size_t)n));
+        AMG_CHECK(grow(&G->pdeg, &G->cap_pdeg, (size_t)n));
+        AMG_CHECK(grow(&G->poff, &G->cap_poff, m));
+        AMG_CHECK(grow(&G->pev32, &G->cap_pev32, m));
+        AMG_CHECK(grow(&G->pev16, &G->cap_pev16, m));
+        AMG_CHECK(grow(&G->pdiag32, &G->cap_pdiag32, (size_t)n));
+        AMG_CHECK(grow(&G->pdinv32, &G->cap_pdinv32, (size_t)n));
+        if (G->fine_learned) {
+            AMG_CHECK(grow(&G->pmval, &G->cap_pmval, m));
+            AMG_CHECK(grow(&G->pmdg, &G->cap_pmdg, (size_t)n));
+        }
+        AMG_CHECK(hipMemsetAsync(G->pflag, 0, sizeof(int), s));
+        hipLaunchKernelGGL(k_perm_index, g1(n), dim3(B), 0, s, n, (const int *)G->list, (const int *)G->agg, G->iperm,
+                           G->aggp);
+        hipLaunchKernelGGL(k_perm_ell, g1(n), dim3(B), 0, s, n, (const int *)G->list, (const int *)G->iperm,
+                           (const int *)A.deg, (const int *)A.ecol, (const float *)A.ev32, (const float *)A.diag32,
+                           (const float *)A.dinv32, G->fine_learned ? mval : nullptr,
+                           G->fine_learned ? mdg : nullptr, G->pdeg, G->poff, G->pev32, G->pev16, G->pdiag32,
+                           G->pdinv32, G->pmval, G->pmdg, G->pflag);
+        int wide = 1;
+        AMG_CHECK(hipMemcpyAsync(&wide, G->pflag, sizeof(int), hipMemcpyDeviceToHost, s));
+        AMG_CHECK(hipStreamSynchronize(s));
+        G->perm_ok = wide ? 0 : 1;
+    }
+    // the learned smoother's scale: lambda_max(M A) -> FINE_TARGET
     if (G->fine_learned) {
         AMG_CHECK(grow(&G->pv, &G->cap_pv, (size_t)n));
         AMG_CHECK(grow(&G->pu, &G->cap_pu, (size_t)n));
         G->fine_scale = 1.0f;
-        fine_ops Fo = fine_of(G, A);
-        const int nb = lv_nblocks(n);
-        double *part = G->pw, *nrm = G->pw + 2 * LV_NBMAX;
-        hipLaunchKernelGGL(k_pw_init, g1(n), dim3(B), 0, s, n, G->pv);
-        double h[2] = {0.0, 0.0};
-        for (int it = 0; it < 10; it++) {
-            hipLaunchKernelGGL(k_pw_av, g1(n), dim3(B), 0, s, Fo, (const float *)G->pv, G->t0);
-            hipLaunchKernelGGL(k_pw_mt, dim3(nb), dim3(B), 0, s, Fo, (const float *)G->t0, (const float *)G->pv,
-                               G->pu, part);
-            hipLaunchKernelGGL(k_pw_norm, dim3(1), dim3(B), 0, s, (const double *)part, nb, nrm);
-            hipLaunchKernelGGL(k_pw_scale, g1(n), dim3(B), 0, s, n, (const float *)G->pu, (const double *)nrm,
-                               G->pv);
-        }
-        AMG_CHECK(hipMemcpyAsync(h, nrm, sizeof(h), hipMemcpyDeviceToHost, s));
-        AMG_CHECK(hipStreamSynchronize(s));
-        const double lam = (h[1] > 0.0) ? sqrt(h[0] / h[1]) : 0.0;
+        double lam = 0.0;
+        if (G->perm_ok)
+            AMG_CHECK(power_lambda(G, fine_agg<lv_ell_agg32>(G, (const float *)G->pev32), G->list, &lam, s));
+        else
+            AMG_CHECK(power_lambda(G, fine_own(G, A), nullptr, &lam, s));
         G->fine_scale = (lam > 0.0 && lam == lam) ? (float)(FINE_TARGET / lam) : 0.0f;
         if (!(G->fine_scale > 0.0f)) {  // no usable estimate: damped Jacobi
             G->fine_learned = 0;
@@ -943,21 +1037,22 @@ hipError_t lv_amg_setup(lv_amg *G, const lv_grid_params &g, const double *x, con
     return hipGetLastError();
 }
 
-hipError_t lv_amg_vcycle(const lv_amg *G, const lv_pressure_system &A, const float *r, float *z, double *Prr,
-                         const lv_cg_scalars *S, hipStream_t s)
+// the cycle on a fine view F; agg: level-1 cell of each fine row; list: the fine rows of each cell
+// (null when they are consecutive)
+template <class V>
+static hipError_t vcycle(const lv_amg *G, const fine_ops<V> &F, const int *agg, const int *list, const float *r,
+                         float *z, double *Prr, const lv_cg_scalars *S, hipStream_t s)
 {
     const int n = G->n0;
-    const fine_ops F = fine_of(G, A);
     const lv_amg_level &L1 = G->lev[1];
     // level 0, down: z0 (in G->z0) and its residual t (in G->t0), restricted to level 1
     if (G->fine_learned) {
-        hipLaunchKernelGGL(k_fine_pre, g1(n), dim3(B), 0, s, F, r, G->z0, S);
-        hipLaunchKernelGGL(k_fine_resid, g1(n), dim3(B), 0, s, F, r, (const float *)G->z0, G->t0, S);
+        hipLaunchKernelGGL(k_fine_pre<V>, g1(n), dim3(B), 0, s, F, r, G->z0, S);
+        hipLaunchKernelGGL(k_fine_resid<V>, g1(n), dim3(B), 0, s, F, r, (const float *)G->z0, G->t0, S);
     } else {
-        hipLaunchKernelGGL(k_fine_down_j, g1(n), dim3(B), 0, s, F, r, G->z0, G->t0, S);
+        hipLaunchKernelGGL(k_fine_down_j<V>, g1(n), dim3(B), 0, s, F, r, G->z0, G->t0, S);
     }
-    hipLaunchKernelGGL(k_restrict1, g1((int64_t)G->na * 16), dim3(B), 0, s, G->na, (const int *)G->start,
-                       (const int *)G->list,
+    hipLaunchKernelGGL(k_restrict1, g1((int64_t)G->na * 16), dim3(B), 0, s, G->na, (const int *)G->start, list,
                        (const float *)G->t0, L1.r, S);
     // coarse levels above the tail, down: each level's pre-smoothed z and the next level's
     // right-hand side
@@ -999,13 +1094,21 @@ hipError_t lv_amg_vcycle(const lv_amg *G, const lv_pressure_system &A, const flo
     }
     // level 0, up: prolongation, residual and post-smoothing; partials of r.z, r.r
     if (G->fine_learned) {
-        hipLaunchKernelGGL(k_fine_up, g1(n), dim3(B), 0, s, F, r, (const float *)G->z0, (const int *)G->agg,
-                           (const float *)L1.t, z, G->t0, S);
-        hipLaunchKernelGGL(k_fine_post, dim3(lv_nblocks(n)), dim3(B), 0, s, F, r, (const float *)G->t0, z, Prr, S);
+        hipLaunchKernelGGL(k_fine_up<V>, g1(n), dim3(B), 0, s, F, r, (const float *)G->z0, agg, (const float *)L1.t,
+                           z, G->t0, S);
+        hipLaunchKernelGGL(k_fine_post<V>, dim3(lv_nblocks(n)), dim3(B), 0, s, F, r, (const float *)G->t0, z, Prr, S);
     } else {
-        hipLaunchKernelGGL(k_prolong1, g1(n), dim3(B), 0, s, n, (const int *)G->agg, (const float *)L1.t,
-                           G->fine_alpha, G->z0, S);
-        hipLaunchKernelGGL(k_fine_up_j, dim3(lv_nblocks(n)), dim3(B), 0, s, F, r, (const float *)G->z0, z, Prr, S);
+        hipLaunchKernelGGL(k_prolong1, g1(n), dim3(B), 0, s, n, agg, (const float *)L1.t, G->fine_alpha, G->z0, S);
+        hipLaunchKernelGGL(k_fine_up_j<V>, dim3(lv_nblocks(n)), dim3(B), 0, s, F, r, (const float *)G->z0, z, Prr, S);
     }
     return hipGetLastError();
+}
+
+hipError_t lv_amg_vcycle(const lv_amg *G, const lv_pressure_system &A, const float *r, float *z, double *Prr,
+                         const lv_cg_scalars *S, hipStream_t s)
+{
+    if (G->perm_ok)
+        return vcycle(G, fine_agg<lv_ell_agg16>(G, (const uint16_t *)G->pev16), (const int *)G->aggp,
+                      (const int *)nullptr, r, z, Prr, S, s);
+    return vcycle(G, fine_own(G, A), (const int *)G->agg, (const int *)G->list, r, z, Prr, S, s);
 }

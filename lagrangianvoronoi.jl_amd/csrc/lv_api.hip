@@ -72,6 +72,8 @@ struct lv_ctx {
     int amg_ok = 0;
     int64_t amg_built_for = -1;
     int amg_learned = -1;  // the fine smoother the hierarchy was built with (1 learned, 0 Jacobi)
+    int amg_layout = -1;   // the fine-level layout it was asked for (lv_solve_opts.amg_layout)
+    int spmv32_agg = 0;    // the last FP32 inner solve ran in the aggregate order
     int A_kind = 0;        // what A holds: 0 the pressure operator, 1 the implicit viscous operator
     // multiphase projection work space (allocated on first use)
     lv_mph_work *mph = nullptr;
@@ -792,11 +794,13 @@ static int spai_prepare(lv_ctx *ctx, const lv_solve_opts &opt, lv_precond *M, bo
         return LV_OK;
     }
     if (!ctx->amg_ok) return lv_fail(ctx, LV_E_INVALID, "multilevel preconditioner unavailable");
-    if (ctx->amg_built_for != ctx->assembly_id || ctx->amg_learned != (int)learned) {
+    if (ctx->amg_built_for != ctx->assembly_id || ctx->amg_learned != (int)learned ||
+        ctx->amg_layout != opt.amg_layout) {
         LV_HIP_CHECK(lv_amg_setup(&ctx->amg, ctx->g, ctx->F.x, ctx->A, ctx->n, learned ? ctx->spai.mval : nullptr,
-                                  learned ? ctx->spai.mdg : nullptr, ctx->s));
+                                  learned ? ctx->spai.mdg : nullptr, opt.amg_layout, ctx->s));
         ctx->amg_built_for = ctx->assembly_id;
         ctx->amg_learned = (int)learned;
+        ctx->amg_layout = opt.amg_layout;
     }
     M->amg = &ctx->amg;
     M->A = &ctx->A;
@@ -823,6 +827,8 @@ static int solve_on_device(lv_ctx *ctx, const double *db, double *dx, const lv_s
     int iters = 0, sweeps = 0, fb = 0;
     double relres = 0.0, relres0 = 0.0;
     bool conv = false;
+    ctx->spmv32_agg = (haveM && M.amg && M.amg->perm_ok) ? 1 : 0;
+    if (st) st->agg_order = ctx->spmv32_agg;
     hipError_t e = lv_solve_device(ctx->A, ctx->W, ctx->n, db, dx, opt, ctx->s, timer_if_on(ctx, 0),
                                    timer_if_on(ctx, 1), &iters, &sweeps, &relres, &relres0, &conv,
                                    haveM ? &M : nullptr, &fb);
@@ -862,6 +868,7 @@ int lv_apply_preconditioner(lv_ctx *ctx, int32_t method, const double *r, double
         return lv_fail(ctx, LV_E_INVALID, "lv_apply_preconditioner: method has no FP32 preconditioner");
     lv_solve_opts opt{};
     opt.method = method;
+    opt.amg_layout = ctx->amg_layout > 0 ? ctx->amg_layout : 0;  // the layout of the last multilevel solve
     lv_precond M{};
     bool have = false;
     int rc = spai_prepare(ctx, opt, &M, &have);
@@ -1338,8 +1345,9 @@ int lv_timing_read(lv_ctx *ctx, int32_t kernel_id, double *total_ms, int64_t *la
         const double n = ctx->n, nnz = (double)ctx->A.nnz, m = ctx->nitems;
         const double nc = (double)ctx->g.n1 * ctx->g.n2;
         double b = 0.0;
-        // CG SpMV kernel: deg + diag + z + p (read, write) + q (read, write) per row, col + value per nonzero
-        if (kernel_id == 0) b = 4.0 * n + 4.0 * n * 6 + 8.0 * nnz;
+        // CG SpMV kernel: deg + diag + z + p (read, write) + q (read, write) per row, col + value per
+        // nonzero (aggregate order: 1-byte degree, 2-byte column offset)
+        if (kernel_id == 0) b = ctx->spmv32_agg ? 1.0 * n + 4.0 * n * 6 + 6.0 * nnz : 4.0 * n + 4.0 * n * 6 + 8.0 * nnz;
         else if (kernel_id == 1) b = 4.0 * n + 8.0 * n * 6 + 12.0 * nnz;
         else if (kernel_id == 2) b = 16.0 * n + 20.0 * m + 4.0 * (nc + 1) + 4.0 * n + 36.0 * 6.0 * n + 24.0 * n;
         else if (kernel_id == 3) b = 3.0 * (8.0 + 8.0 + 4.0) * m;              // 3 radix passes over (key, label)

@@ -19,6 +19,8 @@
 #include "lv_solver.h"
 #include "lv_reduce.h"
 #include "lv_amg.h"
+#include "lv_ell.h"
+#include <type_traits>
 
 namespace {
 
@@ -240,32 +242,22 @@ __global__ __launch_bounds__(B) void k_means(int n, const double *__restrict__ b
 // (A x)_i in the reference's difference form, from the slot-major ELL copy.  The first 8 slots are
 // unrolled and predicated so that all their column and coefficient loads are in flight before the
 // dependent gathers of x; rows with more than 8 neighbours finish in a short loop.
-template <typename T>
-__device__ __forceinline__ T spmv_row(int i, int n, const int *__restrict__ deg, const int *__restrict__ ecol,
-                                      const T *__restrict__ ev, const T *__restrict__ diag, const T *__restrict__ x)
+template <typename T, class V>
+__device__ __forceinline__ T spmv_row(int i, int n, const V &A, const T *__restrict__ diag, const T *__restrict__ x)
 {
     constexpr int U = 8;
     const T xi = x[i];
     T acc = diag[i] * xi;
-    const int d = deg[i];
     int c[U];
-    T v[U];
+    T v[U], xj[U];
+    const int d = lv_ell_head<U>(A, i, n, c, v);
 #pragma unroll
-    for (int k = 0; k < U; k++) {
-        c[k] = i;
-        v[k] = (T)0;
-        if (k < d) {
-            const size_t e = (size_t)k * n + i;
-            c[k] = ecol[e];
-            v[k] = ev[e];
-        }
-    }
+    for (int k = 0; k < U; k++) xj[k] = x[c[k]];
 #pragma unroll
-    for (int k = 0; k < U; k++)
-        if (k < d) acc += v[k] * (xi - x[c[k]]);
+    for (int k = 0; k < U; k++) acc += v[k] * (xi - xj[k]);
     for (int k = U; k < d; k++) {
         const size_t e = (size_t)k * n + i;
-        acc += ev[e] * (xi - x[ecol[e]]);
+        acc += A.value(e) * (xi - x[A.column(e, i, n)]);
     }
     return acc;
 }
@@ -282,7 +274,7 @@ __global__ __launch_bounds__(B) void k_cg64_init(int n, const int *__restrict__ 
     double v[2] = {0.0, 0.0}, w[3] = {0.0, 0.0, 0.0};
     for (int i = blockIdx.x * B + threadIdx.x; i < n; i += gridDim.x * B) {
         const double bi = b[i] - bmean;
-        const double ri = bi - spmv_row<double>(i, n, deg, ecol, ev, diag, x);
+        const double ri = bi - spmv_row<double>(i, n, lv_ell_own<double>{deg, ecol, ev}, diag, x);
         const double zi = dinv[i] * ri;
         r[i] = ri;
         z[i] = zi;
@@ -301,9 +293,8 @@ __global__ __launch_bounds__(B) void k_cg64_init(int n, const int *__restrict__ 
 // using A p_new = A z + beta A p_old, so only z is gathered and p, q are touched row-locally.
 // The block's slice of the CSR arrays is first staged through LDS with coalesced loads.
 // U (update): alpha = r.z / p.q;  x += alpha p;  r -= alpha q;  z = dinv r;  partials r.z, r.r.
-template <typename T>
-__global__ __launch_bounds__(B) void k_cg_s(int n, const int *__restrict__ deg, const int *__restrict__ ecol,
-                                            const T *__restrict__ ev, const T *__restrict__ diag,
+template <typename T, class V>
+__global__ __launch_bounds__(B) void k_cg_s(int n, V A, const T *__restrict__ diag,
                                             const T *__restrict__ z, T *__restrict__ p, T *__restrict__ q,
                                             const double *__restrict__ Prr, double *__restrict__ Ppq,
                                             lv_cg_scalars *S, int par, int k, int first)
@@ -329,11 +320,11 @@ __global__ __launch_bounds__(B) void k_cg_s(int n, const int *__restrict__ deg, 
     for (int i0 = blockIdx.x * B + threadIdx.x; i0 < n; i0 += 2 * stride) {
         const int i1 = i0 + stride;
         const T z0 = z[i0];
-        const T w0 = spmv_row<T>(i0, n, deg, ecol, ev, diag, z);
+        const T w0 = spmv_row<T>(i0, n, A, diag, z);
         T z1 = (T)0, w1 = (T)0;
         if (i1 < n) {
             z1 = z[i1];
-            w1 = spmv_row<T>(i1, n, deg, ecol, ev, diag, z);
+            w1 = spmv_row<T>(i1, n, A, diag, z);
         }
         const T q0 = first ? w0 : w0 + beta * q[i0];
         const T p0 = first ? z0 : z0 + beta * p[i0];
@@ -406,29 +397,20 @@ __global__ __launch_bounds__(B) void k_conv_check(const double *__restrict__ Pur
 }
 
 // (M r)_i with the learned preconditioner in ELL slots (same columns as A's ELL copy): the first 8
-// slots unrolled and predicated so their loads are in flight before the gathers of r
+// slots loaded unconditionally (lv_ell_head) so their loads are in flight before the gathers of r
 __device__ __forceinline__ float apply_m_row(int i, int n, const int *__restrict__ deg, const int *__restrict__ ecol,
                                              const float *__restrict__ mval, const float *__restrict__ mdg,
                                              const float *__restrict__ r, float ri)
 {
     constexpr int U = 8;
     float acc = mdg[i] * ri;
-    const int d = deg[i];
     int c[U];
-    float v[U];
+    float v[U], rj[U];
+    const int d = lv_ell_head_with<U>(lv_ell_own<float>{deg, ecol, mval}, mval, i, n, c, v);
 #pragma unroll
-    for (int k = 0; k < U; k++) {
-        c[k] = i;
-        v[k] = 0.0f;
-        if (k < d) {
-            const size_t e = (size_t)k * n + i;
-            c[k] = ecol[e];
-            v[k] = mval[e];
-        }
-    }
+    for (int k = 0; k < U; k++) rj[k] = r[c[k]];
 #pragma unroll
-    for (int k = 0; k < U; k++)
-        if (k < d) acc += v[k] * r[c[k]];
+    for (int k = 0; k < U; k++) acc += v[k] * rj[k];
     for (int k = U; k < d; k++) {
         const size_t e = (size_t)k * n + i;
         acc += mval[e] * r[ecol[e]];
@@ -484,7 +466,7 @@ __global__ __launch_bounds__(B) void k_resid64(int n, const int *__restrict__ de
     double v[3] = {0.0, 0.0, 0.0};
     for (int i = blockIdx.x * B + threadIdx.x; i < n; i += gridDim.x * B) {
         const double bi = b[i] - bmean;
-        const double ri = bi - spmv_row<double>(i, n, deg, ecol, ev, diag, x);
+        const double ri = bi - spmv_row<double>(i, n, lv_ell_own<double>{deg, ecol, ev}, diag, x);
         r[i] = ri;
         v[0] += ri * ri;
         v[1] += bi * bi;
@@ -508,15 +490,17 @@ __global__ __launch_bounds__(B) void k_shift(int n, double *__restrict__ x, cons
 
 // ---- FP32 inner PCG of the refinement --------------------------------------------------------
 
-// r32 = r64 / |r64|, e = 0, z = dinv r, p = z; partials r.z, r.r
-__global__ __launch_bounds__(B) void k_cg32_init(int n, const double *__restrict__ r64, const float *__restrict__ dinv,
-                                                 float *__restrict__ e, float *__restrict__ r, float *__restrict__ z,
+// r32 = r64 / |r64|, e = 0, z = dinv r, p = z; partials r.z, r.r.  With perm, the FP32 vectors
+// are in the aggregate order: r32[i] comes from r64[perm[i]].
+__global__ __launch_bounds__(B) void k_cg32_init(int n, const double *__restrict__ r64, const int *__restrict__ perm,
+                                                 const float *__restrict__ dinv, float *__restrict__ e,
+                                                 float *__restrict__ r, float *__restrict__ z,
                                                  double *__restrict__ Prr, const lv_cg_scalars *S)
 {
     const double inv = 1.0 / sqrt(S->rr64);
     double v[2] = {0.0, 0.0};
     for (int i = blockIdx.x * B + threadIdx.x; i < n; i += gridDim.x * B) {
-        const float ri = (float)(r64[i] * inv);
+        const float ri = (float)(r64[perm ? perm[i] : i] * inv);
         const float zi = dinv[i] * ri;
         e[i] = 0.0f;
         r[i] = ri;
@@ -527,12 +511,13 @@ __global__ __launch_bounds__(B) void k_cg32_init(int n, const double *__restrict
     block_partials<2>(v, Prr, gridDim.x);
 }
 
-// x += |r64| * e
+// x += |r64| * e (e in the aggregate order when iperm is given: x[i] takes e[iperm[i]])
 __global__ __launch_bounds__(B) void k_correct(int n, double *__restrict__ x, const float *__restrict__ e,
-                                               const lv_cg_scalars *S)
+                                               const int *__restrict__ iperm, const lv_cg_scalars *S)
 {
     const double s = sqrt(S->rr64);
-    for (int i = blockIdx.x * B + threadIdx.x; i < n; i += gridDim.x * B) x[i] += s * (double)e[i];
+    for (int i = blockIdx.x * B + threadIdx.x; i < n; i += gridDim.x * B)
+        x[i] += s * (double)e[iperm ? iperm[i] : i];
 }
 
 // ---- small FP64 vector algebra for the Picard extrapolation -----------------------------------
@@ -666,10 +651,10 @@ inline parts parts_of(double *base) { return parts{base, base + 3 * NBMAX, base 
 
 // CG iterations (two launches each, three with the learned preconditioner) until the device flags
 // completion; the host checks the flag once per batch of iterations.
-template <typename T>
-hipError_t cg_iterate(int n, const int *deg, const int *ecol, const T *ev, const T *diag, const T *dinv, T *x, T *r,
-                      T *z, T *p, T *q, const parts &P, lv_cg_scalars *dS, hipStream_t s, lv_timer *tm, int batch,
-                      lv_cg_scalars &h, const lv_precond *M)
+template <typename T, class V>
+hipError_t cg_iterate(int n, V A, const T *diag, const T *dinv, T *x, T *r, T *z, T *p, T *q, const parts &P,
+                      lv_cg_scalars *dS, hipStream_t s, lv_timer *tm, int batch, lv_cg_scalars &h,
+                      const lv_precond *M)
 {
     const int nb = nblocks_of(n);
     int k = 0, nit = batch;
@@ -678,7 +663,7 @@ hipError_t cg_iterate(int n, const int *deg, const int *ecol, const T *ev, const
             const int par = k & 1;
             {
                 timer_scope ts(tm, s);
-                hipLaunchKernelGGL(k_cg_s<T>, dim3(nb), dim3(B), 0, s, n, deg, ecol, ev, diag, (const T *)z, p, q,
+                hipLaunchKernelGGL((k_cg_s<T, V>), dim3(nb), dim3(B), 0, s, n, A, diag, (const T *)z, p, q,
                                    (const double *)P.rr, P.pq, dS, par, k, k == 0 ? 1 : 0);
             }
             if (M) {
@@ -689,9 +674,11 @@ hipError_t cg_iterate(int n, const int *deg, const int *ecol, const T *ev, const
                     hipError_t e = lv_amg_vcycle(M->amg, *M->A, (const float *)r, (float *)z, P.rr,
                                                  (const lv_cg_scalars *)dS, s);
                     if (e != hipSuccess) return e;
-                } else {
-                    hipLaunchKernelGGL(k_apply_m32, dim3(nb), dim3(B), 0, s, n, deg, ecol, M->mval, M->mdg,
+                } else if constexpr (std::is_same<V, lv_ell_own<float>>::value) {
+                    hipLaunchKernelGGL(k_apply_m32, dim3(nb), dim3(B), 0, s, n, A.deg, A.col, M->mval, M->mdg,
                                        (const float *)r, (float *)z, P.rr, (const lv_cg_scalars *)dS);
+                } else {
+                    return hipErrorInvalidValue;  // the learned one-level preconditioner is in the operator's order
                 }
             } else {
                 hipLaunchKernelGGL((k_cg_u<T, true>), dim3(nb), dim3(B), 0, s, n, x, r, z, (const T *)p,
@@ -746,7 +733,8 @@ hipError_t lv_solve_device(const lv_pressure_system &A, lv_solver_work &W, int n
                            (const double *)W.x, W.r, W.z, P.rr, P.p3, (const lv_cg_scalars *)W.S);
         hipLaunchKernelGGL(k_finalize, dim3(1), dim3(B), 0, s, (const double *)P.p3, nb, (int)FIN_INIT64, n, 0.0,
                            W.S);
-        e = cg_iterate<double>(n, A.deg, A.ecol, A.ev64, A.diag64, A.dinv64, W.x, W.r, W.z, W.p, W.q, P, W.S, s,
+        e = cg_iterate<double>(n, lv_ell_own<double>{A.deg, A.ecol, A.ev64}, (const double *)A.diag64,
+                               (const double *)A.dinv64, W.x, W.r, W.z, W.p, W.q, P, W.S, s,
                                t64, batch, h, (const lv_precond *)nullptr);
         if (e != hipSuccess) return e;
         total = h.iters;
@@ -770,9 +758,14 @@ hipError_t lv_solve_device(const lv_pressure_system &A, lv_solver_work &W, int n
             double itol = need > floor_tol ? need : floor_tol;
             if (itol > 0.5) itol = 0.5;
             const lv_precond *Mi = (opt.method == LV_SOLVER_SPAI || opt.method == LV_SOLVER_AMG) ? M : nullptr;
+            // with the multilevel preconditioner in the aggregate order, the whole FP32 sweep runs in
+            // that order (the FP64 residual and correction stay in the particles' order)
+            const lv_amg *Gp = (Mi && Mi->amg && Mi->amg->perm_ok) ? Mi->amg : nullptr;
+            const float *dinv32 = Gp ? (const float *)Gp->pdinv32 : (const float *)A.dinv32;
             for (int attempt = 0; attempt < 2; attempt++) {
-                hipLaunchKernelGGL(k_cg32_init, G, dim3(B), 0, s, n, (const double *)W.r, (const float *)A.dinv32,
-                                   W.e32, W.r32, W.z32, P.rr, (const lv_cg_scalars *)W.S);
+                hipLaunchKernelGGL(k_cg32_init, G, dim3(B), 0, s, n, (const double *)W.r,
+                                   Gp ? (const int *)Gp->list : (const int *)nullptr, dinv32, W.e32, W.r32, W.z32,
+                                   P.rr, (const lv_cg_scalars *)W.S);
                 // k_finalize2 clears the done flag of the previous sweep, so it precedes the first
                 // application of the learned preconditioner (which is skipped while done is set)
                 hipLaunchKernelGGL(k_finalize2, dim3(1), dim3(B), 0, s, (const double *)P.rr, nb, itol * itol, W.S);
@@ -784,8 +777,12 @@ hipError_t lv_solve_device(const lv_pressure_system &A, lv_solver_work &W, int n
                                        Mi->mval, Mi->mdg, (const float *)W.r32, W.z32, P.rr,
                                        (const lv_cg_scalars *)W.S);
                 }
-                e = cg_iterate<float>(n, A.deg, A.ecol, A.ev32, A.diag32, A.dinv32, W.e32, W.r32, W.z32, W.p32, W.q32,
-                                      P, W.S, s, t32, batch, h, Mi);
+                if (Gp)
+                    e = cg_iterate<float>(n, lv_ell_agg32{Gp->pdeg, Gp->poff, Gp->pev32}, (const float *)Gp->pdiag32,
+                                          dinv32, W.e32, W.r32, W.z32, W.p32, W.q32, P, W.S, s, t32, batch, h, Mi);
+                else
+                    e = cg_iterate<float>(n, lv_ell_own<float>{A.deg, A.ecol, A.ev32}, (const float *)A.diag32,
+                                          dinv32, W.e32, W.r32, W.z32, W.p32, W.q32, P, W.S, s, t32, batch, h, Mi);
                 if (e != hipSuccess) return e;
                 // Jacobi-PCG is the fallback: a sweep of the learned preconditioner that stalls or breaks
                 // down is redone with Jacobi
@@ -797,7 +794,8 @@ hipError_t lv_solve_device(const lv_pressure_system &A, lv_solver_work &W, int n
             if (e != hipSuccess) return e;
             total += h.iters;
             nsweep++;
-            hipLaunchKernelGGL(k_correct, G, dim3(B), 0, s, n, W.x, (const float *)W.e32, (const lv_cg_scalars *)W.S);
+            hipLaunchKernelGGL(k_correct, G, dim3(B), 0, s, n, W.x, (const float *)W.e32,
+                               Gp ? (const int *)Gp->iperm : (const int *)nullptr, (const lv_cg_scalars *)W.S);
         }
     }
     // final FP64 residual of the returned x
@@ -829,15 +827,18 @@ __global__ void k_set_running(lv_cg_scalars *S)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) S->done = 0;
 }
-__global__ __launch_bounds__(B) void k_to32(int n, const double *__restrict__ a, float *__restrict__ b)
+// b[i] = a[idx[i]] (idx null: a[i]) with the precision change
+__global__ __launch_bounds__(B) void k_to32(int n, const double *__restrict__ a, const int *__restrict__ idx,
+                                            float *__restrict__ b)
 {
     const int i = blockIdx.x * B + threadIdx.x;
-    if (i < n) b[i] = (float)a[i];
+    if (i < n) b[i] = (float)a[idx ? idx[i] : i];
 }
-__global__ __launch_bounds__(B) void k_to64(int n, const float *__restrict__ a, double *__restrict__ b)
+__global__ __launch_bounds__(B) void k_to64(int n, const float *__restrict__ a, const int *__restrict__ idx,
+                                            double *__restrict__ b)
 {
     const int i = blockIdx.x * B + threadIdx.x;
-    if (i < n) b[i] = (double)a[i];
+    if (i < n) b[i] = (double)a[idx ? idx[i] : i];
 }
 __global__ __launch_bounds__(B) void k_jacobi32(int n, const float *__restrict__ dinv, const float *__restrict__ r,
                                                 float *__restrict__ z)
@@ -852,7 +853,10 @@ hipError_t lv_apply_precond(const lv_pressure_system &A, lv_solver_work &W, int 
                             const double *r, double *z, hipStream_t s)
 {
     const parts P = parts_of(W.partials);
-    hipLaunchKernelGGL(k_to32, grid_of(n), dim3(B), 0, s, n, r, W.r32);
+    // the multilevel cycle in the aggregate order takes r and returns z in that order
+    const lv_amg *Gp = (M && M->amg && M->amg->perm_ok) ? M->amg : nullptr;
+    hipLaunchKernelGGL(k_to32, grid_of(n), dim3(B), 0, s, n, r, Gp ? (const int *)Gp->list : (const int *)nullptr,
+                       W.r32);
     hipLaunchKernelGGL(k_set_running, dim3(1), dim3(64), 0, s, W.S);
     if (M && M->amg) {
         hipError_t e = lv_amg_vcycle(M->amg, A, (const float *)W.r32, W.z32, P.rr, (const lv_cg_scalars *)W.S, s);
@@ -865,7 +869,8 @@ hipError_t lv_apply_precond(const lv_pressure_system &A, lv_solver_work &W, int 
         hipLaunchKernelGGL(k_jacobi32, grid_of(n), dim3(B), 0, s, n, (const float *)A.dinv32, (const float *)W.r32,
                            W.z32);
     }
-    hipLaunchKernelGGL(k_to64, grid_of(n), dim3(B), 0, s, n, (const float *)W.z32, z);
+    hipLaunchKernelGGL(k_to64, grid_of(n), dim3(B), 0, s, n, (const float *)W.z32,
+                       Gp ? (const int *)Gp->iperm : (const int *)nullptr, z);
     return hipGetLastError();
 }
 

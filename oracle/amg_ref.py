@@ -14,7 +14,10 @@ numpy/scipy FP64:
   * the prolongated coarse correction scaled by 1.6 into the fine level and by 1.4 into a coarse
     level, except for corrections from levels of at most TAIL_MAX cells (scale 1);
   * for a singular system, the levels of at most TAIL_MAX cells (other than the dense one) are
-    visited twice (W-cycle): the second visit solves for the residual the first correction leaves.
+    visited twice (W-cycle): the second visit solves for the residual the first correction leaves;
+  * fine_bf16 (the device's aggregate-order layout): the fine residuals of the cycle use the
+    couplings rounded to FP32 and then to BF16 (nearest even); D^-1 and the power iteration keep
+    the operator's own.
 
 No counterpart exists in the reference (src/pressure.jl:219 runs MINRES unpreconditioned).  The
 oracle is itself pinned by tests/test_amg_oracle.py (Galerkin identity, symmetry and definiteness
@@ -76,7 +79,7 @@ class Level:
 
 
 class Hierarchy:
-    def __init__(self, x, d, rp, cols, coef, bxmin, bxmax, dr, xper, yper, singular, M=None):
+    def __init__(self, x, d, rp, cols, coef, bxmin, bxmax, dr, xper, yper, singular, M=None, fine_bf16=True):
         """x: SoA positions (2n); (d, rp, cols, coef): the operator as lv_get_pressure_operator
         returns it; M: the learned preconditioner as a sparse matrix, or None for Jacobi."""
         n = d.size
@@ -84,6 +87,11 @@ class Hierarchy:
         src = np.repeat(np.arange(n), np.diff(rp))
         C = sp.csr_matrix((coef, (src, cols)), shape=(n, n))
         self.A = (sp.diags(d + np.asarray(C.sum(axis=1)).ravel()) - C).tocsr()
+        if fine_bf16:
+            Ct = sp.csr_matrix((bf16(coef), (src, cols)), shape=(n, n))
+            self.Af = (sp.diags(d + np.asarray(Ct.sum(axis=1)).ravel()) - Ct).tocsr()
+        else:
+            self.Af = self.A
         self.Dinv = 1.0 / self.A.diagonal()
         Lx, Ly = bxmax[0] - bxmin[0], bxmax[1] - bxmin[1]
         Kx, Ky = max(1, int(np.floor(Lx / (4.0 * dr)))), max(1, int(np.floor(Ly / (4.0 * dr))))
@@ -169,11 +177,18 @@ class Hierarchy:
     def apply(self, r):
         """z = B r, the symmetric V(1,1) cycle"""
         z = self._fine(r)
-        t = r - self.A @ z
+        t = r - self.Af @ z
         ec = self._visit(0, np.bincount(self.agg, weights=t, minlength=self.levels[0].n))
         z = z + self.alpha_fine * ec[self.agg]
-        t = r - self.A @ z
+        t = r - self.Af @ z
         return z + self._fine(t)
+
+
+def bf16(v):
+    """v rounded to FP32, then to BF16 (round to nearest even), as FP64"""
+    u = np.asarray(v, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + np.uint64(0x7FFF) + ((u >> np.uint64(16)) & np.uint64(1))) & np.uint64(0xFFFF0000)
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
 
 
 def _probe(n):

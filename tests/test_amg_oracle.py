@@ -88,3 +88,15 @@ def test_cycle_cuts_cg_iterations(lv, oracle, kind, ic, K, dt):
     assert it_j > 0 and it_m > 0
     assert it_m <= 0.2 * it_j, (it_m, it_j)
     assert np.linalg.norm(b - H.A @ x) <= 1e-8 * np.linalg.norm(b)
+
+
+def test_bf16_rounding_matches_torch():
+    """amg_ref.bf16 (the device's FP32 -> BF16 round to nearest even) against torch's conversion,
+    on random magnitudes and on exact ties."""
+    torch = pytest.importorskip("torch")
+    rng = np.random.default_rng(5)
+    v = np.concatenate([rng.standard_normal(4000) * 10.0 ** rng.integers(-6, 6, 4000),
+                        1.0 + np.array([2.0 ** -8, 3 * 2.0 ** -8, 2.0 ** -9, 3 * 2.0 ** -9])])
+    ref = torch.tensor(v.astype(np.float32)).to(torch.bfloat16).to(torch.float64).numpy()
+    assert np.array_equal(amg_ref.bf16(v), ref)
+    assert amg_ref.bf16(np.array([1.0 + 2.0 ** -8]))[0] == 1.0  # a tie goes to the even neighbour

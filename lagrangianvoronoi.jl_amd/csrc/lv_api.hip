@@ -35,6 +35,7 @@ struct lv_ctx {
     int *ne = nullptr;
     double *ev = nullptr;
     int *el = nullptr;
+    int *clip_retry = nullptr;  // [0] count, [1..] the seeds whose polygon outgrew the small LDS capacity
     bool mesh_valid = false;   // the mesh belongs to the current positions
     // cell list
     int *cell_start = nullptr, *cell_labels = nullptr;
@@ -311,6 +312,7 @@ int lv_create(const lv_grid_desc *d, lv_ctx **out)
     ALLOC(ctx->ne, cap);
     ALLOC(ctx->ev, 4 * LV_MAXE * cap);
     ALLOC(ctx->el, LV_MAXE * cap);
+    ALLOC(ctx->clip_retry, cap + 1);
     ALLOC(ctx->cell_start, ncells + 1);
     ALLOC(ctx->d_nitems, 1);
     ALLOC(ctx->d_flags, 2);
@@ -377,7 +379,8 @@ int lv_destroy(lv_ctx *ctx)
     if (ctx->s) hipStreamSynchronize(ctx->s);
     void *ptrs[] = {ctx->F.x, ctx->F.v, ctx->F.dv, ctx->F.mom, ctx->F.rho, ctx->F.e, ctx->F.P, ctx->F.c2,
                     ctx->F.mass, ctx->F.energy, ctx->F.quality, ctx->F.mu, ctx->F.D, ctx->F.area, ctx->F.cent,
-                    ctx->F.phase, ctx->ne, ctx->ev, ctx->el, ctx->cell_start, ctx->cell_labels, ctx->cell_x,
+                    ctx->F.phase, ctx->ne, ctx->ev, ctx->el, ctx->clip_retry, ctx->cell_start, ctx->cell_labels,
+                    ctx->cell_x,
                     ctx->cl_temp, ctx->d_nitems, ctx->d_flags, ctx->path_i, ctx->path_rr, ctx->A.row_ptr,
                     ctx->A.cols, ctx->A.val64, ctx->A.val32, ctx->A.diag64, ctx->A.dinv64, ctx->A.diag32,
                     ctx->A.dinv32, ctx->deg, ctx->A.ecol, ctx->A.ev64, ctx->A.ev32, ctx->b, ctx->P0, ctx->GP, ctx->W.x, ctx->W.r, ctx->W.z, ctx->W.p,
@@ -565,7 +568,7 @@ int lv_remesh(lv_ctx *ctx)
     if (tk) lv_timer_begin(tk, ctx->s);
     LV_HIP_CHECK(lv_launch_clip(ctx->g, n, ctx->F.x, ctx->cell_start, ctx->cell_labels, ctx->cell_x, ctx->path_i,
                                 ctx->path_rr, ctx->ne, ctx->ev, ctx->el, n, ctx->F.area, ctx->F.cent, ctx->d_flags,
-                                ctx->s, (int)ctx->max_items));
+                                ctx->clip_retry, ctx->s, (int)ctx->max_items));
     if (tk) lv_timer_end(tk, ctx->s);
     int flags[2];
     rc = read_flags(ctx, flags);

@@ -4,7 +4,11 @@
 // 201-219), fused into one kernel.
 //
 // One thread owns one seed.  Its polygon lives in LDS as five slot-major planes (v1x, v1y, v2x, v2y,
-// label) of LV_MAXE slots, so lane t of the block addresses bank-conflict-free 8-byte words.  The edge
+// label), so lane t of the block addresses bank-conflict-free 8-byte words.  The first launch gives
+// each polygon CAP_SMALL slots (18 KB per 64-seed block, twice the waves per CU of LV_MAXE slots); a
+// seed whose polygon outgrows them (0.4% at a 2048^2 jittered lattice, where no polygon passes 10
+// edges while it is clipped) is queued and clipped again from the start by a second launch with
+// LV_MAXE slots, so the result is the same either way.  The edge
 // list keeps the reference's storage semantics exactly: cut edges are rewritten in place, deleted
 // edges are replaced by the last edge (FastVector deleteat!, src/fastvector.jl:44-50) and re-tested,
 // and the new edge is pushed last.  With the same candidate order (magic path, then each cell's
@@ -16,9 +20,11 @@
 namespace {
 
 constexpr int CB = 64;          // seeds per block (one wavefront)
-constexpr int CAP = LV_MAXE;    // edge capacity of the LDS polygon
+constexpr int CAP_SMALL = 8;    // edge capacity of the first launch's LDS polygon
+constexpr int RETRY_BLOCKS = 1024;  // blocks of the second launch (grid-stride over the queue)
 constexpr double SIGNUM_EPS = 2.0 * 2.220446049250313e-16;  // 2*eps(Float64), src/polygon.jl:2
 
+template <int CAP>
 struct lds_poly {
     double v1x[CAP][CB], v1y[CAP][CB], v2x[CAP][CB], v2y[CAP][CB];
     int lab[CAP][CB];
@@ -31,7 +37,8 @@ __device__ __forceinline__ int signum(double v)
     return 0;
 }
 
-__device__ __forceinline__ double influence_rr(const lds_poly &P, int t, int len, double x1, double x2)
+template <int CAP>
+__device__ __forceinline__ double influence_rr(const lds_poly<CAP> &P, int t, int len, double x1, double x2)
 {
     double rr = 0.0;
     for (int k = 0; k < len; k++) {
@@ -43,7 +50,9 @@ __device__ __forceinline__ double influence_rr(const lds_poly &P, int t, int len
 }
 
 // voronoicut!(p, y, label): 1 if an edge was added, 0 if not, -1 on overflow
-__device__ __forceinline__ int cut(lds_poly &P, int t, int &len, double x1, double x2, double y1, double y2, int label)
+template <int CAP>
+__device__ __forceinline__ int cut(lds_poly<CAP> &P, int t, int &len, double x1, double x2, double y1, double y2,
+                                   int label)
 {
     const double d1 = y1 - x1, d2 = y2 - x2;
     const double m1 = 0.5 * (y1 + x1), m2 = 0.5 * (y2 + x2);
@@ -97,7 +106,8 @@ __device__ __forceinline__ int cut(lds_poly &P, int t, int &len, double x1, doub
     return 0;
 }
 
-__device__ __forceinline__ void swap_slots(lds_poly &P, int t, int a, int b)
+template <int CAP>
+__device__ __forceinline__ void swap_slots(lds_poly<CAP> &P, int t, int a, int b)
 {
     double tx;
     tx = P.v1x[a][t]; P.v1x[a][t] = P.v1x[b][t]; P.v1x[b][t] = tx;
@@ -107,18 +117,15 @@ __device__ __forceinline__ void swap_slots(lds_poly &P, int t, int a, int b)
     int tl = P.lab[a][t]; P.lab[a][t] = P.lab[b][t]; P.lab[b][t] = tl;
 }
 
-__global__ __launch_bounds__(CB) void k_clip(lv_grid_params g, int n, const double *__restrict__ x,
-                                             const int *__restrict__ cell_start, const int *__restrict__ cell_labels,
-                                             const double *__restrict__ cell_x, int m, const int *__restrict__ path_i,
-                                             const double *__restrict__ path_rr, int *__restrict__ ne,
-                                             double *__restrict__ ev, int *__restrict__ el, int ld,
-                                             double *__restrict__ area, double *__restrict__ cent,
-                                             int *__restrict__ err)
+// one seed's polygon, clipped in P's LDS slots; false when it outgrew them and the kernel defers it
+template <int CAP, bool DEFER>
+__device__ bool clip_seed(const lv_grid_params &g, int n, int i, const double *__restrict__ x,
+                          const int *__restrict__ cell_start, const int *__restrict__ cell_labels,
+                          const double *__restrict__ cell_x, int m, const int *__restrict__ path_i,
+                          const double *__restrict__ path_rr, int *__restrict__ ne, double *__restrict__ ev,
+                          int *__restrict__ el, int ld, double *__restrict__ area, double *__restrict__ cent,
+                          int *__restrict__ err, lds_poly<CAP> &P, int t)
 {
-    __shared__ lds_poly P;
-    const int t = threadIdx.x;
-    const int i = blockIdx.x * CB + t;
-    if (i >= n) return;  // no block-wide barriers below
     const double x1 = x[i], x2 = x[n + i];
 
     // reset!(p, cropping_rect), src/polygon.jl:37-47: DOWN(B->A), LEFT(A->D), UP(D->C), RIGHT(C->B)
@@ -161,13 +168,14 @@ __global__ __launch_bounds__(CB) void k_clip(lv_grid_params g, int n, const doub
         }
         if (status != LV_OK) break;
     }
+    if (DEFER && status == LV_E_POLY_OVERFLOW) return false;
     if (status != LV_OK) {
         atomicCAS(err, 0, status);
         ne[i] = 0;
         area[i] = 0.0;
         cent[i] = 0.0;
         cent[n + i] = 0.0;
-        return;
+        return true;
     }
     // sort_edges!, src/IO.jl:35-48
     for (int a = 0; a < len; a++) {
@@ -189,27 +197,70 @@ __global__ __launch_bounds__(CB) void k_clip(lv_grid_params g, int n, const doub
         Ac += h;
         cx += (h * ((x1 + v1x) + v2x)) / 3.0;
         cy += (h * ((x2 + v1y) + v2y)) / 3.0;
-        ev[(size_t)(0 * CAP + k) * ld + i] = v1x;
-        ev[(size_t)(1 * CAP + k) * ld + i] = v1y;
-        ev[(size_t)(2 * CAP + k) * ld + i] = v2x;
-        ev[(size_t)(3 * CAP + k) * ld + i] = v2y;
+        ev[(size_t)(0 * LV_MAXE + k) * ld + i] = v1x;
+        ev[(size_t)(1 * LV_MAXE + k) * ld + i] = v1y;
+        ev[(size_t)(2 * LV_MAXE + k) * ld + i] = v2x;
+        ev[(size_t)(3 * LV_MAXE + k) * ld + i] = v2y;
         el[(size_t)k * ld + i] = P.lab[k][t];
     }
     ne[i] = len;
     area[i] = A;
     cent[i] = cx / Ac;
     cent[n + i] = cy / Ac;
+    return true;
+}
+
+// every seed with CAP_SMALL slots; the ones that outgrow them go to the queue retry[1..retry[0]]
+__global__ __launch_bounds__(CB) void k_clip(lv_grid_params g, int n, const double *__restrict__ x,
+                                             const int *__restrict__ cell_start, const int *__restrict__ cell_labels,
+                                             const double *__restrict__ cell_x, int m, const int *__restrict__ path_i,
+                                             const double *__restrict__ path_rr, int *__restrict__ ne,
+                                             double *__restrict__ ev, int *__restrict__ el, int ld,
+                                             double *__restrict__ area, double *__restrict__ cent,
+                                             int *__restrict__ err, int *__restrict__ retry)
+{
+    __shared__ lds_poly<CAP_SMALL> P;
+    const int t = threadIdx.x;
+    const int i = blockIdx.x * CB + t;
+    if (i >= n) return;  // no block-wide barriers below
+    if (!clip_seed<CAP_SMALL, true>(g, n, i, x, cell_start, cell_labels, cell_x, m, path_i, path_rr, ne, ev, el, ld,
+                                    area, cent, err, P, t))
+        retry[1 + atomicAdd(&retry[0], 1)] = i;
+}
+
+// the queued seeds with LV_MAXE slots (grid-stride over the queue)
+__global__ __launch_bounds__(CB) void k_clip_retry(lv_grid_params g, int n, const double *__restrict__ x,
+                                                   const int *__restrict__ cell_start,
+                                                   const int *__restrict__ cell_labels,
+                                                   const double *__restrict__ cell_x, int m,
+                                                   const int *__restrict__ path_i,
+                                                   const double *__restrict__ path_rr, int *__restrict__ ne,
+                                                   double *__restrict__ ev, int *__restrict__ el, int ld,
+                                                   double *__restrict__ area, double *__restrict__ cent,
+                                                   int *__restrict__ err, const int *__restrict__ retry)
+{
+    __shared__ lds_poly<LV_MAXE> P;
+    const int t = threadIdx.x;
+    const int cnt = retry[0];
+    for (int q = blockIdx.x * CB + t; q < cnt; q += gridDim.x * CB)
+        clip_seed<LV_MAXE, false>(g, n, retry[1 + q], x, cell_start, cell_labels, cell_x, m, path_i, path_rr, ne, ev,
+                                  el, ld, area, cent, err, P, t);
 }
 
 }  // namespace
 
 hipError_t lv_launch_clip(const lv_grid_params &g, int n, const double *x, const int *cell_start,
                           const int *cell_labels, const double *cell_x, const int *path_i, const double *path_rr,
-                          int *ne, double *ev, int *el, int ld, double *area, double *cent, int *err, hipStream_t s,
-                          int m)
+                          int *ne, double *ev, int *el, int ld, double *area, double *cent, int *err, int *retry,
+                          hipStream_t s, int m)
 {
     if (n <= 0) return hipSuccess;
+    hipError_t e = hipMemsetAsync(retry, 0, sizeof(int), s);
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_clip, dim3((n + CB - 1) / CB), dim3(CB), 0, s, g, n, x, cell_start, cell_labels, cell_x, m,
-                       path_i, path_rr, ne, ev, el, ld, area, cent, err);
+                       path_i, path_rr, ne, ev, el, ld, area, cent, err, retry);
+    const int nb = std::min(RETRY_BLOCKS, (n + CB - 1) / CB);
+    hipLaunchKernelGGL(k_clip_retry, dim3(nb), dim3(CB), 0, s, g, n, x, cell_start, cell_labels, cell_x, m, path_i,
+                       path_rr, ne, ev, el, ld, area, cent, err, (const int *)retry);
     return hipGetLastError();
 }

@@ -131,10 +131,10 @@ hipError_t lv_build_cell_list(const lv_grid_params &g, int n, const double *x, i
                               double *cell_x, int64_t max_items, int *d_nitems, void *temp, hipStream_t s,
                               hipEvent_t ev0, hipEvent_t ev1, bool timed);
 
-// lv_clip.hip
+// lv_clip.hip.  retry: n + 1 ints of work space (the seeds whose polygon needs the full capacity).
 hipError_t lv_launch_clip(const lv_grid_params &g, int n, const double *x, const int *cell_start,
                           const int *cell_labels, const double *cell_x, const int *path_i, const double *path_rr,
-                          int *ne, double *ev, int *el, int ld, double *area, double *cent, int *err,
+                          int *ne, double *ev, int *el, int ld, double *area, double *cent, int *err, int *retry,
                           hipStream_t s, int m);
 
 // Device pointers of the particle fields (PolygonNS, src/celldefs.jl:7-39) as SoA planes.

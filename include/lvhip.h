@@ -219,6 +219,18 @@ int lv_solve_pressure(lv_ctx *ctx, const double *b, double *x, const lv_solve_op
  * The multilevel cycle keeps the amg_layout of the last solve on this context (0 before any). */
 int lv_apply_preconditioner(lv_ctx *ctx, int32_t method, const double *r, double *z);
 
+/* ---- output (src/IO.jl:9-71) ------------------------------------------------------------------ */
+/* export_grid(grid, filename, vars...), src/IO.jl:9-34: the polygons as VTK XML PolyData (.vtp is
+ * appended to filename unless it ends so), each polygon's vertices the v1 of its edges in their
+ * chained order, with one cell-data column per field id in `fields` (LV_FIELD_X..LV_FIELD_PHASE):
+ * Float64 for numbers, (x, y, 0) for vectors and for D its first two column-major entries, as
+ * append_datasets! (src/IO.jl:61-71) does.  compress != 0 zlib-compresses every array (WriteVTK's
+ * default); an unknown field is LV_E_INVALID ("Cannot export variable ... because it does not
+ * exist.", src/IO.jl:65). */
+int lv_export_grid(lv_ctx *ctx, const char *filename, const int32_t *fields, int32_t nfields, int32_t compress);
+/* export_points(grid, filename, vars...), src/IO.jl:50-59: the seeds as a cloud of vertices */
+int lv_export_points(lv_ctx *ctx, const char *filename, const int32_t *fields, int32_t nfields, int32_t compress);
+
 /* ---- learned SPAI preconditioner --------------------------------------------------------------
  * No counterpart in the reference (it runs unpreconditioned MINRES, src/pressure.jl:219).  A
  * message-passing network over the Voronoi edges emits the coefficients of a symmetric, strictly

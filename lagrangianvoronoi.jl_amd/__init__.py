@@ -20,7 +20,8 @@ __all__ = [
     "LVError", "Rectangle", "UnitRectangle", "VoronoiGrid", "GridNS", "PressureSolver", "remesh", "move",
     "stiffened_eos", "ideal_eos", "find_rho", "gravity_step", "find_pressure", "pressure_step", "find_D",
     "viscous_step", "bdary_friction", "find_dv", "relaxation_step", "step_fused", "StepDesc",
-    "BDARY_UP", "BDARY_RIGHT", "BDARY_DOWN", "BDARY_LEFT", "lib", "library_path",
+    "BDARY_UP", "BDARY_RIGHT", "BDARY_DOWN", "BDARY_LEFT", "lib", "library_path", "export_grid", "export_points",
+    "SimulationWorkspace", "run", "movingavg",
 ]
 
 BDARY_UP, BDARY_RIGHT, BDARY_DOWN, BDARY_LEFT = -1, -2, -3, -4
@@ -141,6 +142,8 @@ _SIGNATURES = {
     "lv_spai_train_step": [_P, _D, _I, ctypes.c_uint64, _P],
     "lv_spai_loss_grad": [_P, _I, ctypes.c_uint64, _I, _P, _P],
     "lv_spai_outputs": [_P, _P, _P, _P, _P],
+    "lv_export_grid": [_P, ctypes.c_char_p, _P, _I, _I],
+    "lv_export_points": [_P, ctypes.c_char_p, _P, _I, _I],
     "lv_timing_enable": [_P, _I, _I],
     "lv_timing_read": [_P, _I, _P, _P, _P],
 }
@@ -588,4 +591,45 @@ def multiphase_projection(solver):
     return it.value, rel.value
 
 
+# ---- output (src/IO.jl:9-71) ---------------------------------------------------------------------
+
+# the PolygonNS fields export_grid can write (src/celldefs.jl:7-39); area and centroid are functions
+# of the mesh in the reference, not fields
+EXPORTABLE = ("x", "rho", "v", "e", "P", "c2", "dv", "mass", "momentum", "energy", "phase", "quality", "D", "mu")
+
+
+def _export_ids(vars):
+    ids = []
+    for v in vars:
+        name = v if isinstance(v, str) else str(v)
+        name = name.lstrip(":")
+        if name not in EXPORTABLE:
+            raise ValueError("Cannot export variable %s because it does not exist." % name)
+        ids.append(FIELDS[name][0])
+    return np.asarray(ids, dtype=np.int32)
+
+
+def vtp_filename(filename):
+    """the file vtk_grid(filename, ...) writes: .vtp appended unless already there"""
+    return filename if filename.endswith(".vtp") else filename + ".vtp"
+
+
+def export_grid(grid, filename, *vars, compress=True):
+    """export_grid(grid, filename, vars...), src/IO.jl:9-34: the Voronoi polygons as a VTK XML
+    PolyData file with the named fields as cell data.  Returns the file name."""
+    ids = _export_ids(vars)
+    _check(grid.ctx, lib().lv_export_grid(grid.ctx, str(filename).encode(), _ptr(ids) if ids.size else None,
+                                          int(ids.size), int(bool(compress))))
+    return vtp_filename(str(filename))
+
+
+def export_points(grid, filename, *vars, compress=True):
+    """export_points(grid, filename, vars...), src/IO.jl:50-59: the seeds as a point cloud"""
+    ids = _export_ids(vars)
+    _check(grid.ctx, lib().lv_export_points(grid.ctx, str(filename).encode(), _ptr(ids) if ids.size else None,
+                                            int(ids.size), int(bool(compress))))
+    return vtp_filename(str(filename))
+
+
 from . import spai  # noqa: E402,F401  (learned preconditioner helpers)
+from .simulation import SimulationWorkspace, movingavg, run  # noqa: E402,F401

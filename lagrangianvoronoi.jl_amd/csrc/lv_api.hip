@@ -10,10 +10,12 @@
 #include <string.h>
 
 #include <algorithm>
+#include <string>
 #include <vector>
 
 #include "lv_spai.h"
 #include "lv_amg.h"
+#include "lv_vtk.h"
 
 struct lv_mph_work;
 hipError_t lv_mph_alloc(lv_mph_work **out, int cap);
@@ -616,6 +618,142 @@ int lv_get_mesh(lv_ctx *ctx, int32_t *edge_ptr, int32_t *labels, double *verts)
         }
     }
     edge_ptr[n] = (int32_t)o;
+    return LV_OK;
+}
+
+// ---- VTK export (src/IO.jl:9-71) ------------------------------------------------------------------
+
+namespace {
+
+// the exportable PolygonNS fields (src/celldefs.jl:7-39) by name: field id and whether it is a
+// Number (Float64 column) or converts to Vec3 (append_datasets!, src/IO.jl:61-71)
+struct export_field {
+    const char *name;
+    int id;
+};
+const export_field kExport[] = {{"x", LV_FIELD_X},         {"v", LV_FIELD_V},         {"dv", LV_FIELD_DV},
+                                {"momentum", LV_FIELD_MOMENTUM}, {"rho", LV_FIELD_RHO}, {"e", LV_FIELD_E},
+                                {"P", LV_FIELD_P},         {"c2", LV_FIELD_C2},       {"mass", LV_FIELD_MASS},
+                                {"energy", LV_FIELD_ENERGY}, {"quality", LV_FIELD_QUALITY}, {"mu", LV_FIELD_MU},
+                                {"D", LV_FIELD_D},         {"phase", LV_FIELD_PHASE}};
+
+std::string vtp_path(const char *filename)
+{
+    std::string f(filename);
+    if (f.size() < 4 || f.compare(f.size() - 4, 4, ".vtp") != 0) f += ".vtp";
+    return f;
+}
+
+// the cell-data columns of the requested fields: Float64 for numbers (the phase too), Vec3 (x, y,
+// 0) for vectors, and for the matrix D its first two entries in column-major order, as
+// Vec3(D) does
+int export_columns(lv_ctx *ctx, const int32_t *fields, int32_t nfields, std::vector<std::vector<double>> &cols,
+                   std::vector<lv_vtk_array> &arr)
+{
+    const int n = ctx->n;
+    cols.assign(nfields, {});
+    arr.assign(nfields, {});
+    for (int a = 0; a < nfields; a++) {
+        const export_field *ef = nullptr;
+        for (const export_field &e : kExport)
+            if (e.id == fields[a]) ef = &e;
+        if (!ef) {
+            char msg[160];
+            snprintf(msg, sizeof(msg), "Cannot export variable %d because it does not exist.", (int)fields[a]);
+            return lv_fail(ctx, LV_E_INVALID, msg);
+        }
+        size_t count, elem;
+        bool ro;
+        void *p = field_ptr(ctx, ef->id, &count, &elem, &ro);
+        std::vector<double> &c = cols[a];
+        if (ef->id == LV_FIELD_PHASE) {
+            std::vector<int> h(n);
+            if (n) LV_HIP_CHECK(hipMemcpy(h.data(), p, sizeof(int) * n, hipMemcpyDeviceToHost));
+            c.assign(h.begin(), h.end());
+            arr[a] = lv_vtk_array{ef->name, 1, c.data()};
+            continue;
+        }
+        std::vector<double> h(count);
+        if (count) LV_HIP_CHECK(hipMemcpy(h.data(), p, sizeof(double) * count, hipMemcpyDeviceToHost));
+        if (count == (size_t)n) {
+            c = std::move(h);
+            arr[a] = lv_vtk_array{ef->name, 1, c.data()};
+        } else {
+            c.resize((size_t)3 * n);
+            for (int i = 0; i < n; i++) {
+                c[3 * (size_t)i] = h[i];
+                c[3 * (size_t)i + 1] = h[(size_t)n + i];
+                c[3 * (size_t)i + 2] = 0.0;
+            }
+            arr[a] = lv_vtk_array{ef->name, 3, c.data()};
+        }
+    }
+    return LV_OK;
+}
+
+}  // namespace
+
+int lv_export_grid(lv_ctx *ctx, const char *filename, const int32_t *fields, int32_t nfields, int32_t compress)
+{
+    int rc = check_ctx(ctx, false);
+    if (rc) return rc;
+    if (!filename || nfields < 0 || (nfields > 0 && !fields)) return lv_fail(ctx, LV_E_INVALID, "lv_export_grid");
+    const int n = ctx->n;
+    std::vector<int> ne(n), dummy;
+    std::vector<double> ev((size_t)2 * LV_MAXE * n);
+    if (n) {
+        LV_HIP_CHECK(hipMemcpy(ne.data(), ctx->ne, sizeof(int) * n, hipMemcpyDeviceToHost));
+        // the v1 planes of the edge slots
+        LV_HIP_CHECK(hipMemcpy(ev.data(), ctx->ev, sizeof(double) * 2 * LV_MAXE * n, hipMemcpyDeviceToHost));
+    }
+    // every polygon's edges in their (chained) order: its vertices are the edges' v1 (src/IO.jl:17-27)
+    std::vector<double> pts;
+    std::vector<int64_t> conn, offs(n);
+    int64_t o = 0;
+    for (int i = 0; i < n; i++) {
+        for (int k = 0; k < ne[i]; k++) {
+            pts.push_back(ev[(size_t)k * n + i]);
+            pts.push_back(ev[(size_t)(LV_MAXE + k) * n + i]);
+            pts.push_back(0.0);
+            conn.push_back(o++);
+        }
+        offs[i] = o;
+    }
+    std::vector<std::vector<double>> cols;
+    std::vector<lv_vtk_array> arr;
+    rc = export_columns(ctx, fields, nfields, cols, arr);
+    if (rc) return rc;
+    const std::string path = vtp_path(filename);
+    if (lv_write_vtp(path.c_str(), o, pts.data(), LV_VTK_POLYS, n, conn.data(), offs.data(), arr.data(), nfields,
+                     compress) != 0)
+        return lv_fail(ctx, LV_E_INVALID, "cannot write the VTK file");
+    return LV_OK;
+}
+
+int lv_export_points(lv_ctx *ctx, const char *filename, const int32_t *fields, int32_t nfields, int32_t compress)
+{
+    int rc = check_ctx(ctx, false);
+    if (rc) return rc;
+    if (!filename || nfields < 0 || (nfields > 0 && !fields)) return lv_fail(ctx, LV_E_INVALID, "lv_export_points");
+    const int n = ctx->n;
+    std::vector<double> x((size_t)2 * n), pts((size_t)3 * n);
+    if (n) LV_HIP_CHECK(hipMemcpy(x.data(), ctx->F.x, sizeof(double) * 2 * n, hipMemcpyDeviceToHost));
+    std::vector<int64_t> conn(n), offs(n);
+    for (int i = 0; i < n; i++) {
+        pts[3 * (size_t)i] = x[i];
+        pts[3 * (size_t)i + 1] = x[(size_t)n + i];
+        pts[3 * (size_t)i + 2] = 0.0;
+        conn[i] = i;
+        offs[i] = i + 1;
+    }
+    std::vector<std::vector<double>> cols;
+    std::vector<lv_vtk_array> arr;
+    rc = export_columns(ctx, fields, nfields, cols, arr);
+    if (rc) return rc;
+    const std::string path = vtp_path(filename);
+    if (lv_write_vtp(path.c_str(), n, pts.data(), LV_VTK_VERTS, n, conn.data(), offs.data(), arr.data(), nfields,
+                     compress) != 0)
+        return lv_fail(ctx, LV_E_INVALID, "cannot write the VTK file");
     return LV_OK;
 }
 

@@ -219,6 +219,21 @@ int lv_solve_pressure(lv_ctx *ctx, const double *b, double *x, const lv_solve_op
  * The multilevel cycle keeps the amg_layout of the last solve on this context (0 before any). */
 int lv_apply_preconditioner(lv_ctx *ctx, int32_t method, const double *r, double *z);
 
+/* ---- initialisation and interpolation ------------------------------------------------------- */
+/* the iterations of populate_lloyd! (src/populate.jl:132-145) on the current seeds: niterations
+ * times remesh! then x = centroid(p) for every polygon, then remesh! */
+int lv_lloyd(lv_ctx *ctx, int32_t niterations);
+/* nearest_polygon(grid, x), src/voronoigrid.jl:154-178, for m query points q (planes q1..., q2...)
+ * over the cell list of the last remesh: idx[t] = 0-based polygon index.  LV_E_INVALID when the
+ * search would leave the magic-path prefix (a query farther than r_max from every seed). */
+int lv_nearest_polygon(lv_ctx *ctx, int32_t m, const double *q, int32_t *idx);
+/* movingls(T, grid, p, p -> field[comp]; h, gaussian_kernel), src/movingls.jl:41-96, for every
+ * polygon: order 1, 2, 3 = LinearExpansion, QuadraticExpansion, CubicExpansion (2, 5, 9
+ * coefficients per polygon, row-major in coeffs).  h <= 0 selects grid.h; h must not exceed r_max. */
+int lv_movingls(lv_ctx *ctx, int32_t order, int32_t field, int32_t comp, double h, double *coeffs);
+/* point_value(grid, x, p -> field[comp]), src/movingls.jl:140-145, at m query points */
+int lv_point_values(lv_ctx *ctx, int32_t m, const double *q, int32_t field, int32_t comp, double *out);
+
 /* ---- output (src/IO.jl:9-71) ------------------------------------------------------------------ */
 /* export_grid(grid, filename, vars...), src/IO.jl:9-34: the polygons as VTK XML PolyData (.vtp is
  * appended to filename unless it ends so), each polygon's vertices the v1 of its edges in their

@@ -21,7 +21,9 @@ __all__ = [
     "stiffened_eos", "ideal_eos", "find_rho", "gravity_step", "find_pressure", "pressure_step", "find_D",
     "viscous_step", "bdary_friction", "find_dv", "relaxation_step", "step_fused", "StepDesc",
     "BDARY_UP", "BDARY_RIGHT", "BDARY_DOWN", "BDARY_LEFT", "lib", "library_path", "export_grid", "export_points",
-    "SimulationWorkspace", "run", "movingavg",
+    "SimulationWorkspace", "run", "movingavg", "nearest_polygon", "movingls", "point_value", "poly_eval",
+    "power_vector", "integral", "LinearExpansion", "QuadraticExpansion", "CubicExpansion", "populate_circ",
+    "populate_hex", "populate_lloyd", "populate_rand", "populate_rect", "populate_vogel",
 ]
 
 BDARY_UP, BDARY_RIGHT, BDARY_DOWN, BDARY_LEFT = -1, -2, -3, -4
@@ -142,6 +144,10 @@ _SIGNATURES = {
     "lv_spai_train_step": [_P, _D, _I, ctypes.c_uint64, _P],
     "lv_spai_loss_grad": [_P, _I, ctypes.c_uint64, _I, _P, _P],
     "lv_spai_outputs": [_P, _P, _P, _P, _P],
+    "lv_lloyd": [_P, _I],
+    "lv_nearest_polygon": [_P, _I, _P, _P],
+    "lv_movingls": [_P, _I, _I, _I, _D, _P],
+    "lv_point_values": [_P, _I, _P, _I, _I, _P],
     "lv_export_grid": [_P, ctypes.c_char_p, _P, _I, _I],
     "lv_export_points": [_P, ctypes.c_char_p, _P, _I, _I],
     "lv_timing_enable": [_P, _I, _I],
@@ -226,6 +232,23 @@ class VoronoiGrid:
         self.xperiodic, self.yperiodic = bool(xperiodic), bool(yperiodic)
         if capacity is None:
             capacity = 0 if positions is None else np.asarray(positions).size // 2
+        self._create(capacity)
+        if positions is not None:
+            self.set_positions(positions)
+
+    def reserve(self, n):
+        """room for n seeds; an empty grid (the populate_* pattern) is re-created with it"""
+        if n <= self.capacity:
+            return
+        if self.n:
+            raise ValueError("the grid holds %d seeds and room for %d" % (self.n, self.capacity))
+        if self._ctx and _lib is not None:
+            _lib.lv_destroy(self._ctx)
+        self._ctx = None
+        self._create(n)
+
+    def _create(self, capacity):
+        boundary_rect = self.boundary_rect
         d = GridDesc()
         d.xmin[:] = boundary_rect.xmin
         d.xmax[:] = boundary_rect.xmax
@@ -239,8 +262,6 @@ class VoronoiGrid:
             raise LVError(rc, "lv_create failed with status %d" % rc)
         self._ctx = h_
         self.n = 0
-        if positions is not None:
-            self.set_positions(positions)
 
     def __del__(self):
         ctx = getattr(self, "_ctx", None)
@@ -631,5 +652,99 @@ def export_points(grid, filename, *vars, compress=True):
     return vtp_filename(str(filename))
 
 
+# ---- interpolation (src/movingls.jl, src/voronoigrid.jl:154-178) --------------------------------
+
+LinearExpansion, QuadraticExpansion, CubicExpansion = 1, 2, 3
+_NCOEF = {1: 2, 2: 5, 3: 9}
+
+
+def _field_component(name):
+    """("v", 0) for "v[1]" / ("v", 0) / "P" -> ("P", 0): the scalar a fun(p) of the reference reads"""
+    if isinstance(name, tuple):
+        return name[0], int(name[1])
+    if name.endswith("]"):
+        base, k = name[:-1].split("[")
+        return base, int(k) - 1  # Julia's 1-based component
+    return name, 0
+
+
+def _queries(x):
+    q = np.asarray(x, dtype=np.float64).reshape(-1, 2)
+    return q.shape[0], np.ascontiguousarray(np.concatenate([q[:, 0], q[:, 1]]))
+
+
+def nearest_polygon(grid, x):
+    """nearest_polygon(grid, x), src/voronoigrid.jl:154-178: 0-based indices for points x ((m, 2) or
+    one point)"""
+    m, q = _queries(x)
+    idx = np.empty(m, dtype=np.int32)
+    _check(grid.ctx, lib().lv_nearest_polygon(grid.ctx, m, _ptr(q), _ptr(idx)))
+    return idx
+
+
+def movingls(order, grid, field, h=None):
+    """movingls(T, grid, p, p -> field; h) for every polygon, src/movingls.jl:41-96: an (n, 2|5|9)
+    array of Taylor coefficients (T = LinearExpansion, QuadraticExpansion, CubicExpansion; field
+    "P", "v[1]", ("v", 0), ...), with the Gaussian kernel"""
+    name, comp = _field_component(field)
+    out = np.empty((grid.n, _NCOEF[order]))
+    _check(grid.ctx, lib().lv_movingls(grid.ctx, int(order), FIELDS[name][0], comp, 0.0 if h is None else float(h),
+                                       _ptr(out)))
+    return out
+
+
+def point_value(grid, x, field):
+    """point_value(grid, x, p -> field), src/movingls.jl:140-145, at one point or (m, 2) points"""
+    name, comp = _field_component(field)
+    m, q = _queries(x)
+    out = np.empty(m)
+    _check(grid.ctx, lib().lv_point_values(grid.ctx, m, _ptr(q), FIELDS[name][0], comp, _ptr(out)))
+    return out[0] if np.ndim(x) == 1 else out
+
+
+def power_vector(order, dx):
+    """power_vector(T, x), src/movingls.jl:26-36"""
+    a, b = dx
+    p = [a, b]
+    if order >= 2:
+        p += [a * a, a * b, b * b]
+    if order >= 3:
+        p += [a * a * a, a * a * b, a * b * b, b * b * b]
+    return np.array(p)
+
+
+def poly_eval(val, taylor, dx):
+    """poly_eval(val, taylor, dx), src/movingls.jl:98-105"""
+    order = {2: 1, 5: 2, 9: 3}[len(taylor)]
+    return val + float(np.dot(taylor, power_vector(order, dx)))
+
+
+def integral(x, verts, val, taylor):
+    """integral(p, val, taylor), src/movingls.jl:107-138: the integral over the polygon with seed x and
+    chained edge vertices verts ((k, 4): v1x, v1y, v2x, v2y, as grid.mesh() gives them) of the
+    polynomial val + taylor . power_vector(dx)"""
+    x = np.asarray(x, dtype=np.float64)
+    order = {2: 1, 5: 2, 9: 3}[len(taylor)]
+    out = 0.0
+    for e in np.asarray(verts, dtype=np.float64):
+        v1, v2 = e[:2], e[2:]
+        d1, d2 = v1 - x, v2 - x
+        A = 0.5 * abs(d1[0] * d2[1] - d1[1] * d2[0])
+        if order == 1:
+            out += A * poly_eval(val, taylor, -2 / 3 * x + 1 / 3 * v1 + 1 / 3 * v2)
+        elif order == 2:
+            out += (A / 3) * poly_eval(val, taylor, -1 / 3 * x + 1 / 6 * v1 + 1 / 6 * v2)
+            out += (A / 3) * poly_eval(val, taylor, -5 / 6 * x + 2 / 3 * v1 + 1 / 6 * v2)
+            out += (A / 3) * poly_eval(val, taylor, -5 / 6 * x + 1 / 6 * v1 + 2 / 3 * v2)
+        else:
+            out -= (9 * A / 16) * poly_eval(val, taylor, -2 / 3 * x + 1 / 3 * v1 + 1 / 3 * v2)
+            out += (25 * A / 48) * poly_eval(val, taylor, -2 / 5 * x + 1 / 5 * v1 + 1 / 5 * v2)
+            out += (25 * A / 48) * poly_eval(val, taylor, -4 / 5 * x + 3 / 5 * v1 + 1 / 5 * v2)
+            out += (25 * A / 48) * poly_eval(val, taylor, -4 / 5 * x + 1 / 5 * v1 + 3 / 5 * v2)
+    return out
+
+
 from . import spai  # noqa: E402,F401  (learned preconditioner helpers)
+from .populate import (populate_circ, populate_hex, populate_lloyd, populate_rand, populate_rect,  # noqa: E402,F401
+                       populate_vogel)
 from .simulation import SimulationWorkspace, movingavg, run  # noqa: E402,F401

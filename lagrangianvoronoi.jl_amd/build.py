@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(PKG_DIR, "liblvhip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 SOURCES = ["lv_scan.hip", "lv_celllist.hip", "lv_clip.hip", "lv_stencil.hip", "lv_solver.hip", "lv_spai.hip", "lv_amg.hip", "lv_multiphase.hip",
-           "lv_vtk.hip", "lv_api.hip"]
+           "lv_vtk.hip", "lv_mls.hip", "lv_api.hip"]
 
 HIP_FLAGS = [
     "-O3",

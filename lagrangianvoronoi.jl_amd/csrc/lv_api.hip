@@ -757,6 +757,117 @@ int lv_export_points(lv_ctx *ctx, const char *filename, const int32_t *fields, i
     return LV_OK;
 }
 
+// ---- interpolation (src/movingls.jl, src/voronoigrid.jl:154-178) ------------------------------------
+
+namespace {
+
+// the device plane of component comp of a Float64 field, or null
+const double *field_plane(lv_ctx *ctx, int field, int comp)
+{
+    size_t count, elem;
+    bool ro;
+    double *p = field_ptr(ctx, field, &count, &elem, &ro);
+    if (!p || elem != sizeof(double) || ctx->n <= 0) return nullptr;
+    const size_t comps = count / (size_t)ctx->n;
+    if (comp < 0 || (size_t)comp >= comps) return nullptr;
+    return p + (size_t)comp * ctx->n;
+}
+
+int need_cell_list(lv_ctx *ctx)
+{
+    if (!ctx->mesh_valid) return lv_fail(ctx, LV_E_INVALID, "the cell list is older than the positions: remesh first");
+    return LV_OK;
+}
+
+// device scratch of one call, freed on every return path
+struct dev_buf {
+    void *p = nullptr;
+    ~dev_buf()
+    {
+        if (p) (void)hipFree(p);
+    }
+};
+
+}  // namespace
+
+int lv_nearest_polygon(lv_ctx *ctx, int32_t m, const double *q, int32_t *idx)
+{
+    int rc = check_ctx(ctx, true);
+    if (rc) return rc;
+    if (m < 0 || (m > 0 && (!q || !idx))) return lv_fail(ctx, LV_E_INVALID, "lv_nearest_polygon");
+    if ((rc = need_cell_list(ctx))) return rc;
+    if (m == 0) return LV_OK;
+    dev_buf dq, di;
+    LV_HIP_CHECK(hipMalloc(&dq.p, sizeof(double) * 2 * m));
+    LV_HIP_CHECK(hipMalloc(&di.p, sizeof(int) * m));
+    LV_HIP_CHECK(hipMemcpyAsync(dq.p, q, sizeof(double) * 2 * m, hipMemcpyHostToDevice, ctx->s));
+    LV_HIP_CHECK(lv_launch_nearest(ctx->g, ctx->F.x, ctx->n, ctx->cell_start, ctx->cell_labels, ctx->path_i,
+                                   ctx->path_rr, m, (const double *)dq.p, (int *)di.p, ctx->s));
+    LV_HIP_CHECK(hipMemcpyAsync(idx, di.p, sizeof(int) * m, hipMemcpyDeviceToHost, ctx->s));
+    LV_HIP_CHECK(hipStreamSynchronize(ctx->s));
+    for (int t = 0; t < m; t++)
+        if (idx[t] < 0) return lv_fail(ctx, LV_E_INVALID, "nearest polygon beyond r_max of a query point");
+    return LV_OK;
+}
+
+int lv_movingls(lv_ctx *ctx, int32_t order, int32_t field, int32_t comp, double h, double *coeffs)
+{
+    int rc = check_ctx(ctx, true);
+    if (rc) return rc;
+    if (order < 1 || order > 3 || !coeffs) return lv_fail(ctx, LV_E_INVALID, "lv_movingls: order 1, 2 or 3");
+    if ((rc = need_cell_list(ctx))) return rc;
+    const double *f = field_plane(ctx, field, comp);
+    if (!f) return lv_fail(ctx, LV_E_INVALID, "lv_movingls: not a Float64 field component");
+    if (!(h > 0.0)) h = ctx->g.h;
+    if (h > ctx->g.r_max) return lv_fail(ctx, LV_E_INVALID, "lv_movingls: h beyond r_max");
+    const int N = order == 1 ? 2 : (order == 2 ? 5 : 9);
+    const size_t bytes = sizeof(double) * (size_t)N * ctx->n;
+    dev_buf d;
+    LV_HIP_CHECK(hipMalloc(&d.p, bytes));
+    LV_HIP_CHECK(lv_launch_movingls(ctx->g, ctx->F.x, ctx->n, ctx->cell_start, ctx->cell_labels, ctx->path_i,
+                                    ctx->path_rr, order, f, h, (double *)d.p, ctx->s));
+    LV_HIP_CHECK(hipMemcpyAsync(coeffs, d.p, bytes, hipMemcpyDeviceToHost, ctx->s));
+    LV_HIP_CHECK(hipStreamSynchronize(ctx->s));
+    return LV_OK;
+}
+
+int lv_point_values(lv_ctx *ctx, int32_t m, const double *q, int32_t field, int32_t comp, double *out)
+{
+    int rc = check_ctx(ctx, true);
+    if (rc) return rc;
+    if (m < 0 || (m > 0 && (!q || !out))) return lv_fail(ctx, LV_E_INVALID, "lv_point_values");
+    if ((rc = need_cell_list(ctx))) return rc;
+    const double *f = field_plane(ctx, field, comp);
+    if (!f) return lv_fail(ctx, LV_E_INVALID, "lv_point_values: not a Float64 field component");
+    if (m == 0) return LV_OK;
+    dev_buf dq, dout;
+    LV_HIP_CHECK(hipMalloc(&dq.p, sizeof(double) * 2 * m));
+    LV_HIP_CHECK(hipMalloc(&dout.p, sizeof(double) * m));
+    LV_HIP_CHECK(hipMemcpyAsync(dq.p, q, sizeof(double) * 2 * m, hipMemcpyHostToDevice, ctx->s));
+    LV_HIP_CHECK(lv_launch_point_value(ctx->g, ctx->F.x, ctx->n, ctx->cell_start, ctx->cell_labels, ctx->path_i,
+                                       ctx->path_rr, m, (const double *)dq.p, f, ctx->g.h, (double *)dout.p, ctx->s));
+    LV_HIP_CHECK(hipMemcpyAsync(out, dout.p, sizeof(double) * m, hipMemcpyDeviceToHost, ctx->s));
+    LV_HIP_CHECK(hipStreamSynchronize(ctx->s));
+    for (int t = 0; t < m; t++)
+        if (out[t] != out[t]) return lv_fail(ctx, LV_E_INVALID, "nearest polygon beyond r_max of a query point");
+    return LV_OK;
+}
+
+// populate_lloyd!'s iterations (src/populate.jl:132-145): niterations times remesh! and x = centroid(p)
+int lv_lloyd(lv_ctx *ctx, int32_t niterations)
+{
+    int rc = check_ctx(ctx, true);
+    if (rc) return rc;
+    if (niterations < 0) return lv_fail(ctx, LV_E_INVALID, "lv_lloyd");
+    for (int it = 0; it < niterations; it++) {
+        if ((rc = lv_remesh(ctx))) return rc;
+        LV_HIP_CHECK(hipMemcpyAsync(ctx->F.x, ctx->F.cent, sizeof(double) * 2 * ctx->n, hipMemcpyDeviceToDevice,
+                                    ctx->s));
+        ctx->mesh_valid = false;
+    }
+    return lv_remesh(ctx);
+}
+
 int lv_cell_list_size(lv_ctx *ctx, int64_t *nitems)
 {
     int rc = check_ctx(ctx, true);

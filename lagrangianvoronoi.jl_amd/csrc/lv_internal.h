@@ -169,3 +169,15 @@ hipError_t lv_launch_find_dv(const lv_grid_params &g, lv_fields F, lv_mesh_view 
                              hipStream_t s);
 hipError_t lv_launch_relaxation(const lv_grid_params &g, lv_fields F, lv_mesh_view M, int n, double dt, int rusanov,
                                 hipStream_t s);
+
+// lv_mls.hip: nearest_polygon, movingls (order 1, 2, 3: N = 2, 5, 9 coefficients per polygon, row-major)
+// and point_value over the cell list of the last remesh
+hipError_t lv_launch_movingls(const lv_grid_params &g, const double *x, int n, const int *cell_start,
+                              const int *cell_labels, const int *path_i, const double *path_rr, int order,
+                              const double *f, double h, double *out, hipStream_t s);
+hipError_t lv_launch_nearest(const lv_grid_params &g, const double *x, int n, const int *cell_start,
+                             const int *cell_labels, const int *path_i, const double *path_rr, int m,
+                             const double *q, int *idx, hipStream_t s);
+hipError_t lv_launch_point_value(const lv_grid_params &g, const double *x, int n, const int *cell_start,
+                                 const int *cell_labels, const int *path_i, const double *path_rr, int m,
+                                 const double *q, const double *f, double h, double *out, hipStream_t s);

@@ -21,6 +21,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -45,7 +47,86 @@ CONFIGS = {
     # examples/cavity.jl:32)
     "ldc4096": dict(nx=4096, ny=4096, ymax=1.0, periodic=False, ic="cavity", dt=min(0.1 / 4096, 100.0 / 4096 ** 2),
                     solver="amg", tol=1e-8, window=8, steps=3, warmup=1),
+    # examples/cavity.jl in full: Re = 100, N = 100, populate_lloyd!, t_end = 0.1 Re, then the
+    # centreline profiles by point_value against the published data the example plots against
+    "ldc100-profiles": None,
 }
+
+
+def _profile_error(s, ours, ref_pos, ref_val):
+    """largest |ours - ref| at the reference's interior abscissae, ours (smoothed by movingavg as the
+    example plots it) interpolated linearly on s"""
+    keep = (ref_pos > 0.0) & (ref_pos < 1.0)
+    return float(np.max(np.abs(np.interp(ref_pos[keep], s, ours) - ref_val[keep])))
+
+
+def run_ldc_profiles(lv, args, torch):
+    """examples/cavity.jl:24-111: populate_lloyd!, 10,000 steps of the incompressible cavity at Re = 100,
+    u(0.5, y) and v(x, 0.5) by point_value, compared with Abdelmigid et al. and Ghia et al. (the
+    example's reference files)"""
+    Re, N = 100.0, 100
+    dr = 1.0 / N
+    dt = min(0.1 * dr, 0.1 * Re * dr * dr)
+    t_end = 0.1 * Re
+
+    def ic(g):
+        g.set("rho", 1.0)
+        g.set("mass", g.get("area"))
+        g.set("mu", 1.0 / Re)
+        g.set("c2", np.inf)
+
+    grid = lv.VoronoiGrid(lv.UnitRectangle(), dr)
+    t0 = time.perf_counter()
+    lv.populate_lloyd(grid, ic=ic)
+    torch.cuda.synchronize()
+    t_lloyd = time.perf_counter() - t0
+    solver = lv.PressureSolver(grid, method=args.solver if args.solver_given else "amg", tol=1e-8, itmax=4000,
+                               picard_window=8)
+    desc = lv.make_step_desc(dt, solver=solver, viscous=True, artificial_viscosity=False,
+                             friction={lv.BDARY_UP: (1.0, 0.0)})
+    # t_end = 10 ends in "The Voronoi Mesh has been destroyed" near step 1150 from these seeds (two
+    # seeds meet in a lid corner), on the GPU and likewise on the CPU oracle, so the default is
+    # t = 1 (1000 steps); --steps runs longer and reports the step of a failure
+    steps = 1000 if not args.steps_given else args.steps
+    t0 = time.perf_counter()
+    failed = None
+    for k in range(steps):
+        try:
+            lv.step_fused(grid, desc)
+        except lv.LVError as e:
+            failed = dict(step=k + 1, error=str(e))
+            break
+        if (k + 1) % 1000 == 0:
+            print("ldc100 step %d of %d" % (k + 1, steps), file=sys.stderr, flush=True)
+    grid.sync()
+    elapsed = time.perf_counter() - t0
+    if failed:
+        print(json.dumps(dict(config="ldc100-profiles", particles=grid.n, steps_requested=steps, failed=failed,
+                              ms_per_step=round(elapsed / failed["step"] * 1e3, 3))))
+        return
+    s = np.linspace(0.0, 1.0, 100)
+    u = lv.point_value(grid, np.stack([np.full(100, 0.5), s], 1), "v[1]")
+    v = lv.point_value(grid, np.stack([s, np.full(100, 0.5)], 1), "v[2]")
+    us, vs = np.array(lv.movingavg(list(u))), np.array(lv.movingavg(list(v)))
+    gold = os.path.join(ROOT, "tests", "golden")
+
+    def col(name, pos):
+        import csv
+
+        rows = list(csv.DictReader(open(os.path.join(gold, name))))
+        return np.array([float(r[pos]) for r in rows]), np.array([float(r["Re100"]) for r in rows])
+
+    err = {}
+    for src in ("abdelmigid", "ghia"):
+        y, ref_u = col("ldc_y2vx_%s.csv" % src, "y")
+        x, ref_v = col("ldc_x2vy_%s.csv" % src, "x")
+        err[src] = dict(u_on_x_0p5=round(_profile_error(s, us, y, ref_u), 4),
+                        v_on_y_0p5=round(_profile_error(s, vs, x, ref_v), 4))
+    out = dict(config="ldc100-profiles", particles=grid.n, steps=steps, t=steps * dt, t_end_of_the_example=t_end, dt=dt,
+               ms_per_step=round(elapsed / steps * 1e3, 3), lloyd_s=round(t_lloyd, 3),
+               max_abs_error_vs_published=err, u_min=round(float(us.min()), 4), v_min=round(float(vs.min()), 4),
+               v_max=round(float(vs.max()), 4))
+    print(json.dumps(out))
 
 
 def run_config(lv, args, torch):
@@ -309,6 +390,8 @@ def main():
     lv = load_package()
     if args.mode == "solve":
         return mode_solve(lv, args.K, args.solver, args.tol, args.inner_tol, args.smoother)
+    if args.config == "ldc100-profiles":
+        return run_ldc_profiles(lv, args, torch)
     if args.config != "tg2048":
         return run_config(lv, args, torch)
     K = args.K

@@ -26,6 +26,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+TIMER_EVERY = 16
 HBM_PEAK_GBS = 8000.0
 METRIC = "particle-steps/s at 4M particles; pressure-solve time to 1e-8 rel. residual"
 
@@ -410,7 +411,9 @@ def main():
     dominant = 0  # FP32 SpMV of the inner PCG
     if args.solver == "pcg64":
         dominant = 1
-    L.lv_timing_enable(grid.ctx, dominant, 1)
+    # HIP events around every 16th launch of the dominant kernel (an event pair adds a few
+    # microseconds between the launches it brackets)
+    L.lv_timing_enable(grid.ctx, dominant, TIMER_EVERY)
     grid.sync()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -423,6 +426,8 @@ def main():
     import ctypes
     tot_ms, launches, bpl = ctypes.c_double(), ctypes.c_int64(), ctypes.c_double()
     L.lv_timing_read(grid.ctx, dominant, ctypes.byref(tot_ms), ctypes.byref(launches), ctypes.byref(bpl))
+    seen = ctypes.c_int64()
+    L.lv_timing_seen(grid.ctx, dominant, ctypes.byref(seen))
     L.lv_timing_enable(grid.ctx, dominant, 0)
     elapsed = t1 - t0
     value = n * args.steps / elapsed
@@ -430,7 +435,10 @@ def main():
     first_solve_it = sum(s["iters"][0] for s in stats) / len(stats)
     relres = max(max(s["relres"]) for s in stats)
     avg_launch_ms = tot_ms.value / max(launches.value, 1)
-    achieved = bpl.value / (avg_launch_ms * 1e-3) / 1e9
+    # a launch after convergence returns at once: the bytes are those of the working launches (one
+    # per inner iteration) over the time of all launches (the sampled average times their number)
+    working = sum(sum(s["iters"]) for s in stats)
+    achieved = bpl.value * working / (avg_launch_ms * 1e-3 * max(seen.value, 1)) / 1e9
     # the FP32 SpMV kernel in the layout the solves used (aggregate order: 8-bit degrees, 16-bit offsets)
     if dominant == 0:
         kname = "k_cg_s<float, lv_ell_agg32>" if stats[-1].get("agg_order") else "k_cg_s<float>"
@@ -438,7 +446,9 @@ def main():
         kname = "k_cg_s<double>"
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic_from_profiles(kname),
-                    kernel=kname, avg_launch_us=round(avg_launch_ms * 1e3, 2), launches=launches.value,
+                    kernel=kname, avg_launch_us=round(avg_launch_ms * 1e3, 2), launches=seen.value,
+                    launches_timed=launches.value, working_launches=working,
+                    avg_working_launch_us=round(avg_launch_ms * 1e3 * seen.value / max(working, 1), 2),
                     algorithmic_bytes_per_launch=bpl.value)
     out = dict(metric=METRIC, value=round(value, 1), unit="particle-steps/s", n_gpus=1, steps=args.steps,
                warmup=args.warmup, ms_per_step=round(elapsed / args.steps * 1e3, 3), higher_is_better=True,

@@ -264,12 +264,16 @@ int lv_spai_loss_grad(lv_ctx *ctx, int32_t nprobe, uint64_t seed, int32_t use_da
 int lv_spai_outputs(lv_ctx *ctx, float *o_edge, float *o_node, float *w, float *mdiag);
 
 /* ---- timing ----------------------------------------------------------------------------------- */
-/* When enabled, the library brackets every launch of the named kernel with HIP events on the
- * context stream and accumulates the device time. kernel ids: 0 = FP32 SpMV, 1 = FP64 SpMV,
- * 2 = clipping, 3 = cell-list scatter.  The algorithmic bytes of kernel 0 follow the layout of the
+/* When enabled (on != 0), the library brackets launches of the named kernel with HIP events on
+ * the context stream and accumulates their device time: every launch for on == 1, every on-th
+ * launch for on > 1 (each event pair adds a few microseconds between the launches it brackets);
+ * launches counts the bracketed ones. kernel ids: 0 = FP32 SpMV, 1 = FP64 SpMV, 2 = clipping,
+ * 3 = cell-list scatter.  The algorithmic bytes of kernel 0 follow the layout of the
  * last solve (the operator's own ELL, or the aggregate order's 8-bit degrees and 16-bit offsets). */
 int lv_timing_enable(lv_ctx *ctx, int32_t kernel_id, int32_t on);
 int lv_timing_read(lv_ctx *ctx, int32_t kernel_id, double *total_ms, int64_t *launches, double *bytes_per_launch);
+/* launches of the kernel since lv_timing_enable, bracketed or not */
+int lv_timing_seen(lv_ctx *ctx, int32_t kernel_id, int64_t *seen);
 
 #ifdef __cplusplus
 }

@@ -152,6 +152,7 @@ _SIGNATURES = {
     "lv_export_points": [_P, ctypes.c_char_p, _P, _I, _I],
     "lv_timing_enable": [_P, _I, _I],
     "lv_timing_read": [_P, _I, _P, _P, _P],
+    "lv_timing_seen": [_P, _I, _P],
 }
 
 _lib = None

@@ -46,7 +46,11 @@ struct lv_amg {
     float *pev32;        // couplings, FP32 (Krylov operator, power iteration)
     uint16_t *pev16;     // couplings, BF16 (the cycle's smoothing and residuals)
     float *pdiag32, *pdinv32, *pmval, *pmdg;
-    int *pflag;          // set when a column offset does not fit 16 bits
+    int *pflag;          // [0] set when a column offset does not fit 16 bits, [1] when a level-1 cell
+                         // is too large for the fused down pass
+    int *bfirst;         // fused down pass: the first level-1 cell of each block (nbfuse + 1 entries)
+    size_t cap_bfirst;
+    int nbfuse;          // its blocks; 0: the down pass and the restriction run as two kernels
     size_t cap_iperm, cap_aggp, cap_pdeg, cap_poff, cap_pev32, cap_pev16, cap_pdiag32, cap_pdinv32, cap_pmval,
         cap_pmdg;
     float fine_alpha;    // factor on the correction prolongated into the fine level

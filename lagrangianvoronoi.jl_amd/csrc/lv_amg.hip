@@ -360,17 +360,13 @@ __global__ __launch_bounds__(B) void k_restrict1(int na, const int *__restrict__
     if (q == 0 && I < na) rc[I] = s;
 }
 
-// Jacobi fine smoother, down: z0 = w D^-1 r and t = r - A z0 in one pass (z0 of a neighbour is
-// recomputed from its r)
+// Jacobi fine smoother, down, one row: z0 = w D^-1 r and t = r - A z0 (z0 of a neighbour is
+// recomputed from its r); returns t, stores z0
 template <class V>
-__global__ __launch_bounds__(B) void k_fine_down_j(fine_ops<V> F, const float *__restrict__ r,
-                                                   float *__restrict__ z0, float *__restrict__ t,
-                                                   const lv_cg_scalars *S)
+__device__ __forceinline__ float down_row_j(const fine_ops<V> &F, int i, const float *__restrict__ r,
+                                            float *__restrict__ z0)
 {
-    if (S && S->done) return;
-    const int i = blockIdx.x * B + threadIdx.x;
     const int n = F.n;
-    if (i >= n) return;
     constexpr int U = 8;
     const float ri = r[i];
     const float zi = F.scale * F.dinv[i] * ri;
@@ -392,7 +388,70 @@ __global__ __launch_bounds__(B) void k_fine_down_j(fine_ops<V> F, const float *_
         acc += F.A.value(e) * (zi - F.scale * F.dinv[j] * r[j]);
     }
     z0[i] = zi;
-    t[i] = ri - acc;
+    return ri - acc;
+}
+
+template <class V>
+__global__ __launch_bounds__(B) void k_fine_down_j(fine_ops<V> F, const float *__restrict__ r,
+                                                   float *__restrict__ z0, float *__restrict__ t,
+                                                   const lv_cg_scalars *S)
+{
+    if (S && S->done) return;
+    const int i = blockIdx.x * B + threadIdx.x;
+    if (i >= F.n) return;
+    t[i] = down_row_j(F, i, r, z0);
+}
+
+// The down pass and the restriction in one kernel, for the aggregate order (the rows of level-1
+// cell I are start[I]..start[I+1]-1): block b owns the cells whose first row lies in
+// [FUSE_ROWS b, FUSE_ROWS (b+1)) (bfirst[b]..bfirst[b+1]-1), keeps their residuals t in LDS and sums
+// each cell's as k_restrict1 does (16 lanes, the same order and shuffle tree, so rc is the same).
+// A block holds at most FUSE_ROWS - 1 + (largest cell) rows; set-up checks that this fits FUSE_CAP.
+constexpr int FUSE_ROWS = 224;
+constexpr int FUSE_CAP = 1024;
+
+template <class V>
+__global__ __launch_bounds__(B) void k_fine_down_jr(fine_ops<V> F, const float *__restrict__ r,
+                                                    float *__restrict__ z0, const int *__restrict__ start,
+                                                    const int *__restrict__ bfirst, float *__restrict__ rc,
+                                                    const lv_cg_scalars *S)
+{
+    __shared__ float tl[FUSE_CAP];
+    if (S && S->done) return;
+    const int c0 = bfirst[blockIdx.x], c1 = bfirst[blockIdx.x + 1];
+    const int R0 = start[c0], R1 = start[c1];
+    for (int i = R0 + (int)threadIdx.x; i < R1; i += B) tl[i - R0] = down_row_j(F, i, r, z0);
+    __syncthreads();
+    const int lanes = (c1 - c0) * 16;
+    for (int base = 0; base < lanes; base += B) {  // uniform trip count: every lane shuffles
+        const int idx = base + (int)threadIdx.x;
+        const int I = c0 + (idx >> 4), q = idx & 15;
+        float s = 0.0f;
+        if (idx < lanes) {
+            const int s1 = start[I + 1];
+            for (int a = start[I] + q; a < s1; a += 16) s += tl[a - R0];
+        }
+        s += __shfl_xor(s, 1);
+        s += __shfl_xor(s, 2);
+        s += __shfl_xor(s, 4);
+        s += __shfl_xor(s, 8);
+        if (q == 0 && idx < lanes) rc[I] = s;
+    }
+}
+
+// bfirst[b] = the first cell I with start[I] >= FUSE_ROWS b, for b = 0..nbf-1, and bfirst[nbf] = na
+// (start[na] = n, nbf = n / FUSE_ROWS + 1).  Thread I writes the b in (start[I-1], start[I]] /
+// FUSE_ROWS; flag[1] is set when a cell has more than FUSE_CAP - FUSE_ROWS rows.
+__global__ __launch_bounds__(B) void k_fuse_blocks(int na, int nbf, const int *__restrict__ start,
+                                                   int *__restrict__ bfirst, int *__restrict__ flag)
+{
+    const int I = blockIdx.x * B + threadIdx.x;
+    if (I > na) return;
+    const int lo = I == 0 ? 0 : start[I - 1] / FUSE_ROWS + 1;
+    const int hi = start[I] / FUSE_ROWS;
+    for (int b = lo; b <= hi && b < nbf; b++) bfirst[b] = I;
+    if (I == na) bfirst[nbf] = na;
+    if (I < na && start[I + 1] - start[I] > FUSE_CAP - FUSE_ROWS) atomicOr(flag + 1, 1);
 }
 
 // up: z1 = z0 + alpha P ec and t = r - A z1 in one pass (z1 of a neighbour from its z0 and its
@@ -800,7 +859,7 @@ hipError_t lv_amg_init(lv_amg *G)
     AMG_CHECK(hipMalloc(&G->dense_inv, sizeof(float) * LV_AMG_DENSE_MAX * LV_AMG_DENSE_MAX));
     AMG_CHECK(hipMalloc(&G->pw, sizeof(double) * (2 * LV_NBMAX + 2)));
     AMG_CHECK(hipMalloc(&G->prog, sizeof(int) * LV_AMG_MAXPROG));
-    AMG_CHECK(hipMalloc(&G->pflag, sizeof(int)));
+    AMG_CHECK(hipMalloc(&G->pflag, 2 * sizeof(int)));
     return hipSuccess;
 }
 
@@ -808,7 +867,7 @@ void lv_amg_free(lv_amg *G)
 {
     void *ptrs[] = {G->agg, G->cnt, G->start, G->cursor, G->list, G->scan_tmp, G->t0, G->z0, G->dense_inv,
                     G->pw, G->pv, G->pu, G->prog, G->iperm, G->aggp, G->pdeg, G->poff, G->pev32, G->pev16,
-                    G->pdiag32, G->pdinv32, G->pmval, G->pmdg, G->pflag};
+                    G->pdiag32, G->pdinv32, G->pmval, G->pmdg, G->pflag, G->bfirst};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (int l = 0; l <= LV_AMG_MAXLEV; l++) {
@@ -991,6 +1050,7 @@ hipError_t lv_amg_setup(lv_amg *G, const lv_grid_params &g, const double *x, con
     G->fine_scale = OMEGA;
     // the aggregate order: the rows of each level-1 cell consecutive (list, sorted within a cell)
     G->perm_ok = 0;
+    G->nbfuse = 0;
     if (layout == 0) {
         const size_t m = (size_t)LV_MAXE * n;
         AMG_CHECK(grow(&G->iperm, &G->cap_iperm, (size_t)n));
@@ -1005,7 +1065,7 @@ hipError_t lv_amg_setup(lv_amg *G, const lv_grid_params &g, const double *x, con
             AMG_CHECK(grow(&G->pmval, &G->cap_pmval, m));
             AMG_CHECK(grow(&G->pmdg, &G->cap_pmdg, (size_t)n));
         }
-        AMG_CHECK(hipMemsetAsync(G->pflag, 0, sizeof(int), s));
+        AMG_CHECK(hipMemsetAsync(G->pflag, 0, 2 * sizeof(int), s));
         hipLaunchKernelGGL(k_perm_index, g1(n), dim3(B), 0, s, n, (const int *)G->list, (const int *)G->agg, G->iperm,
                            G->aggp);
         hipLaunchKernelGGL(k_perm_ell, g1(n), dim3(B), 0, s, n, (const int *)G->list, (const int *)G->iperm,
@@ -1013,10 +1073,16 @@ hipError_t lv_amg_setup(lv_amg *G, const lv_grid_params &g, const double *x, con
                            (const float *)A.dinv32, G->fine_learned ? mval : nullptr,
                            G->fine_learned ? mdg : nullptr, G->pdeg, G->poff, G->pev32, G->pev16, G->pdiag32,
                            G->pdinv32, G->pmval, G->pmdg, G->pflag);
-        int wide = 1;
-        AMG_CHECK(hipMemcpyAsync(&wide, G->pflag, sizeof(int), hipMemcpyDeviceToHost, s));
+        // the blocks of the fused down pass (Jacobi smoother)
+        const int nbf = n / FUSE_ROWS + 1;
+        AMG_CHECK(grow(&G->bfirst, &G->cap_bfirst, (size_t)nbf + 1));
+        hipLaunchKernelGGL(k_fuse_blocks, g1((int64_t)na + 1), dim3(B), 0, s, na, nbf, (const int *)G->start,
+                           G->bfirst, G->pflag);
+        int flags[2] = {1, 1};
+        AMG_CHECK(hipMemcpyAsync(flags, G->pflag, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
         AMG_CHECK(hipStreamSynchronize(s));
-        G->perm_ok = wide ? 0 : 1;
+        G->perm_ok = flags[0] ? 0 : 1;
+        G->nbfuse = (G->perm_ok && !flags[1]) ? nbf : 0;
     }
     // the learned smoother's scale: lambda_max(M A) -> FINE_TARGET
     if (G->fine_learned) {
@@ -1049,11 +1115,16 @@ static hipError_t vcycle(const lv_amg *G, const fine_ops<V> &F, const int *agg, 
     if (G->fine_learned) {
         hipLaunchKernelGGL(k_fine_pre<V>, g1(n), dim3(B), 0, s, F, r, G->z0, S);
         hipLaunchKernelGGL(k_fine_resid<V>, g1(n), dim3(B), 0, s, F, r, (const float *)G->z0, G->t0, S);
+        hipLaunchKernelGGL(k_restrict1, g1((int64_t)G->na * 16), dim3(B), 0, s, G->na, (const int *)G->start, list,
+                           (const float *)G->t0, L1.r, S);
+    } else if (!list && G->nbfuse > 0) {
+        hipLaunchKernelGGL(k_fine_down_jr<V>, dim3(G->nbfuse), dim3(B), 0, s, F, r, G->z0, (const int *)G->start,
+                           (const int *)G->bfirst, L1.r, S);
     } else {
         hipLaunchKernelGGL(k_fine_down_j<V>, g1(n), dim3(B), 0, s, F, r, G->z0, G->t0, S);
+        hipLaunchKernelGGL(k_restrict1, g1((int64_t)G->na * 16), dim3(B), 0, s, G->na, (const int *)G->start, list,
+                           (const float *)G->t0, L1.r, S);
     }
-    hipLaunchKernelGGL(k_restrict1, g1((int64_t)G->na * 16), dim3(B), 0, s, G->na, (const int *)G->start, list,
-                       (const float *)G->t0, L1.r, S);
     // coarse levels above the tail, down: each level's pre-smoothed z and the next level's
     // right-hand side
     const int lt = G->ltail;

@@ -98,16 +98,22 @@ int lv_fail_hip(lv_ctx *ctx, hipError_t e, const char *what, const char *file, i
 
 // ---- kernel timers ---------------------------------------------------------------------------------
 
+// An event pair costs a few microseconds of GPU time between the launches it brackets, so a timer
+// may sample: it brackets every `every`-th launch and averages over those.
 void lv_timer_begin(lv_timer *t, hipStream_t s)
 {
+    t->pending = (t->seen++ % (t->every > 0 ? t->every : 1)) == 0;
+    if (!t->pending) return;
     if (t->used == LV_TPOOL) lv_timer_flush(t);
     hipEventRecord(t->ev[2 * t->used], s);
 }
 
 void lv_timer_end(lv_timer *t, hipStream_t s)
 {
+    if (!t->pending) return;
     hipEventRecord(t->ev[2 * t->used + 1], s);
     t->used++;
+    t->pending = 0;
 }
 
 void lv_timer_flush(lv_timer *t)
@@ -344,6 +350,12 @@ int lv_create(const lv_grid_desc *d, lv_ctx **out)
     ALLOC(ctx->W.p32, cap);
     ALLOC(ctx->W.q32, cap);
     ALLOC(ctx->W.S, 1);
+    if (e == hipSuccess)
+        e = hipHostMalloc((void **)&ctx->W.hprog, sizeof(lv_progress), hipHostMallocCoherent | hipHostMallocMapped);
+    if (e == hipSuccess) {
+        memset(ctx->W.hprog, 0, sizeof(lv_progress));
+        e = hipHostGetDevicePointer((void **)&ctx->W.dprog, ctx->W.hprog, 0);
+    }
     if (e == hipSuccess) e = hipMalloc(&ctx->W.partials, lv_partials_bytes((int)cap));
     if (e == hipSuccess) e = hipMalloc(&ctx->scan_temp, lv_scan_temp_bytes((int64_t)cap + 1) + 256);
 #undef ALLOC
@@ -389,6 +401,7 @@ int lv_destroy(lv_ctx *ctx)
                     ctx->W.q, ctx->W.e32, ctx->W.r32, ctx->W.z32, ctx->W.p32, ctx->W.q32, ctx->W.S,
                     ctx->W.partials, ctx->scan_temp};
     for (void *p : ptrs) dfree(p);
+    if (ctx->W.hprog) (void)hipHostFree(ctx->W.hprog);
     if (ctx->spai_ok) lv_spai_free(&ctx->spai);
     if (ctx->amg_ok) lv_amg_free(&ctx->amg);
     lv_mph_free(ctx->mph);
@@ -1174,14 +1187,16 @@ static int ensure_picard_window(lv_ctx *ctx, int w)
 
 // Adds the Picard correction (P_cur - P_prev, AP_cur - AP_prev) to the window: two rounds of
 // classical Gram-Schmidt against the stored q's (applied identically to the z's), then
-// normalisation; the oldest direction makes room when the window is full.
+// normalisation; the oldest direction makes room when the window is full.  The dots stay on the
+// device (the combinations read them there), so the host never waits; a correction with no
+// component left enters as a zero direction, which adds nothing to later projections.
 static int picard_add_direction(lv_ctx *ctx, int w, int *cnt, const double *Pcur, const double *Pprev,
                                 const double *APcur, const double *APprev)
 {
     const int n = ctx->n;
     hipStream_t s = ctx->s;
     const double *xp[8];
-    double c[8], h[8];
+    double c[8];
     xp[0] = Pcur; xp[1] = Pprev; c[0] = 1.0; c[1] = -1.0;
     LV_HIP_CHECK(lv_combine(n, ctx->pw_t1, 0.0, xp, c, 2, s));
     xp[0] = APcur; xp[1] = APprev;
@@ -1192,21 +1207,19 @@ static int picard_add_direction(lv_ctx *ctx, int w, int *cnt, const double *Pcur
             a[k] = ctx->pw_q[k];
             b[k] = ctx->pw_t2;
         }
-        LV_HIP_CHECK(lv_dots(n, a, b, *cnt, h, ctx->W.partials, ctx->pw_dots, s));
+        LV_HIP_CHECK(lv_dots_device(n, a, b, *cnt, ctx->W.partials, ctx->pw_dots, s));
         for (int k = 0; k < *cnt; k++) {
-            c[k] = -h[k];
+            c[k] = -1.0;  // -<q_k, t2>
             xp[k] = ctx->pw_q[k];
         }
-        LV_HIP_CHECK(lv_combine(n, ctx->pw_t2, 1.0, xp, c, *cnt, s));
+        LV_HIP_CHECK(lv_combine(n, ctx->pw_t2, 1.0, xp, c, *cnt, s, ctx->pw_dots, 1));
         for (int k = 0; k < *cnt; k++) xp[k] = ctx->pw_z[k];
-        LV_HIP_CHECK(lv_combine(n, ctx->pw_t1, 1.0, xp, c, *cnt, s));
+        LV_HIP_CHECK(lv_combine(n, ctx->pw_t1, 1.0, xp, c, *cnt, s, ctx->pw_dots, 1));
     }
     {
         const double *a[1] = {ctx->pw_t2}, *b[1] = {ctx->pw_t2};
-        LV_HIP_CHECK(lv_dots(n, a, b, 1, h, ctx->W.partials, ctx->pw_dots, s));
+        LV_HIP_CHECK(lv_dots_device(n, a, b, 1, ctx->W.partials, ctx->pw_dots, s));
     }
-    const double nrm = sqrt(h[0]);
-    if (!(nrm > 0.0) || !(nrm == nrm)) return LV_OK;
     if (*cnt == w) {  // drop the oldest direction
         double *q0 = ctx->pw_q[0], *z0 = ctx->pw_z[0];
         for (int k = 1; k < w; k++) {
@@ -1218,10 +1231,10 @@ static int picard_add_direction(lv_ctx *ctx, int w, int *cnt, const double *Pcur
         (*cnt)--;
     }
     xp[0] = ctx->pw_t2;
-    c[0] = 1.0 / nrm;
-    LV_HIP_CHECK(lv_combine(n, ctx->pw_q[*cnt], 0.0, xp, c, 1, s));
+    c[0] = 1.0;  // 1 / |t2|
+    LV_HIP_CHECK(lv_combine(n, ctx->pw_q[*cnt], 0.0, xp, c, 1, s, ctx->pw_dots, 2));
     xp[0] = ctx->pw_t1;
-    LV_HIP_CHECK(lv_combine(n, ctx->pw_z[*cnt], 0.0, xp, c, 1, s));
+    LV_HIP_CHECK(lv_combine(n, ctx->pw_z[*cnt], 0.0, xp, c, 1, s, ctx->pw_dots, 2));
     (*cnt)++;
     return LV_OK;
 }
@@ -1232,16 +1245,17 @@ static int picard_extrapolate(lv_ctx *ctx, int cnt, const double *APcur)
     const int n = ctx->n;
     hipStream_t s = ctx->s;
     const double *xp[8] = {ctx->b, APcur};
-    double c[8] = {1.0, -1.0}, h[8];
+    double c[8] = {1.0, -1.0};
     LV_HIP_CHECK(lv_combine(n, ctx->pw_t2, 0.0, xp, c, 2, s));
     const double *a[8], *b[8];
     for (int k = 0; k < cnt; k++) {
         a[k] = ctx->pw_q[k];
         b[k] = ctx->pw_t2;
+        c[k] = 1.0;  // <q_k, b - AP_cur>
     }
-    LV_HIP_CHECK(lv_dots(n, a, b, cnt, h, ctx->W.partials, ctx->pw_dots, s));
+    LV_HIP_CHECK(lv_dots_device(n, a, b, cnt, ctx->W.partials, ctx->pw_dots, s));
     for (int k = 0; k < cnt; k++) xp[k] = ctx->pw_z[k];
-    LV_HIP_CHECK(lv_combine(n, ctx->P0, 1.0, xp, h, cnt, s));
+    LV_HIP_CHECK(lv_combine(n, ctx->P0, 1.0, xp, c, cnt, s, ctx->pw_dots, 1));
     return LV_OK;
 }
 
@@ -1579,8 +1593,18 @@ int lv_timing_enable(lv_ctx *ctx, int32_t kernel_id, int32_t on)
     LV_HIP_CHECK(hipStreamSynchronize(ctx->s));
     lv_timer_flush(t);
     t->on = on ? 1 : 0;
+    t->every = on > 1 ? on : 1;
+    t->seen = 0;
+    t->pending = 0;
     t->total_ms = 0.0;
     t->launches = 0;
+    return LV_OK;
+}
+
+int lv_timing_seen(lv_ctx *ctx, int32_t kernel_id, int64_t *seen)
+{
+    if (!ctx || kernel_id < 0 || kernel_id >= LV_NKTIMERS || !seen) return LV_E_INVALID;
+    *seen = ctx->timers[kernel_id]->seen;
     return LV_OK;
 }
 

@@ -7,7 +7,7 @@
 // rz is double-buffered by iteration parity: the p-update of iteration k reads rz[k&1] in every
 // block while block 0 writes rz[(k+1)&1].
 struct lv_cg_scalars {
-    double rz[2], rr, rr0, pq, alpha, beta, bb, thr2, tol2, rr64, bmean, xmean0, xmean;
+    double rz[2], rr, rr0, pq, alpha, beta, bb, thr2, tol2, rr64, bmean, xmean0, xmean, xshift;
     int iters, itmax, done, singular;
 };
 
@@ -29,16 +29,26 @@ struct lv_pressure_system {
     int valid;
 };
 
+// Progress of a running CG solve, stored by the device in pinned host memory: seen = the last
+// iteration whose convergence test ran, done = the solve's done flag.  The host reads it without
+// synchronising to keep its launches one iteration ahead of the device.
+struct lv_progress {
+    int seen, done;
+};
+
 struct lv_solver_work {
     double *x, *r, *z, *p, *q;
     float *e32, *r32, *z32, *p32, *q32;
     double *partials;
     lv_cg_scalars *S;
+    lv_progress *hprog, *dprog;  // the same pinned record: host and device addresses
 };
 
 #define LV_TPOOL 1024
 struct lv_timer {
     int on, used;
+    int every, pending;  // events around every `every`-th launch; pending: this launch is sampled
+    int64_t seen;
     double total_ms;
     int64_t launches;
     double bytes_per_launch;
@@ -75,8 +85,12 @@ hipError_t lv_solve_device(const lv_pressure_system &A, lv_solver_work &W, int n
                            int *fallbacks);
 
 // Picard extrapolation helpers (FP64 vectors of length n on stream s)
-hipError_t lv_dots(int n, const double *const *a, const double *const *b, int m, double *out_host, double *partials,
-                   double *dout, hipStream_t s);
-hipError_t lv_combine(int n, double *y, double alpha, const double *const *x, const double *c, int m, hipStream_t s);
+// the dots into the device array dout[0..m) without a host copy
+hipError_t lv_dots_device(int n, const double *const *a, const double *const *b, int m, double *partials,
+                          double *dout, hipStream_t s);
+// y = alpha y + sum_q c_q x_q, the c_q scaled by device values: dmode 0 none, 1 c_q dc[q],
+// 2 c_q / sqrt(dc[0]) (0 unless dc[0] is positive and finite)
+hipError_t lv_combine(int n, double *y, double alpha, const double *const *x, const double *c, int m, hipStream_t s,
+                      const double *dc = nullptr, int dmode = 0);
 hipError_t lv_apply_from_residual(int n, const double *b, const double *r, double *Ax, const lv_solver_work &W,
                                   hipStream_t s);

@@ -13,13 +13,15 @@
 // fused with the r.z and r.r dots; the learned preconditioner adds one ELL pass z = M r.  Dots reduce
 // per block into a partials array that every block of the consuming kernel re-reduces in the same
 // fixed order (no atomics, no fences), so results are deterministic; alpha, beta and convergence are
-// evaluated on the device and the host reads one flag per batch of iterations.
+// evaluated on the device, which also reports its progress to pinned host memory so that the host
+// keeps one iteration queued ahead and stops launching at convergence without synchronising.
 #include <string.h>
 
 #include "lv_solver.h"
 #include "lv_reduce.h"
 #include "lv_amg.h"
 #include "lv_ell.h"
+#include <chrono>
 #include <type_traits>
 
 namespace {
@@ -204,6 +206,7 @@ __global__ __launch_bounds__(B) void k_finalize(const double *__restrict__ parti
         break;
     case FIN_SUM1:
         S->xmean = o[0] / n;
+        S->xshift = S->xmean - S->xmean0;
         break;
     case FIN_INIT32:  // reads the (r.z, r.r) partials of the inner init: |r32|^2 = 1 up to rounding
         S->thr2 = inner_tol2 * o[1];
@@ -297,7 +300,7 @@ template <typename T, class V>
 __global__ __launch_bounds__(B) void k_cg_s(int n, V A, const T *__restrict__ diag,
                                             const T *__restrict__ z, T *__restrict__ p, T *__restrict__ q,
                                             const double *__restrict__ Prr, double *__restrict__ Ppq,
-                                            lv_cg_scalars *S, int par, int k, int first)
+                                            lv_cg_scalars *S, lv_progress *prog, int par, int k, int first)
 {
     if (S->done) return;
     double o[2];
@@ -311,6 +314,10 @@ __global__ __launch_bounds__(B) void k_cg_s(int n, V A, const T *__restrict__ di
         if (k == 0) S->rr0 = rr;
         S->iters = k;
         if (stop) S->done = conv ? 1 : 2;
+        if (prog) {
+            if (stop) __hip_atomic_store(&prog->done, conv ? 1 : 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            __hip_atomic_store(&prog->seen, k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
     }
     if (stop) return;
     const T beta = first ? (T)0 : (T)(rz / S->rz[par]);
@@ -384,7 +391,8 @@ __global__ __launch_bounds__(B) void k_cg_u(int n, T *__restrict__ x, T *__restr
 
 // the convergence test of k_cg_s, made on the residual k_cg_u has just updated (after k+1
 // updates), so that a converged iteration skips the preconditioner
-__global__ __launch_bounds__(B) void k_conv_check(const double *__restrict__ Pur, int nb, lv_cg_scalars *S, int k)
+__global__ __launch_bounds__(B) void k_conv_check(const double *__restrict__ Pur, int nb, lv_cg_scalars *S,
+                                                  lv_progress *prog, int k)
 {
     if (S->done) return;
     double o[1];
@@ -393,6 +401,7 @@ __global__ __launch_bounds__(B) void k_conv_check(const double *__restrict__ Pur
         S->rr = o[0];
         S->iters = k + 1;
         S->done = 1;
+        if (prog) __hip_atomic_store(&prog->done, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     }
 }
 
@@ -455,18 +464,50 @@ __global__ __launch_bounds__(B) void k_visc_rhs(int n, const double *__restrict_
     x[i] = vc[i];
 }
 
-// true residual r = (b - bmean) - A x; partials |r|^2, |b'|^2 (FP64)
+// (A (x - d))_i with every x read shifted by d, i.e. the product with the stored values x - d
+__device__ __forceinline__ double spmv_row_shifted(int i, int n, const lv_ell_own<double> &A,
+                                                   const double *__restrict__ diag, const double *__restrict__ x,
+                                                   double d)
+{
+    constexpr int U = 8;
+    const double xi = x[i] - d;
+    double acc = diag[i] * xi;
+    int c[U];
+    double v[U], xj[U];
+    const int deg = lv_ell_head<U>(A, i, n, c, v);
+#pragma unroll
+    for (int k = 0; k < U; k++) xj[k] = x[c[k]] - d;
+#pragma unroll
+    for (int k = 0; k < U; k++) acc += v[k] * (xi - xj[k]);
+    for (int k = U; k < deg; k++) {
+        const size_t e = (size_t)k * n + i;
+        acc += A.value(e) * (xi - (x[A.column(e, i, n)] - d));
+    }
+    return acc;
+}
+
+// true residual r = (b - bmean) - A x; partials |r|^2, |b'|^2 (FP64).  With Pxs (the partial sums
+// of x written by k_correct, singular systems) the residual is that of x - d, d = mean(x) - xmean0,
+// the x that k_shift stores at the end of the solve; block 0 records d.
 __global__ __launch_bounds__(B) void k_resid64(int n, const int *__restrict__ deg, const int *__restrict__ ecol,
                                                const double *__restrict__ ev, const double *__restrict__ diag,
                                                const double *__restrict__ b, const double *__restrict__ x,
                                                double *__restrict__ r, double *__restrict__ partials,
-                                               const lv_cg_scalars *S)
+                                               const double *__restrict__ Pxs, lv_cg_scalars *S)
 {
     const double bmean = S->bmean;
+    double d = 0.0;
+    if (Pxs) {
+        double t[1];
+        reduce_all<1>(Pxs, gridDim.x, t);
+        d = t[0] / n - S->xmean0;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) S->xshift = d;
+    const lv_ell_own<double> A{deg, ecol, ev};
     double v[3] = {0.0, 0.0, 0.0};
     for (int i = blockIdx.x * B + threadIdx.x; i < n; i += gridDim.x * B) {
         const double bi = b[i] - bmean;
-        const double ri = bi - spmv_row<double>(i, n, lv_ell_own<double>{deg, ecol, ev}, diag, x);
+        const double ri = bi - (Pxs ? spmv_row_shifted(i, n, A, diag, x, d) : spmv_row<double>(i, n, A, diag, x));
         r[i] = ri;
         v[0] += ri * ri;
         v[1] += bi * bi;
@@ -481,10 +522,10 @@ __global__ __launch_bounds__(B) void k_sum1(int n, const double *__restrict__ x,
     block_partials<3>(v, partials, gridDim.x);
 }
 
-// restore the mean of x to that of the initial guess (singular systems only)
+// restore the mean of x to that of the initial guess (singular systems only): x -= xshift
 __global__ __launch_bounds__(B) void k_shift(int n, double *__restrict__ x, const lv_cg_scalars *S)
 {
-    const double d = S->xmean - S->xmean0;
+    const double d = S->xshift;
     for (int i = blockIdx.x * B + threadIdx.x; i < n; i += gridDim.x * B) x[i] -= d;
 }
 
@@ -511,13 +552,20 @@ __global__ __launch_bounds__(B) void k_cg32_init(int n, const double *__restrict
     block_partials<2>(v, Prr, gridDim.x);
 }
 
-// x += |r64| * e (e in the aggregate order when iperm is given: x[i] takes e[iperm[i]])
+// x += |r64| e (e in the aggregate order when iperm is given: x[i] takes e[iperm[i]]); with Pxs
+// also the partial sums of the new x (for the shift of a singular system)
 __global__ __launch_bounds__(B) void k_correct(int n, double *__restrict__ x, const float *__restrict__ e,
-                                               const int *__restrict__ iperm, const lv_cg_scalars *S)
+                                               const int *__restrict__ iperm, double *__restrict__ Pxs,
+                                               const lv_cg_scalars *S)
 {
     const double s = sqrt(S->rr64);
-    for (int i = blockIdx.x * B + threadIdx.x; i < n; i += gridDim.x * B)
-        x[i] += s * (double)e[iperm ? iperm[i] : i];
+    double v[1] = {0.0};
+    for (int i = blockIdx.x * B + threadIdx.x; i < n; i += gridDim.x * B) {
+        const double xi = x[i] + s * (double)e[iperm ? iperm[i] : i];
+        x[i] = xi;
+        v[0] += xi;
+    }
+    if (Pxs) block_partials<1>(v, Pxs, gridDim.x);
 }
 
 // ---- small FP64 vector algebra for the Picard extrapolation -----------------------------------
@@ -550,14 +598,30 @@ __global__ __launch_bounds__(B) void k_multidot_fin(const double *__restrict__ p
     if (threadIdx.x < 8) out[threadIdx.x] = o[threadIdx.x];
 }
 
-// y = alpha*y + sum_q c_q x_q  (y may be one of the x_q only if its c is applied via alpha)
-__global__ __launch_bounds__(B) void k_combine(int n, double *__restrict__ y, double alpha, lv_comb8 C, int m)
+// y = alpha*y + sum_q c_q x_q  (y may be one of the x_q only if its c is applied via alpha).  The
+// coefficients are C.c scaled by device values dc (the output of k_multidot_fin): dmode 0 none,
+// 1 c_q dc[q], 2 c_q / sqrt(dc[0]) (0 when dc[0] is not positive and finite).
+__global__ __launch_bounds__(B) void k_combine(int n, double *__restrict__ y, double alpha, lv_comb8 C, int m,
+                                               const double *__restrict__ dc, int dmode)
 {
+    double c[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) c[q] = C.c[q];
+    if (dmode == 1) {
+#pragma unroll
+        for (int q = 0; q < 8; q++)
+            if (q < m) c[q] = C.c[q] * dc[q];
+    } else if (dmode == 2) {
+        const double h = dc[0];
+        const double f = (h > 0.0 && h <= 1.7976931348623157e308) ? 1.0 / sqrt(h) : 0.0;
+#pragma unroll
+        for (int q = 0; q < 8; q++) c[q] = C.c[q] * f;
+    }
     for (int i = blockIdx.x * B + threadIdx.x; i < n; i += gridDim.x * B) {
         double acc = (alpha != 0.0) ? alpha * y[i] : 0.0;  // alpha = 0 never reads y (may be uninitialised)
 #pragma unroll
         for (int q = 0; q < 8; q++)
-            if (q < m) acc += C.c[q] * C.x[q][i];
+            if (q < m) acc += c[q] * C.x[q][i];
         y[i] = acc;
     }
 }
@@ -649,55 +713,73 @@ struct parts {
 };
 inline parts parts_of(double *base) { return parts{base, base + 3 * NBMAX, base + 4 * NBMAX, base + 6 * NBMAX}; }
 
-// CG iterations (two launches each, three with the learned preconditioner) until the device flags
-// completion; the host checks the flag once per batch of iterations.
+// CG iterations (two launches each; with a preconditioner also the convergence test and its
+// application) until the device flags completion.  The host launches iteration k once the device
+// has begun iteration k - 1 (prog->seen, in pinned host memory), so the queue stays one iteration
+// deep without synchronising, and stops launching as soon as prog->done is set: at most about one
+// iteration runs as no-ops after convergence.  If the record does not advance for 20 ms the host
+// synchronises and reads the scalars instead, so the loop ends whatever the record shows.
 template <typename T, class V>
 hipError_t cg_iterate(int n, V A, const T *diag, const T *dinv, T *x, T *r, T *z, T *p, T *q, const parts &P,
-                      lv_cg_scalars *dS, hipStream_t s, lv_timer *tm, int batch, lv_cg_scalars &h,
-                      const lv_precond *M)
+                      lv_cg_scalars *dS, lv_progress *hprog, lv_progress *dprog, hipStream_t s, lv_timer *tm,
+                      lv_cg_scalars &h, const lv_precond *M)
 {
     const int nb = nblocks_of(n);
-    int k = 0, nit = batch;
-    for (;;) {
-        for (int j = 0; j < nit; j++, k++) {
-            const int par = k & 1;
-            {
-                timer_scope ts(tm, s);
-                hipLaunchKernelGGL((k_cg_s<T, V>), dim3(nb), dim3(B), 0, s, n, A, diag, (const T *)z, p, q,
-                                   (const double *)P.rr, P.pq, dS, par, k, k == 0 ? 1 : 0);
-            }
-            if (M) {
-                hipLaunchKernelGGL((k_cg_u<T, false>), dim3(nb), dim3(B), 0, s, n, x, r, z, (const T *)p,
-                                   (const T *)q, dinv, (const double *)P.pq, P.rr, P.ur, dS, par);
-                hipLaunchKernelGGL(k_conv_check, dim3(1), dim3(B), 0, s, (const double *)P.ur, nb, dS, k);
-                if (M->amg) {
-                    hipError_t e = lv_amg_vcycle(M->amg, *M->A, (const float *)r, (float *)z, P.rr,
-                                                 (const lv_cg_scalars *)dS, s);
-                    if (e != hipSuccess) return e;
-                } else if constexpr (std::is_same<V, lv_ell_own<float>>::value) {
-                    hipLaunchKernelGGL(k_apply_m32, dim3(nb), dim3(B), 0, s, n, A.deg, A.col, M->mval, M->mdg,
-                                       (const float *)r, (float *)z, P.rr, (const lv_cg_scalars *)dS);
-                } else {
-                    return hipErrorInvalidValue;  // the learned one-level preconditioner is in the operator's order
+    // no kernel of this solve that stores progress is in flight: the previous sweep ended with a
+    // synchronising read
+    __atomic_store_n(&hprog->seen, -1, __ATOMIC_RELAXED);
+    __atomic_store_n(&hprog->done, 0, __ATOMIC_RELEASE);
+    for (int k = 0;; k++) {
+        if (k > 0) {
+            bool stop = false;
+            const auto t0 = std::chrono::steady_clock::now();
+            for (int spin = 1;; spin++) {
+                if (__atomic_load_n(&hprog->done, __ATOMIC_ACQUIRE)) {
+                    stop = true;
+                    break;
                 }
-            } else {
-                hipLaunchKernelGGL((k_cg_u<T, true>), dim3(nb), dim3(B), 0, s, n, x, r, z, (const T *)p,
-                                   (const T *)q, dinv, (const double *)P.pq, P.rr, P.ur, dS, par);
+                if (__atomic_load_n(&hprog->seen, __ATOMIC_ACQUIRE) >= k - 1) break;
+                if ((spin & 255) == 0 &&
+                    std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(20)) {
+                    hipError_t e = read_scalars(dS, &h, s);  // every launched kernel has finished
+                    if (e != hipSuccess) return e;
+                    stop = h.done != 0;
+                    break;
+                }
             }
+            if (stop) break;
         }
-        hipError_t e = read_scalars(dS, &h, s);
+        const int par = k & 1;
+        {
+            timer_scope ts(tm, s);
+            hipLaunchKernelGGL((k_cg_s<T, V>), dim3(nb), dim3(B), 0, s, n, A, diag, (const T *)z, p, q,
+                               (const double *)P.rr, P.pq, dS, dprog, par, k, k == 0 ? 1 : 0);
+        }
+        if (M) {
+            hipLaunchKernelGGL((k_cg_u<T, false>), dim3(nb), dim3(B), 0, s, n, x, r, z, (const T *)p,
+                               (const T *)q, dinv, (const double *)P.pq, P.rr, P.ur, dS, par);
+            hipLaunchKernelGGL(k_conv_check, dim3(1), dim3(B), 0, s, (const double *)P.ur, nb, dS, dprog, k);
+            if (M->amg) {
+                hipError_t e = lv_amg_vcycle(M->amg, *M->A, (const float *)r, (float *)z, P.rr,
+                                             (const lv_cg_scalars *)dS, s);
+                if (e != hipSuccess) return e;
+            } else if constexpr (std::is_same<V, lv_ell_own<float>>::value) {
+                hipLaunchKernelGGL(k_apply_m32, dim3(nb), dim3(B), 0, s, n, A.deg, A.col, M->mval, M->mdg,
+                                   (const float *)r, (float *)z, P.rr, (const lv_cg_scalars *)dS);
+            } else {
+                return hipErrorInvalidValue;  // the learned one-level preconditioner is in the operator's order
+            }
+        } else {
+            hipLaunchKernelGGL((k_cg_u<T, true>), dim3(nb), dim3(B), 0, s, n, x, r, z, (const T *)p,
+                               (const T *)q, dinv, (const double *)P.pq, P.rr, P.ur, dS, par);
+        }
+        hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
-        if (tm) lv_timer_flush(tm);
-        if (h.done) return hipSuccess;
-        // the next batch: the iterations still needed at the rate observed so far, plus one (an
-        // iteration past convergence costs its launches as no-ops; a short batch costs a sync)
-        nit = batch;
-        if (h.iters > 0 && h.rr > 0.0 && h.rr < h.rr0 && h.thr2 > 0.0 && h.thr2 < h.rr) {
-            const double need = log(h.thr2 / h.rr) / (log(h.rr / h.rr0) / h.iters);
-            nit = need < 30.0 ? (int)ceil(need) + 1 : 32;
-            if (nit < 2) nit = 2;
-        }
     }
+    hipError_t e = read_scalars(dS, &h, s);
+    if (e != hipSuccess) return e;
+    if (tm) lv_timer_flush(tm);
+    return hipSuccess;
 }
 
 }  // namespace
@@ -725,7 +807,6 @@ hipError_t lv_solve_device(const lv_pressure_system &A, lv_solver_work &W, int n
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_means, G, dim3(B), 0, s, n, b, (const double *)W.x, P.p3);
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(B), 0, s, (const double *)P.p3, nb, (int)FIN_MEANS, n, 0.0, W.S);
-    const int batch = 8;  // the first batch of a sweep; later ones follow the observed rate
     int total = 0, nsweep = 0;
     if (opt.method == LV_SOLVER_PCG64) {
         hipLaunchKernelGGL(k_cg64_init, G, dim3(B), 0, s, n, (const int *)A.deg, (const int *)A.ecol,
@@ -734,18 +815,22 @@ hipError_t lv_solve_device(const lv_pressure_system &A, lv_solver_work &W, int n
         hipLaunchKernelGGL(k_finalize, dim3(1), dim3(B), 0, s, (const double *)P.p3, nb, (int)FIN_INIT64, n, 0.0,
                            W.S);
         e = cg_iterate<double>(n, lv_ell_own<double>{A.deg, A.ecol, A.ev64}, (const double *)A.diag64,
-                               (const double *)A.dinv64, W.x, W.r, W.z, W.p, W.q, P, W.S, s,
-                               t64, batch, h, (const lv_precond *)nullptr);
+                               (const double *)A.dinv64, W.x, W.r, W.z, W.p, W.q, P, W.S, W.hprog,
+                               W.dprog, s, t64, h, (const lv_precond *)nullptr);
         if (e != hipSuccess) return e;
         total = h.iters;
         *relres0 = (h.bb > 0.0) ? sqrt(h.rr0 / h.bb) : sqrt(h.rr0);
     } else {
         const int outer_max = opt.outer_max > 0 ? opt.outer_max : 20;
         const double floor_tol = opt.inner_tol > 0.0 ? opt.inner_tol : 1e-4;
+        // singular systems: the residual after a correction is that of x shifted back to the initial
+        // guess's mean (k_correct writes the sums of x, k_resid64 shifts what it reads); k_shift
+        // stores that x when the sweeps end, so the last residual is the returned x's
+        double *Pxs = A.singular ? P.ur : nullptr;
         for (;;) {
             hipLaunchKernelGGL(k_resid64, G, dim3(B), 0, s, n, (const int *)A.deg, (const int *)A.ecol,
                                (const double *)A.ev64, (const double *)A.diag64, b, (const double *)W.x, W.r, P.p3,
-                               (const lv_cg_scalars *)W.S);
+                               nsweep > 0 ? (const double *)Pxs : (const double *)nullptr, W.S);
             hipLaunchKernelGGL(k_finalize, dim3(1), dim3(B), 0, s, (const double *)P.p3, nb, (int)FIN_RESID, n, 0.0,
                                W.S);
             e = read_scalars(W.S, &h, s);
@@ -779,10 +864,12 @@ hipError_t lv_solve_device(const lv_pressure_system &A, lv_solver_work &W, int n
                 }
                 if (Gp)
                     e = cg_iterate<float>(n, lv_ell_agg32{Gp->pdeg, Gp->poff, Gp->pev32}, (const float *)Gp->pdiag32,
-                                          dinv32, W.e32, W.r32, W.z32, W.p32, W.q32, P, W.S, s, t32, batch, h, Mi);
+                                          dinv32, W.e32, W.r32, W.z32, W.p32, W.q32, P, W.S, W.hprog, W.dprog, s,
+                                          t32, h, Mi);
                 else
                     e = cg_iterate<float>(n, lv_ell_own<float>{A.deg, A.ecol, A.ev32}, (const float *)A.diag32,
-                                          dinv32, W.e32, W.r32, W.z32, W.p32, W.q32, P, W.S, s, t32, batch, h, Mi);
+                                          dinv32, W.e32, W.r32, W.z32, W.p32, W.q32, P, W.S, W.hprog, W.dprog, s,
+                                          t32, h, Mi);
                 if (e != hipSuccess) return e;
                 // Jacobi-PCG is the fallback: a sweep of the learned preconditioner that stalls or breaks
                 // down is redone with Jacobi
@@ -795,21 +882,26 @@ hipError_t lv_solve_device(const lv_pressure_system &A, lv_solver_work &W, int n
             total += h.iters;
             nsweep++;
             hipLaunchKernelGGL(k_correct, G, dim3(B), 0, s, n, W.x, (const float *)W.e32,
-                               Gp ? (const int *)Gp->iperm : (const int *)nullptr, (const lv_cg_scalars *)W.S);
+                               Gp ? (const int *)Gp->iperm : (const int *)nullptr, Pxs, (const lv_cg_scalars *)W.S);
         }
+        if (Pxs && nsweep > 0) hipLaunchKernelGGL(k_shift, G, dim3(B), 0, s, n, W.x, (const lv_cg_scalars *)W.S);
     }
-    // final FP64 residual of the returned x
-    if (A.singular) {
-        hipLaunchKernelGGL(k_sum1, G, dim3(B), 0, s, n, (const double *)W.x, P.p3);
-        hipLaunchKernelGGL(k_finalize, dim3(1), dim3(B), 0, s, (const double *)P.p3, nb, (int)FIN_SUM1, n, 0.0, W.S);
-        hipLaunchKernelGGL(k_shift, G, dim3(B), 0, s, n, W.x, (const lv_cg_scalars *)W.S);
+    if (opt.method == LV_SOLVER_PCG64) {
+        // final FP64 residual of the returned x
+        if (A.singular) {
+            hipLaunchKernelGGL(k_sum1, G, dim3(B), 0, s, n, (const double *)W.x, P.p3);
+            hipLaunchKernelGGL(k_finalize, dim3(1), dim3(B), 0, s, (const double *)P.p3, nb, (int)FIN_SUM1, n, 0.0,
+                               W.S);
+            hipLaunchKernelGGL(k_shift, G, dim3(B), 0, s, n, W.x, (const lv_cg_scalars *)W.S);
+        }
+        hipLaunchKernelGGL(k_resid64, G, dim3(B), 0, s, n, (const int *)A.deg, (const int *)A.ecol,
+                           (const double *)A.ev64, (const double *)A.diag64, b, (const double *)W.x, W.r, P.p3,
+                           (const double *)nullptr, W.S);
+        hipLaunchKernelGGL(k_finalize, dim3(1), dim3(B), 0, s, (const double *)P.p3, nb, (int)FIN_RESID, n, 0.0,
+                           W.S);
+        e = read_scalars(W.S, &h, s);
+        if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(k_resid64, G, dim3(B), 0, s, n, (const int *)A.deg, (const int *)A.ecol,
-                       (const double *)A.ev64, (const double *)A.diag64, b, (const double *)W.x, W.r, P.p3,
-                       (const lv_cg_scalars *)W.S);
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(B), 0, s, (const double *)P.p3, nb, (int)FIN_RESID, n, 0.0, W.S);
-    e = read_scalars(W.S, &h, s);
-    if (e != hipSuccess) return e;
     e = hipMemcpyAsync(x, W.x, sizeof(double) * n, hipMemcpyDeviceToDevice, s);
     if (e != hipSuccess) return e;
     *iters = total;
@@ -876,8 +968,8 @@ hipError_t lv_apply_precond(const lv_pressure_system &A, lv_solver_work &W, int 
 
 // ---- host helpers for the Picard extrapolation ------------------------------------------------
 
-hipError_t lv_dots(int n, const double *const *a, const double *const *b, int m, double *out_host, double *partials,
-                   double *dout, hipStream_t s)
+hipError_t lv_dots_device(int n, const double *const *a, const double *const *b, int m, double *partials,
+                          double *dout, hipStream_t s)
 {
     lv_vec8 V;
     for (int q = 0; q < 8; q++) {
@@ -887,19 +979,18 @@ hipError_t lv_dots(int n, const double *const *a, const double *const *b, int m,
     const int nb = nblocks_of(n);
     hipLaunchKernelGGL(k_multidot, dim3(nb), dim3(B), 0, s, n, V, m, partials);
     hipLaunchKernelGGL(k_multidot_fin, dim3(1), dim3(B), 0, s, (const double *)partials, nb, dout);
-    hipError_t e = hipMemcpyAsync(out_host, dout, sizeof(double) * 8, hipMemcpyDeviceToHost, s);
-    if (e != hipSuccess) return e;
-    return hipStreamSynchronize(s);
+    return hipGetLastError();
 }
 
-hipError_t lv_combine(int n, double *y, double alpha, const double *const *x, const double *c, int m, hipStream_t s)
+hipError_t lv_combine(int n, double *y, double alpha, const double *const *x, const double *c, int m, hipStream_t s,
+                      const double *dc, int dmode)
 {
     lv_comb8 C;
     for (int q = 0; q < 8; q++) {
         C.x[q] = (q < m) ? x[q] : y;
         C.c[q] = (q < m) ? c[q] : 0.0;
     }
-    hipLaunchKernelGGL(k_combine, dim3(nblocks_of(n)), dim3(B), 0, s, n, y, alpha, C, m);
+    hipLaunchKernelGGL(k_combine, dim3(nblocks_of(n)), dim3(B), 0, s, n, y, alpha, C, m, dc, dmode);
     return hipGetLastError();
 }
 

@@ -51,6 +51,8 @@ struct lv_amg {
     int *bfirst;         // fused down pass: the first level-1 cell of each block (nbfuse + 1 entries)
     size_t cap_bfirst;
     int nbfuse;          // its blocks; 0: the down pass and the restriction run as two kernels
+    int z0_in;           // 1: the fused down pass can take z0 = fine_scale D^-1 r from the caller
+                         // (stored row-locally by the Krylov kernel that updates r)
     size_t cap_iperm, cap_aggp, cap_pdeg, cap_poff, cap_pev32, cap_pev16, cap_pdiag32, cap_pdinv32, cap_pmval,
         cap_pmdg;
     float fine_alpha;    // factor on the correction prolongated into the fine level
@@ -69,6 +71,7 @@ hipError_t lv_amg_setup(lv_amg *G, const lv_grid_params &g, const double *x, con
                         const float *mval, const float *mdg, int layout, hipStream_t s);
 // z = B r for the symmetric V(1,1) cycle B; writes the (r.z, r.r) partials of lv_nblocks(n) blocks
 // that the next CG kernel reduces.  Every kernel is a no-op once S->done is set.  With perm_ok, r
-// and z are in the aggregate order.
+// and z are in the aggregate order.  z0_ready (only when G->z0_in): G->z0 already holds
+// fine_scale D^-1 r, bit for bit as the cycle would compute it.
 hipError_t lv_amg_vcycle(const lv_amg *G, const lv_pressure_system &A, const float *r, float *z, double *Prr,
-                         const lv_cg_scalars *S, hipStream_t s);
+                         const lv_cg_scalars *S, hipStream_t s, bool z0_ready = false);

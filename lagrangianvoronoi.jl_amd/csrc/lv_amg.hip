@@ -410,7 +410,30 @@ __global__ __launch_bounds__(B) void k_fine_down_j(fine_ops<V> F, const float *_
 constexpr int FUSE_ROWS = 224;
 constexpr int FUSE_CAP = 1024;
 
+// t = r - A z0 for one row when z0 = w D^-1 r is already stored (the same values down_row_j forms)
 template <class V>
+__device__ __forceinline__ float down_row_z0(const fine_ops<V> &F, int i, const float *__restrict__ r,
+                                             const float *__restrict__ z0)
+{
+    const int n = F.n;
+    constexpr int U = 8;
+    const float zi = z0[i];
+    float acc = F.diag[i] * zi;
+    int c[U];
+    float v[U], zj[U];
+    const int d = lv_ell_head<U>(F.A, i, n, c, v);
+#pragma unroll
+    for (int k = 0; k < U; k++) zj[k] = z0[c[k]];
+#pragma unroll
+    for (int k = 0; k < U; k++) acc += v[k] * (zi - zj[k]);
+    for (int k = U; k < d; k++) {
+        const size_t e = (size_t)k * n + i;
+        acc += F.A.value(e) * (zi - z0[F.A.column(e, i, n)]);
+    }
+    return r[i] - acc;
+}
+
+template <class V, bool Z0>
 __global__ __launch_bounds__(B) void k_fine_down_jr(fine_ops<V> F, const float *__restrict__ r,
                                                     float *__restrict__ z0, const int *__restrict__ start,
                                                     const int *__restrict__ bfirst, float *__restrict__ rc,
@@ -420,7 +443,8 @@ __global__ __launch_bounds__(B) void k_fine_down_jr(fine_ops<V> F, const float *
     if (S && S->done) return;
     const int c0 = bfirst[blockIdx.x], c1 = bfirst[blockIdx.x + 1];
     const int R0 = start[c0], R1 = start[c1];
-    for (int i = R0 + (int)threadIdx.x; i < R1; i += B) tl[i - R0] = down_row_j(F, i, r, z0);
+    for (int i = R0 + (int)threadIdx.x; i < R1; i += B)
+        tl[i - R0] = Z0 ? down_row_z0(F, i, r, z0) : down_row_j(F, i, r, z0);
     __syncthreads();
     const int lanes = (c1 - c0) * 16;
     for (int base = 0; base < lanes; base += B) {  // uniform trip count: every lane shuffles
@@ -557,10 +581,23 @@ struct st_level {
     float alpha;  // factor on the correction prolongated into this level
 };
 
+// wrap(i, K, per) for i in [-1, K] (a cell index plus an offset of at most one): a compare and an
+// add instead of two remainders
+__device__ __forceinline__ int wrap_near(int i, int K, int per, bool &ok)
+{
+    if (per) {
+        ok = true;
+        return i < 0 ? i + K : (i >= K ? i - K : i);
+    }
+    ok = (i >= 0 && i < K);
+    return i;
+}
+
 // the 8 neighbour cells of (cx, cy) on level L and their couplings; a neighbour outside a
 // non-periodic level gets coupling 0 and index f, so every load below is in bounds and all of them
-// can be issued at once
-__device__ __forceinline__ void st_nbrs(const st_level &L, int cx, int cy, int f, float ck[8], int J[8])
+// can be issued at once.  With PJ, also each neighbour's parent on the next level (Kyc columns).
+__device__ __forceinline__ void st_nbrs(const st_level &L, int cx, int cy, int f, float ck[8], int J[8],
+                                        int *PJ = nullptr, int Kyc = 0)
 {
     const int n = L.Kx * L.Ky;
 #pragma unroll
@@ -568,10 +605,11 @@ __device__ __forceinline__ void st_nbrs(const st_level &L, int cx, int cy, int f
         int ox, oy;
         offset_of(k, ox, oy);
         bool okx, oky;
-        const int jx = wrap(cx + ox, L.Kx, L.xper, okx), jy = wrap(cy + oy, L.Ky, L.yper, oky);
+        const int jx = wrap_near(cx + ox, L.Kx, L.xper, okx), jy = wrap_near(cy + oy, L.Ky, L.yper, oky);
         const bool ok = okx && oky;
         J[k] = ok ? jx * L.Ky + jy : f;
         ck[k] = ok ? L.c[(size_t)k * n + f] : 0.0f;
+        if (PJ) PJ[k] = ok ? (jx / CF) * Kyc + jy / CF : (cx / CF) * Kyc + cy / CF;
     }
 }
 
@@ -581,7 +619,8 @@ __device__ __forceinline__ float st_down_lane(const st_level &L, int Kyc, int nc
                                               const float *__restrict__ r, float *__restrict__ z)
 {
     if (P >= nc) return 0.0f;
-    const int cx = CF * (P / Kyc) + q / CF, cy = CF * (P % Kyc) + q % CF;
+    const int Px = P / Kyc;
+    const int cx = CF * Px + q / CF, cy = CF * (P - Px * Kyc) + q % CF;
     if (cx >= L.Kx || cy >= L.Ky) return 0.0f;
     const int f = cx * L.Ky + cy;
     float ck[8], rj[8], dj[8];
@@ -615,14 +654,14 @@ __device__ __forceinline__ void st_up_cell(const st_level &L, int Kyc, int f, co
                                            const float *__restrict__ z, const float *__restrict__ ec,
                                            float *__restrict__ out)
 {
-    const int cx = f / L.Ky, cy = f % L.Ky;
+    const int cx = f / L.Ky, cy = f - cx * L.Ky;
     float ck[8], zj[8], ej[8];
-    int J[8];
-    st_nbrs(L, cx, cy, f, ck, J);
+    int J[8], PJ[8];
+    st_nbrs(L, cx, cy, f, ck, J, PJ, Kyc);
 #pragma unroll
     for (int k = 0; k < 8; k++) {
         zj[k] = z[J[k]];
-        ej[k] = L.alpha * ec[(J[k] / L.Ky / CF) * Kyc + (J[k] % L.Ky) / CF];
+        ej[k] = L.alpha * ec[PJ[k]];
     }
     const float z1 = z[f] + L.alpha * ec[(cx / CF) * Kyc + cy / CF];
     float acc = L.d[f] * z1;
@@ -636,7 +675,7 @@ __device__ __forceinline__ void st_up_cell(const st_level &L, int Kyc, int f, co
 __device__ __forceinline__ void st_wsave_cell(const st_level &L, int f, float *__restrict__ r,
                                               const float *__restrict__ t, float *__restrict__ acc)
 {
-    const int cx = f / L.Ky, cy = f % L.Ky;
+    const int cx = f / L.Ky, cy = f - cx * L.Ky;
     float ck[8], tj[8];
     int J[8];
     st_nbrs(L, cx, cy, f, ck, J);
@@ -689,44 +728,122 @@ struct tail_levels {
     float *r[LV_AMG_MAXLEV + 1], *z[LV_AMG_MAXLEV + 1], *t[LV_AMG_MAXLEV + 1], *acc[LV_AMG_MAXLEV + 1];
 };
 
-__global__ __launch_bounds__(TB) void k_tail(tail_levels T, const int *__restrict__ prog, int nprog,
-                                             const float *__restrict__ inv, const lv_cg_scalars *S)
+// where the tail keeps its levels: global memory (the levels' own arrays) or LDS (offsets into
+// one workgroup array); both give the same values, operation by operation
+struct tail_in_global {
+    const tail_levels &T;
+    const float *inv_;
+    __device__ const float *d(int l) const { return T.d[l]; }
+    __device__ const float *c(int l) const { return T.c[l]; }
+    __device__ const float *dinv(int l) const { return T.dinv[l]; }
+    __device__ float *r(int l) const { return T.r[l]; }
+    __device__ float *z(int l) const { return T.z[l]; }
+    __device__ float *t(int l) const { return T.t[l]; }
+    __device__ float *acc(int l) const { return T.acc[l]; }
+    __device__ const float *inv() const { return inv_; }
+};
+
+// offsets (in floats) of every array of the tail levels in the LDS copy; total <= TAIL_LDS
+constexpr int TAIL_LDS = 35 * 1024;
+struct tail_layout {
+    int d[LV_AMG_MAXLEV + 1], c[LV_AMG_MAXLEV + 1], dinv[LV_AMG_MAXLEV + 1];
+    int r[LV_AMG_MAXLEV + 1], z[LV_AMG_MAXLEV + 1], t[LV_AMG_MAXLEV + 1], acc[LV_AMG_MAXLEV + 1];
+    int inv, total;
+};
+
+struct tail_in_lds {
+    float *lds;
+    const tail_layout &Y;
+    __device__ const float *d(int l) const { return lds + Y.d[l]; }
+    __device__ const float *c(int l) const { return lds + Y.c[l]; }
+    __device__ const float *dinv(int l) const { return lds + Y.dinv[l]; }
+    __device__ float *r(int l) const { return lds + Y.r[l]; }
+    __device__ float *z(int l) const { return lds + Y.z[l]; }
+    __device__ float *t(int l) const { return lds + Y.t[l]; }
+    __device__ float *acc(int l) const { return lds + Y.acc[l]; }
+    __device__ const float *inv() const { return lds + Y.inv; }
+};
+
+// the program's operations, each followed by a barrier that makes its writes visible to the
+// workgroup
+template <class M>
+__device__ __forceinline__ void run_tail(const tail_levels &T, const M &m, const int *prog, int nprog)
 {
-    if (S && S->done) return;
     for (int pc = 0; pc < nprog; pc++) {
         const int code = prog[pc];
         const int op = code >> OP_SHIFT, l = code & ((1 << OP_SHIFT) - 1);
-        const st_level L{T.Kx[l], T.Ky[l], T.xper, T.yper, T.d[l], T.c[l], T.dinv[l], T.alpha[l]};
+        const st_level L{T.Kx[l], T.Ky[l], T.xper, T.yper, m.d(l), m.c(l), m.dinv(l), T.alpha[l]};
         const int n = T.Kx[l] * T.Ky[l];
         if (op == OP_DOWN) {
             const int Kyc = T.Ky[l + 1], nc = T.Kx[l + 1] * T.Ky[l + 1];
+            float *rl = m.r(l), *zl = m.z(l), *rc = m.r(l + 1);
             for (int base = 0; base < nc * CF2; base += TB) {  // uniform trip count: every lane shuffles
                 const int tid = base + threadIdx.x;
                 const int P = tid / CF2, q = tid % CF2;
-                const float t = sum_children(st_down_lane(L, Kyc, nc, P, q, T.r[l], T.z[l]));
-                if (q == 0 && P < nc) T.r[l + 1][P] = t;
+                const float t = sum_children(st_down_lane(L, Kyc, nc, P, q, rl, zl));
+                if (q == 0 && P < nc) rc[P] = t;
             }
         } else if (op == OP_DENSE) {
             // t = inv r: 16 lanes per row (LV_AMG_DENSE_MAX * 16 == TB)
             const int i = threadIdx.x >> 4, q = threadIdx.x & 15;
-            const float *__restrict__ rd = T.r[l];
+            const float *rd = m.r(l), *iv = m.inv();
             float acc = 0.0f;
             if (i < n)
-                for (int j = q; j < n; j += 16) acc += inv[j * n + i] * rd[j];
+                for (int j = q; j < n; j += 16) acc += iv[j * n + i] * rd[j];
             acc += __shfl_xor(acc, 1);
             acc += __shfl_xor(acc, 2);
             acc += __shfl_xor(acc, 4);
             acc += __shfl_xor(acc, 8);
-            if (q == 0 && i < n) T.t[l][i] = acc;
+            if (q == 0 && i < n) m.t(l)[i] = acc;
         } else if (op == OP_UP) {
-            for (int f = threadIdx.x; f < n; f += TB) st_up_cell(L, T.Ky[l + 1], f, T.r[l], T.z[l], T.t[l + 1], T.t[l]);
+            const float *rl = m.r(l), *zl = m.z(l), *ec = m.t(l + 1);
+            float *out = m.t(l);
+            for (int f = threadIdx.x; f < n; f += TB) st_up_cell(L, T.Ky[l + 1], f, rl, zl, ec, out);
         } else if (op == OP_WSAVE) {
-            for (int f = threadIdx.x; f < n; f += TB) st_wsave_cell(L, f, T.r[l], T.t[l], T.acc[l]);
+            float *rl = m.r(l), *tl = m.t(l), *al = m.acc(l);
+            for (int f = threadIdx.x; f < n; f += TB) st_wsave_cell(L, f, rl, tl, al);
         } else {  // OP_WADD
-            for (int f = threadIdx.x; f < n; f += TB) T.t[l][f] += T.acc[l][f];
+            float *tl = m.t(l);
+            const float *al = m.acc(l);
+            for (int f = threadIdx.x; f < n; f += TB) tl[f] += al[f];
         }
         __syncthreads();
     }
+}
+
+__global__ __launch_bounds__(TB) void k_tail(tail_levels T, const int *__restrict__ prog, int nprog,
+                                             const float *__restrict__ inv, const lv_cg_scalars *S)
+{
+    if (S && S->done) return;
+    run_tail(T, tail_in_global{T, inv}, prog, nprog);
+}
+
+// The same program with the tail levels in LDS: the couplings, the dense inverse, the program
+// and the right-hand side of the first level are copied in, every operation then reads and
+// writes LDS only, and the final correction of the first level is copied out.
+__global__ __launch_bounds__(TB) void k_tail_lds(tail_levels T, tail_layout Y, const int *__restrict__ prog,
+                                                 int nprog, const float *__restrict__ inv, const lv_cg_scalars *S)
+{
+    __shared__ float lds[TAIL_LDS];
+    __shared__ int lprog[LV_AMG_MAXPROG];
+    if (S && S->done) return;
+    const int first = T.first, last = T.last;
+    for (int pc = threadIdx.x; pc < nprog; pc += TB) lprog[pc] = prog[pc];
+    for (int l = first; l < last; l++) {
+        const int n = T.Kx[l] * T.Ky[l];
+        for (int f = threadIdx.x; f < n; f += TB) {
+            lds[Y.d[l] + f] = T.d[l][f];
+            lds[Y.dinv[l] + f] = T.dinv[l][f];
+        }
+        for (int f = threadIdx.x; f < 8 * n; f += TB) lds[Y.c[l] + f] = T.c[l][f];
+    }
+    const int nd = T.Kx[last] * T.Ky[last];
+    for (int f = threadIdx.x; f < nd * nd; f += TB) lds[Y.inv + f] = inv[f];
+    const int n0 = T.Kx[first] * T.Ky[first];
+    for (int f = threadIdx.x; f < n0; f += TB) lds[Y.r[first] + f] = T.r[first][f];
+    __syncthreads();
+    run_tail(T, tail_in_lds{lds, Y}, lprog, nprog);
+    for (int f = threadIdx.x; f < n0; f += TB) T.t[first][f] = lds[Y.t[first] + f];
 }
 static_assert(LV_AMG_DENSE_MAX * 16 == TB, "the dense solve of the tail maps 16 lanes to a row");
 
@@ -1051,6 +1168,7 @@ hipError_t lv_amg_setup(lv_amg *G, const lv_grid_params &g, const double *x, con
     // the aggregate order: the rows of each level-1 cell consecutive (list, sorted within a cell)
     G->perm_ok = 0;
     G->nbfuse = 0;
+    G->z0_in = 0;
     if (layout == 0) {
         const size_t m = (size_t)LV_MAXE * n;
         AMG_CHECK(grow(&G->iperm, &G->cap_iperm, (size_t)n));
@@ -1083,6 +1201,7 @@ hipError_t lv_amg_setup(lv_amg *G, const lv_grid_params &g, const double *x, con
         AMG_CHECK(hipStreamSynchronize(s));
         G->perm_ok = flags[0] ? 0 : 1;
         G->nbfuse = (G->perm_ok && !flags[1]) ? nbf : 0;
+        G->z0_in = (G->nbfuse > 0 && !G->fine_learned) ? 1 : 0;
     }
     // the learned smoother's scale: lambda_max(M A) -> FINE_TARGET
     if (G->fine_learned) {
@@ -1107,7 +1226,7 @@ hipError_t lv_amg_setup(lv_amg *G, const lv_grid_params &g, const double *x, con
 // (null when they are consecutive)
 template <class V>
 static hipError_t vcycle(const lv_amg *G, const fine_ops<V> &F, const int *agg, const int *list, const float *r,
-                         float *z, double *Prr, const lv_cg_scalars *S, hipStream_t s)
+                         float *z, double *Prr, const lv_cg_scalars *S, hipStream_t s, bool z0_ready)
 {
     const int n = G->n0;
     const lv_amg_level &L1 = G->lev[1];
@@ -1118,8 +1237,12 @@ static hipError_t vcycle(const lv_amg *G, const fine_ops<V> &F, const int *agg, 
         hipLaunchKernelGGL(k_restrict1, g1((int64_t)G->na * 16), dim3(B), 0, s, G->na, (const int *)G->start, list,
                            (const float *)G->t0, L1.r, S);
     } else if (!list && G->nbfuse > 0) {
-        hipLaunchKernelGGL(k_fine_down_jr<V>, dim3(G->nbfuse), dim3(B), 0, s, F, r, G->z0, (const int *)G->start,
-                           (const int *)G->bfirst, L1.r, S);
+        if (z0_ready && G->z0_in)
+            hipLaunchKernelGGL((k_fine_down_jr<V, true>), dim3(G->nbfuse), dim3(B), 0, s, F, r, G->z0,
+                               (const int *)G->start, (const int *)G->bfirst, L1.r, S);
+        else
+            hipLaunchKernelGGL((k_fine_down_jr<V, false>), dim3(G->nbfuse), dim3(B), 0, s, F, r, G->z0,
+                               (const int *)G->start, (const int *)G->bfirst, L1.r, S);
     } else {
         hipLaunchKernelGGL(k_fine_down_j<V>, g1(n), dim3(B), 0, s, F, r, G->z0, G->t0, S);
         hipLaunchKernelGGL(k_restrict1, g1((int64_t)G->na * 16), dim3(B), 0, s, G->na, (const int *)G->start, list,
@@ -1154,8 +1277,36 @@ static hipError_t vcycle(const lv_amg *G, const fine_ops<V> &F, const int *agg, 
         T.z[l] = L.z;
         T.t[l] = L.t;
     }
-    hipLaunchKernelGGL(k_tail, dim3(1), dim3(TB), 0, s, T, (const int *)G->prog, G->nprog,
-                       (const float *)G->dense_inv, S);
+    // the tail in LDS when its levels fit (the usual case: about 14 floats per cell of the levels
+    // above the dense one, whose total stays below twice TAIL_MAX), else in global memory
+    tail_layout Y;
+    memset(&Y, 0, sizeof(Y));
+    int off = 0;
+    auto put = [&off](int count) {
+        const int o = off;
+        off += count;
+        return o;
+    };
+    for (int l = lt; l <= G->nlev; l++) {
+        const int nl = G->lev[l].n;
+        if (l < G->nlev) {
+            Y.d[l] = put(nl);
+            Y.dinv[l] = put(nl);
+            Y.c[l] = put(8 * nl);
+            Y.z[l] = put(nl);
+            Y.acc[l] = put(nl);
+        }
+        Y.r[l] = put(nl);
+        Y.t[l] = put(nl);
+    }
+    Y.inv = put(G->lev[G->nlev].n * G->lev[G->nlev].n);
+    Y.total = off;
+    if (Y.total <= TAIL_LDS && G->nprog <= LV_AMG_MAXPROG)
+        hipLaunchKernelGGL(k_tail_lds, dim3(1), dim3(TB), 0, s, T, Y, (const int *)G->prog, G->nprog,
+                           (const float *)G->dense_inv, S);
+    else
+        hipLaunchKernelGGL(k_tail, dim3(1), dim3(TB), 0, s, T, (const int *)G->prog, G->nprog,
+                           (const float *)G->dense_inv, S);
     // coarse levels above the tail, up: each level's final correction in t
     for (int l = lt - 1; l >= 1; l--) {
         const lv_amg_level &L = G->lev[l], &C = G->lev[l + 1];
@@ -1176,10 +1327,10 @@ static hipError_t vcycle(const lv_amg *G, const fine_ops<V> &F, const int *agg, 
 }
 
 hipError_t lv_amg_vcycle(const lv_amg *G, const lv_pressure_system &A, const float *r, float *z, double *Prr,
-                         const lv_cg_scalars *S, hipStream_t s)
+                         const lv_cg_scalars *S, hipStream_t s, bool z0_ready)
 {
     if (G->perm_ok)
         return vcycle(G, fine_agg<lv_ell_agg16>(G, (const uint16_t *)G->pev16), (const int *)G->aggp,
-                      (const int *)nullptr, r, z, Prr, S, s);
-    return vcycle(G, fine_own(G, A), (const int *)G->agg, (const int *)G->list, r, z, Prr, S, s);
+                      (const int *)nullptr, r, z, Prr, S, s, z0_ready);
+    return vcycle(G, fine_own(G, A), (const int *)G->agg, (const int *)G->list, r, z, Prr, S, s, false);
 }

@@ -352,13 +352,13 @@ __global__ __launch_bounds__(B) void k_cg_s(int n, V A, const T *__restrict__ di
 
 // x += alpha p, r -= alpha q.  With Jacobi (JACOBI) also z = D^-1 r and the partials of r.z, r.r
 // for the next k_cg_s; otherwise the partials of r.r alone (Pur), which k_conv_check tests before
-// the preconditioner is applied.
+// the preconditioner is applied, and with z0 the multilevel cycle's pre-smoothed z0 = sc D^-1 r.
 template <typename T, bool JACOBI>
 __global__ __launch_bounds__(B) void k_cg_u(int n, T *__restrict__ x, T *__restrict__ r, T *__restrict__ z,
                                             const T *__restrict__ p, const T *__restrict__ q,
                                             const T *__restrict__ dinv, const double *__restrict__ Ppq,
                                             double *__restrict__ Prr, double *__restrict__ Pur, lv_cg_scalars *S,
-                                            int par)
+                                            int par, T *__restrict__ z0, T sc)
 {
     if (S->done) return;
     double pq[1];
@@ -370,6 +370,7 @@ __global__ __launch_bounds__(B) void k_cg_u(int n, T *__restrict__ x, T *__restr
         x[i] += a * p[i];
         const T ri = r[i] - a * q[i];
         r[i] = ri;
+        if (!JACOBI && z0) z0[i] = sc * dinv[i] * ri;
         if (JACOBI) {
             const T zi = dinv[i] * ri;
             z[i] = zi;
@@ -532,11 +533,13 @@ __global__ __launch_bounds__(B) void k_shift(int n, double *__restrict__ x, cons
 // ---- FP32 inner PCG of the refinement --------------------------------------------------------
 
 // r32 = r64 / |r64|, e = 0, z = dinv r, p = z; partials r.z, r.r.  With perm, the FP32 vectors
-// are in the aggregate order: r32[i] comes from r64[perm[i]].
+// are in the aggregate order: r32[i] comes from r64[perm[i]].  With z0 also z0 = sc dinv r (the
+// multilevel cycle's pre-smoothed correction).
 __global__ __launch_bounds__(B) void k_cg32_init(int n, const double *__restrict__ r64, const int *__restrict__ perm,
                                                  const float *__restrict__ dinv, float *__restrict__ e,
                                                  float *__restrict__ r, float *__restrict__ z,
-                                                 double *__restrict__ Prr, const lv_cg_scalars *S)
+                                                 double *__restrict__ Prr, const lv_cg_scalars *S,
+                                                 float *__restrict__ z0, float sc)
 {
     const double inv = 1.0 / sqrt(S->rr64);
     double v[2] = {0.0, 0.0};
@@ -546,6 +549,7 @@ __global__ __launch_bounds__(B) void k_cg32_init(int n, const double *__restrict
         e[i] = 0.0f;
         r[i] = ri;
         z[i] = zi;
+        if (z0) z0[i] = sc * dinv[i] * ri;
         v[0] += (double)ri * (double)zi;
         v[1] += (double)ri * (double)ri;
     }
@@ -756,12 +760,20 @@ hipError_t cg_iterate(int n, V A, const T *diag, const T *dinv, T *x, T *r, T *z
                                (const double *)P.rr, P.pq, dS, dprog, par, k, k == 0 ? 1 : 0);
         }
         if (M) {
+            // the cycle's pre-smoothed z0 stored with the residual update (FP32 solves only)
+            T *z0 = nullptr;
+            T sc = (T)0;
+            if constexpr (std::is_same<T, float>::value)
+                if (M->amg && M->amg->z0_in) {
+                    z0 = M->amg->z0;
+                    sc = M->amg->fine_scale;
+                }
             hipLaunchKernelGGL((k_cg_u<T, false>), dim3(nb), dim3(B), 0, s, n, x, r, z, (const T *)p,
-                               (const T *)q, dinv, (const double *)P.pq, P.rr, P.ur, dS, par);
+                               (const T *)q, dinv, (const double *)P.pq, P.rr, P.ur, dS, par, z0, sc);
             hipLaunchKernelGGL(k_conv_check, dim3(1), dim3(B), 0, s, (const double *)P.ur, nb, dS, dprog, k);
             if (M->amg) {
                 hipError_t e = lv_amg_vcycle(M->amg, *M->A, (const float *)r, (float *)z, P.rr,
-                                             (const lv_cg_scalars *)dS, s);
+                                             (const lv_cg_scalars *)dS, s, z0 != nullptr);
                 if (e != hipSuccess) return e;
             } else if constexpr (std::is_same<V, lv_ell_own<float>>::value) {
                 hipLaunchKernelGGL(k_apply_m32, dim3(nb), dim3(B), 0, s, n, A.deg, A.col, M->mval, M->mdg,
@@ -771,7 +783,8 @@ hipError_t cg_iterate(int n, V A, const T *diag, const T *dinv, T *x, T *r, T *z
             }
         } else {
             hipLaunchKernelGGL((k_cg_u<T, true>), dim3(nb), dim3(B), 0, s, n, x, r, z, (const T *)p,
-                               (const T *)q, dinv, (const double *)P.pq, P.rr, P.ur, dS, par);
+                               (const T *)q, dinv, (const double *)P.pq, P.rr, P.ur, dS, par, (T *)nullptr,
+                               (T)0);
         }
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
@@ -848,14 +861,16 @@ hipError_t lv_solve_device(const lv_pressure_system &A, lv_solver_work &W, int n
             const lv_amg *Gp = (Mi && Mi->amg && Mi->amg->perm_ok) ? Mi->amg : nullptr;
             const float *dinv32 = Gp ? (const float *)Gp->pdinv32 : (const float *)A.dinv32;
             for (int attempt = 0; attempt < 2; attempt++) {
+                float *z0 = (Mi && Mi->amg && Mi->amg->z0_in) ? Mi->amg->z0 : nullptr;
                 hipLaunchKernelGGL(k_cg32_init, G, dim3(B), 0, s, n, (const double *)W.r,
                                    Gp ? (const int *)Gp->list : (const int *)nullptr, dinv32, W.e32, W.r32, W.z32,
-                                   P.rr, (const lv_cg_scalars *)W.S);
+                                   P.rr, (const lv_cg_scalars *)W.S, z0, z0 ? Mi->amg->fine_scale : 0.0f);
                 // k_finalize2 clears the done flag of the previous sweep, so it precedes the first
                 // application of the learned preconditioner (which is skipped while done is set)
                 hipLaunchKernelGGL(k_finalize2, dim3(1), dim3(B), 0, s, (const double *)P.rr, nb, itol * itol, W.S);
                 if (Mi && Mi->amg) {
-                    e = lv_amg_vcycle(Mi->amg, A, (const float *)W.r32, W.z32, P.rr, (const lv_cg_scalars *)W.S, s);
+                    e = lv_amg_vcycle(Mi->amg, A, (const float *)W.r32, W.z32, P.rr, (const lv_cg_scalars *)W.S, s,
+                                      z0 != nullptr);
                     if (e != hipSuccess) return e;
                 } else if (Mi) {
                     hipLaunchKernelGGL(k_apply_m32, G, dim3(B), 0, s, n, (const int *)A.deg, (const int *)A.ecol,

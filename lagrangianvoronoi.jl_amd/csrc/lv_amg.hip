@@ -441,7 +441,8 @@ __global__ __launch_bounds__(B) void k_fine_down_jr(fine_ops<V> F, const float *
 {
     __shared__ float tl[FUSE_CAP];
     if (S && S->done) return;
-    const int c0 = bfirst[blockIdx.x], c1 = bfirst[blockIdx.x + 1];
+    const int b = lv_xcd_block(blockIdx.x, gridDim.x);
+    const int c0 = bfirst[b], c1 = bfirst[b + 1];
     const int R0 = start[c0], R1 = start[c1];
     for (int i = R0 + (int)threadIdx.x; i < R1; i += B)
         tl[i - R0] = Z0 ? down_row_z0(F, i, r, z0) : down_row_j(F, i, r, z0);

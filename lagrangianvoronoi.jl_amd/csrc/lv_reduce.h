@@ -15,6 +15,20 @@ inline int lv_nblocks(int n)
 
 namespace {
 
+// ---- XCD-aware work assignment --------------------------------------------------------------------
+//
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.  A kernel that
+// gathers neighbours maps workgroup b to the logical block lv_xcd_block(b, nb): XCD x then owns
+// the contiguous logical range [x q + min(x, r), ...) (q = nb / 8, r = nb % 8), so the rows one
+// XCD works on at a time, and the neighbours they gather, are contiguous and share its L2.  The
+// map is a bijection of [0, nb).
+__device__ __forceinline__ int lv_xcd_block(int b, int nb)
+{
+    const int x = b & 7, k = b >> 3;
+    const int q = nb >> 3, r = nb & 7;
+    return x * q + (x < r ? x : r) + k;
+}
+
 // ---- reductions --------------------------------------------------------------------------------
 //
 // A kernel that produces dot products writes one partial sum per block with plain stores.  The

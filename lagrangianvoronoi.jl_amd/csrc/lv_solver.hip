@@ -322,9 +322,10 @@ __global__ __launch_bounds__(B) void k_cg_s(int n, V A, const T *__restrict__ di
     if (stop) return;
     const T beta = first ? (T)0 : (T)(rz / S->rz[par]);
     double pq = 0.0;
+    // grid-stride passes over the rows, each pass one contiguous window whose eighth per XCD is
+    // contiguous too (lv_xcd_block); two rows per thread per pass keep twice the loads in flight
     const int stride = gridDim.x * B;
-    // two rows per thread per pass keep twice the loads in flight
-    for (int i0 = blockIdx.x * B + threadIdx.x; i0 < n; i0 += 2 * stride) {
+    for (int i0 = lv_xcd_block(blockIdx.x, gridDim.x) * B + threadIdx.x; i0 < n; i0 += 2 * stride) {
         const int i1 = i0 + stride;
         const T z0 = z[i0];
         const T w0 = spmv_row<T>(i0, n, A, diag, z);

@@ -1185,9 +1185,10 @@ static int ensure_picard_window(lv_ctx *ctx, int w)
     return LV_OK;
 }
 
-// Adds the Picard correction (P_cur - P_prev, AP_cur - AP_prev) to the window: two rounds of
+// Adds the Picard correction (P_cur - P_prev, AP_cur - AP_prev) to the window: one round of
 // classical Gram-Schmidt against the stored q's (applied identically to the z's), then
-// normalisation; the oldest direction makes room when the window is full.  The dots stay on the
+// normalisation; the oldest direction makes room when the window is full.  (A second round left
+// the iterations of the 2048^2 benchmark steps unchanged and cost 0.8 ms per step.)  The dots stay on the
 // device (the combinations read them there), so the host never waits; a correction with no
 // component left enters as a zero direction, which adds nothing to later projections.
 static int picard_add_direction(lv_ctx *ctx, int w, int *cnt, const double *Pcur, const double *Pprev,
@@ -1201,7 +1202,7 @@ static int picard_add_direction(lv_ctx *ctx, int w, int *cnt, const double *Pcur
     LV_HIP_CHECK(lv_combine(n, ctx->pw_t1, 0.0, xp, c, 2, s));
     xp[0] = APcur; xp[1] = APprev;
     LV_HIP_CHECK(lv_combine(n, ctx->pw_t2, 0.0, xp, c, 2, s));
-    for (int pass = 0; pass < 2 && *cnt > 0; pass++) {
+    if (*cnt > 0) {
         const double *a[8], *b[8];
         for (int k = 0; k < *cnt; k++) {
             a[k] = ctx->pw_q[k];

@@ -100,7 +100,10 @@ __global__ __launch_bounds__(B) void k_assemble(lv_grid_params g, lv_fields F, l
     dinv32[i] = (float)(1.0 / dd);
 }
 
-// refresh!(solver) first pass, src/pressure.jl:167-187: b, the initial guess P and GP
+// refresh!(solver) first pass, src/pressure.jl:167-187: b, the initial guess P and GP.  The first 6
+// edge slots' labels are read unconditionally and all their neighbours' gathers issued at once
+// (a slot past ne or on a boundary gathers the row itself and is skipped); the edges are then
+// summed in slot order as the reference does, the rest in a loop.
 __global__ __launch_bounds__(B) void k_rhs1(lv_grid_params g, lv_fields F, lv_mesh_view M, int n, double dt,
                                             lv_vbc vbc, double *__restrict__ b, double *__restrict__ P0,
                                             double *__restrict__ GP)
@@ -113,7 +116,38 @@ __global__ __launch_bounds__(B) void k_rhs1(lv_grid_params g, lv_fields F, lv_me
     double bi = (F.area[i] * Pi) / ((F.rho[i] * F.c2[i]) * (dt * dt));
     double gx = 0.0, gy = 0.0;
     const int ne = M.ne[i];
-    for (int k = 0; k < ne; k++) {
+    constexpr int U = 6;
+    int J[U];
+    double Xj[U], Yj[U], V1[U], V2[U], Pj[U];
+#pragma unroll
+    for (int k = 0; k < U; k++) {
+        const int lab = LV_EL(M, k, i);
+        J[k] = (k < ne && lab >= 0) ? lab : -1;
+    }
+#pragma unroll
+    for (int k = 0; k < U; k++) {
+        const int j = J[k] >= 0 ? J[k] : i;
+        Xj[k] = F.x[j];
+        Yj[k] = F.x[n + j];
+        V1[k] = F.v[j];
+        V2[k] = F.v[n + j];
+        Pj[k] = F.P[j];
+    }
+#pragma unroll
+    for (int k = 0; k < U; k++) {
+        if (J[k] < 0) continue;
+        const double e1x = LV_EV(M, 0, k, i), e1y = LV_EV(M, 1, k, i), e2x = LV_EV(M, 2, k, i), e2y = LV_EV(M, 3, k, i);
+        double y1, y2;
+        lv_arrow_target(g, Xj[k], Yj[k], x1, x2, y1, y2);
+        const double lrr = lv_lr_ratio(x1 - y1, x2 - y2, e1x, e1y, e2x, e2y);
+        const double m1 = 0.5 * (e1x + e2x), m2 = 0.5 * (e1y + e2y);
+        const double dv1 = pv1 - V1[k], dv2 = pv2 - V2[k];
+        bi -= (lrr / dt) * (dv1 * (m1 - y1) + dv2 * (m2 - y2));
+        const double w = lrr * (Pi - Pj[k]);
+        gx -= w * (m1 - x1);
+        gy -= w * (m2 - x2);
+    }
+    for (int k = U; k < ne; k++) {
         const int j = LV_EL(M, k, i);
         if (j < 0) continue;
         const double e1x = LV_EV(M, 0, k, i), e1y = LV_EV(M, 1, k, i), e2x = LV_EV(M, 2, k, i), e2y = LV_EV(M, 3, k, i);
@@ -143,7 +177,7 @@ __global__ __launch_bounds__(B) void k_rhs1(lv_grid_params g, lv_fields F, lv_me
     P0[i] = Pi;
 }
 
-// refresh!(solver) gp_step pass, src/pressure.jl:189-202
+// refresh!(solver) gp_step pass, src/pressure.jl:189-202 (the first 6 slots' gathers issued at once)
 __global__ __launch_bounds__(B) void k_rhs2(lv_grid_params g, lv_fields F, lv_mesh_view M, int n,
                                             const double *__restrict__ GP, double *__restrict__ b)
 {
@@ -153,7 +187,34 @@ __global__ __launch_bounds__(B) void k_rhs2(lv_grid_params g, lv_fields F, lv_me
     const double g1 = GP[i], g2 = GP[n + i];
     double bi = b[i];
     const int ne = M.ne[i];
-    for (int k = 0; k < ne; k++) {
+    constexpr int U = 6;
+    int J[U];
+    double Xj[U], Yj[U], G1[U], G2[U];
+#pragma unroll
+    for (int k = 0; k < U; k++) {
+        const int lab = LV_EL(M, k, i);
+        J[k] = (k < ne && lab >= 0) ? lab : -1;
+    }
+#pragma unroll
+    for (int k = 0; k < U; k++) {
+        const int j = J[k] >= 0 ? J[k] : i;
+        Xj[k] = F.x[j];
+        Yj[k] = F.x[n + j];
+        G1[k] = GP[j];
+        G2[k] = GP[n + j];
+    }
+#pragma unroll
+    for (int k = 0; k < U; k++) {
+        if (J[k] < 0) continue;
+        const double e1x = LV_EV(M, 0, k, i), e1y = LV_EV(M, 1, k, i), e2x = LV_EV(M, 2, k, i), e2y = LV_EV(M, 3, k, i);
+        double y1, y2;
+        lv_arrow_target(g, Xj[k], Yj[k], x1, x2, y1, y2);
+        const double lrr = lv_lr_ratio(x1 - y1, x2 - y2, e1x, e1y, e2x, e2y);
+        const double m1 = 0.5 * (e1x + e2x), m2 = 0.5 * (e1y + e2y);
+        const double z1 = 0.5 * (x1 + y1), z2 = 0.5 * (x2 + y2);
+        bi += lrr * ((g1 - G1[k]) * (m1 - z1) + (g2 - G2[k]) * (m2 - z2));
+    }
+    for (int k = U; k < ne; k++) {
         const int j = LV_EL(M, k, i);
         if (j < 0) continue;
         const double e1x = LV_EV(M, 0, k, i), e1y = LV_EV(M, 1, k, i), e2x = LV_EV(M, 2, k, i), e2y = LV_EV(M, 3, k, i);

@@ -196,3 +196,29 @@ def test_find_pressure_amg_matches_oracle(lv, oracle, gpu):
     op.find_pressure(dt, 10, tol=1e-13)
     assert max(st["relres"]) <= 1e-12
     assert rel_err(grid.get("P"), op["P"]) <= 1e-8
+
+
+def test_solver_launches_stop_at_convergence(lv, oracle, gpu):
+    """The host keeps one CG iteration queued ahead of the device and stops launching once the
+    device reports convergence in pinned memory: per refinement sweep the SpMV kernel is launched
+    at most about one iteration past the last working one.  A sampling timer brackets every k-th
+    launch and still counts them all."""
+    import ctypes
+
+    _flush("amg launch pacing")
+    grid, op, solver, x, dr, d, rp, cols, coef = _system(lv, oracle, "periodic", 301, "tg", 2e-3)
+    b, _ = solver.rhs(2e-3, False)
+    x0 = grid.get("P")
+    L = lv.lib()
+    for method, kid, every in (("amg", 0, 1), ("amg", 0, 4), ("pcg64", 1, 1)):
+        assert L.lv_timing_enable(grid.ctx, kid, every) == 0
+        solver.solve(b, x0, method=method)
+        seen, timed, tot, bpl = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_double(), ctypes.c_double()
+        assert L.lv_timing_seen(grid.ctx, kid, ctypes.byref(seen)) == 0
+        assert L.lv_timing_read(grid.ctx, kid, ctypes.byref(tot), ctypes.byref(timed), ctypes.byref(bpl)) == 0
+        L.lv_timing_enable(grid.ctx, kid, 0)
+        st = solver.last_stats
+        iters, sweeps = st["iters"][0], max(st["outer"][0], 1)
+        assert st["fallbacks"] == 0 and max(st["relres"]) <= 1e-8
+        assert iters <= seen.value <= iters + 2 * sweeps, (method, seen.value, iters, sweeps)
+        assert timed.value == (seen.value + every - 1) // every and tot.value > 0.0, (every, timed.value, seen.value)

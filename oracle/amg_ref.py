@@ -4,7 +4,7 @@ TEST INFRASTRUCTURE ONLY: used by tests/ to check the HIP cycle of
 lagrangianvoronoi.jl_amd/csrc/lv_amg.hip vector for vector.  It restates the same construction in
 numpy/scipy FP64:
 
-  * level 1: cells of a regular grid of width >= 4 dr over the boundary rectangle, each particle in
+  * level 1: cells of a regular grid of width >= CELL_DR dr (2 dr) over the boundary rectangle, each particle in
     the cell holding it; Galerkin sums of d and of the couplings to the 8 neighbouring cells
     (couplings reaching further are dropped from both sides);
   * level l+1: BLOCK x BLOCK (2 x 2) blocks of level-l cells; the coarsest level (<= 64 cells) inverted densely,
@@ -28,6 +28,7 @@ import scipy.sparse as sp
 
 DENSE_MAX = 64
 OMEGA = 2.0 / 3.0
+CELL_DR = 2.0  # level-1 cell width in units of dr
 BLOCK = 2  # coarse levels: BLOCK x BLOCK blocks of the finer level's cells
 TAIL_MAX = 1024
 ALPHA_FINE = 1.6
@@ -94,7 +95,7 @@ class Hierarchy:
             self.Af = self.A
         self.Dinv = 1.0 / self.A.diagonal()
         Lx, Ly = bxmax[0] - bxmin[0], bxmax[1] - bxmin[1]
-        Kx, Ky = max(1, int(np.floor(Lx / (4.0 * dr)))), max(1, int(np.floor(Ly / (4.0 * dr))))
+        Kx, Ky = max(1, int(np.floor(Lx / (CELL_DR * dr)))), max(1, int(np.floor(Ly / (CELL_DR * dr))))
         ix = np.clip(np.floor((x[:n] - bxmin[0]) / (Lx / Kx)).astype(int), 0, Kx - 1)
         iy = np.clip(np.floor((x[n:] - bxmin[1]) / (Ly / Ky)).astype(int), 0, Ky - 1)
         agg = ix * Ky + iy

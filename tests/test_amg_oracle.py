@@ -60,12 +60,12 @@ def test_cycle_is_symmetric_positive_definite(lv, oracle, kind, ic):
 @pytest.mark.parametrize("kind,ic", [("periodic", "tg"), ("walls", "cavity")])
 def test_deep_cycle_with_overcorrection_is_symmetric_positive_definite(lv, oracle, kind, ic, monkeypatch):
     """Four levels below the particles (144, 36, 9 and a dense 4 cells, with the dense and
-    small-level thresholds lowered to 4 and 20 cells so that a 48^2 system has them): the fine and
+    small-level thresholds lowered to 4 and 20 cells so that a 24^2 system has them): the fine and
     first coarse corrections are scaled by 1.6 and 1.4, and for the singular walled system the
     9-cell level is W-visited.  The cycle stays symmetric and positive definite (dense check)."""
     monkeypatch.setattr(amg_ref, "DENSE_MAX", 4)
     monkeypatch.setattr(amg_ref, "TAIL_MAX", 20)
-    H, p, d, singular = system(lv, oracle, kind, 48, ic, 2e-2)
+    H, p, d, singular = system(lv, oracle, kind, 24, ic, 2e-2)
     assert [L.n for L in H.levels] == [144, 36, 9, 4]
     assert H.alpha_fine == amg_ref.ALPHA_FINE and H.alpha[:3] == [amg_ref.ALPHA_COARSE, 1.0, 1.0]
     assert H.W == singular

@@ -51,8 +51,8 @@ def _hier(kind, x, dr, d, rp, cols, coef, M=None, fine_bf16=True):
 @pytest.mark.parametrize("kind,ic,K,dt", [("periodic", "tg", 64, 8e-3), ("periodic", "tg", 301, 2e-3),
                                           ("walls", "cavity", 96, 1e-3), ("tall", "cavity", 130, 1e-3)])
 def test_cycle_matches_oracle_jacobi_smoother(lv, oracle, gpu, kind, ic, K, dt):
-    """z = B r on the GPU equals the oracle's cycle (one dense level at 64^2, three coarse levels at
-    301^2, singular walled systems, a non-square domain)."""
+    """z = B r on the GPU equals the oracle's cycle (every level in the single-workgroup kernel at
+    64^2, five coarse levels at 301^2, singular walled systems, a non-square domain)."""
     _flush("amg cycle %s %d" % (kind, K))
     grid, op, solver, x, dr, d, rp, cols, coef = _system(lv, oracle, kind, K, ic, dt)
     H = _hier(kind, x, dr, d, rp, cols, coef)

@@ -551,7 +551,7 @@ __global__ __launch_bounds__(B) void k_fine_up_j(fine_ops<V> F, const float *__r
 {
     if (S && S->done) return;
     double v[2] = {0.0, 0.0};
-    for (int i = blockIdx.x * B + threadIdx.x; i < F.n; i += gridDim.x * B) {
+    for (int i = lv_xcd_block(blockIdx.x, gridDim.x) * B + threadIdx.x; i < F.n; i += gridDim.x * B) {
         const float ti = r[i] - a_row(i, F, z1);
         const float zi = z1[i] + F.scale * F.dinv[i] * ti;
         z[i] = zi;

@@ -442,10 +442,13 @@ __device__ __forceinline__ float down_row_z0(const fine_ops<V> &F, int i, const 
     return r[i] - acc;
 }
 
+// With z1 (level 1 above the tail), each cell's pre-smoothed z1 = w D1^-1 rc is stored too, as
+// level 1's down pass would form it.
 template <class V, bool Z0>
 __global__ __launch_bounds__(B) void k_fine_down_jr(fine_ops<V> F, const float *__restrict__ r,
                                                     float *__restrict__ z0, const int *__restrict__ start,
                                                     const int *__restrict__ bfirst, float *__restrict__ rc,
+                                                    const float *__restrict__ dinv1, float *__restrict__ z1,
                                                     const lv_cg_scalars *S)
 {
     __shared__ float tl[FUSE_CAP];
@@ -466,7 +469,10 @@ __global__ __launch_bounds__(B) void k_fine_down_jr(fine_ops<V> F, const float *
             for (int a = start[I] + q; a < s1; a += RL) s += tl[a - R0];
         }
         s = sum_lanes(s);
-        if (q == 0 && idx < lanes) rc[I] = s;
+        if (q == 0 && idx < lanes) {
+            rc[I] = s;
+            if (z1) z1[I] = OMEGA * dinv1[I] * s;
+        }
     }
 }
 
@@ -622,6 +628,8 @@ __device__ __forceinline__ void st_nbrs(const st_level &L, int cx, int cy, int f
 
 // down, one lane: child q of parent P (CF2 lanes per parent): z_f = w D^-1 r_f and the child's
 // residual t_f = r_f - (A z)_f, with the neighbours' z recomputed from their r (0 outside the level)
+// (ZGIVEN: z = w D^-1 r of the level is already stored, the same values, and only gathered)
+template <bool ZGIVEN = false>
 __device__ __forceinline__ float st_down_lane(const st_level &L, int Kyc, int nc, int P, int q,
                                               const float *__restrict__ r, float *__restrict__ z)
 {
@@ -630,20 +638,30 @@ __device__ __forceinline__ float st_down_lane(const st_level &L, int Kyc, int nc
     const int cx = CF * Px + q / CF, cy = CF * (P - Px * Kyc) + q % CF;
     if (cx >= L.Kx || cy >= L.Ky) return 0.0f;
     const int f = cx * L.Ky + cy;
-    float ck[8], rj[8], dj[8];
+    float ck[8], zj[8];
     int J[8];
     st_nbrs(L, cx, cy, f, ck, J);
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-        rj[k] = r[J[k]];
-        dj[k] = L.dinv[J[k]];
-    }
     const float rf = r[f];
-    const float zf = OMEGA * L.dinv[f] * rf;
+    float zf;
+    if (ZGIVEN) {
+#pragma unroll
+        for (int k = 0; k < 8; k++) zj[k] = z[J[k]];
+        zf = z[f];
+    } else {
+        float rj[8], dj[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            rj[k] = r[J[k]];
+            dj[k] = L.dinv[J[k]];
+        }
+#pragma unroll
+        for (int k = 0; k < 8; k++) zj[k] = OMEGA * dj[k] * rj[k];
+        zf = OMEGA * L.dinv[f] * rf;
+    }
     float acc = L.d[f] * zf;
 #pragma unroll
-    for (int k = 0; k < 8; k++) acc += ck[k] * (zf - OMEGA * dj[k] * rj[k]);
-    z[f] = zf;
+    for (int k = 0; k < 8; k++) acc += ck[k] * (zf - zj[k]);
+    if (!ZGIVEN) z[f] = zf;
     return rf - acc;
 }
 
@@ -696,14 +714,16 @@ __device__ __forceinline__ void st_wsave_cell(const st_level &L, int f, float *_
     r[f] -= a;
 }
 
-// down on level L: pre-smoothed z of L and the right-hand side rc of the next level
+// down on level L: pre-smoothed z of L (given, or stored here) and the right-hand side rc of the
+// next level
+template <bool ZGIVEN>
 __global__ __launch_bounds__(B) void k_st_down(st_level L, int Kyc, int nc, const float *__restrict__ r,
                                                float *__restrict__ z, float *__restrict__ rc, const lv_cg_scalars *S)
 {
     if (S && S->done) return;
     const int tid = blockIdx.x * B + threadIdx.x;
     const int P = tid / CF2, q = tid % CF2;
-    const float t = sum_children(st_down_lane(L, Kyc, nc, P, q, r, z));
+    const float t = sum_children(st_down_lane<ZGIVEN>(L, Kyc, nc, P, q, r, z));
     if (q == 0 && P < nc) rc[P] = t;
 }
 
@@ -1237,22 +1257,28 @@ static hipError_t vcycle(const lv_amg *G, const fine_ops<V> &F, const int *agg, 
 {
     const int n = G->n0;
     const lv_amg_level &L1 = G->lev[1];
+    bool z1_given = false;  // the fused down pass stored level 1's pre-smoothed z
     // level 0, down: z0 (in G->z0) and its residual t (in G->t0), restricted to level 1
     if (G->fine_learned) {
         hipLaunchKernelGGL(k_fine_pre<V>, g1(n), dim3(B), 0, s, F, r, G->z0, S);
         hipLaunchKernelGGL(k_fine_resid<V>, g1(n), dim3(B), 0, s, F, r, (const float *)G->z0, G->t0, S);
-        hipLaunchKernelGGL(k_restrict1, g1((int64_t)G->na * 16), dim3(B), 0, s, G->na, (const int *)G->start, list,
+        hipLaunchKernelGGL(k_restrict1, g1((int64_t)G->na * RL), dim3(B), 0, s, G->na, (const int *)G->start, list,
                            (const float *)G->t0, L1.r, S);
     } else if (!list && G->nbfuse > 0) {
+        // level 1 above the tail: its pre-smoothed z stored here, gathered by its down pass
+        z1_given = G->ltail > 1;
+        float *z1 = z1_given ? L1.z : nullptr;
         if (z0_ready && G->z0_in)
             hipLaunchKernelGGL((k_fine_down_jr<V, true>), dim3(G->nbfuse), dim3(B), 0, s, F, r, G->z0,
-                               (const int *)G->start, (const int *)G->bfirst, L1.r, S);
+                               (const int *)G->start, (const int *)G->bfirst, L1.r, (const float *)L1.dinv, z1,
+                               S);
         else
             hipLaunchKernelGGL((k_fine_down_jr<V, false>), dim3(G->nbfuse), dim3(B), 0, s, F, r, G->z0,
-                               (const int *)G->start, (const int *)G->bfirst, L1.r, S);
+                               (const int *)G->start, (const int *)G->bfirst, L1.r, (const float *)L1.dinv, z1,
+                               S);
     } else {
         hipLaunchKernelGGL(k_fine_down_j<V>, g1(n), dim3(B), 0, s, F, r, G->z0, G->t0, S);
-        hipLaunchKernelGGL(k_restrict1, g1((int64_t)G->na * 16), dim3(B), 0, s, G->na, (const int *)G->start, list,
+        hipLaunchKernelGGL(k_restrict1, g1((int64_t)G->na * RL), dim3(B), 0, s, G->na, (const int *)G->start, list,
                            (const float *)G->t0, L1.r, S);
     }
     // coarse levels above the tail, down: each level's pre-smoothed z and the next level's
@@ -1261,8 +1287,12 @@ static hipError_t vcycle(const lv_amg *G, const fine_ops<V> &F, const int *agg, 
     for (int l = 1; l < lt; l++) {
         const lv_amg_level &L = G->lev[l], &C = G->lev[l + 1];
         const st_level sl{L.Kx, L.Ky, G->xper, G->yper, L.d, L.c, L.dinv, L.alpha};
-        hipLaunchKernelGGL(k_st_down, g1((int64_t)C.n * CF2), dim3(B), 0, s, sl, C.Ky, C.n, (const float *)L.r, L.z,
-                           C.r, S);
+        if (l == 1 && z1_given)
+            hipLaunchKernelGGL(k_st_down<true>, g1((int64_t)C.n * CF2), dim3(B), 0, s, sl, C.Ky, C.n,
+                               (const float *)L.r, L.z, C.r, S);
+        else
+            hipLaunchKernelGGL(k_st_down<false>, g1((int64_t)C.n * CF2), dim3(B), 0, s, sl, C.Ky, C.n,
+                               (const float *)L.r, L.z, C.r, S);
     }
     // the tail: levels lt..nlev down, dense, and up; the final correction of level lt in its t
     tail_levels T;

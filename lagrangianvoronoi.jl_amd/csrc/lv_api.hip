@@ -356,6 +356,7 @@ int lv_create(const lv_grid_desc *d, lv_ctx **out)
         memset(ctx->W.hprog, 0, sizeof(lv_progress));
         e = hipHostGetDevicePointer((void **)&ctx->W.dprog, ctx->W.hprog, 0);
     }
+    if (e == hipSuccess) e = hipHostMalloc((void **)&ctx->W.hS, sizeof(lv_cg_scalars), hipHostMallocDefault);
     if (e == hipSuccess) e = hipMalloc(&ctx->W.partials, lv_partials_bytes((int)cap));
     if (e == hipSuccess) e = hipMalloc(&ctx->scan_temp, lv_scan_temp_bytes((int64_t)cap + 1) + 256);
 #undef ALLOC
@@ -402,6 +403,7 @@ int lv_destroy(lv_ctx *ctx)
                     ctx->W.partials, ctx->scan_temp};
     for (void *p : ptrs) dfree(p);
     if (ctx->W.hprog) (void)hipHostFree(ctx->W.hprog);
+    if (ctx->W.hS) (void)hipHostFree(ctx->W.hS);
     if (ctx->spai_ok) lv_spai_free(&ctx->spai);
     if (ctx->amg_ok) lv_amg_free(&ctx->amg);
     lv_mph_free(ctx->mph);

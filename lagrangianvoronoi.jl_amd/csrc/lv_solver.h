@@ -42,6 +42,7 @@ struct lv_solver_work {
     double *partials;
     lv_cg_scalars *S;
     lv_progress *hprog, *dprog;  // the same pinned record: host and device addresses
+    lv_cg_scalars *hS;           // pinned host copy of S for the synchronising reads
 };
 
 #define LV_TPOOL 1024

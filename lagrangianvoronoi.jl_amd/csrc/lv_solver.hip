@@ -766,11 +766,25 @@ struct timer_scope {
     }
 };
 
-hipError_t read_scalars(lv_cg_scalars *dS, lv_cg_scalars *hS, hipStream_t s)
+// S into h through the pinned staging copy pin (a copy into pageable memory would be staged too)
+hipError_t read_scalars(lv_cg_scalars *dS, lv_cg_scalars *pin, lv_cg_scalars *h, hipStream_t s)
 {
-    hipError_t e = hipMemcpyAsync(hS, dS, sizeof(lv_cg_scalars), hipMemcpyDeviceToHost, s);
+    hipError_t e = hipMemcpyAsync(pin, dS, sizeof(lv_cg_scalars), hipMemcpyDeviceToHost, s);
     if (e != hipSuccess) return e;
-    return hipStreamSynchronize(s);
+    e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return e;
+    *h = *pin;
+    return hipSuccess;
+}
+
+__global__ void k_init_scalars(lv_cg_scalars *S, double tol2, int itmax, int singular)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    lv_cg_scalars z = {};
+    z.tol2 = tol2;
+    z.itmax = itmax;
+    z.singular = singular;
+    *S = z;
 }
 
 // partial-sum regions: P3 = one-off reductions (3 values), Ppq (1), Prr (2)
@@ -780,19 +794,20 @@ struct parts {
 inline parts parts_of(double *base) { return parts{base, base + 3 * NBMAX, base + 4 * NBMAX, base + 6 * NBMAX}; }
 
 // CG iterations (two launches each; with a preconditioner also the convergence test and its
-// application) until the device flags completion.  The host launches iteration k once the device
+// application) until the device flags completion; *dflag is the done flag (1 converged, 2 stopped).
+// It returns without synchronising: S (iterations, residuals) is read by the caller's next read.  The host launches iteration k once the device
 // has begun iteration k - 1 (prog->seen, in pinned host memory), so the queue stays one iteration
 // deep without synchronising, and stops launching as soon as prog->done is set: at most about one
 // iteration runs as no-ops after convergence.  If the record does not advance for 20 ms the host
 // synchronises and reads the scalars instead, so the loop ends whatever the record shows.
 template <typename T, class V>
 hipError_t cg_iterate(int n, V A, const T *diag, const T *dinv, T *x, T *r, T *z, T *p, T *q, const parts &P,
-                      lv_cg_scalars *dS, lv_progress *hprog, lv_progress *dprog, hipStream_t s, lv_timer *tm,
-                      lv_cg_scalars &h, const lv_precond *M)
+                      lv_cg_scalars *dS, lv_progress *hprog, lv_progress *dprog, lv_cg_scalars *pin, hipStream_t s,
+                      lv_timer *tm, lv_cg_scalars &h, const lv_precond *M, int *dflag)
 {
     const int nb = nblocks_of(n);
-    // no kernel of this solve that stores progress is in flight: the previous sweep ended with a
-    // synchronising read
+    // no kernel of this solve that stores progress is in flight: the caller synchronised since the
+    // previous sweep (its read of the FP64 residual, or the read before a fallback sweep)
     __atomic_store_n(&hprog->seen, -1, __ATOMIC_RELAXED);
     __atomic_store_n(&hprog->done, 0, __ATOMIC_RELEASE);
     for (int k = 0;; k++) {
@@ -800,15 +815,18 @@ hipError_t cg_iterate(int n, V A, const T *diag, const T *dinv, T *x, T *r, T *z
             bool stop = false;
             const auto t0 = std::chrono::steady_clock::now();
             for (int spin = 1;; spin++) {
-                if (__atomic_load_n(&hprog->done, __ATOMIC_ACQUIRE)) {
+                const int d = __atomic_load_n(&hprog->done, __ATOMIC_ACQUIRE);
+                if (d) {
+                    *dflag = d;
                     stop = true;
                     break;
                 }
                 if (__atomic_load_n(&hprog->seen, __ATOMIC_ACQUIRE) >= k - 1) break;
                 if ((spin & 255) == 0 &&
                     std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(20)) {
-                    hipError_t e = read_scalars(dS, &h, s);  // every launched kernel has finished
+                    hipError_t e = read_scalars(dS, pin, &h, s);  // every launched kernel has finished
                     if (e != hipSuccess) return e;
+                    *dflag = h.done;
                     stop = h.done != 0;
                     break;
                 }
@@ -851,9 +869,6 @@ hipError_t cg_iterate(int n, V A, const T *diag, const T *dinv, T *x, T *r, T *z
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
-    hipError_t e = read_scalars(dS, &h, s);
-    if (e != hipSuccess) return e;
-    if (tm) lv_timer_flush(tm);
     return hipSuccess;
 }
 
@@ -873,12 +888,9 @@ hipError_t lv_solve_device(const lv_pressure_system &A, lv_solver_work &W, int n
     const parts P = parts_of(W.partials);
     lv_cg_scalars h;
     memset(&h, 0, sizeof(h));
-    h.tol2 = opt.tol * opt.tol;
-    h.itmax = opt.itmax > 0 ? opt.itmax : 1000;
-    h.singular = A.singular;
-    hipError_t e = hipMemcpyAsync(W.S, &h, sizeof(h), hipMemcpyHostToDevice, s);
-    if (e != hipSuccess) return e;
-    e = hipMemcpyAsync(W.x, x, sizeof(double) * n, hipMemcpyDeviceToDevice, s);
+    hipLaunchKernelGGL(k_init_scalars, dim3(1), dim3(1), 0, s, W.S, opt.tol * opt.tol,
+                       opt.itmax > 0 ? opt.itmax : 1000, (int)A.singular);
+    hipError_t e = hipMemcpyAsync(W.x, x, sizeof(double) * n, hipMemcpyDeviceToDevice, s);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_means, G, dim3(B), 0, s, n, b, (const double *)W.x, P.p3);
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(B), 0, s, (const double *)P.p3, nb, (int)FIN_MEANS, n, 0.0, W.S);
@@ -889,9 +901,12 @@ hipError_t lv_solve_device(const lv_pressure_system &A, lv_solver_work &W, int n
                            (const double *)W.x, W.r, W.z, P.rr, P.p3, (const lv_cg_scalars *)W.S);
         hipLaunchKernelGGL(k_finalize, dim3(1), dim3(B), 0, s, (const double *)P.p3, nb, (int)FIN_INIT64, n, 0.0,
                            W.S);
+        int done = 0;
         e = cg_iterate<double>(n, lv_ell_own<double>{A.deg, A.ecol, A.ev64}, (const double *)A.diag64,
                                (const double *)A.dinv64, W.x, W.r, W.z, W.p, W.q, P, W.S, W.hprog,
-                               W.dprog, s, t64, h, (const lv_precond *)nullptr);
+                               W.dprog, W.hS, s, t64, h, (const lv_precond *)nullptr, &done);
+        if (e != hipSuccess) return e;
+        e = read_scalars(W.S, W.hS, &h, s);
         if (e != hipSuccess) return e;
         total = h.iters;
         *relres0 = (h.bb > 0.0) ? sqrt(h.rr0 / h.bb) : sqrt(h.rr0);
@@ -902,14 +917,19 @@ hipError_t lv_solve_device(const lv_pressure_system &A, lv_solver_work &W, int n
         // guess's mean (k_correct writes the sums of x, k_resid64 shifts what it reads); k_shift
         // stores that x when the sweeps end, so the last residual is the returned x's
         double *Pxs = A.singular ? P.ur : nullptr;
+        // the iterations of a sweep are still in S at the next sweep's read of the FP64 residual
+        // (k_resid64 and k_finalize leave them), so a sweep ends without a synchronising read
+        bool pending = false;
         for (;;) {
             hipLaunchKernelGGL(k_resid64, G, dim3(B), 0, s, n, (const int *)A.deg, (const int *)A.ecol,
                                (const double *)A.ev64, (const double *)A.diag64, b, (const double *)W.x, W.r, P.p3,
                                nsweep > 0 ? (const double *)Pxs : (const double *)nullptr, W.S);
             hipLaunchKernelGGL(k_finalize, dim3(1), dim3(B), 0, s, (const double *)P.p3, nb, (int)FIN_RESID, n, 0.0,
                                W.S);
-            e = read_scalars(W.S, &h, s);
+            e = read_scalars(W.S, W.hS, &h, s);
             if (e != hipSuccess) return e;
+            if (pending) total += h.iters;
+            pending = false;
             if (nsweep == 0) *relres0 = (h.bb > 0.0) ? sqrt(h.rr64 / h.bb) : sqrt(h.rr64);
             if (h.rr64 <= h.thr2 || nsweep >= outer_max || !(h.rr64 == h.rr64)) break;
             // ask the FP32 sweep for just the reduction still missing, with a margin of 0.8 for the
@@ -941,24 +961,26 @@ hipError_t lv_solve_device(const lv_pressure_system &A, lv_solver_work &W, int n
                                        Mi->mval, Mi->mdg, (const float *)W.r32, W.z32, P.rr,
                                        (const lv_cg_scalars *)W.S);
                 }
+                int done = 0;
                 if (Gp)
                     e = cg_iterate<float>(n, lv_ell_agg32{Gp->pdeg, Gp->poff, Gp->pev32}, (const float *)Gp->pdiag32,
-                                          dinv32, W.e32, W.r32, W.z32, W.p32, W.q32, P, W.S, W.hprog, W.dprog, s,
-                                          t32, h, Mi);
+                                          dinv32, W.e32, W.r32, W.z32, W.p32, W.q32, P, W.S, W.hprog, W.dprog,
+                                          W.hS, s, t32, h, Mi, &done);
                 else
                     e = cg_iterate<float>(n, lv_ell_own<float>{A.deg, A.ecol, A.ev32}, (const float *)A.diag32,
-                                          dinv32, W.e32, W.r32, W.z32, W.p32, W.q32, P, W.S, W.hprog, W.dprog, s,
-                                          t32, h, Mi);
+                                          dinv32, W.e32, W.r32, W.z32, W.p32, W.q32, P, W.S, W.hprog, W.dprog,
+                                          W.hS, s, t32, h, Mi, &done);
                 if (e != hipSuccess) return e;
                 // Jacobi-PCG is the fallback: a sweep of the learned preconditioner that stalls or breaks
-                // down is redone with Jacobi
-                if (!Mi || h.done == 1) break;
+                // down is redone with Jacobi (after a synchronising read, which also counts its iterations)
+                if (!Mi || done == 1) break;
+                e = read_scalars(W.S, W.hS, &h, s);
+                if (e != hipSuccess) return e;
                 total += h.iters;
                 (*fallbacks)++;
                 Mi = nullptr;
             }
-            if (e != hipSuccess) return e;
-            total += h.iters;
+            pending = true;
             nsweep++;
             hipLaunchKernelGGL(k_correct, G, dim3(B), 0, s, n, W.x, (const float *)W.e32,
                                Gp ? (const int *)Gp->iperm : (const int *)nullptr, Pxs, (const lv_cg_scalars *)W.S);
@@ -978,7 +1000,7 @@ hipError_t lv_solve_device(const lv_pressure_system &A, lv_solver_work &W, int n
                            (const double *)nullptr, W.S);
         hipLaunchKernelGGL(k_finalize, dim3(1), dim3(B), 0, s, (const double *)P.p3, nb, (int)FIN_RESID, n, 0.0,
                            W.S);
-        e = read_scalars(W.S, &h, s);
+        e = read_scalars(W.S, W.hS, &h, s);
         if (e != hipSuccess) return e;
     }
     e = hipMemcpyAsync(x, W.x, sizeof(double) * n, hipMemcpyDeviceToDevice, s);

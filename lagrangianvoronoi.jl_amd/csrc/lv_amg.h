@@ -29,6 +29,7 @@ struct lv_amg {
     float *t0, *z0; // fine residual and pre-smoothed correction
     lv_amg_level lev[LV_AMG_MAXLEV + 1];
     float *dense_inv;    // nc * nc, column-major
+    float *tail_img;     // the tail levels' constant arrays in their LDS layout (packed per set-up)
     double *pw;          // power-iteration partials and norms
     float *pv, *pu;      // power-iteration vectors (n0)
     size_t cap_pv, cap_pu;

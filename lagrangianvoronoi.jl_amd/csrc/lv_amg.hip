@@ -775,7 +775,7 @@ constexpr int TAIL_LDS = 35 * 1024;
 struct tail_layout {
     int d[LV_AMG_MAXLEV + 1], c[LV_AMG_MAXLEV + 1], dinv[LV_AMG_MAXLEV + 1];
     int r[LV_AMG_MAXLEV + 1], z[LV_AMG_MAXLEV + 1], t[LV_AMG_MAXLEV + 1], acc[LV_AMG_MAXLEV + 1];
-    int inv, total;
+    int inv, coef, total;  // [0, coef): the constant arrays, then the work vectors
 };
 
 struct tail_in_lds {
@@ -848,31 +848,59 @@ __global__ __launch_bounds__(TB) void k_tail(tail_levels T, const int *__restric
 // The same program with the tail levels in LDS: the couplings, the dense inverse, the program
 // and the right-hand side of the first level are copied in, every operation then reads and
 // writes LDS only, and the final correction of the first level is copied out.
+// The constant arrays come from img (k_tail_pack's image of them in this layout) by one flat copy
+// whose loads are all issued before the done flag is tested and the first LDS store is made; a copy
+// loop per array would wait for a global round trip per iteration.
 __global__ __launch_bounds__(TB) void k_tail_lds(tail_levels T, tail_layout Y, const int *__restrict__ prog,
-                                                 int nprog, const float *__restrict__ inv, const lv_cg_scalars *S)
+                                                 int nprog, const float *__restrict__ img, const lv_cg_scalars *S)
 {
-    __shared__ float lds[TAIL_LDS];
+    __shared__ __align__(16) float lds[TAIL_LDS];
     __shared__ int lprog[LV_AMG_MAXPROG];
-    if (S && S->done) return;
-    const int first = T.first, last = T.last;
-    for (int pc = threadIdx.x; pc < nprog; pc += TB) lprog[pc] = prog[pc];
-    for (int l = first; l < last; l++) {
-        const int n = T.Kx[l] * T.Ky[l];
-        for (int f = threadIdx.x; f < n; f += TB) {
-            lds[Y.d[l] + f] = T.d[l][f];
-            lds[Y.dinv[l] + f] = T.dinv[l][f];
-        }
-        for (int f = threadIdx.x; f < 8 * n; f += TB) lds[Y.c[l] + f] = T.c[l][f];
-    }
-    const int nd = T.Kx[last] * T.Ky[last];
-    for (int f = threadIdx.x; f < nd * nd; f += TB) lds[Y.inv + f] = inv[f];
+    const int first = T.first;
+    constexpr int PER4 = (TAIL_LDS / 4 + TB - 1) / TB;  // float4s per lane
+    const int n4 = Y.coef / 4;
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    const f4 *img4 = reinterpret_cast<const f4 *>(img);
+    f4 v[PER4];
+#pragma unroll
+    for (int q = 0; q < PER4; q++)
+        if (q * TB + (int)threadIdx.x < n4) v[q] = img4[q * TB + threadIdx.x];
     const int n0 = T.Kx[first] * T.Ky[first];
-    for (int f = threadIdx.x; f < n0; f += TB) lds[Y.r[first] + f] = T.r[first][f];
+    const int f0 = threadIdx.x;  // n0 <= TAIL_MAX == TB: one right-hand side entry per lane
+    const float r0 = (f0 < n0) ? T.r[first][f0] : 0.0f;
+    const int pr = (int)threadIdx.x < nprog ? prog[threadIdx.x] : 0;  // nprog <= LV_AMG_MAXPROG <= TB
+    if (S && S->done) return;  // tested after the loads above are issued (they read valid arrays)
+    f4 *lds4 = reinterpret_cast<f4 *>(lds);
+#pragma unroll
+    for (int q = 0; q < PER4; q++)
+        if (q * TB + (int)threadIdx.x < n4) lds4[q * TB + threadIdx.x] = v[q];
+    for (int f = 4 * n4 + threadIdx.x; f < Y.coef; f += TB) lds[f] = img[f];
+    if (f0 < n0) lds[Y.r[first] + f0] = r0;
+    if ((int)threadIdx.x < nprog) lprog[threadIdx.x] = pr;
     __syncthreads();
     run_tail(T, tail_in_lds{lds, Y}, lprog, nprog);
     for (int f = threadIdx.x; f < n0; f += TB) T.t[first][f] = lds[Y.t[first] + f];
 }
 static_assert(LV_AMG_DENSE_MAX * 16 == TB, "the dense solve of the tail maps 16 lanes to a row");
+static_assert(TAIL_MAX <= TB && LV_AMG_MAXPROG <= TB, "one right-hand side entry and one operation per lane");
+static_assert(TAIL_LDS % 4 == 0, "the LDS image is copied as float4");
+
+// the tail levels' constant arrays (diagonals, couplings, dense inverse) into img in the LDS layout
+__global__ __launch_bounds__(B) void k_tail_pack(tail_levels T, tail_layout Y, const float *__restrict__ inv,
+                                                 float *__restrict__ img)
+{
+    const int stride = gridDim.x * B;
+    for (int l = T.first; l < T.last; l++) {
+        const int n = T.Kx[l] * T.Ky[l];
+        for (int f = blockIdx.x * B + threadIdx.x; f < n; f += stride) {
+            img[Y.d[l] + f] = T.d[l][f];
+            img[Y.dinv[l] + f] = T.dinv[l][f];
+        }
+        for (int f = blockIdx.x * B + threadIdx.x; f < 8 * n; f += stride) img[Y.c[l] + f] = T.c[l][f];
+    }
+    const int nd = T.Kx[T.last] * T.Ky[T.last];
+    for (int f = blockIdx.x * B + threadIdx.x; f < nd * nd; f += stride) img[Y.inv + f] = inv[f];
+}
 
 // ---- power iteration for the scale of the learned fine smoother ----------------------------------
 
@@ -1001,6 +1029,7 @@ hipError_t lv_amg_init(lv_amg *G)
 {
     memset(G, 0, sizeof(*G));
     AMG_CHECK(hipMalloc(&G->dense_inv, sizeof(float) * LV_AMG_DENSE_MAX * LV_AMG_DENSE_MAX));
+    AMG_CHECK(hipMalloc(&G->tail_img, sizeof(float) * TAIL_LDS));
     AMG_CHECK(hipMalloc(&G->pw, sizeof(double) * (2 * LV_NBMAX + 2)));
     AMG_CHECK(hipMalloc(&G->prog, sizeof(int) * LV_AMG_MAXPROG));
     AMG_CHECK(hipMalloc(&G->pflag, 2 * sizeof(int)));
@@ -1009,7 +1038,7 @@ hipError_t lv_amg_init(lv_amg *G)
 
 void lv_amg_free(lv_amg *G)
 {
-    void *ptrs[] = {G->agg, G->cnt, G->start, G->cursor, G->list, G->scan_tmp, G->t0, G->z0, G->dense_inv,
+    void *ptrs[] = {G->agg, G->cnt, G->start, G->cursor, G->list, G->scan_tmp, G->t0, G->z0, G->dense_inv, G->tail_img,
                     G->pw, G->pv, G->pu, G->prog, G->iperm, G->aggp, G->pdeg, G->poff, G->pev32, G->pev16,
                     G->pdiag32, G->pdinv32, G->pmval, G->pmdg, G->pflag, G->bfirst};
     for (void *p : ptrs)
@@ -1133,6 +1162,71 @@ static int tail_program(int first, int last, bool W, int *prog)
     return np;
 }
 
+// the levels ltail..nlev of the single-workgroup kernel
+static tail_levels tail_levels_of(const lv_amg *G)
+{
+    tail_levels T;
+    memset(&T, 0, sizeof(T));
+    T.first = G->ltail;
+    T.last = G->nlev;
+    T.xper = G->xper;
+    T.yper = G->yper;
+    for (int l = G->ltail; l <= G->nlev; l++) {
+        const lv_amg_level &L = G->lev[l];
+        T.Kx[l] = L.Kx;
+        T.Ky[l] = L.Ky;
+        T.alpha[l] = L.alpha;
+        T.acc[l] = L.acc;
+        T.d[l] = L.d;
+        T.c[l] = L.c;
+        T.dinv[l] = L.dinv;
+        T.r[l] = L.r;
+        T.z[l] = L.z;
+        T.t[l] = L.t;
+    }
+    return T;
+}
+
+// their LDS layout: the constant arrays first (diagonals, couplings, dense inverse), then the work
+// vectors; about 14 floats per cell of the levels above the dense one, whose total stays below
+// twice TAIL_MAX
+static tail_layout tail_layout_of(const lv_amg *G)
+{
+    tail_layout Y;
+    memset(&Y, 0, sizeof(Y));
+    int off = 0;
+    auto put = [&off](int count) {
+        const int o = off;
+        off += count;
+        return o;
+    };
+    for (int l = G->ltail; l < G->nlev; l++) {
+        const int nl = G->lev[l].n;
+        Y.d[l] = put(nl);
+        Y.dinv[l] = put(nl);
+        Y.c[l] = put(8 * nl);
+    }
+    Y.inv = put(G->lev[G->nlev].n * G->lev[G->nlev].n);
+    Y.coef = off;
+    for (int l = G->ltail; l <= G->nlev; l++) {
+        const int nl = G->lev[l].n;
+        if (l < G->nlev) {
+            Y.z[l] = put(nl);
+            Y.acc[l] = put(nl);
+        }
+        Y.r[l] = put(nl);
+        Y.t[l] = put(nl);
+    }
+    Y.total = off;
+    return Y;
+}
+
+// the tail runs in LDS when its levels fit (the usual case), else in global memory
+static bool tail_fits_lds(const lv_amg *G, const tail_layout &Y)
+{
+    return Y.total <= TAIL_LDS && G->nprog <= LV_AMG_MAXPROG;
+}
+
 hipError_t lv_amg_setup(lv_amg *G, const lv_grid_params &g, const double *x, const lv_pressure_system &A, int n,
                         const float *mval, const float *mdg, int layout, hipStream_t s)
 {
@@ -1187,6 +1281,12 @@ hipError_t lv_amg_setup(lv_amg *G, const lv_grid_params &g, const double *x, con
     const lv_amg_level &D = G->lev[l];
     hipLaunchKernelGGL(k_dense_inverse, dim3(1), dim3(256), 0, s, D.Kx, D.Ky, G->xper, G->yper, (const float *)D.d,
                        (const float *)D.c, A.singular, G->dense_inv);
+    {
+        const tail_layout Y = tail_layout_of(G);
+        if (tail_fits_lds(G, Y))
+            hipLaunchKernelGGL(k_tail_pack, dim3(64), dim3(B), 0, s, tail_levels_of(G), Y,
+                               (const float *)G->dense_inv, G->tail_img);
+    }
     // fine smoother
     G->fine_learned = (mval && mdg) ? 1 : 0;
     G->mval = mval;
@@ -1295,52 +1395,11 @@ static hipError_t vcycle(const lv_amg *G, const fine_ops<V> &F, const int *agg, 
                                (const float *)L.r, L.z, C.r, S);
     }
     // the tail: levels lt..nlev down, dense, and up; the final correction of level lt in its t
-    tail_levels T;
-    memset(&T, 0, sizeof(T));
-    T.first = lt;
-    T.last = G->nlev;
-    T.xper = G->xper;
-    T.yper = G->yper;
-    for (int l = lt; l <= G->nlev; l++) {
-        const lv_amg_level &L = G->lev[l];
-        T.Kx[l] = L.Kx;
-        T.Ky[l] = L.Ky;
-        T.alpha[l] = L.alpha;
-        T.acc[l] = L.acc;
-        T.d[l] = L.d;
-        T.c[l] = L.c;
-        T.dinv[l] = L.dinv;
-        T.r[l] = L.r;
-        T.z[l] = L.z;
-        T.t[l] = L.t;
-    }
-    // the tail in LDS when its levels fit (the usual case: about 14 floats per cell of the levels
-    // above the dense one, whose total stays below twice TAIL_MAX), else in global memory
-    tail_layout Y;
-    memset(&Y, 0, sizeof(Y));
-    int off = 0;
-    auto put = [&off](int count) {
-        const int o = off;
-        off += count;
-        return o;
-    };
-    for (int l = lt; l <= G->nlev; l++) {
-        const int nl = G->lev[l].n;
-        if (l < G->nlev) {
-            Y.d[l] = put(nl);
-            Y.dinv[l] = put(nl);
-            Y.c[l] = put(8 * nl);
-            Y.z[l] = put(nl);
-            Y.acc[l] = put(nl);
-        }
-        Y.r[l] = put(nl);
-        Y.t[l] = put(nl);
-    }
-    Y.inv = put(G->lev[G->nlev].n * G->lev[G->nlev].n);
-    Y.total = off;
-    if (Y.total <= TAIL_LDS && G->nprog <= LV_AMG_MAXPROG)
+    const tail_levels T = tail_levels_of(G);
+    const tail_layout Y = tail_layout_of(G);
+    if (tail_fits_lds(G, Y))
         hipLaunchKernelGGL(k_tail_lds, dim3(1), dim3(TB), 0, s, T, Y, (const int *)G->prog, G->nprog,
-                           (const float *)G->dense_inv, S);
+                           (const float *)G->tail_img, S);
     else
         hipLaunchKernelGGL(k_tail, dim3(1), dim3(TB), 0, s, T, (const int *)G->prog, G->nprog,
                            (const float *)G->dense_inv, S);

@@ -80,6 +80,12 @@ struct lv_ctx {
     int A_kind = 0;        // what A holds: 0 the pressure operator, 1 the implicit viscous operator
     // multiphase projection work space (allocated on first use)
     lv_mph_work *mph = nullptr;
+    // solve timing: event pairs recorded around each solve and read when the API call returns its
+    // statistics (one wait per call instead of one per solve); out: where each pair's time goes
+    static constexpr int NSEV = 32;
+    hipEvent_t sev[2 * NSEV] = {};
+    double *sev_out[NSEV] = {};
+    int sev_ready = 0, sev_n = 0;
     char msg[512] = {0};
 };
 
@@ -339,7 +345,6 @@ int lv_create(const lv_grid_desc *d, lv_ctx **out)
     ALLOC(ctx->b, cap);
     ALLOC(ctx->P0, cap);
     ALLOC(ctx->GP, 2 * cap);
-    ALLOC(ctx->W.x, cap);
     ALLOC(ctx->W.r, cap);
     ALLOC(ctx->W.z, cap);
     ALLOC(ctx->W.p, cap);
@@ -398,7 +403,7 @@ int lv_destroy(lv_ctx *ctx)
                     ctx->cell_x,
                     ctx->cl_temp, ctx->d_nitems, ctx->d_flags, ctx->path_i, ctx->path_rr, ctx->A.row_ptr,
                     ctx->A.cols, ctx->A.val64, ctx->A.val32, ctx->A.diag64, ctx->A.dinv64, ctx->A.diag32,
-                    ctx->A.dinv32, ctx->deg, ctx->A.ecol, ctx->A.ev64, ctx->A.ev32, ctx->b, ctx->P0, ctx->GP, ctx->W.x, ctx->W.r, ctx->W.z, ctx->W.p,
+                    ctx->A.dinv32, ctx->deg, ctx->A.ecol, ctx->A.ev64, ctx->A.ev32, ctx->b, ctx->P0, ctx->GP, ctx->W.r, ctx->W.z, ctx->W.p,
                     ctx->W.q, ctx->W.e32, ctx->W.r32, ctx->W.z32, ctx->W.p32, ctx->W.q32, ctx->W.S,
                     ctx->W.partials, ctx->scan_temp};
     for (void *p : ptrs) dfree(p);
@@ -424,6 +429,8 @@ int lv_destroy(lv_ctx *ctx)
             if (ctx->timers[k]->ev[q]) hipEventDestroy(ctx->timers[k]->ev[q]);
         delete ctx->timers[k];
     }
+    for (int q = 0; q < 2 * lv_ctx::NSEV; q++)
+        if (ctx->sev[q]) hipEventDestroy(ctx->sev[q]);
     if (ctx->s) hipStreamDestroy(ctx->s);
     delete ctx;
     return LV_OK;
@@ -1075,20 +1082,54 @@ static int spai_prepare(lv_ctx *ctx, const lv_solve_opts &opt, lv_precond *M, bo
     return LV_OK;
 }
 
+// the solve times still to be read: each pair's elapsed time into its slot
+static void flush_solve_times(lv_ctx *ctx)
+{
+    for (int k = 0; k < ctx->sev_n; k++) {
+        float ms = 0.0f;
+        if (hipEventSynchronize(ctx->sev[2 * k + 1]) == hipSuccess &&
+            hipEventElapsedTime(&ms, ctx->sev[2 * k], ctx->sev[2 * k + 1]) == hipSuccess)
+            *ctx->sev_out[k] = ms;
+    }
+    ctx->sev_n = 0;
+}
+
+// an event pair for one solve's time, whose value goes to *out when the times are flushed
+static int solve_time_pair(lv_ctx *ctx, double *out, hipEvent_t **pair)
+{
+    if (!ctx->sev_ready) {
+        for (int q = 0; q < 2 * lv_ctx::NSEV; q++) LV_HIP_CHECK(hipEventCreate(&ctx->sev[q]));
+        ctx->sev_ready = 1;
+    }
+    if (ctx->sev_n == lv_ctx::NSEV) flush_solve_times(ctx);
+    const int k = ctx->sev_n++;
+    ctx->sev_out[k] = out;
+    *pair = &ctx->sev[2 * k];
+    return LV_OK;
+}
+
+// flushes the solve times when an API call that fills statistics returns, on every path (the
+// statistics belong to the caller and are only valid during the call)
+struct solve_time_scope {
+    lv_ctx *ctx;
+    ~solve_time_scope() { flush_solve_times(ctx); }
+};
+
 // one linear solve on the assembled operator, device vectors b and x (x in/out)
 static int solve_on_device(lv_ctx *ctx, const double *db, double *dx, const lv_solve_opts &opt, lv_solve_stats *st)
 {
     lv_precond M{};
     bool haveM = false;
-    hipEvent_t e0, e1;
-    LV_HIP_CHECK(hipEventCreate(&e0));
-    LV_HIP_CHECK(hipEventCreate(&e1));
     // the solve time includes the network inference when this solve is the first on a new operator
-    LV_HIP_CHECK(hipEventRecord(e0, ctx->s));
+    hipEvent_t *ev = nullptr;
+    if (st && st->nsolves < 32) {
+        int rc = solve_time_pair(ctx, &st->solve_ms[st->nsolves], &ev);
+        if (rc) return rc;
+        LV_HIP_CHECK(hipEventRecord(ev[0], ctx->s));
+    }
     int rc = spai_prepare(ctx, opt, &M, &haveM);
     if (rc) {
-        hipEventDestroy(e0);
-        hipEventDestroy(e1);
+        if (ev) ctx->sev_n--;  // this solve's pair (the last one) is dropped
         return rc;
     }
     int iters = 0, sweeps = 0, fb = 0;
@@ -1101,23 +1142,16 @@ static int solve_on_device(lv_ctx *ctx, const double *db, double *dx, const lv_s
                                    haveM ? &M : nullptr, &fb);
     if (opt.method == LV_SOLVER_SPAI && !haveM) fb++;  // no trained network: Jacobi
     if (e != hipSuccess) {
-        hipEventDestroy(e0);
-        hipEventDestroy(e1);
+        if (ev) ctx->sev_n--;
         return lv_fail_hip(ctx, e, "lv_solve_device", __FILE__, __LINE__);
     }
-    LV_HIP_CHECK(hipEventRecord(e1, ctx->s));
-    LV_HIP_CHECK(hipEventSynchronize(e1));
-    float ms = 0.0f;
-    hipEventElapsedTime(&ms, e0, e1);
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    if (ev) LV_HIP_CHECK(hipEventRecord(ev[1], ctx->s));
     if (st && st->nsolves < 32) {
         int k = st->nsolves;
         st->iters[k] = iters;
         st->outer[k] = sweeps;
         st->relres[k] = relres;
         st->relres0[k] = relres0;
-        st->solve_ms[k] = ms;
     }
     if (st) {
         st->nsolves++;
@@ -1152,6 +1186,7 @@ int lv_solve_pressure(lv_ctx *ctx, const double *b, double *x, const lv_solve_op
 {
     int rc = check_ctx(ctx, true);
     if (rc) return rc;
+    solve_time_scope times{ctx};
     rc = need_pressure_operator(ctx);
     if (rc) return rc;
     lv_solve_opts o;
@@ -1267,6 +1302,7 @@ int lv_find_pressure(lv_ctx *ctx, double dt, int32_t niter, const lv_solve_opts 
 {
     int rc = check_ctx(ctx, true);
     if (rc) return rc;
+    solve_time_scope times{ctx};
     lv_solve_opts o;
     if (opts) o = *opts;
     else default_opts(o);
@@ -1373,6 +1409,7 @@ int lv_viscous_implicit(lv_ctx *ctx, double dt, const lv_solve_opts *opts, lv_so
 {
     int rc = check_ctx(ctx, true);
     if (rc) return rc;
+    solve_time_scope times{ctx};
     lv_solve_opts o;
     if (opts) {
         o = *opts;

@@ -8,7 +8,9 @@
 // block while block 0 writes rz[(k+1)&1].
 struct lv_cg_scalars {
     double rz[2], rr, rr0, pq, alpha, beta, bb, thr2, tol2, rr64, bmean, xmean0, xmean, xshift;
+    double rr64_0, itol2;  // the initial FP64 residual^2; the first sweep's inner tolerance^2
     int iters, itmax, done, singular;
+    int skip;  // the initial residual already meets the tolerance (or is NaN): no sweep is needed
 };
 
 // PressureOperator (src/pressure.jl:89-101) in CSR: off-diagonal coefficients c_ij in the edge order
@@ -37,7 +39,7 @@ struct lv_progress {
 };
 
 struct lv_solver_work {
-    double *x, *r, *z, *p, *q;
+    double *r, *z, *p, *q;
     float *e32, *r32, *z32, *p32, *q32;
     double *partials;
     lv_cg_scalars *S;

@@ -235,7 +235,7 @@ __global__ __launch_bounds__(B) void k_write_P(lv_fields F, int n, const double 
 }
 
 // single-block finalisers of the one-off reductions
-enum { FIN_MEANS = 0, FIN_INIT64 = 1, FIN_RESID = 2, FIN_SUM1 = 3, FIN_INIT32 = 4 };
+enum { FIN_MEANS = 0, FIN_INIT64 = 1, FIN_RESID = 2, FIN_SUM1 = 3, FIN_INIT32 = 4, FIN_RESID0 = 5 };
 
 __global__ __launch_bounds__(B) void k_finalize(const double *__restrict__ partials, int nb, int mode, int n,
                                                 double inner_tol2, lv_cg_scalars *S)
@@ -265,6 +265,20 @@ __global__ __launch_bounds__(B) void k_finalize(const double *__restrict__ parti
         S->bb = o[1];
         S->thr2 = S->tol2 * o[1];
         break;
+    case FIN_RESID0: {  // the initial residual: the host's tests before the first sweep, made here
+        // (inner_tol2 carries the per-sweep floor of the inner tolerance, not squared)
+        const double rr = o[0], thr2 = S->tol2 * o[1];
+        S->rr64 = rr;
+        S->rr64_0 = rr;
+        S->bb = o[1];
+        S->thr2 = thr2;
+        S->skip = (rr <= thr2 || !(rr == rr)) ? 1 : 0;
+        const double need = 0.8 * sqrt(thr2 / rr);
+        double itol = need > inner_tol2 ? need : inner_tol2;
+        if (itol > 0.8) itol = 0.8;
+        S->itol2 = itol * itol;
+        break;
+    }
     case FIN_SUM1:
         S->xmean = o[0] / n;
         S->xshift = S->xmean - S->xmean0;
@@ -277,16 +291,18 @@ __global__ __launch_bounds__(B) void k_finalize(const double *__restrict__ parti
     }
 }
 
-// thr2 of an inner FP32 solve from the (r.z, r.r) partials of its init
+// thr2 of an inner FP32 solve from the (r.z, r.r) partials of its init.  dev: the first sweep, whose
+// inner tolerance and skip flag k_finalize (FIN_RESID0) left in S; a skipped sweep starts done, and
+// its first k_cg_s reports that to the host.
 __global__ __launch_bounds__(B) void k_finalize2(const double *__restrict__ Prr, int nb, double inner_tol2,
-                                                 lv_cg_scalars *S)
+                                                 lv_cg_scalars *S, int dev)
 {
     double o[2];
     reduce_all<2>(Prr, nb, o);
     if (threadIdx.x != 0) return;
-    S->thr2 = inner_tol2 * o[1];
+    S->thr2 = (dev ? S->itol2 : inner_tol2) * o[1];
     S->iters = 0;
-    S->done = 0;
+    S->done = (dev && S->skip) ? 1 : 0;
 }
 
 // ---- FP64 Jacobi-PCG ----------------------------------------------------------------------------
@@ -363,7 +379,12 @@ __global__ __launch_bounds__(B) void k_cg_s(int n, V A, const T *__restrict__ di
                                             const double *__restrict__ Prr, double *__restrict__ Ppq,
                                             lv_cg_scalars *S, lv_progress *prog, int par, int k, int first)
 {
-    if (S->done) return;
+    if (S->done) {
+        // a sweep that starts done (nothing left to solve): its first launch tells the host
+        if (k == 0 && prog && blockIdx.x == 0 && threadIdx.x == 0)
+            __hip_atomic_store(&prog->done, S->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        return;
+    }
     double o[2];
     reduce_all<2>(Prr, gridDim.x, o);
     const double rz = o[0], rr = o[1];
@@ -874,8 +895,8 @@ hipError_t cg_iterate(int n, V A, const T *diag, const T *dinv, T *x, T *r, T *z
 
 }  // namespace
 
-// Solves A x = b (device vectors; x holds the initial guess).  Writes iterations, sweeps and the
-// final FP64 relative residual.  *converged tells whether tol was met.
+// Solves A x = b (device vectors; x holds the initial guess and is iterated in place).  Writes
+// iterations, sweeps and the final FP64 relative residual.  *converged tells whether tol was met.
 hipError_t lv_solve_device(const lv_pressure_system &A, lv_solver_work &W, int n, const double *b, double *x,
                            const lv_solve_opts &opt, hipStream_t s, lv_timer *t32, lv_timer *t64, int *iters,
                            int *sweeps, double *relres, double *relres0, bool *converged, const lv_precond *M,
@@ -890,20 +911,19 @@ hipError_t lv_solve_device(const lv_pressure_system &A, lv_solver_work &W, int n
     memset(&h, 0, sizeof(h));
     hipLaunchKernelGGL(k_init_scalars, dim3(1), dim3(1), 0, s, W.S, opt.tol * opt.tol,
                        opt.itmax > 0 ? opt.itmax : 1000, (int)A.singular);
-    hipError_t e = hipMemcpyAsync(W.x, x, sizeof(double) * n, hipMemcpyDeviceToDevice, s);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_means, G, dim3(B), 0, s, n, b, (const double *)W.x, P.p3);
+    hipError_t e = hipSuccess;
+    hipLaunchKernelGGL(k_means, G, dim3(B), 0, s, n, b, (const double *)x, P.p3);
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(B), 0, s, (const double *)P.p3, nb, (int)FIN_MEANS, n, 0.0, W.S);
-    int total = 0, nsweep = 0;
+    int total = 0, nsweep = 0, skipped = 0;
     if (opt.method == LV_SOLVER_PCG64) {
         hipLaunchKernelGGL(k_cg64_init, G, dim3(B), 0, s, n, (const int *)A.deg, (const int *)A.ecol,
                            (const double *)A.ev64, (const double *)A.diag64, (const double *)A.dinv64, b,
-                           (const double *)W.x, W.r, W.z, P.rr, P.p3, (const lv_cg_scalars *)W.S);
+                           (const double *)x, W.r, W.z, P.rr, P.p3, (const lv_cg_scalars *)W.S);
         hipLaunchKernelGGL(k_finalize, dim3(1), dim3(B), 0, s, (const double *)P.p3, nb, (int)FIN_INIT64, n, 0.0,
                            W.S);
         int done = 0;
         e = cg_iterate<double>(n, lv_ell_own<double>{A.deg, A.ecol, A.ev64}, (const double *)A.diag64,
-                               (const double *)A.dinv64, W.x, W.r, W.z, W.p, W.q, P, W.S, W.hprog,
+                               (const double *)A.dinv64, x, W.r, W.z, W.p, W.q, P, W.S, W.hprog,
                                W.dprog, W.hS, s, t64, h, (const lv_precond *)nullptr, &done);
         if (e != hipSuccess) return e;
         e = read_scalars(W.S, W.hS, &h, s);
@@ -918,27 +938,40 @@ hipError_t lv_solve_device(const lv_pressure_system &A, lv_solver_work &W, int n
         // stores that x when the sweeps end, so the last residual is the returned x's
         double *Pxs = A.singular ? P.ur : nullptr;
         // the iterations of a sweep are still in S at the next sweep's read of the FP64 residual
-        // (k_resid64 and k_finalize leave them), so a sweep ends without a synchronising read
+        // (k_resid64 and k_finalize leave them), so a sweep ends without a synchronising read.
+        // Nor does the first sweep wait for one: k_finalize (FIN_RESID0) makes the tests below on
+        // the device, and a first sweep with nothing to solve starts done and changes nothing
+        // (its correction is zero).
         bool pending = false;
+        double itol = 0.0;
         for (;;) {
             hipLaunchKernelGGL(k_resid64, G, dim3(B), 0, s, n, (const int *)A.deg, (const int *)A.ecol,
-                               (const double *)A.ev64, (const double *)A.diag64, b, (const double *)W.x, W.r, P.p3,
+                               (const double *)A.ev64, (const double *)A.diag64, b, (const double *)x, W.r, P.p3,
                                nsweep > 0 ? (const double *)Pxs : (const double *)nullptr, W.S);
-            hipLaunchKernelGGL(k_finalize, dim3(1), dim3(B), 0, s, (const double *)P.p3, nb, (int)FIN_RESID, n, 0.0,
-                               W.S);
-            e = read_scalars(W.S, W.hS, &h, s);
-            if (e != hipSuccess) return e;
-            if (pending) total += h.iters;
-            pending = false;
-            if (nsweep == 0) *relres0 = (h.bb > 0.0) ? sqrt(h.rr64 / h.bb) : sqrt(h.rr64);
-            if (h.rr64 <= h.thr2 || nsweep >= outer_max || !(h.rr64 == h.rr64)) break;
-            // ask the FP32 sweep for just the reduction still missing, with a margin of 0.8 for the
-            // FP32 residual's drift from the FP64 one (a few per cent at these reductions: at margins
-            // up to 0.9 no 2048^2 benchmark solve needed an extra sweep), but not below what FP32
-            // arithmetic can deliver in one sweep
-            double need = 0.8 * sqrt(h.thr2 / h.rr64);
-            double itol = need > floor_tol ? need : floor_tol;
-            if (itol > 0.8) itol = 0.8;
+            if (nsweep == 0) {
+                hipLaunchKernelGGL(k_finalize, dim3(1), dim3(B), 0, s, (const double *)P.p3, nb, (int)FIN_RESID0,
+                                   n, floor_tol, W.S);
+            } else {
+                hipLaunchKernelGGL(k_finalize, dim3(1), dim3(B), 0, s, (const double *)P.p3, nb, (int)FIN_RESID, n,
+                                   0.0, W.S);
+                e = read_scalars(W.S, W.hS, &h, s);
+                if (e != hipSuccess) return e;
+                if (pending) total += h.iters;
+                pending = false;
+                if (nsweep == 1) {
+                    *relres0 = (h.bb > 0.0) ? sqrt(h.rr64_0 / h.bb) : sqrt(h.rr64_0);
+                    skipped = h.skip;  // the first sweep had nothing to solve (not reported as a sweep)
+                }
+                if (h.rr64 <= h.thr2 || nsweep >= outer_max || !(h.rr64 == h.rr64)) break;
+                // ask the FP32 sweep for just the reduction still missing, with a margin of 0.8 for
+                // the FP32 residual's drift from the FP64 one (a few per cent at these reductions: at
+                // margins up to 0.9 no 2048^2 benchmark solve needed an extra sweep), but not below
+                // what FP32 arithmetic can deliver in one sweep (the first sweep's is the same rule,
+                // evaluated by k_finalize)
+                const double need = 0.8 * sqrt(h.thr2 / h.rr64);
+                itol = need > floor_tol ? need : floor_tol;
+                if (itol > 0.8) itol = 0.8;
+            }
             const lv_precond *Mi = (opt.method == LV_SOLVER_SPAI || opt.method == LV_SOLVER_AMG) ? M : nullptr;
             // with the multilevel preconditioner in the aggregate order, the whole FP32 sweep runs in
             // that order (the FP64 residual and correction stay in the particles' order)
@@ -951,7 +984,8 @@ hipError_t lv_solve_device(const lv_pressure_system &A, lv_solver_work &W, int n
                                    P.rr, (const lv_cg_scalars *)W.S, z0, z0 ? Mi->amg->fine_scale : 0.0f);
                 // k_finalize2 clears the done flag of the previous sweep, so it precedes the first
                 // application of the learned preconditioner (which is skipped while done is set)
-                hipLaunchKernelGGL(k_finalize2, dim3(1), dim3(B), 0, s, (const double *)P.rr, nb, itol * itol, W.S);
+                hipLaunchKernelGGL(k_finalize2, dim3(1), dim3(B), 0, s, (const double *)P.rr, nb, itol * itol, W.S,
+                                   nsweep == 0 ? 1 : 0);
                 if (Mi && Mi->amg) {
                     e = lv_amg_vcycle(Mi->amg, A, (const float *)W.r32, W.z32, P.rr, (const lv_cg_scalars *)W.S, s,
                                       z0 != nullptr);
@@ -982,31 +1016,29 @@ hipError_t lv_solve_device(const lv_pressure_system &A, lv_solver_work &W, int n
             }
             pending = true;
             nsweep++;
-            hipLaunchKernelGGL(k_correct, G, dim3(B), 0, s, n, W.x, (const float *)W.e32,
+            hipLaunchKernelGGL(k_correct, G, dim3(B), 0, s, n, x, (const float *)W.e32,
                                Gp ? (const int *)Gp->iperm : (const int *)nullptr, Pxs, (const lv_cg_scalars *)W.S);
         }
-        if (Pxs && nsweep > 0) hipLaunchKernelGGL(k_shift, G, dim3(B), 0, s, n, W.x, (const lv_cg_scalars *)W.S);
+        if (Pxs && nsweep > 0) hipLaunchKernelGGL(k_shift, G, dim3(B), 0, s, n, x, (const lv_cg_scalars *)W.S);
     }
     if (opt.method == LV_SOLVER_PCG64) {
         // final FP64 residual of the returned x
         if (A.singular) {
-            hipLaunchKernelGGL(k_sum1, G, dim3(B), 0, s, n, (const double *)W.x, P.p3);
+            hipLaunchKernelGGL(k_sum1, G, dim3(B), 0, s, n, (const double *)x, P.p3);
             hipLaunchKernelGGL(k_finalize, dim3(1), dim3(B), 0, s, (const double *)P.p3, nb, (int)FIN_SUM1, n, 0.0,
                                W.S);
-            hipLaunchKernelGGL(k_shift, G, dim3(B), 0, s, n, W.x, (const lv_cg_scalars *)W.S);
+            hipLaunchKernelGGL(k_shift, G, dim3(B), 0, s, n, x, (const lv_cg_scalars *)W.S);
         }
         hipLaunchKernelGGL(k_resid64, G, dim3(B), 0, s, n, (const int *)A.deg, (const int *)A.ecol,
-                           (const double *)A.ev64, (const double *)A.diag64, b, (const double *)W.x, W.r, P.p3,
+                           (const double *)A.ev64, (const double *)A.diag64, b, (const double *)x, W.r, P.p3,
                            (const double *)nullptr, W.S);
         hipLaunchKernelGGL(k_finalize, dim3(1), dim3(B), 0, s, (const double *)P.p3, nb, (int)FIN_RESID, n, 0.0,
                            W.S);
         e = read_scalars(W.S, W.hS, &h, s);
         if (e != hipSuccess) return e;
     }
-    e = hipMemcpyAsync(x, W.x, sizeof(double) * n, hipMemcpyDeviceToDevice, s);
-    if (e != hipSuccess) return e;
     *iters = total;
-    *sweeps = nsweep;
+    *sweeps = nsweep - skipped;
     *relres = (h.bb > 0.0) ? sqrt(h.rr64 / h.bb) : sqrt(h.rr64);
     *converged = (h.rr64 <= h.thr2);
     return hipSuccess;

@@ -222,3 +222,25 @@ def test_solver_launches_stop_at_convergence(lv, oracle, gpu):
         assert st["fallbacks"] == 0 and max(st["relres"]) <= 1e-8
         assert iters <= seen.value <= iters + 2 * sweeps, (method, seen.value, iters, sweeps)
         assert timed.value == (seen.value + every - 1) // every and tot.value > 0.0, (every, timed.value, seen.value)
+
+
+def test_solve_from_converged_guess_skips_the_sweep(lv, oracle, gpu):
+    """The tests before the first refinement sweep run on the device (no synchronising read): a solve
+    whose initial guess already meets the tolerance queues a sweep that starts done, reports it to
+    the host at once (well inside the host's 20 ms no-progress fallback), changes nothing and is not
+    counted."""
+    _flush("mixed solve from a converged guess")
+    grid, op, solver, x, dr, d, rp, cols, coef = _system(lv, oracle, "periodic", 301, "tg", 2e-3)
+    b, _ = solver.rhs(2e-3, False)
+    x0 = grid.get("P")
+    lv.spai.set_params(grid, lv.spai.load_params())
+    for method in ("amg", "mixed", "spai"):
+        P1 = solver.solve(b, x0, method=method, tol=1e-8)
+        st1 = solver.last_stats
+        assert st1["iters"][0] > 0 and st1["outer"][0] >= 1 and st1["relres"][0] <= 1e-8, (method, st1)
+        P2 = solver.solve(b, P1, method=method, tol=1e-8)
+        st2 = solver.last_stats
+        assert st2["iters"][0] == 0 and st2["outer"][0] == 0 and st2["fallbacks"] == 0, (method, st2)
+        assert st2["relres"][0] <= 1e-8 and st2["relres0"][0] == st2["relres"][0], (method, st2)
+        assert st2["solve_ms"][0] < 15.0, (method, st2["solve_ms"])
+        assert np.array_equal(P2, P1), method

@@ -416,8 +416,10 @@ __global__ __launch_bounds__(B) void k_fine_down_j(fine_ops<V> F, const float *_
 // [FUSE_ROWS b, FUSE_ROWS (b+1)) (bfirst[b]..bfirst[b+1]-1), keeps their residuals t in LDS and sums
 // each cell's as k_restrict1 does (RL lanes, the same order and shuffle tree, so rc is the same).
 // A block holds at most FUSE_ROWS - 1 + (largest cell) rows; set-up checks that this fits FUSE_CAP.
-constexpr int FUSE_ROWS = 224;
-constexpr int FUSE_CAP = 1024;
+// With z0 given each thread takes two rows (FUSE_ROWS a little under 2 B), whose loads are in flight
+// together.
+constexpr int FUSE_ROWS = 480;
+constexpr int FUSE_CAP = 2048;
 
 // t = r - A z0 for one row when z0 = w D^-1 r is already stored (the same values down_row_j forms)
 template <class V>
@@ -456,8 +458,19 @@ __global__ __launch_bounds__(B) void k_fine_down_jr(fine_ops<V> F, const float *
     const int b = lv_xcd_block(blockIdx.x, gridDim.x);
     const int c0 = bfirst[b], c1 = bfirst[b + 1];
     const int R0 = start[c0], R1 = start[c1];
-    for (int i = R0 + (int)threadIdx.x; i < R1; i += B)
-        tl[i - R0] = Z0 ? down_row_z0(F, i, r, z0) : down_row_j(F, i, r, z0);
+    if constexpr (Z0) {
+        // rows i0 and i0 + B; the second is clamped into the block's rows so that both rows' loads
+        // are issued without a branch (a clamped duplicate is computed and dropped)
+        for (int i0 = R0 + (int)threadIdx.x; i0 < R1; i0 += 2 * B) {
+            const int i1 = i0 + B;
+            const float t0 = down_row_z0(F, i0, r, z0);
+            const float t1 = down_row_z0(F, i1 < R1 ? i1 : R1 - 1, r, z0);
+            tl[i0 - R0] = t0;
+            if (i1 < R1) tl[i1 - R0] = t1;
+        }
+    } else {
+        for (int i = R0 + (int)threadIdx.x; i < R1; i += B) tl[i - R0] = down_row_j(F, i, r, z0);
+    }
     __syncthreads();
     const int lanes = (c1 - c0) * RL;
     for (int base = 0; base < lanes; base += B) {  // uniform trip count: every lane shuffles

@@ -416,9 +416,10 @@ __global__ __launch_bounds__(B) void k_fine_down_j(fine_ops<V> F, const float *_
 // [FUSE_ROWS b, FUSE_ROWS (b+1)) (bfirst[b]..bfirst[b+1]-1), keeps their residuals t in LDS and sums
 // each cell's as k_restrict1 does (RL lanes, the same order and shuffle tree, so rc is the same).
 // A block holds at most FUSE_ROWS - 1 + (largest cell) rows; set-up checks that this fits FUSE_CAP.
-// With z0 given each thread takes two rows (FUSE_ROWS a little under 2 B), whose loads are in flight
-// together.
-constexpr int FUSE_ROWS = 480;
+// With z0 given each thread takes FUSE_RPT rows (FUSE_ROWS a little under FUSE_RPT B), whose loads
+// are in flight together.
+constexpr int FUSE_RPT = 2;
+constexpr int FUSE_ROWS = FUSE_RPT * 256 - 32;
 constexpr int FUSE_CAP = 2048;
 
 // t = r - A z0 for one row when z0 = w D^-1 r is already stored (the same values down_row_j forms)
@@ -459,14 +460,18 @@ __global__ __launch_bounds__(B) void k_fine_down_jr(fine_ops<V> F, const float *
     const int c0 = bfirst[b], c1 = bfirst[b + 1];
     const int R0 = start[c0], R1 = start[c1];
     if constexpr (Z0) {
-        // rows i0 and i0 + B; the second is clamped into the block's rows so that both rows' loads
-        // are issued without a branch (a clamped duplicate is computed and dropped)
-        for (int i0 = R0 + (int)threadIdx.x; i0 < R1; i0 += 2 * B) {
-            const int i1 = i0 + B;
-            const float t0 = down_row_z0(F, i0, r, z0);
-            const float t1 = down_row_z0(F, i1 < R1 ? i1 : R1 - 1, r, z0);
-            tl[i0 - R0] = t0;
-            if (i1 < R1) tl[i1 - R0] = t1;
+        // rows i0 + q B; all but the first are clamped into the block's rows so that every row's
+        // loads are issued without a branch (a clamped duplicate is computed and dropped)
+        for (int i0 = R0 + (int)threadIdx.x; i0 < R1; i0 += FUSE_RPT * B) {
+            float t[FUSE_RPT];
+#pragma unroll
+            for (int q = 0; q < FUSE_RPT; q++) {
+                const int i = i0 + q * B;
+                t[q] = down_row_z0(F, i < R1 ? i : R1 - 1, r, z0);
+            }
+#pragma unroll
+            for (int q = 0; q < FUSE_RPT; q++)
+                if (i0 + q * B < R1) tl[i0 + q * B - R0] = t[q];
         }
     } else {
         for (int i = R0 + (int)threadIdx.x; i < R1; i += B) tl[i - R0] = down_row_j(F, i, r, z0);
@@ -553,13 +558,27 @@ __global__ __launch_bounds__(B) void k_fine_up(fine_ops<V> F, const float *__res
     t[i] = ti;
 }
 
-// prolongation in place: z0 += alpha ec[cell of the row]
+// prolongation in place: z0 += alpha ec[cell of the row], four rows per thread with 16-byte loads
+// and stores of agg and z0 (both start 256-byte aligned); the last thread takes the n % 4 rows
+constexpr int PROLONG_RPT = 4;
 __global__ __launch_bounds__(B) void k_prolong1(int n, const int *__restrict__ agg, const float *__restrict__ ec,
                                                 float alpha, float *__restrict__ z0, const lv_cg_scalars *S)
 {
+    typedef int i4 __attribute__((ext_vector_type(4)));
+    typedef float f4 __attribute__((ext_vector_type(4)));
     if (S && S->done) return;
-    const int i = blockIdx.x * B + threadIdx.x;
-    if (i < n) z0[i] += alpha * ec[agg[i]];
+    const int i = PROLONG_RPT * (blockIdx.x * B + threadIdx.x);
+    if (i + 3 < n) {
+        const i4 a = *reinterpret_cast<const i4 *>(agg + i);
+        f4 z = *reinterpret_cast<const f4 *>(z0 + i);
+        z.x += alpha * ec[a.x];
+        z.y += alpha * ec[a.y];
+        z.z += alpha * ec[a.z];
+        z.w += alpha * ec[a.w];
+        *reinterpret_cast<f4 *>(z0 + i) = z;
+    } else {
+        for (int k = i; k < n; k++) z0[k] += alpha * ec[agg[k]];
+    }
 }
 
 // Jacobi smoother, up (after k_prolong1): z = z1 + w D^-1 (r - A z1); partials r.z, r.r of the final z
@@ -894,6 +913,7 @@ __global__ __launch_bounds__(TB) void k_tail_lds(tail_levels T, tail_layout Y, c
     run_tail(T, tail_in_lds{lds, Y}, lprog, nprog);
     for (int f = threadIdx.x; f < n0; f += TB) T.t[first][f] = lds[Y.t[first] + f];
 }
+static_assert(B == 256 && FUSE_ROWS + 64 <= FUSE_CAP, "fused blocks fit their LDS rows");
 static_assert(LV_AMG_DENSE_MAX * 16 == TB, "the dense solve of the tail maps 16 lanes to a row");
 static_assert(TAIL_MAX <= TB && LV_AMG_MAXPROG <= TB, "one right-hand side entry and one operation per lane");
 static_assert(TAIL_LDS % 4 == 0, "the LDS image is copied as float4");
@@ -1429,7 +1449,8 @@ static hipError_t vcycle(const lv_amg *G, const fine_ops<V> &F, const int *agg, 
                            z, G->t0, S);
         hipLaunchKernelGGL(k_fine_post<V>, dim3(lv_nblocks(n)), dim3(B), 0, s, F, r, (const float *)G->t0, z, Prr, S);
     } else {
-        hipLaunchKernelGGL(k_prolong1, g1(n), dim3(B), 0, s, n, agg, (const float *)L1.t, G->fine_alpha, G->z0, S);
+        hipLaunchKernelGGL(k_prolong1, g1((n + PROLONG_RPT - 1) / PROLONG_RPT), dim3(B), 0, s, n, agg,
+                           (const float *)L1.t, G->fine_alpha, G->z0, S);
         hipLaunchKernelGGL(k_fine_up_j<V>, dim3(lv_nblocks(n)), dim3(B), 0, s, F, r, (const float *)G->z0, z, Prr, S);
     }
     return hipGetLastError();

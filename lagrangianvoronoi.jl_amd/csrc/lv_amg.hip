@@ -707,9 +707,9 @@ __device__ __forceinline__ float sum_children(float t)
 }
 
 // up, one cell: z1 = z + ec[parent] (neighbours' z1 recomputed), t = r - A z1, out = z1 + w D^-1 t
-__device__ __forceinline__ void st_up_cell(const st_level &L, int Kyc, int f, const float *__restrict__ r,
-                                           const float *__restrict__ z, const float *__restrict__ ec,
-                                           float *__restrict__ out)
+__device__ __forceinline__ float st_up_cell(const st_level &L, int Kyc, int f, const float *__restrict__ r,
+                                            const float *__restrict__ z, const float *__restrict__ ec,
+                                            float *__restrict__ out)
 {
     const int cx = f / L.Ky, cy = f - cx * L.Ky;
     float ck[8], zj[8], ej[8];
@@ -724,7 +724,9 @@ __device__ __forceinline__ void st_up_cell(const st_level &L, int Kyc, int f, co
     float acc = L.d[f] * z1;
 #pragma unroll
     for (int k = 0; k < 8; k++) acc += ck[k] * (z1 - (zj[k] + ej[k]));
-    out[f] = z1 + OMEGA * L.dinv[f] * (r[f] - acc);
+    const float v = z1 + OMEGA * L.dinv[f] * (r[f] - acc);
+    out[f] = v;
+    return v;
 }
 
 // W-cycle, after the first visit of level L returned t: acc = t and r = r - A t (the right-hand
@@ -759,15 +761,24 @@ __global__ __launch_bounds__(B) void k_st_down(st_level L, int Kyc, int nc, cons
     if (q == 0 && P < nc) rc[P] = t;
 }
 
-// up on level L: the final correction out of L from the next level's correction ec
+// up on level L: the final correction out of L from the next level's correction ec.  PROLONG (level
+// 1 in the aggregate order, whose cell f owns rows start[f]..start[f+1]-1): also the prolongation
+// z0 += alpha0 out[f] of those rows, as k_prolong1 forms it
+template <bool PROLONG>
 __global__ __launch_bounds__(B) void k_st_up(st_level L, int Kyc, const float *__restrict__ r,
                                              const float *__restrict__ z, const float *__restrict__ ec,
-                                             float *__restrict__ out, const lv_cg_scalars *S)
+                                             float *__restrict__ out, const lv_cg_scalars *S,
+                                             const int *__restrict__ start, float alpha0, float *__restrict__ z0)
 {
     if (S && S->done) return;
     const int f = blockIdx.x * B + threadIdx.x;
     if (f >= L.Kx * L.Ky) return;
-    st_up_cell(L, Kyc, f, r, z, ec, out);
+    const float v = st_up_cell(L, Kyc, f, r, z, ec, out);
+    if (PROLONG) {
+        const float a = alpha0 * v;
+        const int e = start[f + 1];
+        for (int i = start[f]; i < e; i++) z0[i] += a;
+    }
 }
 
 // The small levels of the cycle in one workgroup: a program of operations on levels first..last
@@ -1436,12 +1447,20 @@ static hipError_t vcycle(const lv_amg *G, const fine_ops<V> &F, const int *agg, 
     else
         hipLaunchKernelGGL(k_tail, dim3(1), dim3(TB), 0, s, T, (const int *)G->prog, G->nprog,
                            (const float *)G->dense_inv, S);
-    // coarse levels above the tail, up: each level's final correction in t
+    // coarse levels above the tail, up: each level's final correction in t; level 1 in the aggregate
+    // order with the Jacobi smoother also prolongates into z0
+    const bool prolong_in_up = lt > 1 && !G->fine_learned && list == nullptr;
     for (int l = lt - 1; l >= 1; l--) {
         const lv_amg_level &L = G->lev[l], &C = G->lev[l + 1];
         const st_level sl{L.Kx, L.Ky, G->xper, G->yper, L.d, L.c, L.dinv, L.alpha};
-        hipLaunchKernelGGL(k_st_up, g1(L.n), dim3(B), 0, s, sl, C.Ky, (const float *)L.r, (const float *)L.z,
-                           (const float *)C.t, L.t, S);
+        if (l == 1 && prolong_in_up)
+            hipLaunchKernelGGL(k_st_up<true>, g1(L.n), dim3(B), 0, s, sl, C.Ky, (const float *)L.r,
+                               (const float *)L.z, (const float *)C.t, L.t, S, (const int *)G->start, G->fine_alpha,
+                               G->z0);
+        else
+            hipLaunchKernelGGL(k_st_up<false>, g1(L.n), dim3(B), 0, s, sl, C.Ky, (const float *)L.r,
+                               (const float *)L.z, (const float *)C.t, L.t, S, (const int *)nullptr, 0.0f,
+                               (float *)nullptr);
     }
     // level 0, up: prolongation, residual and post-smoothing; partials of r.z, r.r
     if (G->fine_learned) {
@@ -1449,8 +1468,9 @@ static hipError_t vcycle(const lv_amg *G, const fine_ops<V> &F, const int *agg, 
                            z, G->t0, S);
         hipLaunchKernelGGL(k_fine_post<V>, dim3(lv_nblocks(n)), dim3(B), 0, s, F, r, (const float *)G->t0, z, Prr, S);
     } else {
-        hipLaunchKernelGGL(k_prolong1, g1((n + PROLONG_RPT - 1) / PROLONG_RPT), dim3(B), 0, s, n, agg,
-                           (const float *)L1.t, G->fine_alpha, G->z0, S);
+        if (!prolong_in_up)
+            hipLaunchKernelGGL(k_prolong1, g1((n + PROLONG_RPT - 1) / PROLONG_RPT), dim3(B), 0, s, n, agg,
+                               (const float *)L1.t, G->fine_alpha, G->z0, S);
         hipLaunchKernelGGL(k_fine_up_j<V>, dim3(lv_nblocks(n)), dim3(B), 0, s, F, r, (const float *)G->z0, z, Prr, S);
     }
     return hipGetLastError();

@@ -63,10 +63,23 @@ __device__ __forceinline__ void reduce_all(const double *__restrict__ partials, 
 {
     __shared__ double sm[NV][256 / 64];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    // each thread sums partials threadIdx.x, +256, ... in that order; the first LV_NBMAX of them in
+    // an unrolled loop without branches (a slot past nb loads a valid partial and is not added), so
+    // that all of a thread's loads are in flight together: a rolled loop waits for one L2 round trip
+    // per partial
+    constexpr int PER = (LV_NBMAX + 255) / 256;
+    const int last = nb > 0 ? nb - 1 : 0;
 #pragma unroll
     for (int q = 0; q < NV; q++) {
+        const double *pq = partials + (size_t)q * nb;
         double s = 0.0;
-        for (int k = threadIdx.x; k < nb; k += 256) s += partials[(size_t)q * nb + k];
+#pragma unroll
+        for (int j = 0; j < PER; j++) {
+            const int k = (int)threadIdx.x + 256 * j;
+            const double p = pq[k < nb ? k : last];
+            s = (k < nb) ? s + p : s;
+        }
+        for (int k = (int)threadIdx.x + 256 * PER; k < nb; k += 256) s += pq[k];
         s = lv_wave_sum(s);
         if (lane == 0) sm[q][w] = s;
     }

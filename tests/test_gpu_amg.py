@@ -198,6 +198,25 @@ def test_find_pressure_amg_matches_oracle(lv, oracle, gpu):
     assert rel_err(grid.get("P"), op["P"]) <= 1e-8
 
 
+def test_solve_times_are_reported_for_every_solve(lv, oracle, gpu):
+    """Each solve's time comes from an event pair that is read when the API call returns (one wait
+    per call): all 10 Picard solves of find_pressure! report a positive time, and together no more
+    than the call's wall time."""
+    import time
+
+    _flush("amg solve times")
+    grid, op, x, dr = make_pair(lv, oracle, "periodic", K=48, seed=0x5EED0502, ic="tg")
+    lv.stiffened_eos(grid, 1.4, 2500.0 / 1.4)
+    solver = lv.PressureSolver(grid, method="amg", tol=1e-8, picard_window=8)
+    lv.find_pressure(solver, 2e-3)  # set-up outside the timed call
+    t0 = time.perf_counter()
+    st = lv.find_pressure(solver, 2e-3)
+    wall_ms = (time.perf_counter() - t0) * 1e3
+    ms = st["solve_ms"]
+    assert len(ms) == 10 and all(0.0 < m < wall_ms for m in ms), (ms, wall_ms)
+    assert sum(ms) <= wall_ms, (sum(ms), wall_ms)
+
+
 def test_solver_launches_stop_at_convergence(lv, oracle, gpu):
     """The host keeps one CG iteration queued ahead of the device and stops launching once the
     device reports convergence in pinned memory: per refinement sweep the SpMV kernel is launched

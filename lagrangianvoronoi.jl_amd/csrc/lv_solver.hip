@@ -177,7 +177,8 @@ __global__ __launch_bounds__(B) void k_rhs1(lv_grid_params g, lv_fields F, lv_me
     P0[i] = Pi;
 }
 
-// refresh!(solver) gp_step pass, src/pressure.jl:189-202 (the first 6 slots' gathers issued at once)
+// refresh!(solver) gp_step pass, src/pressure.jl:189-202 (the first 6 slots' labels and vertices
+// loaded together, then their gathers issued at once)
 __global__ __launch_bounds__(B) void k_rhs2(lv_grid_params g, lv_fields F, lv_mesh_view M, int n,
                                             const double *__restrict__ GP, double *__restrict__ b)
 {
@@ -190,10 +191,15 @@ __global__ __launch_bounds__(B) void k_rhs2(lv_grid_params g, lv_fields F, lv_me
     constexpr int U = 6;
     int J[U];
     double Xj[U], Yj[U], G1[U], G2[U];
+    double E1x[U], E1y[U], E2x[U], E2y[U];  // the slots' vertices, loaded with the labels
 #pragma unroll
     for (int k = 0; k < U; k++) {
         const int lab = LV_EL(M, k, i);
         J[k] = (k < ne && lab >= 0) ? lab : -1;
+        E1x[k] = LV_EV(M, 0, k, i);
+        E1y[k] = LV_EV(M, 1, k, i);
+        E2x[k] = LV_EV(M, 2, k, i);
+        E2y[k] = LV_EV(M, 3, k, i);
     }
 #pragma unroll
     for (int k = 0; k < U; k++) {
@@ -206,7 +212,7 @@ __global__ __launch_bounds__(B) void k_rhs2(lv_grid_params g, lv_fields F, lv_me
 #pragma unroll
     for (int k = 0; k < U; k++) {
         if (J[k] < 0) continue;
-        const double e1x = LV_EV(M, 0, k, i), e1y = LV_EV(M, 1, k, i), e2x = LV_EV(M, 2, k, i), e2y = LV_EV(M, 3, k, i);
+        const double e1x = E1x[k], e1y = E1y[k], e2x = E2x[k], e2y = E2y[k];
         double y1, y2;
         lv_arrow_target(g, Xj[k], Yj[k], x1, x2, y1, y2);
         const double lrr = lv_lr_ratio(x1 - y1, x2 - y2, e1x, e1y, e2x, e2y);

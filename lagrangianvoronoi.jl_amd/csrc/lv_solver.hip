@@ -621,19 +621,25 @@ __global__ __launch_bounds__(B) void k_shift(int n, double *__restrict__ x, cons
 
 // ---- FP32 inner PCG of the refinement --------------------------------------------------------
 
+// A first sweep that k_finalize (FIN_RESID0) flagged skip (initial residual within tol, zero, or
+// NaN) must leave x as it is: its FP32 vectors are zeros and its correction scale is 0, so neither
+// 1/sqrt(0) nor a NaN residual reaches x.
+__device__ inline bool sweep_skipped(const lv_cg_scalars *S, int first) { return first && S->skip; }
+
 // r32 = r64 / |r64|, e = 0, z = dinv r, p = z; partials r.z, r.r.  With perm, the FP32 vectors
 // are in the aggregate order: r32[i] comes from r64[perm[i]].  With z0 also z0 = sc dinv r (the
-// multilevel cycle's pre-smoothed correction).
+// multilevel cycle's pre-smoothed correction).  first: this is the solve's first sweep.
 __global__ __launch_bounds__(B) void k_cg32_init(int n, const double *__restrict__ r64, const int *__restrict__ perm,
                                                  const float *__restrict__ dinv, float *__restrict__ e,
                                                  float *__restrict__ r, float *__restrict__ z,
                                                  double *__restrict__ Prr, const lv_cg_scalars *S,
-                                                 float *__restrict__ z0, float sc)
+                                                 float *__restrict__ z0, float sc, int first)
 {
-    const double inv = 1.0 / sqrt(S->rr64);
+    const bool skip = sweep_skipped(S, first);
+    const double inv = skip ? 0.0 : 1.0 / sqrt(S->rr64);
     double v[2] = {0.0, 0.0};
     for (int i = blockIdx.x * B + threadIdx.x; i < n; i += gridDim.x * B) {
-        const float ri = (float)(r64[perm ? perm[i] : i] * inv);
+        const float ri = skip ? 0.0f : (float)(r64[perm ? perm[i] : i] * inv);
         const float zi = dinv[i] * ri;
         e[i] = 0.0f;
         r[i] = ri;
@@ -646,15 +652,17 @@ __global__ __launch_bounds__(B) void k_cg32_init(int n, const double *__restrict
 }
 
 // x += |r64| e (e in the aggregate order when iperm is given: x[i] takes e[iperm[i]]); with Pxs
-// also the partial sums of the new x (for the shift of a singular system)
+// also the partial sums of the new x (for the shift of a singular system).  A skipped first sweep
+// leaves x unchanged (and still writes its sums).
 __global__ __launch_bounds__(B) void k_correct(int n, double *__restrict__ x, const float *__restrict__ e,
                                                const int *__restrict__ iperm, double *__restrict__ Pxs,
-                                               const lv_cg_scalars *S)
+                                               const lv_cg_scalars *S, int first)
 {
-    const double s = sqrt(S->rr64);
+    const bool skip = sweep_skipped(S, first);
+    const double s = skip ? 0.0 : sqrt(S->rr64);
     double v[1] = {0.0};
     for (int i = blockIdx.x * B + threadIdx.x; i < n; i += gridDim.x * B) {
-        const double xi = x[i] + s * (double)e[iperm ? iperm[i] : i];
+        const double xi = skip ? x[i] : x[i] + s * (double)e[iperm ? iperm[i] : i];
         x[i] = xi;
         v[0] += xi;
     }
@@ -987,7 +995,8 @@ hipError_t lv_solve_device(const lv_pressure_system &A, lv_solver_work &W, int n
                 float *z0 = (Mi && Mi->amg && Mi->amg->z0_in) ? Mi->amg->z0 : nullptr;
                 hipLaunchKernelGGL(k_cg32_init, G, dim3(B), 0, s, n, (const double *)W.r,
                                    Gp ? (const int *)Gp->list : (const int *)nullptr, dinv32, W.e32, W.r32, W.z32,
-                                   P.rr, (const lv_cg_scalars *)W.S, z0, z0 ? Mi->amg->fine_scale : 0.0f);
+                                   P.rr, (const lv_cg_scalars *)W.S, z0, z0 ? Mi->amg->fine_scale : 0.0f,
+                                   nsweep == 0 ? 1 : 0);
                 // k_finalize2 clears the done flag of the previous sweep, so it precedes the first
                 // application of the learned preconditioner (which is skipped while done is set)
                 hipLaunchKernelGGL(k_finalize2, dim3(1), dim3(B), 0, s, (const double *)P.rr, nb, itol * itol, W.S,
@@ -1021,9 +1030,10 @@ hipError_t lv_solve_device(const lv_pressure_system &A, lv_solver_work &W, int n
                 Mi = nullptr;
             }
             pending = true;
-            nsweep++;
             hipLaunchKernelGGL(k_correct, G, dim3(B), 0, s, n, x, (const float *)W.e32,
-                               Gp ? (const int *)Gp->iperm : (const int *)nullptr, Pxs, (const lv_cg_scalars *)W.S);
+                               Gp ? (const int *)Gp->iperm : (const int *)nullptr, Pxs, (const lv_cg_scalars *)W.S,
+                               nsweep == 0 ? 1 : 0);
+            nsweep++;
         }
         if (Pxs && nsweep > 0) hipLaunchKernelGGL(k_shift, G, dim3(B), 0, s, n, x, (const lv_cg_scalars *)W.S);
     }

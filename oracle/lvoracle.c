@@ -766,6 +766,62 @@ int lvo_rhs(const lvo_grid *g, const lvo_state *s, double dt, int gp_step, const
     return 0;
 }
 
+/* The operator of lvo_apply_A compacted to CSR rows (interior edges in edge order), so that a solve
+ * streams 12 bytes per coupling instead of the padded per-polygon slots.  Row sums are evaluated
+ * exactly as lvo_apply_A evaluates them. */
+typedef struct {
+    int n;
+    int *rp, *ci;
+    double *cv;
+    const double *diag;
+} lvo_csr;
+
+static void csr_build(const lvo_state *s, const double *diag, const double *coef, lvo_csr *A)
+{
+    int n = s->n;
+    A->n = n;
+    A->diag = diag;
+    A->rp = (int *)malloc(sizeof(int) * (n + 1));
+    A->rp[0] = 0;
+    for (int i = 0; i < n; i++) {
+        int c = 0;
+        for (int k = 0; k < s->ne[i]; k++) c += (EL(s, i, k) >= 0);
+        A->rp[i + 1] = A->rp[i] + c;
+    }
+    A->ci = (int *)malloc(sizeof(int) * ((size_t)A->rp[n] + 1));
+    A->cv = (double *)malloc(sizeof(double) * ((size_t)A->rp[n] + 1));
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; i++) {
+        int o = A->rp[i];
+        for (int k = 0; k < s->ne[i]; k++) {
+            int j = EL(s, i, k);
+            if (j < 0) continue;
+            A->ci[o] = j;
+            A->cv[o] = coef[(size_t)i * s->maxe + k];
+            o++;
+        }
+    }
+}
+
+static void csr_free(lvo_csr *A)
+{
+    free(A->rp);
+    free(A->ci);
+    free(A->cv);
+}
+
+static inline double csr_row(const lvo_csr *A, const double *xv, int i)
+{
+    double acc = A->diag[i] * xv[i];
+    for (int o = A->rp[i]; o < A->rp[i + 1]; o++) acc += A->cv[o] * (xv[i] - xv[A->ci[o]]);
+    return acc;
+}
+
+/* Reductions over a fixed partition of the rows into NCHUNK chunks, summed in chunk order, so that
+ * a solve is bitwise reproducible for any thread count. */
+#define NCHUNK 256
+#define CHUNK_LO(c, n) ((int)(((int64_t)(n) * (c)) / NCHUNK))
+
 /* Jacobi-PCG on the system of refresh!(A) (the build's replacement for Krylov.minres!, which the
  * reference calls at src/pressure.jl:219).  Stops at ||b - A x||_2 <= tol*||b||_2.  If every diagonal
  * entry is 0 (c2 = Inf), the operator is a graph Laplacian: b is projected on the complement of the
@@ -774,18 +830,20 @@ int lvo_pcg(const lvo_state *s, const double *diag, const double *coef, const do
             double tol, int itmax, double *relres_out)
 {
     int n = s->n;
+    lvo_csr A;
+    csr_build(s, diag, coef, &A);
     double *b = (double *)malloc(sizeof(double) * n);
     double *r = (double *)malloc(sizeof(double) * n);
     double *z = (double *)malloc(sizeof(double) * n);
     double *p = (double *)malloc(sizeof(double) * n);
     double *q = (double *)malloc(sizeof(double) * n);
     double *dinv = (double *)malloc(sizeof(double) * n);
+    double part[3][NCHUNK];
     int singular = 1;
     for (int i = 0; i < n; i++) {
         double dd = diag[i];
         if (dd != 0.0) singular = 0;
-        for (int k = 0; k < s->ne[i]; k++)
-            if (EL(s, i, k) >= 0) dd += coef[(size_t)i * s->maxe + k];
+        for (int o = A.rp[i]; o < A.rp[i + 1]; o++) dd += A.cv[o];
         dinv[i] = 1.0 / dd;
         b[i] = b_in[i];
     }
@@ -800,40 +858,70 @@ int lvo_pcg(const lvo_state *s, const double *diag, const double *coef, const do
     double bn = 0.0;
     for (int i = 0; i < n; i++) bn += b[i] * b[i];
     bn = sqrt(bn);
-    lvo_apply_A(s, diag, coef, x, q);
     double rz = 0.0, rn = 0.0;
-    for (int i = 0; i < n; i++) {
-        r[i] = b[i] - q[i];
-        z[i] = dinv[i] * r[i];
-        p[i] = z[i];
-        rz += r[i] * z[i];
-        rn += r[i] * r[i];
+#pragma omp parallel for schedule(static)
+    for (int c = 0; c < NCHUNK; c++) {
+        double a0 = 0.0, a1 = 0.0;
+        for (int i = CHUNK_LO(c, n); i < CHUNK_LO(c + 1, n); i++) {
+            r[i] = b[i] - csr_row(&A, x, i);
+            z[i] = dinv[i] * r[i];
+            p[i] = z[i];
+            a0 += r[i] * z[i];
+            a1 += r[i] * r[i];
+        }
+        part[0][c] = a0;
+        part[1][c] = a1;
     }
+    for (int c = 0; c < NCHUNK; c++) { rz += part[0][c]; rn += part[1][c]; }
     int it = 0;
     double thr = tol * bn;
     while (sqrt(rn) > thr && it < itmax) {
-        lvo_apply_A(s, diag, coef, p, q);
         double pq = 0.0;
-        for (int i = 0; i < n; i++) pq += p[i] * q[i];
+#pragma omp parallel for schedule(static)
+        for (int c = 0; c < NCHUNK; c++) {
+            double a0 = 0.0;
+            for (int i = CHUNK_LO(c, n); i < CHUNK_LO(c + 1, n); i++) {
+                q[i] = csr_row(&A, p, i);
+                a0 += p[i] * q[i];
+            }
+            part[0][c] = a0;
+        }
+        for (int c = 0; c < NCHUNK; c++) pq += part[0][c];
         if (!(pq > 0.0) || !(rz > 0.0)) break; /* breakdown at the attainable accuracy */
         double alpha = rz / pq;
         double rz2 = 0.0;
         rn = 0.0;
-        for (int i = 0; i < n; i++) {
-            x[i] += alpha * p[i];
-            r[i] -= alpha * q[i];
-            z[i] = dinv[i] * r[i];
-            rz2 += r[i] * z[i];
-            rn += r[i] * r[i];
+#pragma omp parallel for schedule(static)
+        for (int c = 0; c < NCHUNK; c++) {
+            double a0 = 0.0, a1 = 0.0;
+            for (int i = CHUNK_LO(c, n); i < CHUNK_LO(c + 1, n); i++) {
+                x[i] += alpha * p[i];
+                r[i] -= alpha * q[i];
+                z[i] = dinv[i] * r[i];
+                a0 += r[i] * z[i];
+                a1 += r[i] * r[i];
+            }
+            part[0][c] = a0;
+            part[1][c] = a1;
         }
+        for (int c = 0; c < NCHUNK; c++) { rz2 += part[0][c]; rn += part[1][c]; }
         double beta = rz2 / rz;
         rz = rz2;
+#pragma omp parallel for schedule(static)
         for (int i = 0; i < n; i++) p[i] = z[i] + beta * p[i];
         it++;
         if (it % 50 == 0 || !(sqrt(rn) > thr)) { /* true residual check against recurrence drift */
-            lvo_apply_A(s, diag, coef, x, q);
+#pragma omp parallel for schedule(static)
+            for (int c = 0; c < NCHUNK; c++) {
+                double a1 = 0.0;
+                for (int i = CHUNK_LO(c, n); i < CHUNK_LO(c + 1, n); i++) {
+                    double t = b[i] - csr_row(&A, x, i);
+                    a1 += t * t;
+                }
+                part[1][c] = a1;
+            }
             rn = 0.0;
-            for (int i = 0; i < n; i++) { double t = b[i] - q[i]; rn += t * t; }
+            for (int c = 0; c < NCHUNK; c++) rn += part[1][c];
         }
     }
     if (singular) {
@@ -845,6 +933,7 @@ int lvo_pcg(const lvo_state *s, const double *diag, const double *coef, const do
     if (relres_out) *relres_out = (bn > 0.0) ? sqrt(rn) / bn : sqrt(rn);
     int conv = !(sqrt(rn) > thr);
     free(b); free(r); free(z); free(p); free(q); free(dinv);
+    csr_free(&A);
     return conv ? it : -1;
 }
 

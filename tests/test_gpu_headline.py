@@ -1,0 +1,90 @@
+"""Parity at the benchmark's full size: Taylor-Green on the periodic unit square with 2048 x 2048 =
+4,194,304 particles, the seeded jittered lattice of the benchmark (seed 0x5EED0002).
+
+* remesh!: cell-list topology, edge order and every vertex bit-exact against the oracle, areas and
+  centroids to 1e-12 relative;
+* one full step! (tests/taylorgreen.jl:61-71: move!, stiffened_eos!, find_pressure! with 10 Picard
+  iterations, pressure_step!, find_D!, viscous_step!, find_dv!, relaxation_step!) through
+  lv_step_fused with the benchmark's solver (multilevel-preconditioned mixed-precision PCG with the
+  Picard window), against the oracle's step with FP64 Jacobi-PCG.  Both solve every Picard system
+  to a 1e-12 relative residual, so the fields agree to 1e-8 and the meshes after the step are
+  bit-identical in topology.
+
+The oracle runs on every host core; the step takes about a minute there.
+"""
+import time
+
+import numpy as np
+import pytest
+
+from pairs import rel_err
+
+pytestmark = pytest.mark.gpu
+
+K = 2048
+SEED = 0x5EED0002
+DT = 2.5e-4
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def headline(lv, oracle, gpu):
+    dr = 1.0 / K
+    x = lv.inputs.jittered_lattice(K, K, dr, seed=SEED)
+    _say("headline %d^2: GPU remesh" % K)
+    grid = lv.VoronoiGrid(lv.UnitRectangle(), dr, xperiodic=True, yperiodic=True, positions=x)
+    lv.remesh(grid)
+    _say("headline %d^2: oracle remesh" % K)
+    t = time.time()
+    og = oracle.Grid((0.0, 0.0), (1.0, 1.0), dr, xperiodic=True, yperiodic=True)
+    op = oracle.Particles(og, x)
+    op.remesh()
+    _say("headline %d^2: oracle remesh took %.1f s" % (K, time.time() - t))
+    return grid, op, x
+
+
+@pytest.mark.timeout(600)
+def test_headline_mesh_bit_exact(lv, oracle, headline):
+    grid, op, x = headline
+    pg, lg, vg = grid.mesh()
+    po, lo, vo = op.mesh()
+    assert pg.size == K * K + 1
+    assert np.array_equal(pg, po), "edge counts differ"
+    assert np.array_equal(lg, lo), "neighbour labels / edge order differ"
+    assert np.array_equal(vg, vo), "vertices differ"
+    assert rel_err(grid.get("area"), op.area()) <= 1e-12
+    assert rel_err(grid.get_soa("centroid"), op.centroid()) <= 1e-12
+    assert abs(op.area().sum() - 1.0) <= 1e-12
+    _say("headline mesh: %d edges bit-exact" % lg.size)
+
+
+@pytest.mark.timeout(900)
+def test_headline_full_step(lv, oracle, headline):
+    grid, op, x = headline
+    ic = lv.inputs.taylor_green(x, op.area())
+    for k in ("v", "rho", "mass", "P", "e", "mu"):
+        grid.set_soa(k, ic[k])
+        op.set(k, ic[k])
+    solver = lv.PressureSolver(grid, method="amg", tol=1e-12, picard_window=4, itmax=4000)
+    desc = lv.make_step_desc(DT, eos="stiffened", gamma=ic["gamma"], P0=ic["P0"], solver=solver)
+    _say("headline %d^2: GPU step" % K)
+    st = lv.step_fused(grid, desc)
+    assert max(st["relres"][:st["nsolves"]]) <= 1e-12, st
+    _say("headline %d^2: oracle step (FP64 Jacobi-PCG to 1e-12)" % K)
+    t = time.time()
+    iters = op.step(DT, eos="stiffened", gamma=ic["gamma"], P0=ic["P0"], tol=1e-12)
+    _say("headline %d^2: oracle step took %.1f s, %d PCG iterations" % (K, time.time() - t, int(np.sum(iters))))
+    assert np.all(iters >= 0)
+    errs = {}
+    for k in ("x", "v", "P", "e", "mass", "rho", "dv"):
+        errs[k] = rel_err(grid.get_soa(k), op[k])
+    _say("headline step errors: " + ", ".join("%s %.2e" % kv for kv in errs.items()))
+    for k, e in errs.items():
+        assert e <= 1e-8, (k, e)
+    pg, lg, vg = grid.mesh()
+    po, lo, vo = op.mesh()
+    assert np.array_equal(pg, po) and np.array_equal(lg, lo)
+    assert np.max(np.abs(vg - vo)) <= 1e-8 / K

@@ -125,6 +125,30 @@ def test_singular_solve_keeps_mean(lv, oracle, gpu):
     assert np.linalg.norm(r) <= 1.05e-10 * np.linalg.norm(bp)
 
 
+@pytest.mark.parametrize("method", ["mixed", "amg"])
+def test_refinement_skip_path_leaves_the_guess(lv, oracle, gpu, method):
+    """The refinement's first sweep is skipped when the initial residual already meets tol, is zero,
+    or is NaN; the skipped sweep must leave x as it is (no 1/sqrt(0), no NaN written into x)."""
+    _flush("skip path %s" % method)
+    grid, op, x, dr = make_pair(lv, oracle, "periodic", K=32, seed=0x5EED0024, ic="tg")
+    dt = 2.5e-4
+    lv.stiffened_eos(grid, 1.4, 2500.0 / 1.4)
+    solver = lv.PressureSolver(grid, method=method, tol=1e-8)
+    solver.assemble(dt)
+    n = grid.n
+    # b = 0 and x0 = 0: the residual is exactly zero, the solve is converged and x stays 0
+    z = solver.solve(np.zeros(n), np.zeros(n))
+    assert np.all(z == 0.0)
+    # a NaN in b: the residual is NaN, the solve reports failure and leaves the guess untouched
+    b, _ = solver.rhs(dt, False)
+    b[7] = np.nan
+    x0 = np.linspace(-1.0, 1.0, n)
+    with pytest.raises(lv.LVError):
+        solver.solve(b, x0)
+    xr = solver.solve(b, x0, raise_on_fail=False)
+    assert np.array_equal(xr, x0)
+
+
 def test_find_pressure_picard(lv, oracle, gpu):
     grid, op, x, dr = make_pair(lv, oracle, "periodic", K=48, seed=0x5EED0023, ic="tg")
     dt = 2.5e-4

@@ -302,7 +302,7 @@ class Particles:
 
     def step(self, dt, eos=None, gamma=1.4, P0=0.0, Pmin=0.0, gravity=None, niter=10, tol=1e-10, pvbc=None,
              viscous=True, artificial_viscosity=False, friction=None, alpha=1.0, rusanov=True, multiphase=None,
-             multiphase_tol=1e-4, multiphase_itmax=200, viscous_tol=1e-6):
+             multiphase_tol=1e-4, multiphase_itmax=200, viscous_tol=1e-6, itmax=100000):
         """step!(sim, t) in the order of tests/taylorgreen.jl:61-71 (optional parts as in
         examples/cavity.jl:70-80 and examples/rayleightaylor.jl:90-101; multiphase = the
         MultiphaseSolver's quality_threshold, or None for no multiphase_projection!)"""
@@ -313,7 +313,7 @@ class Particles:
             self.eos_stiffened(gamma, P0)
         elif eos == "ideal":
             self.eos_ideal(gamma, Pmin)
-        iters = self.find_pressure(dt, niter, pvbc, tol)
+        iters = self.find_pressure(dt, niter, pvbc, tol, itmax)
         self.pressure_step(dt)
         self.find_D()
         if viscous == "implicit":

@@ -7,10 +7,11 @@
   iterations, pressure_step!, find_D!, viscous_step!, find_dv!, relaxation_step!) through
   lv_step_fused with the benchmark's solver (multilevel-preconditioned mixed-precision PCG with the
   Picard window), against the oracle's step with FP64 Jacobi-PCG.  Both solve every Picard system
-  to a 1e-12 relative residual, so the fields agree to 1e-8 and the meshes after the step are
-  bit-identical in topology.
+  to a 1e-11 relative residual (FP64 Jacobi-PCG attains about 3e-12 on this system and stalls
+  there; two FP64 solves to 1e-11 from different guesses differ by about 2e-12 in P), so the fields
+  agree to 1e-8 and the meshes after the step are bit-identical in topology.
 
-The oracle runs on every host core; the step takes about a minute there.
+The oracle runs on every host core; its step takes about a minute.
 """
 import time
 
@@ -61,21 +62,22 @@ def test_headline_mesh_bit_exact(lv, oracle, headline):
     _say("headline mesh: %d edges bit-exact" % lg.size)
 
 
-@pytest.mark.timeout(900)
+@pytest.mark.timeout(600)
 def test_headline_full_step(lv, oracle, headline):
     grid, op, x = headline
     ic = lv.inputs.taylor_green(x, op.area())
     for k in ("v", "rho", "mass", "P", "e", "mu"):
         grid.set_soa(k, ic[k])
         op.set(k, ic[k])
-    solver = lv.PressureSolver(grid, method="amg", tol=1e-12, picard_window=4, itmax=4000)
+    solver = lv.PressureSolver(grid, method="amg", tol=1e-11, picard_window=4, itmax=4000)
     desc = lv.make_step_desc(DT, eos="stiffened", gamma=ic["gamma"], P0=ic["P0"], solver=solver)
     _say("headline %d^2: GPU step" % K)
     st = lv.step_fused(grid, desc)
-    assert max(st["relres"][:st["nsolves"]]) <= 1e-12, st
-    _say("headline %d^2: oracle step (FP64 Jacobi-PCG to 1e-12)" % K)
+    assert max(st["relres"]) <= 1e-11, st
+    _say("headline %d^2: GPU step done, %d inner iterations; oracle step (FP64 Jacobi-PCG to 1e-11)"
+         % (K, st["total_iters"]))
     t = time.time()
-    iters = op.step(DT, eos="stiffened", gamma=ic["gamma"], P0=ic["P0"], tol=1e-12)
+    iters = op.step(DT, eos="stiffened", gamma=ic["gamma"], P0=ic["P0"], tol=1e-11, itmax=20000)
     _say("headline %d^2: oracle step took %.1f s, %d PCG iterations" % (K, time.time() - t, int(np.sum(iters))))
     assert np.all(iters >= 0)
     errs = {}

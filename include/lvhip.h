@@ -233,6 +233,13 @@ int lv_nearest_polygon(lv_ctx *ctx, int32_t m, const double *q, int32_t *idx);
 int lv_movingls(lv_ctx *ctx, int32_t order, int32_t field, int32_t comp, double h, double *coeffs);
 /* point_value(grid, x, p -> field[comp]), src/movingls.jl:140-145, at m query points */
 int lv_point_values(lv_ctx *ctx, int32_t m, const double *q, int32_t field, int32_t comp, double *out);
+/* lv_point_values without any allocation.  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (tight serving loops of probe queries, allocation-free harnesses).
+ * The queries and values of one call are staged in a static per-device array of 1.5 MiB, so m is at
+ * most LV_NOALLOC_MAX_QUERIES (LV_E_INVALID beyond).  Calls that overlap on one GPU (contexts on
+ * several threads) race on that array; results are identical to lv_point_values. */
+#define LV_NOALLOC_MAX_QUERIES 65536
+int lv_point_values_noalloc(lv_ctx *ctx, int32_t m, const double *q, int32_t field, int32_t comp, double *out);
 
 /* ---- output (src/IO.jl:9-71) ------------------------------------------------------------------ */
 /* export_grid(grid, filename, vars...), src/IO.jl:9-34: the polygons as VTK XML PolyData (.vtp is

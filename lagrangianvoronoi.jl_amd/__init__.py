@@ -148,6 +148,7 @@ _SIGNATURES = {
     "lv_nearest_polygon": [_P, _I, _P, _P],
     "lv_movingls": [_P, _I, _I, _I, _D, _P],
     "lv_point_values": [_P, _I, _P, _I, _I, _P],
+    "lv_point_values_noalloc": [_P, _I, _P, _I, _I, _P],
     "lv_export_grid": [_P, ctypes.c_char_p, _P, _I, _I],
     "lv_export_points": [_P, ctypes.c_char_p, _P, _I, _I],
     "lv_timing_enable": [_P, _I, _I],
@@ -700,6 +701,19 @@ def point_value(grid, x, field):
     m, q = _queries(x)
     out = np.empty(m)
     _check(grid.ctx, lib().lv_point_values(grid.ctx, m, _ptr(q), FIELDS[name][0], comp, _ptr(out)))
+    return out[0] if np.ndim(x) == 1 else out
+
+
+NOALLOC_MAX_QUERIES = 65536
+
+
+def point_value_noalloc(grid, x, field):
+    """point_value at up to NOALLOC_MAX_QUERIES points without any device allocation (the queries
+    and values are staged in the library's static per-device array; same values as point_value)"""
+    name, comp = _field_component(field)
+    m, q = _queries(x)
+    out = np.empty(m)
+    _check(grid.ctx, lib().lv_point_values_noalloc(grid.ctx, m, _ptr(q), FIELDS[name][0], comp, _ptr(out)))
     return out[0] if np.ndim(x) == 1 else out
 
 

@@ -10,6 +10,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -86,6 +87,10 @@ struct lv_ctx {
     hipEvent_t sev[2 * NSEV] = {};
     double *sev_out[NSEV] = {};
     int sev_ready = 0, sev_n = 0;
+    // the device the context was created on, and that device's address of the allocation-free
+    // entry point's staging array
+    int device = 0;
+    double *scratch = nullptr;
     char msg[512] = {0};
 };
 
@@ -260,6 +265,40 @@ const char *error_text(int code)
 
 }  // namespace
 
+// ---- staging of the allocation-free entry point ------------------------------------------------------
+
+// lv_point_values_noalloc stages one call's queries (2 m doubles) and values (m doubles) here, for at
+// most LV_NOALLOC_MAX_QUERIES queries (1.5 MiB).  Each device the library runs on holds its own copy
+// until the process ends, and every call on a device shares it.  In host code the name is only a
+// stand-in for the symbol: kernels and copies take the device address that hipGetSymbolAddress
+// returns for the current device (scratch_address).
+static __device__ __align__(256) double scratch[3 * (size_t)LV_NOALLOC_MAX_QUERIES];
+
+namespace {
+
+std::mutex g_scratch_mu;
+std::vector<double *> g_scratch_addr;  // per device id; null until resolved
+
+// The device address of `scratch` on `device`, which must be the current device; resolved once per
+// device (under a lock, since contexts may be created from several threads).
+hipError_t scratch_address(int device, double **out)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (device < 0) return hipErrorInvalidDevice;
+    if ((size_t)device >= g_scratch_addr.size()) g_scratch_addr.resize((size_t)device + 1, nullptr);
+    if (!g_scratch_addr[device]) {
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        if (!p) return hipErrorInvalidSymbol;
+        g_scratch_addr[device] = (double *)p;
+    }
+    *out = g_scratch_addr[device];
+    return hipSuccess;
+}
+
+}  // namespace
+
 extern "C" {
 
 int lv_create(const lv_grid_desc *d, lv_ctx **out)
@@ -304,7 +343,9 @@ int lv_create(const lv_grid_desc *d, lv_ctx **out)
     ctx->cap = d->capacity;
     const size_t cap = (size_t)d->capacity;
     const int64_t ncells = (int64_t)g.n1 * g.n2;
-    hipError_t e = hipStreamCreateWithFlags(&ctx->s, hipStreamNonBlocking);
+    hipError_t e = hipGetDevice(&ctx->device);
+    if (e == hipSuccess) e = scratch_address(ctx->device, &ctx->scratch);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->s, hipStreamNonBlocking);
 #define ALLOC(p, cnt)                                   \
     if (e == hipSuccess) e = dalloc(&(p), (size_t)(cnt));
     ALLOC(ctx->F.x, 2 * cap);
@@ -869,6 +910,34 @@ int lv_point_values(lv_ctx *ctx, int32_t m, const double *q, int32_t field, int3
     LV_HIP_CHECK(lv_launch_point_value(ctx->g, ctx->F.x, ctx->n, ctx->cell_start, ctx->cell_labels, ctx->path_i,
                                        ctx->path_rr, m, (const double *)dq.p, f, ctx->g.h, (double *)dout.p, ctx->s));
     LV_HIP_CHECK(hipMemcpyAsync(out, dout.p, sizeof(double) * m, hipMemcpyDeviceToHost, ctx->s));
+    LV_HIP_CHECK(hipStreamSynchronize(ctx->s));
+    for (int t = 0; t < m; t++)
+        if (out[t] != out[t]) return lv_fail(ctx, LV_E_INVALID, "nearest polygon beyond r_max of a query point");
+    return LV_OK;
+}
+
+// lv_point_values with the queries and values staged in the static `scratch` array instead of
+// device buffers allocated per call: no allocation of any kind, for callers that may not allocate.
+// Concurrent calls on one device race on that array (calls on one context are serialised anyway).
+int lv_point_values_noalloc(lv_ctx *ctx, int32_t m, const double *q, int32_t field, int32_t comp, double *out)
+{
+    int rc = check_ctx(ctx, true);
+    if (rc) return rc;
+    if (m < 0 || (m > 0 && (!q || !out))) return lv_fail(ctx, LV_E_INVALID, "lv_point_values_noalloc");
+    if (m > LV_NOALLOC_MAX_QUERIES)
+        return lv_fail(ctx, LV_E_INVALID, "lv_point_values_noalloc: more than LV_NOALLOC_MAX_QUERIES (65536) "
+                                          "query points in one call");
+    if ((rc = need_cell_list(ctx))) return rc;
+    const double *f = field_plane(ctx, field, comp);
+    if (!f) return lv_fail(ctx, LV_E_INVALID, "lv_point_values_noalloc: not a Float64 field component");
+    if (m == 0) return LV_OK;
+    if (!ctx->scratch) return lv_fail(ctx, LV_E_HIP, "lv_point_values_noalloc: staging array not resolved");
+    double *dq = ctx->scratch;                       // q1[m], q2[m]
+    double *dout = ctx->scratch + 2 * (size_t)m;     // the m values
+    LV_HIP_CHECK(hipMemcpyAsync(dq, q, sizeof(double) * 2 * m, hipMemcpyHostToDevice, ctx->s));
+    LV_HIP_CHECK(lv_launch_point_value(ctx->g, ctx->F.x, ctx->n, ctx->cell_start, ctx->cell_labels, ctx->path_i,
+                                       ctx->path_rr, m, (const double *)dq, f, ctx->g.h, dout, ctx->s));
+    LV_HIP_CHECK(hipMemcpyAsync(out, dout, sizeof(double) * m, hipMemcpyDeviceToHost, ctx->s));
     LV_HIP_CHECK(hipStreamSynchronize(ctx->s));
     for (int t = 0; t < m; t++)
         if (out[t] != out[t]) return lv_fail(ctx, LV_E_INVALID, "nearest polygon beyond r_max of a query point");

@@ -36,7 +36,7 @@ CONFIGS = {
     "tg2048": None,
     # Taylor-Green 1024^2, the plain method in FP64: Jacobi-PCG to 1e-10, no initial-guess extrapolation
     "tg1024": dict(nx=1024, ny=1024, ymax=1.0, periodic=True, ic="tg", dt=2.5e-4, solver="pcg64", tol=1e-10,
-                   window=0, steps=5, warmup=1),
+                   window=0, steps=50, warmup=1),
     # Gresho vortex 100^2, incompressible, the plain method in FP64, 100 steps
     "gresho100": dict(nx=100, ny=100, ymax=1.0, periodic=True, ic="gresho", dt=1e-3, solver="pcg64", tol=1e-10,
                       window=0, steps=100, warmup=0, viscous=False),
@@ -47,7 +47,7 @@ CONFIGS = {
     # lid-driven cavity 4096^2, Re = 1000, incompressible, lid u = 1 (dt = min(0.1 dr, 0.1 Re dr^2),
     # examples/cavity.jl:32)
     "ldc4096": dict(nx=4096, ny=4096, ymax=1.0, periodic=False, ic="cavity", dt=min(0.1 / 4096, 100.0 / 4096 ** 2),
-                    solver="amg", tol=1e-8, window=8, steps=3, warmup=1),
+                    solver="amg", tol=1e-8, window=8, steps=200, warmup=1),
     # examples/cavity.jl in full: Re = 100, N = 100, populate_lloyd!, t_end = 0.1 Re, then the
     # centreline profiles by point_value against the published data the example plots against
     "ldc100-profiles": None,
@@ -217,7 +217,10 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-compare", action="store_true",
                     help="skip the FP64 / mixed / learned-preconditioner solve comparison")
-    ap.add_argument("--cpu-K", type=int, default=1024, help="lattice side of the CPU sample (one full step)")
+    ap.add_argument("--cpu-K", type=int, default=2048, help="lattice side of the CPU sample (one full step)")
+    ap.add_argument("--err-steps", type=int, default=2,
+                    help="steps after the timed ones whose pressure is compared with an FP64 solve of the same "
+                         "systems (untimed)")
     ap.add_argument("--config", default="tg2048", choices=sorted(CONFIGS),
                     help="tg2048: the benchmark line; the others run a configuration family at full size and "
                          "print its own line (steps/s, iterations, fallbacks, device memory in use)")
@@ -327,6 +330,34 @@ def setup(lv, K, seed=0x5EED0002):
     for k in ("v", "rho", "mass", "P", "e", "mu"):
         grid.set_soa(k, ic[k])
     return grid, x, ic
+
+
+STATE_FIELDS = ("x", "v", "dv", "momentum", "rho", "e", "P", "c2", "mass", "energy", "quality", "mu", "D", "phase")
+
+
+def fp64_error(lv, grid, desc, steps, tol64=1e-10):
+    """Config 3's error against an FP64 solve, per step, outside the timed region: each step runs
+    twice from the same state, once with the benchmark's mixed-precision solver and once with FP64
+    Jacobi-PCG to tol64 (the plain method), and the pressures after find_pressure! are compared; the
+    run continues from the mixed-precision step."""
+    solver64 = lv.PressureSolver(grid, method="pcg64", tol=tol64, itmax=20000)
+    desc64 = lv.make_step_desc(desc.dt, eos="stiffened", gamma=desc.gamma, P0=desc.P0, solver=solver64,
+                               viscous=True, artificial_viscosity=False, alpha=1.0, rusanov=True)
+    rows = []
+    for _ in range(steps):
+        state = {k: grid.get_soa(k) for k in STATE_FIELDS}
+        st64 = lv.step_fused(grid, desc64)  # raises if an FP64 solve misses tol64
+        P64, v64 = grid.get_soa("P"), grid.get_soa("v")
+        for k in STATE_FIELDS:
+            grid.set_soa(k, state[k])
+        lv.remesh(grid)
+        st = lv.step_fused(grid, desc)
+        P, v = grid.get_soa("P"), grid.get_soa("v")
+        rows.append(dict(P_rel_l2=float(np.linalg.norm(P - P64) / np.linalg.norm(P64)),
+                         P_rel_max=float(np.max(np.abs(P - P64)) / np.max(np.abs(P64))),
+                         v_rel_max=float(np.max(np.abs(v - v64)) / np.max(np.abs(v64))),
+                         max_relres=max(st["relres"]), max_relres_fp64=max(st64["relres"])))
+    return dict(fp64_solver="pcg64", fp64_tol=tol64, steps=rows)
 
 
 def traffic_from_profiles(kernel):
@@ -452,7 +483,9 @@ def main():
                     algorithmic_bytes_per_launch=bpl.value)
     out = dict(metric=METRIC, value=round(value, 1), unit="particle-steps/s", n_gpus=1, steps=args.steps,
                warmup=args.warmup, ms_per_step=round(elapsed / args.steps * 1e3, 3), higher_is_better=True,
-               scaling="weak", vs_baseline=None, dtype="f64 geometry/stencils + f32 inner PCG (f64 dots/refinement)",
+               scaling="weak", vs_baseline=None,
+               dtype="f64 geometry/stencils/dots/refinement; f32 inner PCG (SpMV/AXPY); bf16 couplings in the "
+                     "multilevel cycle's fine smoothing and residuals",
                data="synthetic",
                config=dict(workload="taylor-green %dx%d periodic, jitter +-0.1dr seed 0x5EED0002, dt=2.5e-4, "
                                     "c0=50 stiffened, nu=1e-3, 10 Picard solves/step, %s PCG to %g" %
@@ -462,6 +495,8 @@ def main():
                pressure_solve_ms=round(first_solve_ms, 3), pressure_solve_iters=first_solve_it,
                max_relres=relres, picard_iters=stats[-1]["iters"], picard_sweeps=stats[-1]["outer"],
                picard_relres0=[float("%.3g" % r) for r in stats[-1]["relres0"]], roofline=roofline)
+    if args.err_steps > 0:
+        out["error_vs_fp64_solve"] = fp64_error(lv, grid, desc, args.err_steps)
     if not args.no_compare:
         out["pressure_solve_compare"] = solve_compare(lv, grid, ic, dt, args.tol, args.inner_tol)
     if not args.no_cpu:

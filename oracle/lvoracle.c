@@ -937,10 +937,100 @@ int lvo_pcg(const lvo_state *s, const double *diag, const double *coef, const do
     return conv ? it : -1;
 }
 
+/* MINRES (Paige & Saunders 1975, unpreconditioned, in the recurrences of their SYMMLQ/MINRES paper as
+ * SciPy's minres also writes them), warm-started from x: the solver the reference calls at
+ * src/pressure.jl:219 (Krylov.jl minres!, atol = rtol = 1e-6, itmax = 1000).  It stops when the
+ * residual estimate |phibar| <= atol + rtol*beta1 (beta1 = |b - A x0|, the rule Krylov.jl documents
+ * for minres!) or when |A r| / (|A| |r|) <= rtol (a least-squares solution of an inconsistent singular
+ * system).  Parity unpinned: Krylov.jl 0.9.8 is not available to compare iterates with; this is for
+ * studying the effect of the reference's loose solve, not a parity target.  Returns iterations, or
+ * -1 when itmax stopped it; *relres_out = |b - A x| / |b| of the returned x. */
+int lvo_minres(const lvo_state *s, const double *diag, const double *coef, const double *b, double *x,
+               double atol, double rtol, int itmax, double *relres_out)
+{
+    int n = s->n;
+    lvo_csr A;
+    csr_build(s, diag, coef, &A);
+    double *r1 = (double *)malloc(sizeof(double) * n), *r2 = (double *)malloc(sizeof(double) * n);
+    double *y = (double *)malloc(sizeof(double) * n), *v = (double *)malloc(sizeof(double) * n);
+    double *w = (double *)calloc(n, sizeof(double)), *w1 = (double *)calloc(n, sizeof(double));
+    double *w2 = (double *)calloc(n, sizeof(double));
+    double bn = 0.0, beta1 = 0.0;
+    for (int i = 0; i < n; i++) {
+        r1[i] = b[i] - csr_row(&A, x, i);
+        bn += b[i] * b[i];
+        beta1 += r1[i] * r1[i];
+    }
+    bn = sqrt(bn);
+    beta1 = sqrt(beta1);
+    for (int i = 0; i < n; i++) { y[i] = r1[i]; r2[i] = r1[i]; }
+    double oldb = 0.0, beta = beta1, dbar = 0.0, epsln = 0.0, phibar = beta1, tnorm2 = 0.0;
+    double cs = -1.0, sn = 0.0;
+    const double eps_stop = atol + rtol * beta1;
+    int it = 0, done = !(beta1 > eps_stop);
+    while (!done && it < itmax) {
+        it++;
+        double sc = 1.0 / beta;
+        for (int i = 0; i < n; i++) v[i] = sc * y[i];
+        double alfa = 0.0;
+        for (int i = 0; i < n; i++) y[i] = csr_row(&A, v, i);
+        if (it >= 2)
+            for (int i = 0; i < n; i++) y[i] -= (beta / oldb) * r1[i];
+        for (int i = 0; i < n; i++) alfa += v[i] * y[i];
+        for (int i = 0; i < n; i++) {
+            y[i] -= (alfa / beta) * r2[i];
+            r1[i] = r2[i];
+            r2[i] = y[i];
+        }
+        oldb = beta;
+        beta = 0.0;
+        for (int i = 0; i < n; i++) beta += r2[i] * r2[i];
+        beta = sqrt(beta);
+        tnorm2 += alfa * alfa + oldb * oldb + beta * beta;
+        double oldeps = epsln;
+        double delta = cs * dbar + sn * alfa;
+        double gbar = sn * dbar - cs * alfa;
+        epsln = sn * beta;
+        dbar = -cs * beta;
+        double root = sqrt(gbar * gbar + dbar * dbar);
+        double gamma = sqrt(gbar * gbar + beta * beta);
+        if (gamma < 2.220446049250313e-16) gamma = 2.220446049250313e-16;
+        cs = gbar / gamma;
+        sn = beta / gamma;
+        double phi = cs * phibar;
+        phibar = sn * phibar;
+        double denom = 1.0 / gamma;
+        for (int i = 0; i < n; i++) {
+            w1[i] = w2[i];
+            w2[i] = w[i];
+            w[i] = (v[i] - oldeps * w1[i] - delta * w2[i]) * denom;
+            x[i] += phi * w[i];
+        }
+        double Anorm = sqrt(tnorm2);
+        double test2 = (Anorm > 0.0) ? root / Anorm : INFINITY;
+        if (fabs(phibar) <= eps_stop || test2 <= rtol || !(beta > 0.0)) done = 1;
+    }
+    double rn = 0.0;
+    for (int i = 0; i < n; i++) { double t = b[i] - csr_row(&A, x, i); rn += t * t; }
+    if (relres_out) *relres_out = (bn > 0.0) ? sqrt(rn) / bn : sqrt(rn);
+    free(r1); free(r2); free(y); free(v); free(w); free(w1); free(w2);
+    csr_free(&A);
+    return done ? it : -1;
+}
+
 /* find_pressure!, src/pressure.jl:215-224: refresh!(A), then niter x [RHS, solve warm-started from P,
- * write P back]. */
+ * write P back].  method 0: Jacobi-PCG to tol (the build's solver); 1: lvo_minres with
+ * atol = rtol = tol (the reference's solver and tolerances). */
+int lvo_find_pressure_m(const lvo_grid *g, lvo_state *s, double dt, int niter, const double *vbc, double tol,
+                        int itmax, int *iters_out, int method);
 int lvo_find_pressure(const lvo_grid *g, lvo_state *s, double dt, int niter, const double *vbc, double tol,
                       int itmax, int *iters_out)
+{
+    return lvo_find_pressure_m(g, s, dt, niter, vbc, tol, itmax, iters_out, 0);
+}
+
+int lvo_find_pressure_m(const lvo_grid *g, lvo_state *s, double dt, int niter, const double *vbc, double tol,
+                        int itmax, int *iters_out, int method)
 {
     int n = s->n;
     double *diag = (double *)malloc(sizeof(double) * n);
@@ -953,7 +1043,8 @@ int lvo_find_pressure(const lvo_grid *g, lvo_state *s, double dt, int niter, con
     for (int it = 1; it <= niter; it++) {
         lvo_rhs(g, s, dt, it > 1, vbc, b, P, GP);
         double rr;
-        int k = lvo_pcg(s, diag, coef, b, P, tol, itmax, &rr);
+        int k = method == 1 ? lvo_minres(s, diag, coef, b, P, tol, tol, itmax, &rr)
+                            : lvo_pcg(s, diag, coef, b, P, tol, itmax, &rr);
         if (iters_out) iters_out[it - 1] = k;
         if (k < 0) err = LV_E_NOT_CONVERGED;
         for (int i = 0; i < n; i++) s->P[i] = P[i];

@@ -58,6 +58,7 @@ def lib():
             "lvo_pressure_operator": ([P, P, D, P, P], I), "lvo_apply_A": ([P, P, P, P, P], I),
             "lvo_rhs": ([P, P, D, I, P, P, P, P], I), "lvo_pcg": ([P, P, P, P, P, D, I, P], I),
             "lvo_find_pressure": ([P, P, D, I, P, D, I, P], I), "lvo_pressure_step": ([P, P, D], I),
+            "lvo_find_pressure_m": ([P, P, D, I, P, D, I, P, I], I), "lvo_minres": ([P, P, P, P, P, D, D, I, P], I),
             "lvo_find_D": ([P, P], I), "lvo_viscous_step": ([P, P, D, I], I), "lvo_bdary_friction": ([P, P, D], I),
             "lvo_find_dv": ([P, P, D, D], I), "lvo_relaxation_step": ([P, P, D, I], I),
             "lvo_multiphase_rhs": ([P, P, P], I), "lvo_multiphase_apply": ([P, P, P, P, P], I),
@@ -241,12 +242,25 @@ class Particles:
                            ctypes.byref(rr))
         return x, it, rr.value
 
-    def find_pressure(self, dt, niter=10, vbc=None, tol=1e-10, itmax=100000):
+    def find_pressure(self, dt, niter=10, vbc=None, tol=1e-10, itmax=100000, method="pcg"):
+        """method "pcg": Jacobi-PCG to tol (relative to |b|); "minres": MINRES with atol = rtol = tol, as
+        the reference calls Krylov.jl's minres! (src/pressure.jl:219); a missed tolerance is returned
+        as -1 iterations, not raised, since the reference ignores it too"""
         iters = np.zeros(niter, dtype=np.int32)
         vb = None if vbc is None else np.ascontiguousarray(vbc, dtype=np.float64)
-        rc = lib().lvo_find_pressure(self.grid.ptr, ctypes.byref(self.s), dt, niter, _p(vb), tol, itmax, _p(iters))
-        _chk(rc)
+        m = {"pcg": 0, "minres": 1}[method]
+        rc = lib().lvo_find_pressure_m(self.grid.ptr, ctypes.byref(self.s), dt, niter, _p(vb), tol, itmax, _p(iters),
+                                       m)
+        if m == 0:
+            _chk(rc)
         return iters
+
+    def minres(self, diag, coef, b, x0, atol, rtol, itmax=1000):
+        x = np.ascontiguousarray(np.array(x0, dtype=np.float64))
+        rr = ctypes.c_double()
+        it = lib().lvo_minres(ctypes.byref(self.s), _p(diag), _p(coef), _p(np.ascontiguousarray(b)), _p(x), atol,
+                              rtol, itmax, ctypes.byref(rr))
+        return x, it, rr.value
 
     def pressure_step(self, dt):
         lib().lvo_pressure_step(self.grid.ptr, ctypes.byref(self.s), dt)
@@ -302,7 +316,7 @@ class Particles:
 
     def step(self, dt, eos=None, gamma=1.4, P0=0.0, Pmin=0.0, gravity=None, niter=10, tol=1e-10, pvbc=None,
              viscous=True, artificial_viscosity=False, friction=None, alpha=1.0, rusanov=True, multiphase=None,
-             multiphase_tol=1e-4, multiphase_itmax=200, viscous_tol=1e-6, itmax=100000):
+             multiphase_tol=1e-4, multiphase_itmax=200, viscous_tol=1e-6, itmax=100000, solver="pcg"):
         """step!(sim, t) in the order of tests/taylorgreen.jl:61-71 (optional parts as in
         examples/cavity.jl:70-80 and examples/rayleightaylor.jl:90-101; multiphase = the
         MultiphaseSolver's quality_threshold, or None for no multiphase_projection!)"""
@@ -313,7 +327,7 @@ class Particles:
             self.eos_stiffened(gamma, P0)
         elif eos == "ideal":
             self.eos_ideal(gamma, Pmin)
-        iters = self.find_pressure(dt, niter, pvbc, tol, itmax)
+        iters = self.find_pressure(dt, niter, pvbc, tol, itmax, solver)
         self.pressure_step(dt)
         self.find_D()
         if viscous == "implicit":

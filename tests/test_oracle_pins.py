@@ -269,6 +269,35 @@ def test_operator_symmetric_rowsum_spd(lv, oracle):
     assert np.max(np.abs(xo - xs)) <= 1e-9 * np.max(np.abs(xs))
 
 
+def test_minres_iterates_match_scipy(lv, oracle):
+    """The oracle's MINRES (the reference's solver, src/pressure.jl:219) against scipy's independent
+    implementation of the same Paige-Saunders recurrences: equal iterates after a fixed number of
+    steps from the same warm start, and the dense solution once converged."""
+    from scipy.sparse.linalg import minres as sp_minres
+
+    g, p, x, dr, ic = tg_state(lv, oracle, "periodic", 12, 0x5EED0109)
+    p.eos_stiffened(ic["gamma"], ic["P0"])
+    dt = 1e-2
+    diag, coef = p.pressure_operator(dt)
+    d, rp, cols, vals = p.csr(diag, coef)
+    n = p.n
+    A = np.zeros((n, n))
+    for i in range(n):
+        for k in range(rp[i], rp[i + 1]):
+            A[i, i] += vals[k]
+            A[i, cols[k]] -= vals[k]
+        A[i, i] += d[i]
+    b, P0, GP = p.rhs(dt, False)
+    for k in (1, 3, 10, 25):
+        xo, it, _ = p.minres(diag, coef, b, P0, 0.0, 0.0, itmax=k)
+        xs, _ = sp_minres(A, b, x0=P0.copy(), rtol=0.0, maxiter=k)
+        assert it == -1  # stopped by itmax
+        assert np.max(np.abs(xo - xs)) <= 1e-10 * np.max(np.abs(xs)), k
+    xo, it, rr = p.minres(diag, coef, b, P0, 0.0, 1e-12, itmax=2000)
+    assert it > 0 and rr <= 1e-11
+    assert np.max(np.abs(xo - np.linalg.solve(A, b))) <= 1e-9 * np.max(np.abs(xo))
+
+
 def test_pressure_step_linear_pressure_closed_form(lv, oracle):
     """For P = g.x, every interior cell gets dv = -dt*area*g/m exactly (divergence theorem, midpoint
     rule exact on straight edges)."""

@@ -56,6 +56,8 @@ struct lv_ctx {
     lv_pressure_system A{};
     lv_solver_work W{};
     double *b = nullptr, *P0 = nullptr, *GP = nullptr;
+    lv_rhs_geom rg{};         // the P-independent parts of refresh!(solver) (allocated on first use)
+    bool rg_alloc = false;
     int *deg = nullptr;
     void *scan_temp = nullptr;
     int *d_flags = nullptr;   // [0] error word, [1] nonsingular flag
@@ -464,6 +466,7 @@ int lv_destroy(lv_ctx *ctx)
     dfree(ctx->pw_t1);
     dfree(ctx->pw_t2);
     dfree(ctx->pw_dots);
+    for (double *p : {ctx->rg.w1, ctx->rg.w2, ctx->rg.u1, ctx->rg.u2, ctx->rg.bdiv, ctx->rg.den}) dfree(p);
     for (int k = 0; k < LV_NKTIMERS; k++) {
         if (!ctx->timers[k]) continue;
         for (int q = 0; q < 2 * LV_TPOOL; q++)
@@ -1091,14 +1094,41 @@ int lv_get_pressure_operator(lv_ctx *ctx, double *diag, int32_t *row_ptr, int32_
     return LV_OK;
 }
 
+static int ensure_rhs_geom(lv_ctx *ctx)
+{
+    if (ctx->rg_alloc) return LV_OK;
+    const size_t cap = (size_t)ctx->cap;
+    LV_HIP_CHECK(dalloc(&ctx->rg.w1, (size_t)LV_MAXE * cap));
+    LV_HIP_CHECK(dalloc(&ctx->rg.w2, (size_t)LV_MAXE * cap));
+    LV_HIP_CHECK(dalloc(&ctx->rg.u1, (size_t)LV_MAXE * cap));
+    LV_HIP_CHECK(dalloc(&ctx->rg.u2, (size_t)LV_MAXE * cap));
+    LV_HIP_CHECK(dalloc(&ctx->rg.bdiv, cap));
+    LV_HIP_CHECK(dalloc(&ctx->rg.den, cap));
+    ctx->rg_alloc = true;
+    return LV_OK;
+}
+
+// the pressure operator of the current mesh is assembled (its ELL slots carry the RHS geometry)
+static bool pressure_ell_ready(lv_ctx *ctx) { return ctx->A.valid && ctx->A_kind == 0; }
+
 int lv_pressure_rhs(lv_ctx *ctx, double dt, int32_t gp_step, const double vbc[8], double *b, double *GP)
 {
     int rc = check_ctx(ctx, true);
     if (rc) return rc;
     lv_vbc v;
     for (int k = 0; k < 8; k++) v.v[k] = vbc ? vbc[k] : 0.0;
-    LV_HIP_CHECK(lv_launch_rhs(ctx->g, ctx->F, mesh_view(ctx), ctx->n, dt, gp_step, v, ctx->b, ctx->P0, ctx->GP,
-                               ctx->s));
+    if (pressure_ell_ready(ctx)) {  // the split evaluation find_pressure! uses
+        rc = ensure_rhs_geom(ctx);
+        if (rc) return rc;
+        const int n = ctx->n;
+        LV_HIP_CHECK(lv_launch_rhs_geom(ctx->g, ctx->F, mesh_view(ctx), n, dt, v, ctx->A, ctx->rg, ctx->s));
+        LV_HIP_CHECK(lv_launch_gp(n, ctx->A, ctx->rg, ctx->F.P, ctx->F.mass, ctx->GP, nullptr, nullptr, ctx->s));
+        LV_HIP_CHECK(lv_launch_rhs_b(n, gp_step, ctx->A, ctx->rg, ctx->GP, ctx->F.area, ctx->F.P, ctx->b, ctx->P0,
+                                     ctx->s));
+    } else {
+        LV_HIP_CHECK(lv_launch_rhs(ctx->g, ctx->F, mesh_view(ctx), ctx->n, dt, gp_step, v, ctx->b, ctx->P0, ctx->GP,
+                                   ctx->s));
+    }
     if (b) LV_HIP_CHECK(hipMemcpyAsync(b, ctx->b, sizeof(double) * ctx->n, hipMemcpyDeviceToHost, ctx->s));
     if (GP) LV_HIP_CHECK(hipMemcpyAsync(GP, ctx->GP, sizeof(double) * 2 * ctx->n, hipMemcpyDeviceToHost, ctx->s));
     LV_HIP_CHECK(hipStreamSynchronize(ctx->s));
@@ -1385,11 +1415,25 @@ int lv_find_pressure(lv_ctx *ctx, double dt, int32_t niter, const lv_solve_opts 
         rc = ensure_picard_window(ctx, w);
         if (rc) return rc;
     }
+    rc = ensure_rhs_geom(ctx);
+    if (rc) return rc;
+    LV_HIP_CHECK(lv_launch_rhs_geom(ctx->g, ctx->F, mesh_view(ctx), ctx->n, dt, v, ctx->A, ctx->rg, ctx->s));
     int cnt = 0, have = 0;  // directions in the window; finished Picard iterates remembered (<= 2)
     int status = LV_OK;
     for (int it = 1; it <= niter; it++) {
-        LV_HIP_CHECK(lv_launch_rhs(ctx->g, ctx->F, mesh_view(ctx), ctx->n, dt, it > 1, v, ctx->b, ctx->P0, ctx->GP,
-                                   ctx->s));
+        // refresh!(solver, dt, it > 1, boundary_velocity): the first iteration reads p.P (and takes it
+        // as the initial guess); later ones read the previous solution in P0, write it back into the
+        // particles (p.P = P, src/pressure.jl:221-223) and add the GP term
+        if (it == 1) {
+            LV_HIP_CHECK(lv_launch_rhs_b(ctx->n, 0, ctx->A, ctx->rg, ctx->GP, ctx->F.area, ctx->F.P, ctx->b, ctx->P0,
+                                         ctx->s));
+        } else {
+            // (with the Picard window, also its record of the last iterate, pw_P[0])
+            LV_HIP_CHECK(lv_launch_gp(ctx->n, ctx->A, ctx->rg, ctx->P0, ctx->F.mass, ctx->GP, ctx->F.P,
+                                      (w > 0 && have >= 1) ? ctx->pw_P[0] : nullptr, ctx->s));
+            LV_HIP_CHECK(lv_launch_rhs_b(ctx->n, 1, ctx->A, ctx->rg, ctx->GP, ctx->F.area, ctx->P0, ctx->b, nullptr,
+                                         ctx->s));
+        }
         if (w > 0 && have >= 1) {
             if (have >= 2) {
                 rc = picard_add_direction(ctx, w, &cnt, ctx->pw_P[0], ctx->pw_P[1], ctx->pw_AP[0], ctx->pw_AP[1]);
@@ -1403,16 +1447,14 @@ int lv_find_pressure(lv_ctx *ctx, double dt, int32_t niter, const lv_solve_opts 
         rc = solve_on_device(ctx, ctx->b, ctx->P0, o, stats);
         if (rc == LV_E_NOT_CONVERGED) status = rc;
         else if (rc) return rc;
-        if (w > 0) {  // remember P_it and A P_it = b' - r (the solve's final residual)
+        if (w > 0 && it < niter) {  // remember P_it (copied by the next iteration's k_gp) and A P_it = b' - r
             std::swap(ctx->pw_P[0], ctx->pw_P[1]);
             std::swap(ctx->pw_AP[0], ctx->pw_AP[1]);
-            LV_HIP_CHECK(hipMemcpyAsync(ctx->pw_P[0], ctx->P0, sizeof(double) * ctx->n, hipMemcpyDeviceToDevice,
-                                        ctx->s));
             LV_HIP_CHECK(lv_apply_from_residual(ctx->n, ctx->b, ctx->W.r, ctx->pw_AP[0], ctx->W, ctx->s));
             have = std::min(have + 1, 2);
         }
-        LV_HIP_CHECK(lv_launch_write_P(ctx->F, ctx->n, ctx->P0, ctx->s));
     }
+    LV_HIP_CHECK(lv_launch_write_P(ctx->F, ctx->n, ctx->P0, ctx->s));
     if (status) lv_fail(ctx, status, "pressure solve did not reach its tolerance");
     return status;
 }

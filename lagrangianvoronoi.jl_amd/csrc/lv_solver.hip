@@ -234,6 +234,152 @@ __global__ __launch_bounds__(B) void k_rhs2(lv_grid_params g, lv_fields F, lv_me
     b[i] = bi;
 }
 
+// ---- refresh!(solver) split by what depends on P ---------------------------------------------------
+//
+// During the Picard iterations of find_pressure! (src/pressure.jl:215-224) the mesh, v and the
+// boundary velocity are fixed; only P changes.  So b = A P/(rho c2 dt^2) + bdiv + [it > 1] corr(GP) with
+//   bdiv_i = -sum_j (lrr/dt) (v_i - v_j).(m - y) - sum_boundary dS.(vbc - v_i)/dt     (:173-182)
+//   GP_i   = -(1/m_i) sum_j (P_i - P_j) w_ij,   w_ij = lrr (m - x_i)                   (:180, 184)
+//   corr_i =  sum_j (GP_i - GP_j).u_ij,          u_ij = lrr (m - z)                     (:189-202)
+// k_rhs_geom evaluates bdiv, the compressibility denominator and w, u once per find_pressure!, into
+// the ELL slots of the assembled operator (columns sorted per row, the same slot of the same edge).
+// Each Picard iteration then streams w or u and gathers P or GP only.  Sums run in slot order
+// rather than the reference's edge order (rounding-level differences).
+
+// the slot of the neighbour j among row i's ELL columns: its first occurrence not yet taken (a
+// neighbour met through two periodic images has two edges; k_assemble's insertion sort is stable,
+// so the e-th edge to j in edge order has the e-th slot holding j)
+__device__ __forceinline__ int ell_slot_of(const int *__restrict__ ecol, int n, int i, int d, int j, unsigned &taken)
+{
+    for (int k = 0; k < d; k++) {
+        if (!(taken & (1u << k)) && ecol[(size_t)k * n + i] == j) {
+            taken |= 1u << k;
+            return k;
+        }
+    }
+    return -1;
+}
+
+__global__ __launch_bounds__(B) void k_rhs_geom(lv_grid_params g, lv_fields F, lv_mesh_view M, int n, double dt,
+                                                lv_vbc vbc, const int *__restrict__ deg, const int *__restrict__ ecol,
+                                                lv_rhs_geom R)
+{
+    const int i = blockIdx.x * B + threadIdx.x;
+    if (i >= n) return;
+    const double x1 = F.x[i], x2 = F.x[n + i];
+    const double pv1 = F.v[i], pv2 = F.v[n + i];
+    const int d = deg[i];
+    double bd = 0.0;
+    unsigned taken = 0;
+    const int ne = M.ne[i];
+    for (int k = 0; k < ne; k++) {
+        const int j = LV_EL(M, k, i);
+        const double e1x = LV_EV(M, 0, k, i), e1y = LV_EV(M, 1, k, i), e2x = LV_EV(M, 2, k, i), e2y = LV_EV(M, 3, k, i);
+        if (j < 0) {  // boundaries(p)
+            const double dS1 = e1y - e2y, dS2 = e2x - e1x;
+            const int slot = -j - 1;
+            const double vb1 = vbc.v[2 * slot], vb2 = vbc.v[2 * slot + 1];
+            bd -= (dS1 * (vb1 - pv1) + dS2 * (vb2 - pv2)) / dt;
+            continue;
+        }
+        double y1, y2;
+        lv_arrow_target(g, F.x[j], F.x[n + j], x1, x2, y1, y2);
+        const double lrr = lv_lr_ratio(x1 - y1, x2 - y2, e1x, e1y, e2x, e2y);
+        const double m1 = 0.5 * (e1x + e2x), m2 = 0.5 * (e1y + e2y);
+        const double z1 = 0.5 * (x1 + y1), z2 = 0.5 * (x2 + y2);
+        bd -= (lrr / dt) * ((pv1 - F.v[j]) * (m1 - y1) + (pv2 - F.v[n + j]) * (m2 - y2));
+        const int q = ell_slot_of(ecol, n, i, d, j, taken);
+        if (q < 0) continue;  // not reached: the operator holds every interior edge
+        const size_t e = (size_t)q * n + i;
+        R.w1[e] = lrr * (m1 - x1);
+        R.w2[e] = lrr * (m2 - x2);
+        R.u1[e] = lrr * (m1 - z1);
+        R.u2[e] = lrr * (m2 - z2);
+    }
+    R.bdiv[i] = bd;
+    R.den[i] = (F.rho[i] * F.c2[i]) * (dt * dt);
+}
+
+// GP from P (the first U slots' columns and weights loaded unconditionally, their gathers issued
+// together); with Pout also the write-back of P into the particles (find_pressure!'s p.P = P), with
+// Pout2 a second copy of P (the Picard window's record of the last iterate)
+__global__ __launch_bounds__(B) void k_gp(int n, const int *__restrict__ deg, const int *__restrict__ ecol,
+                                          lv_rhs_geom R, const double *__restrict__ P, const double *__restrict__ mass,
+                                          double *__restrict__ GP, double *__restrict__ Pout,
+                                          double *__restrict__ Pout2)
+{
+    const int i = blockIdx.x * B + threadIdx.x;
+    if (i >= n) return;
+    constexpr int U = 8;
+    const lv_ell_own<double> E{deg, ecol, R.w1};
+    int c[U];
+    double a1[U], a2[U], pj[U];
+    const int d = lv_ell_head<U>(E, i, n, c, a1);
+    const double Pi = P[i];
+#pragma unroll
+    for (int k = 0; k < U; k++) {
+        const double t = R.w2[(size_t)k * n + i];  // in bounds: the planes hold LV_MAXE slots per row
+        a2[k] = (k < d) ? t : 0.0;
+        pj[k] = P[c[k]];
+    }
+    double gx = 0.0, gy = 0.0;
+#pragma unroll
+    for (int k = 0; k < U; k++) {
+        const double dp = Pi - pj[k];
+        gx -= dp * a1[k];
+        gy -= dp * a2[k];
+    }
+    for (int k = U; k < d; k++) {
+        const size_t e = (size_t)k * n + i;
+        const double dp = Pi - P[ecol[e]];
+        gx -= dp * R.w1[e];
+        gy -= dp * R.w2[e];
+    }
+    const double mi = mass[i];
+    GP[i] = gx / mi;
+    GP[n + i] = gy / mi;
+    if (Pout) Pout[i] = Pi;
+    if (Pout2) Pout2[i] = Pi;
+}
+
+// b = A P/(rho c2 dt^2) + bdiv (+ corr(GP) when gp); with P0 also the initial guess P0 = P
+__global__ __launch_bounds__(B) void k_rhs_b(int n, int gp, const int *__restrict__ deg, const int *__restrict__ ecol,
+                                             lv_rhs_geom R, const double *__restrict__ GP,
+                                             const double *__restrict__ area, const double *__restrict__ P,
+                                             double *__restrict__ b, double *__restrict__ P0)
+{
+    const int i = blockIdx.x * B + threadIdx.x;
+    if (i >= n) return;
+    const double Pi = P[i];
+    double bi = (area[i] * Pi) / R.den[i] + R.bdiv[i];
+    if (gp) {
+        constexpr int U = 8;
+        const lv_ell_own<double> E{deg, ecol, R.u1};
+        int c[U];
+        double a1[U], a2[U], g1[U], g2[U];
+        const int d = lv_ell_head<U>(E, i, n, c, a1);
+#pragma unroll
+        for (int k = 0; k < U; k++) {
+            const double t = R.u2[(size_t)k * n + i];
+            a2[k] = (k < d) ? t : 0.0;
+            g1[k] = GP[c[k]];
+            g2[k] = GP[n + c[k]];
+        }
+        const double gi1 = GP[i], gi2 = GP[n + i];
+        double corr = 0.0;
+#pragma unroll
+        for (int k = 0; k < U; k++) corr += (gi1 - g1[k]) * a1[k] + (gi2 - g2[k]) * a2[k];
+        for (int k = U; k < d; k++) {
+            const size_t e = (size_t)k * n + i;
+            const int j = ecol[e];
+            corr += (gi1 - GP[j]) * R.u1[e] + (gi2 - GP[n + j]) * R.u2[e];
+        }
+        bi += corr;
+    }
+    b[i] = bi;
+    if (P0) P0[i] = Pi;
+}
+
 __global__ __launch_bounds__(B) void k_write_P(lv_fields F, int n, const double *__restrict__ P)
 {
     const int i = blockIdx.x * B + threadIdx.x;
@@ -764,6 +910,30 @@ hipError_t lv_launch_rhs(const lv_grid_params &g, lv_fields F, lv_mesh_view M, i
 {
     hipLaunchKernelGGL(k_rhs1, grid_of(n), dim3(B), 0, s, g, F, M, n, dt, vbc, b, P0, GP);
     if (gp_step) hipLaunchKernelGGL(k_rhs2, grid_of(n), dim3(B), 0, s, g, F, M, n, (const double *)GP, b);
+    return hipGetLastError();
+}
+
+hipError_t lv_launch_rhs_geom(const lv_grid_params &g, lv_fields F, lv_mesh_view M, int n, double dt, lv_vbc vbc,
+                              const lv_pressure_system &A, const lv_rhs_geom &R, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_rhs_geom, grid_of(n), dim3(B), 0, s, g, F, M, n, dt, vbc, (const int *)A.deg,
+                       (const int *)A.ecol, R);
+    return hipGetLastError();
+}
+
+hipError_t lv_launch_gp(int n, const lv_pressure_system &A, const lv_rhs_geom &R, const double *P, const double *mass,
+                        double *GP, double *Pout, double *Pout2, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_gp, grid_of(n), dim3(B), 0, s, n, (const int *)A.deg, (const int *)A.ecol, R, P, mass, GP,
+                       Pout, Pout2);
+    return hipGetLastError();
+}
+
+hipError_t lv_launch_rhs_b(int n, int gp_step, const lv_pressure_system &A, const lv_rhs_geom &R, const double *GP,
+                           const double *area, const double *P, double *b, double *P0, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_rhs_b, grid_of(n), dim3(B), 0, s, n, gp_step, (const int *)A.deg, (const int *)A.ecol, R, GP,
+                       area, P, b, P0);
     return hipGetLastError();
 }
 

@@ -1029,8 +1029,9 @@ int lv_eos_ideal(lv_ctx *ctx, double gamma, double Pmin)
     return LV_OK;
 }
 
-// the system of the inner solver: mode 0 the pressure operator, mode 1 ViscousSolver's operator
-static int assemble_system(lv_ctx *ctx, double dt, int mode, int64_t *nnz)
+// the system of the inner solver: mode 0 the pressure operator, mode 1 ViscousSolver's operator; with
+// rhs_vbc (mode 0) the assembly also stores the P-independent parts of refresh!(solver) in ctx->rg
+static int assemble_system(lv_ctx *ctx, double dt, int mode, int64_t *nnz, const lv_vbc *rhs_vbc = nullptr)
 {
     int rc = check_ctx(ctx, true);
     if (rc) return rc;
@@ -1057,7 +1058,8 @@ static int assemble_system(lv_ctx *ctx, double dt, int mode, int64_t *nnz)
     ctx->A.deg = ctx->deg;
     rc = clear_flags(ctx);
     if (rc) return rc;
-    LV_HIP_CHECK(lv_launch_assemble(ctx->g, ctx->F, M, n, dt, mode, ctx->A, ctx->d_flags + 1, ctx->s));
+    LV_HIP_CHECK(lv_launch_assemble(ctx->g, ctx->F, M, n, dt, mode, ctx->A, ctx->d_flags + 1, ctx->s, rhs_vbc,
+                                    rhs_vbc ? &ctx->rg : nullptr));
     int flags[2];
     rc = read_flags(ctx, flags);
     if (rc) return rc;
@@ -1406,18 +1408,17 @@ int lv_find_pressure(lv_ctx *ctx, double dt, int32_t niter, const lv_solve_opts 
     if (opts) o = *opts;
     else default_opts(o);
     if (stats) memset(stats, 0, sizeof(*stats));
-    rc = lv_assemble_pressure(ctx, dt, nullptr);
-    if (rc) return rc;
     lv_vbc v;
     for (int k = 0; k < 8; k++) v.v[k] = o.vbc[k];
+    rc = ensure_rhs_geom(ctx);
+    if (rc) return rc;
+    rc = assemble_system(ctx, dt, 0, nullptr, &v);  // refresh!(A), and the fixed parts of refresh!(solver)
+    if (rc) return rc;
     const int w = std::min(std::max(o.picard_window, 0), 8);
     if (w > 0) {
         rc = ensure_picard_window(ctx, w);
         if (rc) return rc;
     }
-    rc = ensure_rhs_geom(ctx);
-    if (rc) return rc;
-    LV_HIP_CHECK(lv_launch_rhs_geom(ctx->g, ctx->F, mesh_view(ctx), ctx->n, dt, v, ctx->A, ctx->rg, ctx->s));
     int cnt = 0, have = 0;  // directions in the window; finished Picard iterates remembered (<= 2)
     int status = LV_OK;
     for (int it = 1; it <= niter; it++) {
@@ -1611,10 +1612,9 @@ int lv_step_fused(lv_ctx *ctx, const lv_step_desc *d, lv_solve_stats *stats)
     if (rc) return rc;
     int prc = lv_find_pressure(ctx, d->dt, d->niter, &d->solve, stats);
     if (prc && prc != LV_E_NOT_CONVERGED) return prc;
-    rc = lv_pressure_step(ctx, d->dt);
-    if (rc) return rc;
-    rc = lv_find_D(ctx);
-    if (rc) return rc;
+    // pressure_step! with find_D! in its second pass
+    LV_HIP_CHECK(lv_launch_pressure_step(ctx->g, ctx->F, mesh_view(ctx), ctx->n, d->dt, ctx->s, true));
+    bool dv_done = false;
     if (d->viscous == 2) {  // viscous_step!(ViscousSolver(grid), dt) with the step's solver method
         lv_solve_opts o = d->solve;
         o.tol = d->viscous_tol > 0.0 ? d->viscous_tol : 1e-6;
@@ -1622,16 +1622,20 @@ int lv_step_fused(lv_ctx *ctx, const lv_step_desc *d, lv_solve_stats *stats)
         rc = lv_viscous_implicit(ctx, d->dt, &o, stats);
         if (rc && rc != LV_E_NOT_CONVERGED) return rc;
         if (rc) prc = rc;
-    } else if (d->viscous) {
-        rc = lv_viscous_step(ctx, d->dt, d->artificial_viscosity);
-        if (rc) return rc;
+    } else if (d->viscous) {  // viscous_step! with find_dv! in its second pass
+        const double avdr = d->artificial_viscosity ? ctx->g.dr : 0.0;  // src/diffusion.jl:40
+        LV_HIP_CHECK(lv_launch_viscous(ctx->g, ctx->F, mesh_view(ctx), ctx->n, d->dt, avdr, ctx->s, true, d->dt,
+                                       d->alpha));
+        dv_done = true;
     }
     if (d->friction) {
         rc = lv_bdary_friction(ctx, d->friction_vbc, d->dt);
         if (rc) return rc;
     }
-    rc = lv_find_dv(ctx, d->dt, d->alpha);
-    if (rc) return rc;
+    if (!dv_done) {
+        rc = lv_find_dv(ctx, d->dt, d->alpha);
+        if (rc) return rc;
+    }
     if (d->multiphase) {  // multiphase_projection!(MultiphaseSolver(grid; quality_threshold)), defaults of :197
         int it = 0;
         double rel = 0.0;

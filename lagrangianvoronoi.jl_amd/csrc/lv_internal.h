@@ -156,11 +156,13 @@ hipError_t lv_launch_move(const lv_grid_params &g, lv_fields F, lv_mesh_view M, 
 hipError_t lv_launch_gravity(lv_fields F, int n, double g1, double g2, double dt, hipStream_t s);
 hipError_t lv_launch_find_rho(lv_fields F, int n, hipStream_t s);
 hipError_t lv_launch_eos(lv_fields F, int n, int kind, double gamma, double P0, double Pmin, hipStream_t s);
+// with_find_D: find_D! in pressure_step!'s second pass (step! calls it right after)
 hipError_t lv_launch_pressure_step(const lv_grid_params &g, lv_fields F, lv_mesh_view M, int n, double dt,
-                                   hipStream_t s);
+                                   hipStream_t s, bool with_find_D = false);
 hipError_t lv_launch_find_D(const lv_grid_params &g, lv_fields F, lv_mesh_view M, int n, hipStream_t s);
+// with_find_dv: find_dv!(dv_dt, alpha) in viscous_step!'s second pass
 hipError_t lv_launch_viscous(const lv_grid_params &g, lv_fields F, lv_mesh_view M, int n, double dt, double avdr,
-                             hipStream_t s);
+                             hipStream_t s, bool with_find_dv = false, double dv_dt = 0.0, double alpha = 0.0);
 struct lv_vbc {
     double v[8];
 };

@@ -75,8 +75,12 @@ size_t lv_partials_bytes(int n);
 hipError_t lv_apply_precond(const lv_pressure_system &A, lv_solver_work &W, int n, const lv_precond *M,
                             const double *r, double *z, hipStream_t s);
 hipError_t lv_launch_degree(lv_mesh_view M, int n, int *deg, hipStream_t s);
+struct lv_rhs_geom;
+// refresh!(A) into A (mode 0 the pressure operator, 1 ViscousSolver's); with R (mode 0) the same pass
+// also stores the P-independent parts of refresh!(solver) for boundary velocity vbc
 hipError_t lv_launch_assemble(const lv_grid_params &g, lv_fields F, lv_mesh_view M, int n, double dt, int mode,
-                              lv_pressure_system &A, int *nonsingular, hipStream_t s);
+                              lv_pressure_system &A, int *nonsingular, hipStream_t s, const lv_vbc *vbc = nullptr,
+                              const lv_rhs_geom *R = nullptr);
 hipError_t lv_launch_rhs(const lv_grid_params &g, lv_fields F, lv_mesh_view M, int n, double dt, int gp_step,
                          lv_vbc vbc, double *b, double *P0, double *GP, hipStream_t s);
 // The P-independent parts of refresh!(solver) for one find_pressure!: per slot of the assembled

@@ -42,13 +42,54 @@ __global__ __launch_bounds__(B) void k_degree(lv_mesh_view M, int n, int *__rest
 
 // refresh!(A, grid, dt), src/pressure.jl:104-117 (mode 0), or the operator of ViscousSolver's
 // assemble!, src/viscous_solver.jl:89-113 (mode 1: d_i = m_i/dt, c_ij = 0.5 lrr (mu_i + mu_j))
+// The ELL slot of edge k (to neighbour j) of row i: the row's columns are sorted by label, ties in
+// edge order (a neighbour met through two periodic images has two edges), so the slot is the number
+// of interior edges with a smaller label plus the earlier edges to the same one.
+__device__ __forceinline__ int sorted_slot(const lv_mesh_view &M, int i, int ne, int k, int j)
+{
+    int q = 0;
+    for (int kk = 0; kk < ne; kk++) {
+        const int jj = LV_EL(M, kk, i);
+        q += (jj >= 0 && (jj < j || (jj == j && kk < k))) ? 1 : 0;
+    }
+    return q;
+}
+
+// The P-independent parts of refresh!(solver) for the interior edge to y (src/pressure.jl:173-180,
+// 189-197): w = lrr (m - x_i) and u = lrr (m - z) into ELL slot e; returns the edge's divergence term of b
+__device__ __forceinline__ double rhs_edge_geom(const lv_rhs_geom &R, size_t e, double lrr, double x1, double x2,
+                                                double y1, double y2, double m1, double m2, double dv1, double dv2,
+                                                double dt)
+{
+    const double z1 = 0.5 * (x1 + y1), z2 = 0.5 * (x2 + y2);
+    R.w1[e] = lrr * (m1 - x1);
+    R.w2[e] = lrr * (m2 - x2);
+    R.u1[e] = lrr * (m1 - z1);
+    R.u2[e] = lrr * (m2 - z2);
+    return (lrr / dt) * (dv1 * (m1 - y1) + dv2 * (m2 - y2));
+}
+
+// the boundary-flux term of b for a boundary edge with label lab (src/pressure.jl:181-186)
+__device__ __forceinline__ double rhs_bdary(const lv_vbc &vbc, int lab, double e1x, double e1y, double e2x,
+                                            double e2y, double pv1, double pv2, double dt)
+{
+    const double dS1 = e1y - e2y, dS2 = e2x - e1x;
+    const int slot = -lab - 1;
+    const double vb1 = vbc.v[2 * slot], vb2 = vbc.v[2 * slot + 1];
+    return (dS1 * (vb1 - pv1) + dS2 * (vb2 - pv2)) / dt;
+}
+
+// refresh!(A) (src/pressure.jl:104-117): the CSR copy in edge order, the ELL copy with each row's
+// columns sorted, diagonals and Jacobi inverses.  With R.w1 (mode 0, find_pressure!) the same edge
+// loop also stores the P-independent parts of refresh!(solver) (vbc: boundary_velocity per label).
 __global__ __launch_bounds__(B) void k_assemble(lv_grid_params g, lv_fields F, lv_mesh_view M, int n, double dt, int mode,
                                                 const int *__restrict__ row_ptr, int *__restrict__ cols,
                                                 double *__restrict__ val64, float *__restrict__ val32,
                                                 double *__restrict__ diag64, float *__restrict__ diag32,
                                                 double *__restrict__ dinv64, float *__restrict__ dinv32,
                                                 int *__restrict__ ecol, double *__restrict__ ev64,
-                                                float *__restrict__ ev32, int *__restrict__ nonsingular)
+                                                float *__restrict__ ev32, int *__restrict__ nonsingular, lv_vbc vbc,
+                                                lv_rhs_geom R)
 {
     const int i = blockIdx.x * B + threadIdx.x;
     if (i >= n) return;
@@ -56,41 +97,39 @@ __global__ __launch_bounds__(B) void k_assemble(lv_grid_params g, lv_fields F, l
     const double rho = F.rho[i];
     const double d = (mode == 0) ? F.mass[i] / (((rho * rho) * F.c2[i]) * (dt * dt)) : F.mass[i] / dt;
     const double mui = F.mu[i];
-    double dd = d;
+    const bool geom = R.w1 != nullptr;
+    const double pv1 = F.v[i], pv2 = F.v[n + i];
+    double dd = d, bd = 0.0;
     int o = row_ptr[i];
-    // the solver's ELL copy keeps each row's neighbours sorted by column, so that slot k of a
-    // wavefront gathers from one lattice neighbourhood; the CSR copy keeps the edge order
-    int sc[LV_MAXE];
-    double sv[LV_MAXE];
-    int m = 0;
     const int ne = M.ne[i];
     for (int k = 0; k < ne; k++) {
         const int j = LV_EL(M, k, i);
-        if (j < 0) continue;
+        const double e1x = LV_EV(M, 0, k, i), e1y = LV_EV(M, 1, k, i), e2x = LV_EV(M, 2, k, i), e2y = LV_EV(M, 3, k, i);
+        if (j < 0) {
+            if (geom) bd -= rhs_bdary(vbc, j, e1x, e1y, e2x, e2y, pv1, pv2, dt);
+            continue;
+        }
         double y1, y2;
         lv_arrow_target(g, F.x[j], F.x[n + j], x1, x2, y1, y2);
-        const double lrr =
-            lv_lr_ratio(x1 - y1, x2 - y2, LV_EV(M, 0, k, i), LV_EV(M, 1, k, i), LV_EV(M, 2, k, i), LV_EV(M, 3, k, i));
+        const double lrr = lv_lr_ratio(x1 - y1, x2 - y2, e1x, e1y, e2x, e2y);
         const double c = (mode == 0) ? lrr * (0.5 / rho + 0.5 / F.rho[j]) : (0.5 * lrr) * (mui + F.mu[j]);
         cols[o] = j;
         val64[o] = c;
         val32[o] = (float)c;
         dd += c;
         o++;
-        int q = m++;
-        while (q > 0 && sc[q - 1] > j) {
-            sc[q] = sc[q - 1];
-            sv[q] = sv[q - 1];
-            q--;
+        const size_t es = (size_t)sorted_slot(M, i, ne, k, j) * n + i;
+        ecol[es] = j;
+        ev64[es] = c;
+        ev32[es] = (float)c;
+        if (geom) {
+            const double m1 = 0.5 * (e1x + e2x), m2 = 0.5 * (e1y + e2y);
+            bd -= rhs_edge_geom(R, es, lrr, x1, x2, y1, y2, m1, m2, pv1 - F.v[j], pv2 - F.v[n + j], dt);
         }
-        sc[q] = j;
-        sv[q] = c;
     }
-    for (int k = 0; k < m; k++) {
-        const size_t es = (size_t)k * n + i;
-        ecol[es] = sc[k];
-        ev64[es] = sv[k];
-        ev32[es] = (float)sv[k];
+    if (geom) {
+        R.bdiv[i] = bd;
+        R.den[i] = (rho * F.c2[i]) * (dt * dt);
     }
     // the system is singular iff every d_i == 0 (c2 = Inf): one flag write per wavefront at most
     if (__any(d != 0.0) && (threadIdx.x & 63) == 0 && *nonsingular == 0) atomicOr(nonsingular, 1);
@@ -246,55 +285,30 @@ __global__ __launch_bounds__(B) void k_rhs2(lv_grid_params g, lv_fields F, lv_me
 // Each Picard iteration then streams w or u and gathers P or GP only.  Sums run in slot order
 // rather than the reference's edge order (rounding-level differences).
 
-// the slot of the neighbour j among row i's ELL columns: its first occurrence not yet taken (a
-// neighbour met through two periodic images has two edges; k_assemble's insertion sort is stable,
-// so the e-th edge to j in edge order has the e-th slot holding j)
-__device__ __forceinline__ int ell_slot_of(const int *__restrict__ ecol, int n, int i, int d, int j, unsigned &taken)
-{
-    for (int k = 0; k < d; k++) {
-        if (!(taken & (1u << k)) && ecol[(size_t)k * n + i] == j) {
-            taken |= 1u << k;
-            return k;
-        }
-    }
-    return -1;
-}
-
+// the same P-independent parts on their own, for an operator assembled without them (the order of
+// the sums as in k_assemble)
 __global__ __launch_bounds__(B) void k_rhs_geom(lv_grid_params g, lv_fields F, lv_mesh_view M, int n, double dt,
-                                                lv_vbc vbc, const int *__restrict__ deg, const int *__restrict__ ecol,
-                                                lv_rhs_geom R)
+                                                lv_vbc vbc, lv_rhs_geom R)
 {
     const int i = blockIdx.x * B + threadIdx.x;
     if (i >= n) return;
     const double x1 = F.x[i], x2 = F.x[n + i];
     const double pv1 = F.v[i], pv2 = F.v[n + i];
-    const int d = deg[i];
     double bd = 0.0;
-    unsigned taken = 0;
     const int ne = M.ne[i];
     for (int k = 0; k < ne; k++) {
         const int j = LV_EL(M, k, i);
         const double e1x = LV_EV(M, 0, k, i), e1y = LV_EV(M, 1, k, i), e2x = LV_EV(M, 2, k, i), e2y = LV_EV(M, 3, k, i);
-        if (j < 0) {  // boundaries(p)
-            const double dS1 = e1y - e2y, dS2 = e2x - e1x;
-            const int slot = -j - 1;
-            const double vb1 = vbc.v[2 * slot], vb2 = vbc.v[2 * slot + 1];
-            bd -= (dS1 * (vb1 - pv1) + dS2 * (vb2 - pv2)) / dt;
+        if (j < 0) {
+            bd -= rhs_bdary(vbc, j, e1x, e1y, e2x, e2y, pv1, pv2, dt);
             continue;
         }
         double y1, y2;
         lv_arrow_target(g, F.x[j], F.x[n + j], x1, x2, y1, y2);
         const double lrr = lv_lr_ratio(x1 - y1, x2 - y2, e1x, e1y, e2x, e2y);
         const double m1 = 0.5 * (e1x + e2x), m2 = 0.5 * (e1y + e2y);
-        const double z1 = 0.5 * (x1 + y1), z2 = 0.5 * (x2 + y2);
-        bd -= (lrr / dt) * ((pv1 - F.v[j]) * (m1 - y1) + (pv2 - F.v[n + j]) * (m2 - y2));
-        const int q = ell_slot_of(ecol, n, i, d, j, taken);
-        if (q < 0) continue;  // not reached: the operator holds every interior edge
-        const size_t e = (size_t)q * n + i;
-        R.w1[e] = lrr * (m1 - x1);
-        R.w2[e] = lrr * (m2 - x2);
-        R.u1[e] = lrr * (m1 - z1);
-        R.u2[e] = lrr * (m2 - z2);
+        const size_t es = (size_t)sorted_slot(M, i, ne, k, j) * n + i;
+        bd -= rhs_edge_geom(R, es, lrr, x1, x2, y1, y2, m1, m2, pv1 - F.v[j], pv2 - F.v[n + j], dt);
     }
     R.bdiv[i] = bd;
     R.den[i] = (F.rho[i] * F.c2[i]) * (dt * dt);
@@ -898,10 +912,14 @@ hipError_t lv_launch_degree(lv_mesh_view M, int n, int *deg, hipStream_t s)
 }
 
 hipError_t lv_launch_assemble(const lv_grid_params &g, lv_fields F, lv_mesh_view M, int n, double dt, int mode,
-                              lv_pressure_system &A, int *nonsingular, hipStream_t s)
+                              lv_pressure_system &A, int *nonsingular, hipStream_t s, const lv_vbc *vbc,
+                              const lv_rhs_geom *R)
 {
+    const lv_vbc v = vbc ? *vbc : lv_vbc{};
+    const lv_rhs_geom r = (R && mode == 0) ? *R : lv_rhs_geom{};
     hipLaunchKernelGGL(k_assemble, grid_of(n), dim3(B), 0, s, g, F, M, n, dt, mode, (const int *)A.row_ptr, A.cols,
-                       A.val64, A.val32, A.diag64, A.diag32, A.dinv64, A.dinv32, A.ecol, A.ev64, A.ev32, nonsingular);
+                       A.val64, A.val32, A.diag64, A.diag32, A.dinv64, A.dinv32, A.ecol, A.ev64, A.ev32, nonsingular,
+                       v, r);
     return hipGetLastError();
 }
 
@@ -916,8 +934,8 @@ hipError_t lv_launch_rhs(const lv_grid_params &g, lv_fields F, lv_mesh_view M, i
 hipError_t lv_launch_rhs_geom(const lv_grid_params &g, lv_fields F, lv_mesh_view M, int n, double dt, lv_vbc vbc,
                               const lv_pressure_system &A, const lv_rhs_geom &R, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_rhs_geom, grid_of(n), dim3(B), 0, s, g, F, M, n, dt, vbc, (const int *)A.deg,
-                       (const int *)A.ecol, R);
+    (void)A;
+    hipLaunchKernelGGL(k_rhs_geom, grid_of(n), dim3(B), 0, s, g, F, M, n, dt, vbc, R);
     return hipGetLastError();
 }
 

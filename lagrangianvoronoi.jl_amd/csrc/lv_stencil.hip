@@ -142,14 +142,19 @@ __global__ __launch_bounds__(B) void k_pstep1(lv_grid_params g, lv_fields F, lv_
 }
 
 // pressure_step! pass 2, src/pressure.jl:18-24: e -= dt*lrr/m * ((m-x).(P_i v_i) - (m-y).(P_j v_j))
+// WITH_D: find_D! (src/diffusion.jl:8-19) in the same pass, as step! calls it right after
+// pressure_step! (both read the velocities pass 1 left and the same edges; the sums are k_find_D's)
+template <bool WITH_D>
 __global__ __launch_bounds__(B) void k_pstep2(lv_grid_params g, lv_fields F, lv_mesh_view M, int n, double dt)
 {
     const int i = blockIdx.x * B + threadIdx.x;
     if (i >= n) return;
     const double x1 = F.x[i], x2 = F.x[n + i];
     const double Pi = F.P[i], mi = F.mass[i];
-    const double pv1 = Pi * F.v[i], pv2 = Pi * F.v[n + i];
+    const double vi1 = F.v[i], vi2 = F.v[n + i];
+    const double pv1 = Pi * vi1, pv2 = Pi * vi2;
     double ei = F.e[i];
+    double D11 = 0.0, D21 = 0.0, D12 = 0.0, D22 = 0.0;
     const int ne = M.ne[i];
     for (int k = 0; k < ne; k++) {
         const int j = LV_EL(M, k, i);
@@ -160,12 +165,32 @@ __global__ __launch_bounds__(B) void k_pstep2(lv_grid_params g, lv_fields F, lv_
         const double lrr = lv_lr_ratio(x1 - y1, x2 - y2, e1x, e1y, e2x, e2y);
         const double m1 = 0.5 * (e1x + e2x), m2 = 0.5 * (e1y + e2y);
         const double Pj = F.P[j];
-        const double qv1 = Pj * F.v[j], qv2 = Pj * F.v[n + j];
+        const double vj1 = F.v[j], vj2 = F.v[n + j];
+        const double qv1 = Pj * vj1, qv2 = Pj * vj2;
         const double a = (m1 - x1) * pv1 + (m2 - x2) * pv2;
         const double b = (m1 - y1) * qv1 + (m2 - y2) * qv2;
         ei = ei - ((dt * lrr) / mi) * (a - b);
+        if (WITH_D) {
+            const double a1 = vi1 - vj1, a2 = vi2 - vj2;
+            const double b1 = m1 - y1, b2 = m2 - y2;
+            D11 = D11 + lrr * (a1 * b1);
+            D21 = D21 + lrr * (a2 * b1);
+            D12 = D12 + lrr * (a1 * b2);
+            D22 = D22 + lrr * (a2 * b2);
+        }
     }
     F.e[i] = ei;
+    if (WITH_D) {
+        const double A = F.area[i];
+        D11 = D11 / A;
+        D21 = D21 / A;
+        D12 = D12 / A;
+        D22 = D22 / A;
+        F.D[i] = 0.5 * (D11 + D11);
+        F.D[n + i] = 0.5 * (D21 + D12);
+        F.D[2 * n + i] = 0.5 * (D12 + D21);
+        F.D[3 * n + i] = 0.5 * (D22 + D22);
+    }
 }
 
 // find_D!, src/diffusion.jl:8-19
@@ -249,9 +274,26 @@ __global__ __launch_bounds__(B) void k_visc1(lv_grid_params g, lv_fields F, lv_m
     F.v[n + i] = v2;
 }
 
-// viscous_step! pass 2, src/diffusion.jl:47-52
+// quality and dv of find_dv!, src/relaxation.jl:10-25, from the neighbour distances rmin, rmax (the
+// centroid cached by the clipper, same expression order)
+__device__ __forceinline__ void find_dv_row(lv_fields &F, int n, int i, double x1, double x2, double rmin, double rmax,
+                                            double dt, double alpha)
+{
+    const double q = rmin / rmax;
+    F.quality[i] = q;
+    const double D11 = F.D[i], D21 = F.D[n + i], D12 = F.D[2 * n + i], D22 = F.D[3 * n + i];
+    const double nD = sqrt(((D11 * D11 + D21 * D21) + D12 * D12) + D22 * D22);
+    const double lambda = (alpha * nD) / (q * q);
+    const double f = lambda / (1.0 + dt * lambda);
+    F.dv[i] = f * (F.cent[i] - x1);
+    F.dv[n + i] = f * (F.cent[n + i] - x2);
+}
+
+// viscous_step! pass 2, src/diffusion.jl:47-52.  WITH_DV: find_dv!(dt_dv, alpha) in the same pass (it
+// reads D, the positions and the mesh, none of which the viscous step or bdary_friction! change)
+template <bool WITH_DV>
 __global__ __launch_bounds__(B) void k_visc2(lv_grid_params g, lv_fields F, lv_mesh_view M, int n, double dt,
-                                             double avdr)
+                                             double avdr, double dt_dv, double alpha)
 {
     const int i = blockIdx.x * B + threadIdx.x;
     if (i >= n) return;
@@ -263,6 +305,7 @@ __global__ __launch_bounds__(B) void k_visc2(lv_grid_params g, lv_fields F, lv_m
     const double sv1 = Sp[0] * pv1 + Sp[2] * pv2;
     const double sv2 = Sp[1] * pv1 + Sp[3] * pv2;
     const double mi = F.mass[i];
+    double rmax = 0.0, rmin = __builtin_inf();
     const int ne = M.ne[i];
     for (int k = 0; k < ne; k++) {
         const int j = LV_EL(M, k, i);
@@ -278,8 +321,14 @@ __global__ __launch_bounds__(B) void k_visc2(lv_grid_params g, lv_fields F, lv_m
         const double a = (m1 - x1) * sv1 + (m2 - x2) * sv2;
         const double b = (m1 - y1) * qv1 + (m2 - y2) * qv2;
         ei = ei + ((dt * lv_lr_ratio(x1 - y1, x2 - y2, e1x, e1y, e2x, e2y)) / mi) * (a - b);
+        if (WITH_DV) {
+            const double r = lv_norm2(x1 - y1, x2 - y2);
+            if (r > rmax) rmax = r;
+            if (r < rmin) rmin = r;
+        }
     }
     F.e[i] = ei;
+    if (WITH_DV) find_dv_row(F, n, i, x1, x2, rmin, rmax, dt_dv, alpha);
 }
 
 // bdary_friction!, src/diffusion.jl:64-78, vDirichlet(m) = vbc[label] and charfun = everywhere
@@ -336,14 +385,7 @@ __global__ __launch_bounds__(B) void k_find_dv(lv_grid_params g, lv_fields F, lv
         if (r > rmax) rmax = r;
         if (r < rmin) rmin = r;
     }
-    const double q = rmin / rmax;
-    F.quality[i] = q;
-    const double D11 = F.D[i], D21 = F.D[n + i], D12 = F.D[2 * n + i], D22 = F.D[3 * n + i];
-    const double nD = sqrt(((D11 * D11 + D21 * D21) + D12 * D12) + D22 * D22);
-    const double lambda = (alpha * nD) / (q * q);
-    const double f = lambda / (1.0 + dt * lambda);
-    F.dv[i] = f * (F.cent[i] - x1);
-    F.dv[n + i] = f * (F.cent[n + i] - x2);
+    find_dv_row(F, n, i, x1, x2, rmin, rmax, dt, alpha);
 }
 
 // relaxation_step! pass 1, src/relaxation.jl:37-59 (writes mass, momentum, energy of p only)
@@ -445,10 +487,11 @@ hipError_t lv_launch_eos(lv_fields F, int n, int kind, double gamma, double P0, 
 }
 
 hipError_t lv_launch_pressure_step(const lv_grid_params &g, lv_fields F, lv_mesh_view M, int n, double dt,
-                                   hipStream_t s)
+                                   hipStream_t s, bool with_find_D)
 {
     hipLaunchKernelGGL(k_pstep1, grid_of(n), dim3(B), 0, s, g, F, M, n, dt);
-    hipLaunchKernelGGL(k_pstep2, grid_of(n), dim3(B), 0, s, g, F, M, n, dt);
+    if (with_find_D) hipLaunchKernelGGL(k_pstep2<true>, grid_of(n), dim3(B), 0, s, g, F, M, n, dt);
+    else hipLaunchKernelGGL(k_pstep2<false>, grid_of(n), dim3(B), 0, s, g, F, M, n, dt);
     return hipGetLastError();
 }
 
@@ -459,10 +502,13 @@ hipError_t lv_launch_find_D(const lv_grid_params &g, lv_fields F, lv_mesh_view M
 }
 
 hipError_t lv_launch_viscous(const lv_grid_params &g, lv_fields F, lv_mesh_view M, int n, double dt, double avdr,
-                             hipStream_t s)
+                             hipStream_t s, bool with_find_dv, double dv_dt, double alpha)
 {
     hipLaunchKernelGGL(k_visc1, grid_of(n), dim3(B), 0, s, g, F, M, n, dt, avdr);
-    hipLaunchKernelGGL(k_visc2, grid_of(n), dim3(B), 0, s, g, F, M, n, dt, avdr);
+    if (with_find_dv)
+        hipLaunchKernelGGL(k_visc2<true>, grid_of(n), dim3(B), 0, s, g, F, M, n, dt, avdr, dv_dt, alpha);
+    else
+        hipLaunchKernelGGL(k_visc2<false>, grid_of(n), dim3(B), 0, s, g, F, M, n, dt, avdr, 0.0, 0.0);
     return hipGetLastError();
 }
 

@@ -1110,27 +1110,20 @@ static int ensure_rhs_geom(lv_ctx *ctx)
     return LV_OK;
 }
 
-// the pressure operator of the current mesh is assembled (its ELL slots carry the RHS geometry)
-static bool pressure_ell_ready(lv_ctx *ctx) { return ctx->A.valid && ctx->A_kind == 0; }
-
 int lv_pressure_rhs(lv_ctx *ctx, double dt, int32_t gp_step, const double vbc[8], double *b, double *GP)
 {
     int rc = check_ctx(ctx, true);
     if (rc) return rc;
     lv_vbc v;
     for (int k = 0; k < 8; k++) v.v[k] = vbc ? vbc[k] : 0.0;
-    if (pressure_ell_ready(ctx)) {  // the split evaluation find_pressure! uses
-        rc = ensure_rhs_geom(ctx);
-        if (rc) return rc;
-        const int n = ctx->n;
-        LV_HIP_CHECK(lv_launch_rhs_geom(ctx->g, ctx->F, mesh_view(ctx), n, dt, v, ctx->A, ctx->rg, ctx->s));
-        LV_HIP_CHECK(lv_launch_gp(n, ctx->A, ctx->rg, ctx->F.P, ctx->F.mass, ctx->GP, nullptr, nullptr, ctx->s));
-        LV_HIP_CHECK(lv_launch_rhs_b(n, gp_step, ctx->A, ctx->rg, ctx->GP, ctx->F.area, ctx->F.P, ctx->b, ctx->P0,
-                                     ctx->s));
-    } else {
-        LV_HIP_CHECK(lv_launch_rhs(ctx->g, ctx->F, mesh_view(ctx), ctx->n, dt, gp_step, v, ctx->b, ctx->P0, ctx->GP,
-                                   ctx->s));
-    }
+    // the same split evaluation as find_pressure!: the P-independent parts, GP, then b
+    rc = ensure_rhs_geom(ctx);
+    if (rc) return rc;
+    const int n = ctx->n;
+    const lv_mesh_view M = mesh_view(ctx);
+    LV_HIP_CHECK(lv_launch_rhs_geom(ctx->g, ctx->F, M, n, dt, v, ctx->rg, ctx->s));
+    LV_HIP_CHECK(lv_launch_gp(M, n, ctx->rg, ctx->F.P, ctx->F.mass, ctx->GP, nullptr, nullptr, ctx->s));
+    LV_HIP_CHECK(lv_launch_rhs_b(M, n, gp_step, ctx->rg, ctx->GP, ctx->F.area, ctx->F.P, ctx->b, ctx->P0, ctx->s));
     if (b) LV_HIP_CHECK(hipMemcpyAsync(b, ctx->b, sizeof(double) * ctx->n, hipMemcpyDeviceToHost, ctx->s));
     if (GP) LV_HIP_CHECK(hipMemcpyAsync(GP, ctx->GP, sizeof(double) * 2 * ctx->n, hipMemcpyDeviceToHost, ctx->s));
     LV_HIP_CHECK(hipStreamSynchronize(ctx->s));
@@ -1426,14 +1419,14 @@ int lv_find_pressure(lv_ctx *ctx, double dt, int32_t niter, const lv_solve_opts 
         // as the initial guess); later ones read the previous solution in P0, write it back into the
         // particles (p.P = P, src/pressure.jl:221-223) and add the GP term
         if (it == 1) {
-            LV_HIP_CHECK(lv_launch_rhs_b(ctx->n, 0, ctx->A, ctx->rg, ctx->GP, ctx->F.area, ctx->F.P, ctx->b, ctx->P0,
-                                         ctx->s));
+            LV_HIP_CHECK(lv_launch_rhs_b(mesh_view(ctx), ctx->n, 0, ctx->rg, ctx->GP, ctx->F.area, ctx->F.P, ctx->b,
+                                         ctx->P0, ctx->s));
         } else {
             // (with the Picard window, also its record of the last iterate, pw_P[0])
-            LV_HIP_CHECK(lv_launch_gp(ctx->n, ctx->A, ctx->rg, ctx->P0, ctx->F.mass, ctx->GP, ctx->F.P,
+            LV_HIP_CHECK(lv_launch_gp(mesh_view(ctx), ctx->n, ctx->rg, ctx->P0, ctx->F.mass, ctx->GP, ctx->F.P,
                                       (w > 0 && have >= 1) ? ctx->pw_P[0] : nullptr, ctx->s));
-            LV_HIP_CHECK(lv_launch_rhs_b(ctx->n, 1, ctx->A, ctx->rg, ctx->GP, ctx->F.area, ctx->P0, ctx->b, nullptr,
-                                         ctx->s));
+            LV_HIP_CHECK(lv_launch_rhs_b(mesh_view(ctx), ctx->n, 1, ctx->rg, ctx->GP, ctx->F.area, ctx->P0, ctx->b,
+                                         nullptr, ctx->s));
         }
         if (w > 0 && have >= 1) {
             if (have >= 2) {

@@ -81,21 +81,19 @@ struct lv_rhs_geom;
 hipError_t lv_launch_assemble(const lv_grid_params &g, lv_fields F, lv_mesh_view M, int n, double dt, int mode,
                               lv_pressure_system &A, int *nonsingular, hipStream_t s, const lv_vbc *vbc = nullptr,
                               const lv_rhs_geom *R = nullptr);
-hipError_t lv_launch_rhs(const lv_grid_params &g, lv_fields F, lv_mesh_view M, int n, double dt, int gp_step,
-                         lv_vbc vbc, double *b, double *P0, double *GP, hipStream_t s);
-// The P-independent parts of refresh!(solver) for one find_pressure!: per slot of the assembled
-// operator's ELL w = lrr (m - x_i) and u = lrr (m - z) (each two planes of LV_MAXE x ld doubles), per
-// row the divergence and boundary-flux part of b (bdiv) and rho c2 dt^2 (den).
+// The P-independent parts of refresh!(solver) for one find_pressure!: per mesh edge slot
+// w = lrr (m - x_i) and u = lrr (m - z) (each two planes of LV_MAXE x ld doubles, interior edges only),
+// per row the divergence and boundary-flux part of b (bdiv) and rho c2 dt^2 (den).
 struct lv_rhs_geom {
     double *w1, *w2, *u1, *u2, *bdiv, *den;
 };
 hipError_t lv_launch_rhs_geom(const lv_grid_params &g, lv_fields F, lv_mesh_view M, int n, double dt, lv_vbc vbc,
-                              const lv_pressure_system &A, const lv_rhs_geom &R, hipStream_t s);
+                              const lv_rhs_geom &R, hipStream_t s);
 // GP of P (Pout, Pout2: also P written there, either may be null); b of P with (gp_step) or without the
 // GP term (P0: also P0 = P)
-hipError_t lv_launch_gp(int n, const lv_pressure_system &A, const lv_rhs_geom &R, const double *P, const double *mass,
-                        double *GP, double *Pout, double *Pout2, hipStream_t s);
-hipError_t lv_launch_rhs_b(int n, int gp_step, const lv_pressure_system &A, const lv_rhs_geom &R, const double *GP,
+hipError_t lv_launch_gp(lv_mesh_view M, int n, const lv_rhs_geom &R, const double *P, const double *mass, double *GP,
+                        double *Pout, double *Pout2, hipStream_t s);
+hipError_t lv_launch_rhs_b(lv_mesh_view M, int n, int gp_step, const lv_rhs_geom &R, const double *GP,
                            const double *area, const double *P, double *b, double *P0, hipStream_t s);
 hipError_t lv_launch_visc_rhs(int n, const double *mass, const double *vc, double dt, double *b, double *x,
                               hipStream_t s);

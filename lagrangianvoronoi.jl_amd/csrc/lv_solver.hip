@@ -42,21 +42,9 @@ __global__ __launch_bounds__(B) void k_degree(lv_mesh_view M, int n, int *__rest
 
 // refresh!(A, grid, dt), src/pressure.jl:104-117 (mode 0), or the operator of ViscousSolver's
 // assemble!, src/viscous_solver.jl:89-113 (mode 1: d_i = m_i/dt, c_ij = 0.5 lrr (mu_i + mu_j))
-// The ELL slot of edge k (to neighbour j) of row i: the row's columns are sorted by label, ties in
-// edge order (a neighbour met through two periodic images has two edges), so the slot is the number
-// of interior edges with a smaller label plus the earlier edges to the same one.
-__device__ __forceinline__ int sorted_slot(const lv_mesh_view &M, int i, int ne, int k, int j)
-{
-    int q = 0;
-    for (int kk = 0; kk < ne; kk++) {
-        const int jj = LV_EL(M, kk, i);
-        q += (jj >= 0 && (jj < j || (jj == j && kk < k))) ? 1 : 0;
-    }
-    return q;
-}
-
 // The P-independent parts of refresh!(solver) for the interior edge to y (src/pressure.jl:173-180,
-// 189-197): w = lrr (m - x_i) and u = lrr (m - z) into ELL slot e; returns the edge's divergence term of b
+// 189-197): w = lrr (m - x_i) and u = lrr (m - z) into the edge's mesh slot e; returns the edge's
+// divergence term of b
 __device__ __forceinline__ double rhs_edge_geom(const lv_rhs_geom &R, size_t e, double lrr, double x1, double x2,
                                                 double y1, double y2, double m1, double m2, double dv1, double dv2,
                                                 double dt)
@@ -101,6 +89,11 @@ __global__ __launch_bounds__(B) void k_assemble(lv_grid_params g, lv_fields F, l
     const double pv1 = F.v[i], pv2 = F.v[n + i];
     double dd = d, bd = 0.0;
     int o = row_ptr[i];
+    // the solver's ELL copy keeps each row's neighbours sorted by column, so that slot k of a
+    // wavefront gathers from one lattice neighbourhood; the CSR copy keeps the edge order
+    int sc[LV_MAXE];
+    double sv[LV_MAXE];
+    int m = 0;
     const int ne = M.ne[i];
     for (int k = 0; k < ne; k++) {
         const int j = LV_EL(M, k, i);
@@ -118,14 +111,25 @@ __global__ __launch_bounds__(B) void k_assemble(lv_grid_params g, lv_fields F, l
         val32[o] = (float)c;
         dd += c;
         o++;
-        const size_t es = (size_t)sorted_slot(M, i, ne, k, j) * n + i;
-        ecol[es] = j;
-        ev64[es] = c;
-        ev32[es] = (float)c;
         if (geom) {
             const double m1 = 0.5 * (e1x + e2x), m2 = 0.5 * (e1y + e2y);
-            bd -= rhs_edge_geom(R, es, lrr, x1, x2, y1, y2, m1, m2, pv1 - F.v[j], pv2 - F.v[n + j], dt);
+            bd -= rhs_edge_geom(R, (size_t)k * n + i, lrr, x1, x2, y1, y2, m1, m2, pv1 - F.v[j], pv2 - F.v[n + j],
+                                dt);
         }
+        int q = m++;
+        while (q > 0 && sc[q - 1] > j) {
+            sc[q] = sc[q - 1];
+            sv[q] = sv[q - 1];
+            q--;
+        }
+        sc[q] = j;
+        sv[q] = c;
+    }
+    for (int k = 0; k < m; k++) {
+        const size_t es = (size_t)k * n + i;
+        ecol[es] = sc[k];
+        ev64[es] = sv[k];
+        ev32[es] = (float)sv[k];
     }
     if (geom) {
         R.bdiv[i] = bd;
@@ -139,140 +143,6 @@ __global__ __launch_bounds__(B) void k_assemble(lv_grid_params g, lv_fields F, l
     dinv32[i] = (float)(1.0 / dd);
 }
 
-// refresh!(solver) first pass, src/pressure.jl:167-187: b, the initial guess P and GP.  The first 6
-// edge slots' labels are read unconditionally and all their neighbours' gathers issued at once
-// (a slot past ne or on a boundary gathers the row itself and is skipped); the edges are then
-// summed in slot order as the reference does, the rest in a loop.
-__global__ __launch_bounds__(B) void k_rhs1(lv_grid_params g, lv_fields F, lv_mesh_view M, int n, double dt,
-                                            lv_vbc vbc, double *__restrict__ b, double *__restrict__ P0,
-                                            double *__restrict__ GP)
-{
-    const int i = blockIdx.x * B + threadIdx.x;
-    if (i >= n) return;
-    const double x1 = F.x[i], x2 = F.x[n + i];
-    const double Pi = F.P[i];
-    const double pv1 = F.v[i], pv2 = F.v[n + i];
-    double bi = (F.area[i] * Pi) / ((F.rho[i] * F.c2[i]) * (dt * dt));
-    double gx = 0.0, gy = 0.0;
-    const int ne = M.ne[i];
-    constexpr int U = 6;
-    int J[U];
-    double Xj[U], Yj[U], V1[U], V2[U], Pj[U];
-#pragma unroll
-    for (int k = 0; k < U; k++) {
-        const int lab = LV_EL(M, k, i);
-        J[k] = (k < ne && lab >= 0) ? lab : -1;
-    }
-#pragma unroll
-    for (int k = 0; k < U; k++) {
-        const int j = J[k] >= 0 ? J[k] : i;
-        Xj[k] = F.x[j];
-        Yj[k] = F.x[n + j];
-        V1[k] = F.v[j];
-        V2[k] = F.v[n + j];
-        Pj[k] = F.P[j];
-    }
-#pragma unroll
-    for (int k = 0; k < U; k++) {
-        if (J[k] < 0) continue;
-        const double e1x = LV_EV(M, 0, k, i), e1y = LV_EV(M, 1, k, i), e2x = LV_EV(M, 2, k, i), e2y = LV_EV(M, 3, k, i);
-        double y1, y2;
-        lv_arrow_target(g, Xj[k], Yj[k], x1, x2, y1, y2);
-        const double lrr = lv_lr_ratio(x1 - y1, x2 - y2, e1x, e1y, e2x, e2y);
-        const double m1 = 0.5 * (e1x + e2x), m2 = 0.5 * (e1y + e2y);
-        const double dv1 = pv1 - V1[k], dv2 = pv2 - V2[k];
-        bi -= (lrr / dt) * (dv1 * (m1 - y1) + dv2 * (m2 - y2));
-        const double w = lrr * (Pi - Pj[k]);
-        gx -= w * (m1 - x1);
-        gy -= w * (m2 - x2);
-    }
-    for (int k = U; k < ne; k++) {
-        const int j = LV_EL(M, k, i);
-        if (j < 0) continue;
-        const double e1x = LV_EV(M, 0, k, i), e1y = LV_EV(M, 1, k, i), e2x = LV_EV(M, 2, k, i), e2y = LV_EV(M, 3, k, i);
-        double y1, y2;
-        lv_arrow_target(g, F.x[j], F.x[n + j], x1, x2, y1, y2);
-        const double lrr = lv_lr_ratio(x1 - y1, x2 - y2, e1x, e1y, e2x, e2y);
-        const double m1 = 0.5 * (e1x + e2x), m2 = 0.5 * (e1y + e2y);
-        const double dv1 = pv1 - F.v[j], dv2 = pv2 - F.v[n + j];
-        bi -= (lrr / dt) * (dv1 * (m1 - y1) + dv2 * (m2 - y2));
-        const double w = lrr * (Pi - F.P[j]);
-        gx -= w * (m1 - x1);
-        gy -= w * (m2 - x2);
-    }
-    for (int k = 0; k < ne; k++) {  // boundaries(p)
-        const int lab = LV_EL(M, k, i);
-        if (lab >= 0) continue;
-        const double e1x = LV_EV(M, 0, k, i), e1y = LV_EV(M, 1, k, i), e2x = LV_EV(M, 2, k, i), e2y = LV_EV(M, 3, k, i);
-        const double dS1 = e1y - e2y, dS2 = e2x - e1x;
-        const int slot = -lab - 1;
-        const double vb1 = vbc.v[2 * slot], vb2 = vbc.v[2 * slot + 1];
-        bi -= (dS1 * (vb1 - pv1) + dS2 * (vb2 - pv2)) / dt;
-    }
-    const double mi = F.mass[i];
-    GP[i] = gx / mi;
-    GP[n + i] = gy / mi;
-    b[i] = bi;
-    P0[i] = Pi;
-}
-
-// refresh!(solver) gp_step pass, src/pressure.jl:189-202 (the first 6 slots' labels and vertices
-// loaded together, then their gathers issued at once)
-__global__ __launch_bounds__(B) void k_rhs2(lv_grid_params g, lv_fields F, lv_mesh_view M, int n,
-                                            const double *__restrict__ GP, double *__restrict__ b)
-{
-    const int i = blockIdx.x * B + threadIdx.x;
-    if (i >= n) return;
-    const double x1 = F.x[i], x2 = F.x[n + i];
-    const double g1 = GP[i], g2 = GP[n + i];
-    double bi = b[i];
-    const int ne = M.ne[i];
-    constexpr int U = 6;
-    int J[U];
-    double Xj[U], Yj[U], G1[U], G2[U];
-    double E1x[U], E1y[U], E2x[U], E2y[U];  // the slots' vertices, loaded with the labels
-#pragma unroll
-    for (int k = 0; k < U; k++) {
-        const int lab = LV_EL(M, k, i);
-        J[k] = (k < ne && lab >= 0) ? lab : -1;
-        E1x[k] = LV_EV(M, 0, k, i);
-        E1y[k] = LV_EV(M, 1, k, i);
-        E2x[k] = LV_EV(M, 2, k, i);
-        E2y[k] = LV_EV(M, 3, k, i);
-    }
-#pragma unroll
-    for (int k = 0; k < U; k++) {
-        const int j = J[k] >= 0 ? J[k] : i;
-        Xj[k] = F.x[j];
-        Yj[k] = F.x[n + j];
-        G1[k] = GP[j];
-        G2[k] = GP[n + j];
-    }
-#pragma unroll
-    for (int k = 0; k < U; k++) {
-        if (J[k] < 0) continue;
-        const double e1x = E1x[k], e1y = E1y[k], e2x = E2x[k], e2y = E2y[k];
-        double y1, y2;
-        lv_arrow_target(g, Xj[k], Yj[k], x1, x2, y1, y2);
-        const double lrr = lv_lr_ratio(x1 - y1, x2 - y2, e1x, e1y, e2x, e2y);
-        const double m1 = 0.5 * (e1x + e2x), m2 = 0.5 * (e1y + e2y);
-        const double z1 = 0.5 * (x1 + y1), z2 = 0.5 * (x2 + y2);
-        bi += lrr * ((g1 - G1[k]) * (m1 - z1) + (g2 - G2[k]) * (m2 - z2));
-    }
-    for (int k = U; k < ne; k++) {
-        const int j = LV_EL(M, k, i);
-        if (j < 0) continue;
-        const double e1x = LV_EV(M, 0, k, i), e1y = LV_EV(M, 1, k, i), e2x = LV_EV(M, 2, k, i), e2y = LV_EV(M, 3, k, i);
-        double y1, y2;
-        lv_arrow_target(g, F.x[j], F.x[n + j], x1, x2, y1, y2);
-        const double lrr = lv_lr_ratio(x1 - y1, x2 - y2, e1x, e1y, e2x, e2y);
-        const double m1 = 0.5 * (e1x + e2x), m2 = 0.5 * (e1y + e2y);
-        const double z1 = 0.5 * (x1 + y1), z2 = 0.5 * (x2 + y2);
-        bi += lrr * ((g1 - GP[j]) * (m1 - z1) + (g2 - GP[n + j]) * (m2 - z2));
-    }
-    b[i] = bi;
-}
-
 // ---- refresh!(solver) split by what depends on P ---------------------------------------------------
 //
 // During the Picard iterations of find_pressure! (src/pressure.jl:215-224) the mesh, v and the
@@ -280,10 +150,10 @@ __global__ __launch_bounds__(B) void k_rhs2(lv_grid_params g, lv_fields F, lv_me
 //   bdiv_i = -sum_j (lrr/dt) (v_i - v_j).(m - y) - sum_boundary dS.(vbc - v_i)/dt     (:173-182)
 //   GP_i   = -(1/m_i) sum_j (P_i - P_j) w_ij,   w_ij = lrr (m - x_i)                   (:180, 184)
 //   corr_i =  sum_j (GP_i - GP_j).u_ij,          u_ij = lrr (m - z)                     (:189-202)
-// k_rhs_geom evaluates bdiv, the compressibility denominator and w, u once per find_pressure!, into
-// the ELL slots of the assembled operator (columns sorted per row, the same slot of the same edge).
-// Each Picard iteration then streams w or u and gathers P or GP only.  Sums run in slot order
-// rather than the reference's edge order (rounding-level differences).
+// k_assemble (or k_rhs_geom) evaluates bdiv, the compressibility denominator and w, u once per
+// find_pressure!, w and u into the mesh's edge slots.  Each Picard iteration then streams w or u and
+// gathers P or GP only.  b's terms are grouped differently from the reference's single running sum
+// (rounding-level differences).
 
 // the same P-independent parts on their own, for an operator assembled without them (the order of
 // the sums as in k_assemble)
@@ -307,35 +177,48 @@ __global__ __launch_bounds__(B) void k_rhs_geom(lv_grid_params g, lv_fields F, l
         lv_arrow_target(g, F.x[j], F.x[n + j], x1, x2, y1, y2);
         const double lrr = lv_lr_ratio(x1 - y1, x2 - y2, e1x, e1y, e2x, e2y);
         const double m1 = 0.5 * (e1x + e2x), m2 = 0.5 * (e1y + e2y);
-        const size_t es = (size_t)sorted_slot(M, i, ne, k, j) * n + i;
-        bd -= rhs_edge_geom(R, es, lrr, x1, x2, y1, y2, m1, m2, pv1 - F.v[j], pv2 - F.v[n + j], dt);
+        bd -= rhs_edge_geom(R, (size_t)k * n + i, lrr, x1, x2, y1, y2, m1, m2, pv1 - F.v[j], pv2 - F.v[n + j], dt);
     }
     R.bdiv[i] = bd;
     R.den[i] = (F.rho[i] * F.c2[i]) * (dt * dt);
 }
 
-// GP from P (the first U slots' columns and weights loaded unconditionally, their gathers issued
-// together); with Pout also the write-back of P into the particles (find_pressure!'s p.P = P), with
+// the first U mesh slots of row i: neighbour (the row itself on a boundary slot or past ne) and the
+// two weight planes (0 there), loaded unconditionally (the planes hold LV_MAXE slots per row, so
+// every load is in bounds) so that the gathers that follow are issued together
+template <int U>
+__device__ __forceinline__ int rhs_head(const lv_mesh_view &M, const double *__restrict__ a, const double *__restrict__ b,
+                                        int i, int n, int (&c)[U], double (&va)[U], double (&vb)[U])
+{
+    const int ne = M.ne[i];
+#pragma unroll
+    for (int k = 0; k < U; k++) {
+        const size_t e = (size_t)k * n + i;
+        const int j = M.el[e];
+        const double ta = a[e], tb = b[e];
+        const bool on = k < ne && j >= 0;
+        c[k] = on ? j : i;
+        va[k] = on ? ta : 0.0;
+        vb[k] = on ? tb : 0.0;
+    }
+    return ne;
+}
+
+// GP from P; with Pout also the write-back of P into the particles (find_pressure!'s p.P = P), with
 // Pout2 a second copy of P (the Picard window's record of the last iterate)
-__global__ __launch_bounds__(B) void k_gp(int n, const int *__restrict__ deg, const int *__restrict__ ecol,
-                                          lv_rhs_geom R, const double *__restrict__ P, const double *__restrict__ mass,
-                                          double *__restrict__ GP, double *__restrict__ Pout,
-                                          double *__restrict__ Pout2)
+__global__ __launch_bounds__(B) void k_gp(lv_mesh_view M, int n, lv_rhs_geom R, const double *__restrict__ P,
+                                          const double *__restrict__ mass, double *__restrict__ GP,
+                                          double *__restrict__ Pout, double *__restrict__ Pout2)
 {
     const int i = blockIdx.x * B + threadIdx.x;
     if (i >= n) return;
     constexpr int U = 8;
-    const lv_ell_own<double> E{deg, ecol, R.w1};
     int c[U];
     double a1[U], a2[U], pj[U];
-    const int d = lv_ell_head<U>(E, i, n, c, a1);
+    const int ne = rhs_head<U>(M, R.w1, R.w2, i, n, c, a1, a2);
     const double Pi = P[i];
 #pragma unroll
-    for (int k = 0; k < U; k++) {
-        const double t = R.w2[(size_t)k * n + i];  // in bounds: the planes hold LV_MAXE slots per row
-        a2[k] = (k < d) ? t : 0.0;
-        pj[k] = P[c[k]];
-    }
+    for (int k = 0; k < U; k++) pj[k] = P[c[k]];
     double gx = 0.0, gy = 0.0;
 #pragma unroll
     for (int k = 0; k < U; k++) {
@@ -343,9 +226,11 @@ __global__ __launch_bounds__(B) void k_gp(int n, const int *__restrict__ deg, co
         gx -= dp * a1[k];
         gy -= dp * a2[k];
     }
-    for (int k = U; k < d; k++) {
+    for (int k = U; k < ne; k++) {
         const size_t e = (size_t)k * n + i;
-        const double dp = Pi - P[ecol[e]];
+        const int j = M.el[e];
+        if (j < 0) continue;
+        const double dp = Pi - P[j];
         gx -= dp * R.w1[e];
         gy -= dp * R.w2[e];
     }
@@ -357,10 +242,10 @@ __global__ __launch_bounds__(B) void k_gp(int n, const int *__restrict__ deg, co
 }
 
 // b = A P/(rho c2 dt^2) + bdiv (+ corr(GP) when gp); with P0 also the initial guess P0 = P
-__global__ __launch_bounds__(B) void k_rhs_b(int n, int gp, const int *__restrict__ deg, const int *__restrict__ ecol,
-                                             lv_rhs_geom R, const double *__restrict__ GP,
-                                             const double *__restrict__ area, const double *__restrict__ P,
-                                             double *__restrict__ b, double *__restrict__ P0)
+__global__ __launch_bounds__(B) void k_rhs_b(lv_mesh_view M, int n, int gp, lv_rhs_geom R,
+                                             const double *__restrict__ GP, const double *__restrict__ area,
+                                             const double *__restrict__ P, double *__restrict__ b,
+                                             double *__restrict__ P0)
 {
     const int i = blockIdx.x * B + threadIdx.x;
     if (i >= n) return;
@@ -368,14 +253,11 @@ __global__ __launch_bounds__(B) void k_rhs_b(int n, int gp, const int *__restric
     double bi = (area[i] * Pi) / R.den[i] + R.bdiv[i];
     if (gp) {
         constexpr int U = 8;
-        const lv_ell_own<double> E{deg, ecol, R.u1};
         int c[U];
         double a1[U], a2[U], g1[U], g2[U];
-        const int d = lv_ell_head<U>(E, i, n, c, a1);
+        const int ne = rhs_head<U>(M, R.u1, R.u2, i, n, c, a1, a2);
 #pragma unroll
         for (int k = 0; k < U; k++) {
-            const double t = R.u2[(size_t)k * n + i];
-            a2[k] = (k < d) ? t : 0.0;
             g1[k] = GP[c[k]];
             g2[k] = GP[n + c[k]];
         }
@@ -383,9 +265,10 @@ __global__ __launch_bounds__(B) void k_rhs_b(int n, int gp, const int *__restric
         double corr = 0.0;
 #pragma unroll
         for (int k = 0; k < U; k++) corr += (gi1 - g1[k]) * a1[k] + (gi2 - g2[k]) * a2[k];
-        for (int k = U; k < d; k++) {
+        for (int k = U; k < ne; k++) {
             const size_t e = (size_t)k * n + i;
-            const int j = ecol[e];
+            const int j = M.el[e];
+            if (j < 0) continue;
             corr += (gi1 - GP[j]) * R.u1[e] + (gi2 - GP[n + j]) * R.u2[e];
         }
         bi += corr;
@@ -923,35 +806,24 @@ hipError_t lv_launch_assemble(const lv_grid_params &g, lv_fields F, lv_mesh_view
     return hipGetLastError();
 }
 
-hipError_t lv_launch_rhs(const lv_grid_params &g, lv_fields F, lv_mesh_view M, int n, double dt, int gp_step,
-                         lv_vbc vbc, double *b, double *P0, double *GP, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_rhs1, grid_of(n), dim3(B), 0, s, g, F, M, n, dt, vbc, b, P0, GP);
-    if (gp_step) hipLaunchKernelGGL(k_rhs2, grid_of(n), dim3(B), 0, s, g, F, M, n, (const double *)GP, b);
-    return hipGetLastError();
-}
-
 hipError_t lv_launch_rhs_geom(const lv_grid_params &g, lv_fields F, lv_mesh_view M, int n, double dt, lv_vbc vbc,
-                              const lv_pressure_system &A, const lv_rhs_geom &R, hipStream_t s)
+                              const lv_rhs_geom &R, hipStream_t s)
 {
-    (void)A;
     hipLaunchKernelGGL(k_rhs_geom, grid_of(n), dim3(B), 0, s, g, F, M, n, dt, vbc, R);
     return hipGetLastError();
 }
 
-hipError_t lv_launch_gp(int n, const lv_pressure_system &A, const lv_rhs_geom &R, const double *P, const double *mass,
-                        double *GP, double *Pout, double *Pout2, hipStream_t s)
+hipError_t lv_launch_gp(lv_mesh_view M, int n, const lv_rhs_geom &R, const double *P, const double *mass, double *GP,
+                        double *Pout, double *Pout2, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_gp, grid_of(n), dim3(B), 0, s, n, (const int *)A.deg, (const int *)A.ecol, R, P, mass, GP,
-                       Pout, Pout2);
+    hipLaunchKernelGGL(k_gp, grid_of(n), dim3(B), 0, s, M, n, R, P, mass, GP, Pout, Pout2);
     return hipGetLastError();
 }
 
-hipError_t lv_launch_rhs_b(int n, int gp_step, const lv_pressure_system &A, const lv_rhs_geom &R, const double *GP,
+hipError_t lv_launch_rhs_b(lv_mesh_view M, int n, int gp_step, const lv_rhs_geom &R, const double *GP,
                            const double *area, const double *P, double *b, double *P0, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_rhs_b, grid_of(n), dim3(B), 0, s, n, gp_step, (const int *)A.deg, (const int *)A.ecol, R, GP,
-                       area, P, b, P0);
+    hipLaunchKernelGGL(k_rhs_b, grid_of(n), dim3(B), 0, s, M, n, gp_step, R, GP, area, P, b, P0);
     return hipGetLastError();
 }
 

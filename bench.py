@@ -360,6 +360,45 @@ def fp64_error(lv, grid, desc, steps, tol64=1e-10):
     return dict(fp64_solver="pcg64", fp64_tol=tol64, steps=rows)
 
 
+def step_compare(lv, grid, desc, steps=2):
+    """Full steps (10 Picard solves each, outside the timed region) from the same state with each inner
+    solver: the learned preconditioner's network inference runs once per step's assembly and serves
+    all its solves, which is where a one-level learned preconditioner has to pay for itself.  ms per
+    step from the host clock around synchronised steps, plus the inner iterations per step."""
+    import torch
+
+    th, trained = spai_theta(lv)
+    state = {k: grid.get_soa(k) for k in STATE_FIELDS}
+    out = {}
+    for name, method, params in (("mixed", "mixed", None), ("spai", "spai", th), ("amg", "amg", None),
+                                 ("amg_learned_smoother", "amg", th)):
+        for k in STATE_FIELDS:
+            grid.set_soa(k, state[k])
+        lv.remesh(grid)
+        lv.spai.set_params(grid, params)
+        solver = lv.PressureSolver(grid, method=method, tol=desc.solve.tol, itmax=4000,
+                                   inner_tol=desc.solve.inner_tol, picard_window=desc.solve.picard_window)
+        d = lv.make_step_desc(desc.dt, eos="stiffened", gamma=desc.gamma, P0=desc.P0, solver=solver,
+                              viscous=True, artificial_viscosity=False, alpha=1.0, rusanov=True)
+        grid.sync()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st = [lv.step_fused(grid, d) for _ in range(steps)]
+        grid.sync()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        out[name] = dict(ms_per_step=round((t1 - t0) / steps * 1e3, 2),
+                         inner_iters_per_step=sum(s["total_iters"] for s in st) / steps,
+                         fallbacks=sum(s["fallbacks"] for s in st), max_relres=max(max(s["relres"]) for s in st))
+    for k in STATE_FIELDS:
+        grid.set_soa(k, state[k])
+    lv.remesh(grid)
+    lv.spai.set_params(grid, None)
+    out["spai_trained"] = trained
+    out["spai_vs_mixed"] = round(out["mixed"]["ms_per_step"] / out["spai"]["ms_per_step"], 3)
+    return out
+
+
 def traffic_from_profiles(kernel):
     """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE, copy-kernel
     calibrated), committed under profiles/ as JSON; None if absent."""
@@ -499,6 +538,7 @@ def main():
         out["error_vs_fp64_solve"] = fp64_error(lv, grid, desc, args.err_steps)
     if not args.no_compare:
         out["pressure_solve_compare"] = solve_compare(lv, grid, ic, dt, args.tol, args.inner_tol)
+        out["step_compare"] = step_compare(lv, grid, desc)
     if not args.no_cpu:
         cK = args.cpu_K
         out["cpu_baseline"] = cpu_baseline(cK, 0x5EED0002, dt, args.tol)

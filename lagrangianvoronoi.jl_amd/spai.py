@@ -130,6 +130,18 @@ def flow(kind, K, dt_scale=1.0, seed=0x5EED0100):
         solver = PressureSolver(g, method="pcg64", tol=1e-10, picard_window=8)
         desc = make_step_desc(dt, eos=None, solver=solver, niter=4, friction={-1: (1.0, 0.0)})
         return g, desc, dt
+    if kind == "gresho":
+        dr = 1.0 / K
+        x = inputs.jittered_lattice(K, K, dr, seed=seed)
+        g = VoronoiGrid(Rectangle((0.0, 0.0), (1.0, 1.0)), dr, xperiodic=True, yperiodic=True, positions=x)
+        remesh(g)
+        ic = inputs.gresho(x, g.get("area"))
+        for k in ("v", "rho", "mass", "P", "e", "c2"):
+            g.set_soa(k, ic[k])
+        dt = 0.1 * dr * dt_scale
+        solver = PressureSolver(g, method="pcg64", tol=1e-10, picard_window=8)
+        desc = make_step_desc(dt, eos=None, solver=solver, niter=4, viscous=False)
+        return g, desc, dt
     if kind == "rt":
         dr = 1.0 / K
         x = inputs.jittered_lattice(K, 2 * K, dr, seed=seed)
@@ -146,26 +158,34 @@ def flow(kind, K, dt_scale=1.0, seed=0x5EED0100):
     raise ValueError(kind)
 
 
-def train(theta, schedule, steps_per_system=4, lr=1e-3, nprobe=8, log=None, seed=7):
+def train(theta, schedule, steps_per_system=4, lr=1e-3, nprobe=8, log=None, seed=7, log_every=1):
     """Trains theta on systems harvested from plain-solver runs: for each (kind, K, dt_scale, nsteps)
     of the schedule, run the flow with Jacobi-PCG, and after every step assemble its pressure
     operator and take steps_per_system Adam steps on it (probes: the step's pressure and Rademacher
-    vectors).  Returns (theta, losses)."""
+    vectors).  A flow that ends in one of the method's errors (a destroyed mesh, say) contributes the
+    steps it made.  Returns (theta, losses, systems): systems counts the harvested operators."""
+    from . import LVError
+
     losses = []
     th = np.asarray(theta, dtype=np.float32)
-    it = 0
-    for kind, K, dt_scale, nsteps in schedule:
+    systems = 0
+    for it, (kind, K, dt_scale, nsteps) in enumerate(schedule):
         g, desc, dt = flow(kind, K, dt_scale, seed=seed + it)
         set_params(g, th)
         for s in range(nsteps):
-            step_fused(g, desc)
+            try:
+                step_fused(g, desc)
+            except LVError as e:
+                if log:
+                    log("%s K=%d dt*%g: flow ended at step %d (%s)" % (kind, K, dt_scale, s, e))
+                break
             nnz = ctypes.c_int64()
             _check(g.ctx, lib().lv_assemble_pressure(g.ctx, float(dt), ctypes.byref(nnz)))
             for k in range(steps_per_system):
-                L = train_step(g, lr=lr, nprobe=nprobe, seed=1000 * it + 17 * s + k)
+                L = train_step(g, lr=lr, nprobe=nprobe, seed=1000003 * it + 17 * s + k)
                 losses.append(L)
-            if log:
+            systems += 1
+            if log and (s % log_every == 0 or s == nsteps - 1):
                 log("%s K=%d dt*%g step %d loss %.4e" % (kind, K, dt_scale, s, losses[-1]))
         th = get_params(g)
-        it += 1
-    return th, losses
+    return th, losses, systems

@@ -150,19 +150,30 @@ def test_spai_without_parameters_uses_jacobi(lv, oracle, gpu):
     assert solver.last_stats["fallbacks"] == 1 and max(solver.last_stats["relres"]) <= 1e-9
 
 
-def test_training_lowers_loss_and_gains_over_jacobi(lv, oracle, gpu):
-    """Adam on systems harvested from plain-solver runs lowers the probe loss; on a held-out system
-    of the benchmark regime (c0 dt / dr = 25.6) the trained network meets the FP64 tolerance with at
-    most 0.65x the inner iterations of Jacobi (the method without it), and within 10% of the
-    untrained network's (on a homogeneous mesh the Neumann form it starts from is already close to
-    the best a preconditioner on A's own pattern can do: at most a 4x smaller condition number, so
-    2x fewer iterations)."""
+def test_training_on_the_gpu_lowers_the_loss(lv, oracle, gpu):
+    """Adam through the hand-written forward and backward kernels, on systems harvested from
+    plain-solver runs of three flows, lowers the probe loss."""
     th0 = lv.spai.init_params(0)
-    th, losses = lv.spai.train(th0, [("tg", 48, 1.0, 6), ("cavity", 40, 1.0, 4), ("tg", 48, 0.5, 6)],
-                               steps_per_system=8, lr=3e-4, nprobe=8)
+    th, losses, systems = lv.spai.train(th0, [("tg", 48, 1.0, 6), ("cavity", 40, 1.0, 4), ("tg", 48, 0.5, 6)],
+                                        steps_per_system=8, lr=3e-4, nprobe=8)
+    assert systems == 16
     first, last = np.mean(losses[:8]), np.mean(losses[-8:])
     assert last < 0.97 * first, (first, last)
-    g, desc, dt = lv.spai.flow("tg", 96, 1.0, seed=0x5EED0999)
+
+
+def test_trained_network_gains_over_jacobi(lv, oracle, gpu):
+    """The committed network (trained on 20,000 systems harvested from plain-solver runs of every
+    configuration family, lagrangianvoronoi.jl_amd/train_spai.py) on a held-out system of the benchmark
+    regime (Taylor-Green 128^2, c0 dt/dr of the 2048^2 benchmark, a seed not used in training): it meets
+    the FP64 tolerance in at most 0.6x the inner iterations of Jacobi-PCG (the method without it), and
+    in no more than the untrained network's.  A preconditioner on A's own pattern can shrink the
+    condition number by at most 4x, i.e. halve the iterations; the untrained network is the Neumann
+    form I + N, which already reaches about 1.9x, so training lowers the probe loss (by 41% over the
+    20,000 systems) but the iterations by at most about 1% (169 against 169 here, 250 against 252 at
+    256^2)."""
+    th = lv.spai.load_params()
+    assert th is not None
+    g, desc, dt = lv.spai.flow("tg", 128, 1.0, seed=0x5EED0999)
     lv.stiffened_eos(g, 1.4, 2500.0 / 1.4)
     solver = lv.PressureSolver(g, method="mixed", tol=1e-8, itmax=4000)
     solver.assemble(dt)
@@ -171,11 +182,11 @@ def test_training_lowers_loss_and_gains_over_jacobi(lv, oracle, gpu):
     solver.solve(b, x0, method="mixed")
     it_jac = solver.last_stats["iters"][0]
     its = {}
-    for name, theta in (("untrained", th0), ("trained", th)):
+    for name, theta in (("untrained", lv.spai.init_params(0)), ("trained", th)):
         lv.spai.set_params(g, theta)
         solver.solve(b, x0, method="spai")
         assert max(solver.last_stats["relres"]) <= 1e-8 and solver.last_stats["fallbacks"] == 0
         its[name] = solver.last_stats["iters"][0]
     print("inner iterations: jacobi %d, untrained %d, trained %d" % (it_jac, its["untrained"], its["trained"]))
-    assert its["trained"] <= 0.65 * it_jac, (its, it_jac)
-    assert its["trained"] <= 1.1 * its["untrained"], its
+    assert its["trained"] <= 0.6 * it_jac, (its, it_jac)
+    assert its["trained"] <= its["untrained"], its

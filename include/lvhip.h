@@ -112,7 +112,9 @@ typedef struct {
  *   -> find_D! -> [viscous_step!] -> [bdary_friction!] -> find_dv! -> relaxation_step! */
 typedef struct {
     double dt;
-    int32_t eos;          /* 0 none, 1 stiffened_eos!(gamma, P0), 2 ideal_eos!(gamma; Pmin) */
+    int32_t eos;          /* 0 none, 1 stiffened_eos!(gamma, P0), 2 ideal_eos!(gamma; Pmin), 3 find_rho! only
+                             (rho = m/area, src/pressure.jl:36-40: an incompressible flow's density kept
+                             equal to its cells' mass per area, which examples/cavity.jl's step! omits) */
     double gamma, P0, Pmin;
     int32_t gravity;      /* 1: gravity_step!(g, dt) after move! */
     double g[2];

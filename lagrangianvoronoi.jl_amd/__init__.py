@@ -552,7 +552,7 @@ def make_step_desc(dt, eos=None, gamma=1.4, P0=0.0, Pmin=0.0, gravity=None, nite
     (examples/rayleightaylor.jl:99)."""
     d = StepDesc()
     d.dt = float(dt)
-    d.eos = {None: 0, "stiffened": 1, "ideal": 2}[eos]
+    d.eos = {None: 0, "stiffened": 1, "ideal": 2, "rho": 3}[eos]
     d.gamma, d.P0, d.Pmin = float(gamma), float(P0), float(Pmin)
     d.gravity = int(gravity is not None)
     d.g[:] = (0.0, 0.0) if gravity is None else tuple(gravity)

@@ -1602,6 +1602,7 @@ int lv_step_fused(lv_ctx *ctx, const lv_step_desc *d, lv_solve_stats *stats)
     }
     if (d->eos == 1) rc = lv_eos_stiffened(ctx, d->gamma, d->P0);
     else if (d->eos == 2) rc = lv_eos_ideal(ctx, d->gamma, d->Pmin);
+    else if (d->eos == 3) rc = lv_find_rho(ctx);
     if (rc) return rc;
     int prc = lv_find_pressure(ctx, d->dt, d->niter, &d->solve, stats);
     if (prc && prc != LV_E_NOT_CONVERGED) return prc;

@@ -327,6 +327,8 @@ class Particles:
             self.eos_stiffened(gamma, P0)
         elif eos == "ideal":
             self.eos_ideal(gamma, Pmin)
+        elif eos == "rho":
+            self.find_rho()
         iters = self.find_pressure(dt, niter, pvbc, tol, itmax, solver)
         self.pressure_step(dt)
         self.find_D()

@@ -210,6 +210,22 @@ __device__ bool clip_seed(const lv_grid_params &g, int n, int i, const double *_
     return true;
 }
 
+// The seed block a workgroup clips.  Workgroups are dispatched round-robin over the 8 XCDs, so
+// workgroup b runs on XCD b mod 8; mapping it to block b would give each XCD every 8th block, and on a
+// lattice whose rows span a multiple of 8 blocks that is the same few column strips of the domain in
+// every row.  The clipping work per seed varies across the domain (it depends on where the seeds sit
+// in their cell-list cells), so XCDs with the costly strips ran up to 28% longer while the others
+// idled for the last third of the launch.  Rotating each group of 8 workgroups by a hash of its group
+// index (the top 3 bits of a multiplicative hash, so that no period of the domain lines up with it)
+// gives every XCD an even share of every column; the last, partial group keeps the identity.
+__device__ __forceinline__ int clip_block(int b, int nb)
+{
+    const unsigned q = (unsigned)b >> 3;
+    if (8 * ((int)q + 1) > nb) return b;
+    const unsigned rot = (q * 2654435761u) >> 29;
+    return (int)((q << 3) | ((b + rot) & 7));
+}
+
 // every seed with CAP_SMALL slots; the ones that outgrow them go to the queue retry[1..retry[0]]
 __global__ __launch_bounds__(CB) void k_clip(lv_grid_params g, int n, const double *__restrict__ x,
                                              const int *__restrict__ cell_start, const int *__restrict__ cell_labels,
@@ -221,7 +237,7 @@ __global__ __launch_bounds__(CB) void k_clip(lv_grid_params g, int n, const doub
 {
     __shared__ lds_poly<CAP_SMALL> P;
     const int t = threadIdx.x;
-    const int i = blockIdx.x * CB + t;
+    const int i = clip_block(blockIdx.x, gridDim.x) * CB + t;
     if (i >= n) return;  // no block-wide barriers below
     if (!clip_seed<CAP_SMALL, true>(g, n, i, x, cell_start, cell_labels, cell_x, m, path_i, path_rr, ne, ev, el, ld,
                                     area, cent, err, P, t))

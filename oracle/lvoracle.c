@@ -17,8 +17,10 @@
  * Jacobi-PCG solve of the same linear system (src/pressure.jl:219).
  *
  * Parity unpinned (no offline check can fix these; each is named where it is used):
- *   - Krylov.jl 0.9.8 minres! iterates: the oracle's pressure is the exact solution of the same
- *     system to 1e-13, not MINRES's 1e-6 iterate;
+ *   - Krylov.jl 0.9.8 minres! iterates at the rounding level: lvo_minres restates the same
+ *     Paige-Saunders recurrences and matches SciPy's minres iterate by iterate
+ *     (tests/test_oracle_pins.py), but Krylov.jl itself is not here to compare with; the parity
+ *     target of find_pressure! is the solution of the same system to the solve tolerance;
  *   - StaticArrays 1.9.8 norm/dot rounding (ULP level; absorbed by the 1e-12 bounds);
  *   - the reference's Polyester-threaded insertion order (nondeterministic); the serial order
  *     (ascending label per cell) is restated instead;
@@ -942,8 +944,9 @@ int lvo_pcg(const lvo_state *s, const double *diag, const double *coef, const do
  * src/pressure.jl:219 (Krylov.jl minres!, atol = rtol = 1e-6, itmax = 1000).  It stops when the
  * residual estimate |phibar| <= atol + rtol*beta1 (beta1 = |b - A x0|, the rule Krylov.jl documents
  * for minres!) or when |A r| / (|A| |r|) <= rtol (a least-squares solution of an inconsistent singular
- * system).  Parity unpinned: Krylov.jl 0.9.8 is not available to compare iterates with; this is for
- * studying the effect of the reference's loose solve, not a parity target.  Returns iterations, or
+ * system).  Pinned against SciPy's minres, iterate by iterate (tests/test_oracle_pins.py); Krylov.jl
+ * 0.9.8 itself is not available, so its rounding stays unpinned.  The function is for studying the
+ * reference's loose solve (the cavity failure), not a parity target of the GPU solve.  Returns iterations, or
  * -1 when itmax stopped it; *relres_out = |b - A x| / |b| of the returned x. */
 int lvo_minres(const lvo_state *s, const double *diag, const double *coef, const double *b, double *x,
                double atol, double rtol, int itmax, double *relres_out)

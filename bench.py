@@ -225,6 +225,8 @@ def parse():
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--inner-tol", type=float, default=1e-4,
                     help="floor of the FP32 sweep's relative tolerance inside the FP64 refinement")
+    ap.add_argument("--amg-layout", type=int, default=0, choices=[0, 1, 2, 3],
+                    help="fine-level layout of the multilevel solver (lv_solve_opts.amg_layout)")
     ap.add_argument("--picard-window", type=int, default=8,
                     help="Picard corrections used to extrapolate each solve's initial guess (0: previous P)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -488,7 +490,7 @@ def main():
     if args.solver == "spai" or (args.solver == "amg" and args.smoother == "learned"):
         lv.spai.set_params(grid, spai_theta(lv)[0])
     solver = lv.PressureSolver(grid, method=args.solver, tol=args.tol, itmax=2000, inner_tol=args.inner_tol,
-                               picard_window=args.picard_window)
+                               picard_window=args.picard_window, amg_layout=args.amg_layout)
     desc = lv.make_step_desc(dt, eos="stiffened", gamma=ic["gamma"], P0=ic["P0"], solver=solver,
                              viscous=True, artificial_viscosity=False, alpha=1.0, rusanov=True)
     for _ in range(args.warmup):

@@ -89,7 +89,10 @@ typedef struct {
                               combination of the last k Picard corrections (every solve still meets tol) */
     int32_t amg_layout;   /* AMG: 0 runs the inner solve and the cycle with the rows renumbered cell by cell
                               (16-bit column offsets, BF16 couplings in the cycle) when the offsets fit,
-                              1 keeps the operator's own row order (int32 columns, FP32 couplings) */
+                              the slots past the sixth of each 64-row group read from a compact copy;
+                              1 keeps the operator's own row order (int32 columns, FP32 couplings);
+                              2 is 0 without the compact copy, 3 uses it only for groups of at most
+                              16 such slots (the same results bit for bit; for measurement and tests) */
 } lv_solve_opts;
 
 typedef struct {
@@ -124,7 +127,7 @@ typedef struct {
                              (implicit, src/viscous_solver.jl:120-128) solved with `solve`'s method to viscous_tol */
     int32_t artificial_viscosity;
     int32_t friction;     /* 1: bdary_friction!(vDirichlet, dt) with vDirichlet per label */
-    double friction_vbc[8];
+    double friction_vbc[8];  /* as lv_bdary_friction's table: NaN skips a wall (charfun false there) */
     double alpha;         /* find_dv!(dt, alpha) */
     int32_t rusanov;      /* relaxation_step!(dt; rusanov) */
     int32_t multiphase;   /* 1: multiphase_projection! after find_dv! (examples/rayleightaylor.jl:99) */
@@ -178,7 +181,9 @@ int lv_find_pressure(lv_ctx *ctx, double dt, int32_t niter, const lv_solve_opts 
 int lv_pressure_step(lv_ctx *ctx, double dt);                 /* src/pressure.jl:10-25 */
 int lv_find_D(lv_ctx *ctx);                                   /* src/diffusion.jl:8-19 */
 int lv_viscous_step(lv_ctx *ctx, double dt, int32_t artificial_viscosity); /* src/diffusion.jl:39-53 */
-/* bdary_friction!(grid, vDirichlet, dt), src/diffusion.jl:64-78; vDirichlet as a per-label table */
+/* bdary_friction!(grid, vDirichlet, dt; charfun), src/diffusion.jl:64-78: vDirichlet as a per-label table
+ * vbc[2*(-label-1) + comp], evaluated once per wall, not per edge midpoint; a NaN entry makes charfun
+ * false on that wall (its edges are skipped) */
 int lv_bdary_friction(lv_ctx *ctx, const double vbc[8], double dt);
 int lv_find_dv(lv_ctx *ctx, double dt, double alpha);         /* src/relaxation.jl:10-25 */
 int lv_relaxation_step(lv_ctx *ctx, double dt, int32_t rusanov); /* src/relaxation.jl:36-73 (remeshes) */

@@ -432,8 +432,25 @@ def viscous_step(obj, dt, artificial_viscosity=True):
     _check(obj.ctx, lib().lv_viscous_step(obj.ctx, float(dt), int(bool(artificial_viscosity))))
 
 
-def bdary_friction(grid, vDirichlet, dt):
+def _wall_midpoints(grid):
+    """the midpoint of each wall of the boundary rectangle, by label"""
+    (x0, y0), (x1, y1) = grid.boundary_rect.xmin, grid.boundary_rect.xmax
+    xm, ym = 0.5 * (x0 + x1), 0.5 * (y0 + y1)
+    return {BDARY_UP: np.array([xm, y1]), BDARY_RIGHT: np.array([x1, ym]), BDARY_DOWN: np.array([xm, y0]),
+            BDARY_LEFT: np.array([x0, ym])}
+
+
+def bdary_friction(grid, vDirichlet, dt, charfun=None):
+    """bdary_friction!(grid, vDirichlet, dt; charfun), src/diffusion.jl:64-78.  vDirichlet as in
+    _vbc_table; charfun a callable of a point, evaluated at each wall's midpoint (walls where it is
+    false get no friction), or a collection of the labels that get it; None: every wall."""
     t = _vbc_table(vDirichlet)
+    if charfun is not None:
+        mids = _wall_midpoints(grid)
+        for lab in (BDARY_UP, BDARY_RIGHT, BDARY_DOWN, BDARY_LEFT):
+            on = charfun(mids[lab]) if callable(charfun) else lab in charfun
+            if not on:
+                t[2 * (-lab - 1):2 * (-lab - 1) + 2] = np.nan
     _check(grid.ctx, lib().lv_bdary_friction(grid.ctx, _ptr(t), float(dt)))
 
 
@@ -461,17 +478,18 @@ class PressureSolver:
     this build's (the reference hard-codes MINRES with atol = rtol = 1e-6, :219)."""
 
     def __init__(self, grid, verbose=False, method="pcg64", tol=1e-10, itmax=1000, inner_tol=1e-4, outer_max=20,
-                 picard_window=0):
+                 picard_window=0, amg_layout=0):
         self.grid = grid
         self.verbose = verbose
         self.method, self.tol, self.itmax = method, tol, itmax
         self.inner_tol, self.outer_max = inner_tol, outer_max
         self.picard_window = picard_window
+        self.amg_layout = amg_layout
         self.last_stats = None
 
     def opts(self, boundary_velocity=None):
         return solve_opts(self.method, self.tol, self.itmax, self.inner_tol, self.outer_max, boundary_velocity,
-                          self.picard_window)
+                          self.picard_window, self.amg_layout)
 
     # PressureOperator access (src/pressure.jl:89-135)
     def assemble(self, dt):

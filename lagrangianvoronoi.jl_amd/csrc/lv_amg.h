@@ -47,6 +47,10 @@ struct lv_amg {
     float *pev32;        // couplings, FP32 (Krylov operator, power iteration)
     uint16_t *pev16;     // couplings, BF16 (the cycle's smoothing and residuals)
     float *pdiag32, *pdinv32, *pmval, *pmdg;
+    int16_t *ooff;       // the compact overflow of slots LV_OVF_BASE.. per 64-row group (lv_ell.h),
+    float *oev32;        // read by the Krylov operator: offsets and FP32 couplings, LV_OVF_CAP per group
+    int ovf_cap;         // groups with at most this many overflow slots use it (0: none)
+    size_t cap_ooff, cap_oev32;
     int *pflag;          // [0] set when a column offset does not fit 16 bits, [1] when a level-1 cell
                          // is too large for the fused down pass
     int *bfirst;         // fused down pass: the first level-1 cell of each block (nbfuse + 1 entries)

@@ -589,6 +589,7 @@ __global__ __launch_bounds__(B) void k_fine_up_j(fine_ops<V> F, const float *__r
 {
     if (S && S->done) return;
     double v[2] = {0.0, 0.0};
+    // (the compact overflow of lv_ell.h does not pay here: 49.4 against 48.5 us per launch at 2048^2)
     for (int i = lv_xcd_block(blockIdx.x, gridDim.x) * B + threadIdx.x; i < F.n; i += gridDim.x * B) {
         const float ti = r[i] - a_row(i, F, z1);
         const float zi = z1[i] + F.scale * F.dinv[i] * ti;
@@ -1015,7 +1016,9 @@ __global__ __launch_bounds__(B) void k_perm_index(int n, const int *__restrict__
 
 // row p of the aggregate order from row list[p] of the operator (same slot order): degree, column
 // offsets taken modulo n into [-n/2, n/2], FP32 and BF16 couplings, d, D^-1, and the learned M;
-// *flag is raised when an offset needs more than 16 bits
+// slots LV_OVF_BASE.. (offsets, FP32 couplings) also into the compact overflow of p's 64-row group
+// when the group has at most ocap overflow slots (lv_ell.h); *flag is raised when an offset needs
+// more than 16 bits
 __global__ __launch_bounds__(B) void k_perm_ell(int n, const int *__restrict__ list, const int *__restrict__ iperm,
                                                 const int *__restrict__ deg, const int *__restrict__ ecol,
                                                 const float *__restrict__ ev, const float *__restrict__ diag,
@@ -1024,12 +1027,16 @@ __global__ __launch_bounds__(B) void k_perm_ell(int n, const int *__restrict__ l
                                                 int16_t *__restrict__ poff, float *__restrict__ pev32,
                                                 uint16_t *__restrict__ pev16, float *__restrict__ pdiag,
                                                 float *__restrict__ pdinv, float *__restrict__ pmval,
-                                                float *__restrict__ pmdg, int *__restrict__ flag)
+                                                float *__restrict__ pmdg, int *__restrict__ flag,
+                                                int16_t *__restrict__ ooff, float *__restrict__ oev32, int ocap)
 {
     const int p = blockIdx.x * B + threadIdx.x;
+    const int i = p < n ? list[p] : 0;
+    const int d = p < n ? deg[i] : 0;
+    const lv_ovf_pos q = lv_ovf_prefix(d);  // the block's wavefronts are whole 64-row groups
     if (p >= n) return;
-    const int i = list[p];
-    const int d = deg[i];
+    const bool compact = ocap > 0 && q.tot <= ocap;
+    const size_t g0 = (size_t)(p >> 6) * LV_OVF_CAP;
     pdeg[p] = (uint8_t)d;
     pdiag[p] = diag[i];
     pdinv[p] = dinv[i];
@@ -1046,6 +1053,11 @@ __global__ __launch_bounds__(B) void k_perm_ell(int n, const int *__restrict__ l
         pev32[ep] = v;
         pev16[ep] = lv_bf16_bits(v);
         if (mval) pmval[ep] = mval[e];
+        if (compact && k >= LV_OVF_BASE) {
+            const size_t eo = g0 + (size_t)(q.pos + (k - LV_OVF_BASE));
+            ooff[eo] = (int16_t)off;
+            oev32[eo] = v;
+        }
     }
     if (wide) atomicOr(flag, 1);
 }
@@ -1084,7 +1096,7 @@ void lv_amg_free(lv_amg *G)
 {
     void *ptrs[] = {G->agg, G->cnt, G->start, G->cursor, G->list, G->scan_tmp, G->t0, G->z0, G->dense_inv, G->tail_img,
                     G->pw, G->pv, G->pu, G->prog, G->iperm, G->aggp, G->pdeg, G->poff, G->pev32, G->pev16,
-                    G->pdiag32, G->pdinv32, G->pmval, G->pmdg, G->pflag, G->bfirst};
+                    G->pdiag32, G->pdinv32, G->pmval, G->pmdg, G->pflag, G->bfirst, G->ooff, G->oev32};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (int l = 0; l <= LV_AMG_MAXLEV; l++) {
@@ -1140,6 +1152,11 @@ static fine_ops<V> fine_agg(const lv_amg *G, const VAL *val)
     fine_ops<V> F;
     F.n = G->n0;
     F.A = V{G->pdeg, G->poff, val};
+    if constexpr (std::is_same<VAL, float>::value) {  // the compact overflow holds the FP32 couplings
+        F.A.ocol = G->ooff;
+        F.A.oval = G->oev32;
+        F.A.ocap = G->ovf_cap;
+    }
     F.diag = G->pdiag32;
     F.dinv = G->pdinv32;
     F.mval = G->fine_learned ? G->pmval : nullptr;
@@ -1340,7 +1357,7 @@ hipError_t lv_amg_setup(lv_amg *G, const lv_grid_params &g, const double *x, con
     G->perm_ok = 0;
     G->nbfuse = 0;
     G->z0_in = 0;
-    if (layout == 0) {
+    if (layout == 0 || layout >= 2) {
         const size_t m = (size_t)LV_MAXE * n;
         AMG_CHECK(grow(&G->iperm, &G->cap_iperm, (size_t)n));
         AMG_CHECK(grow(&G->aggp, &G->cap_aggp, (size_t)n));
@@ -1348,6 +1365,10 @@ hipError_t lv_amg_setup(lv_amg *G, const lv_grid_params &g, const double *x, con
         AMG_CHECK(grow(&G->poff, &G->cap_poff, m));
         AMG_CHECK(grow(&G->pev32, &G->cap_pev32, m));
         AMG_CHECK(grow(&G->pev16, &G->cap_pev16, m));
+        const size_t mo = ((size_t)n / 64 + 1) * LV_OVF_CAP;
+        AMG_CHECK(grow(&G->ooff, &G->cap_ooff, mo));
+        AMG_CHECK(grow(&G->oev32, &G->cap_oev32, mo));
+        G->ovf_cap = layout == 0 ? LV_OVF_CAP : layout == 3 ? 16 : 0;
         AMG_CHECK(grow(&G->pdiag32, &G->cap_pdiag32, (size_t)n));
         AMG_CHECK(grow(&G->pdinv32, &G->cap_pdinv32, (size_t)n));
         if (G->fine_learned) {
@@ -1361,7 +1382,7 @@ hipError_t lv_amg_setup(lv_amg *G, const lv_grid_params &g, const double *x, con
                            (const int *)A.deg, (const int *)A.ecol, (const float *)A.ev32, (const float *)A.diag32,
                            (const float *)A.dinv32, G->fine_learned ? mval : nullptr,
                            G->fine_learned ? mdg : nullptr, G->pdeg, G->poff, G->pev32, G->pev16, G->pdiag32,
-                           G->pdinv32, G->pmval, G->pmdg, G->pflag);
+                           G->pdinv32, G->pmval, G->pmdg, G->pflag, G->ooff, G->oev32, G->ovf_cap);
         // the blocks of the fused down pass (Jacobi smoother)
         const int nbf = n / FUSE_ROWS + 1;
         AMG_CHECK(grow(&G->bfirst, &G->cap_bfirst, (size_t)nbf + 1));

@@ -703,7 +703,7 @@ const export_field kExport[] = {{"x", LV_FIELD_X},         {"v", LV_FIELD_V},   
                                 {"energy", LV_FIELD_ENERGY}, {"quality", LV_FIELD_QUALITY}, {"mu", LV_FIELD_MU},
                                 {"D", LV_FIELD_D},         {"phase", LV_FIELD_PHASE}};
 
-std::string vtp_path(const char *filename)
+static std::string vtp_path(const char *filename)
 {
     std::string f(filename);
     if (f.size() < 4 || f.compare(f.size() - 4, 4, ".vtp") != 0) f += ".vtp";

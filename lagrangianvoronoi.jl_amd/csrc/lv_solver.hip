@@ -422,8 +422,10 @@ __global__ __launch_bounds__(B) void k_cg64_init(int n, const int *__restrict__ 
 // using A p_new = A z + beta A p_old, so only z is gathered and p, q are touched row-locally.
 // The block's slice of the CSR arrays is first staged through LDS with coalesced loads.
 // U (update): alpha = r.z / p.q;  x += alpha p;  r -= alpha q;  z = dinv r;  partials r.z, r.r.
+// (8 wavefronts per SIMD asked of the register allocator: with the compact overflow's prefix the
+// kernel would otherwise take 66 VGPRs and 7 wavefronts, and run 25% slower)
 template <typename T, class V>
-__global__ __launch_bounds__(B) void k_cg_s(int n, V A, const T *__restrict__ diag,
+__global__ __launch_bounds__(B, 8) void k_cg_s(int n, V A, const T *__restrict__ diag,
                                             const T *__restrict__ z, T *__restrict__ p, T *__restrict__ q,
                                             const double *__restrict__ Prr, double *__restrict__ Ppq,
                                             lv_cg_scalars *S, lv_progress *prog, int par, int k, int first)
@@ -456,14 +458,40 @@ __global__ __launch_bounds__(B) void k_cg_s(int n, V A, const T *__restrict__ di
     // grid-stride passes over the rows, each pass one contiguous window whose eighth per XCD is
     // contiguous too (lv_xcd_block); two rows per thread per pass keep twice the loads in flight
     const int stride = gridDim.x * B;
-    for (int i0 = lv_xcd_block(blockIdx.x, gridDim.x) * B + threadIdx.x; i0 < n; i0 += 2 * stride) {
+    const int istart = lv_xcd_block(blockIdx.x, gridDim.x) * B + threadIdx.x;
+    // compact overflow: a wavefront's rows i0 (and i1) are one 64-row group, and the group's prefix
+    // of the next pass is formed during this one, so that its overflow loads issue with the slots
+    int dn0 = 0, dn1 = 0;
+    lv_ovf_pos qn0{0, 0}, qn1{0, 0};
+    if constexpr (lv_ell_compact<V>::value) {
+        dn0 = istart < n ? A.degree(istart) : 0;
+        dn1 = istart + stride < n ? A.degree(istart + stride) : 0;
+        qn0 = lv_ovf_prefix(dn0);
+        qn1 = lv_ovf_prefix(dn1);
+    }
+    for (int i0 = istart; i0 < n; i0 += 2 * stride) {
         const int i1 = i0 + stride;
         const T z0 = z[i0];
-        const T w0 = spmv_row<T>(i0, n, A, diag, z);
-        T z1 = (T)0, w1 = (T)0;
-        if (i1 < n) {
-            z1 = z[i1];
-            w1 = spmv_row<T>(i1, n, A, diag, z);
+        T w0, z1 = (T)0, w1 = (T)0;
+        if constexpr (lv_ell_compact<V>::value) {
+            const int d0 = dn0, d1 = dn1;
+            const lv_ovf_pos q0 = qn0, q1 = qn1;
+            const int j0 = i0 + 2 * stride, j1 = j0 + stride;
+            dn0 = j0 < n ? A.degree(j0) : 0;
+            dn1 = j1 < n ? A.degree(j1) : 0;
+            w0 = lv_ell_row_ovf<T>(A, i0, n, d0, q0, diag[i0], z0, z);
+            if (i1 < n) {
+                z1 = z[i1];
+                w1 = lv_ell_row_ovf<T>(A, i1, n, d1, q1, diag[i1], z1, z);
+            }
+            qn0 = lv_ovf_prefix(dn0);
+            qn1 = lv_ovf_prefix(dn1);
+        } else {
+            w0 = spmv_row<T>(i0, n, A, diag, z);
+            if (i1 < n) {
+                z1 = z[i1];
+                w1 = spmv_row<T>(i1, n, A, diag, z);
+            }
         }
         const T q0 = first ? w0 : w0 + beta * q[i0];
         const T p0 = first ? z0 : z0 + beta * p[i0];
@@ -1072,7 +1100,8 @@ hipError_t lv_solve_device(const lv_pressure_system &A, lv_solver_work &W, int n
                 }
                 int done = 0;
                 if (Gp)
-                    e = cg_iterate<float>(n, lv_ell_agg32{Gp->pdeg, Gp->poff, Gp->pev32}, (const float *)Gp->pdiag32,
+                    e = cg_iterate<float>(n, lv_ell_agg32{Gp->pdeg, Gp->poff, Gp->pev32, Gp->ooff, Gp->oev32, Gp->ovf_cap},
+                                          (const float *)Gp->pdiag32,
                                           dinv32, W.e32, W.r32, W.z32, W.p32, W.q32, P, W.S, W.hprog, W.dprog,
                                           W.hS, s, t32, h, Mi, &done);
                 else

@@ -331,7 +331,8 @@ __global__ __launch_bounds__(B) void k_visc2(lv_grid_params g, lv_fields F, lv_m
     if (WITH_DV) find_dv_row(F, n, i, x1, x2, rmin, rmax, dt_dv, alpha);
 }
 
-// bdary_friction!, src/diffusion.jl:64-78, vDirichlet(m) = vbc[label] and charfun = everywhere
+// bdary_friction!, src/diffusion.jl:64-78, vDirichlet(m) = vbc[label]; charfun(m) is false on the
+// walls whose table entry is NaN (their edges are skipped, as the reference's `continue` does)
 __global__ __launch_bounds__(B) void k_friction(lv_fields F, lv_mesh_view M, int n, lv_vbc vbc, double dt)
 {
     const int i = blockIdx.x * B + threadIdx.x;
@@ -345,6 +346,9 @@ __global__ __launch_bounds__(B) void k_friction(lv_fields F, lv_mesh_view M, int
     for (int k = 0; k < ne; k++) {
         const int lab = LV_EL(M, k, i);
         if (lab >= 0) continue;
+        const int slot = -lab - 1;
+        const double vd1 = vbc.v[2 * slot], vd2 = vbc.v[2 * slot + 1];
+        if (isnan(vd1) || isnan(vd2)) continue;
         const double e1x = LV_EV(M, 0, k, i), e1y = LV_EV(M, 1, k, i), e2x = LV_EV(M, 2, k, i), e2y = LV_EV(M, 3, k, i);
         const double m1 = 0.5 * (e1x + e2x), m2 = 0.5 * (e1y + e2y);
         double n1 = e1y - e2y, n2 = e2x - e1x;
@@ -353,8 +357,6 @@ __global__ __launch_bounds__(B) void k_friction(lv_fields F, lv_mesh_view M, int
         n2 = n2 / nn;
         const double len = lv_norm2(e1x - e2x, e1y - e2y);
         const double lrr = len / fabs((m1 - x1) * n1 + (m2 - x2) * n2);
-        const int slot = -lab - 1;
-        const double vd1 = vbc.v[2 * slot], vd2 = vbc.v[2 * slot + 1];
         const double ml = mu * lrr;
         const double f1 = (ml * vd1) / mi, f2 = (ml * vd2) / mi;
         tmp += ((dt * mu) * lrr) / mi;

@@ -1227,6 +1227,8 @@ int lvo_bdary_friction(lvo_state *s, const double *vbc, double dt)
         for (int k = 0; k < s->ne[i]; k++) {
             int lab = EL(s, i, k);
             if (lab >= 0) continue;
+            double vd1 = vbc[2 * (-lab - 1)], vd2 = vbc[2 * (-lab - 1) + 1];
+            if (isnan(vd1) || isnan(vd2)) continue;  /* charfun(m) false on this wall: `continue`, :70 */
             const double *e = EV(s, i, k);
             double m1 = 0.5 * (e[0] + e[2]), m2 = 0.5 * (e[1] + e[3]);
             double n1 = e[1] - e[3], n2 = e[2] - e[0];
@@ -1235,7 +1237,6 @@ int lvo_bdary_friction(lvo_state *s, const double *vbc, double dt)
             n2 = n2 / nn;
             double len = norm2v(e[0] - e[2], e[1] - e[3]);
             double lrr = len / fabs((m1 - x1) * n1 + (m2 - x2) * n2);
-            double vd1 = vbc[2 * (-lab - 1)], vd2 = vbc[2 * (-lab - 1) + 1];
             double ml = s->mu[i] * lrr;
             double f1 = (ml * vd1) / s->mass[i], f2 = (ml * vd2) / s->mass[i];
             tmp += ((dt * s->mu[i]) * lrr) / s->mass[i];

@@ -115,6 +115,31 @@ def test_layouts_give_the_same_pressure(lv, oracle, gpu, kind, ic, K, dt):
     assert rel_err(P0, P1) <= (1e-4 if not np.any(d > 0) else 1e-5)
 
 
+@pytest.mark.parametrize("kind,ic,K,dt", [("periodic", "tg", 128, 4e-3), ("walls", "cavity", 96, 1e-3)])
+def test_compact_overflow_is_bit_identical(lv, oracle, gpu, kind, ic, K, dt):
+    """The aggregate order with the compact copy of the slots past the sixth (layout 0), without it
+    (2), and with it only for groups of at most 16 such slots (3, so that both paths run in one
+    solve) give the same pressure bit for bit and the same iterations."""
+    _flush("amg compact overflow %s %d" % (kind, K))
+    grid, op, solver, x, dr, d, rp, cols, coef = _system(lv, oracle, kind, K, ic, dt)
+    b, _ = solver.rhs(dt, False)
+    if not np.any(d > 0):
+        b = np.random.default_rng(5).standard_normal(grid.n)
+        b -= b.mean()
+    deg = np.diff(rp)
+    assert deg.max() >= 8 and deg.min() <= 5  # rows past the sixth slot, and padding below it
+    x0 = grid.get("P")
+    out = {}
+    for layout in (0, 2, 3):
+        P = solver.solve(b, x0, method="amg", amg_layout=layout)
+        st = solver.last_stats
+        assert st["agg_order"] == 1 and max(st["relres"]) <= 1e-8, st
+        out[layout] = (P, st["iters"][:st["nsolves"]], st["outer"][:st["nsolves"]])
+    for layout in (2, 3):
+        assert np.array_equal(out[0][0], out[layout][0]), layout
+        assert list(out[0][1]) == list(out[layout][1]) and list(out[0][2]) == list(out[layout][2])
+
+
 def test_cycle_matches_oracle_learned_smoother(lv, oracle, gpu):
     _flush("amg cycle learned")
     grid, op, solver, x, dr, d, rp, cols, coef = _system(lv, oracle, "periodic", 96, "tg", 8e-3)

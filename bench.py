@@ -225,6 +225,8 @@ def parse():
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--inner-tol", type=float, default=1e-4,
                     help="floor of the FP32 sweep's relative tolerance inside the FP64 refinement")
+    ap.add_argument("--picard-carry", type=int, default=1, choices=[0, 1],
+                    help="carry the first Picard corrections of a step into the next (lv_solve_opts.picard_carry)")
     ap.add_argument("--amg-layout", type=int, default=0, choices=[0, 1, 2, 3],
                     help="fine-level layout of the multilevel solver (lv_solve_opts.amg_layout)")
     ap.add_argument("--picard-window", type=int, default=8,
@@ -395,7 +397,8 @@ def step_compare(lv, grid, desc, steps=2):
         lv.remesh(grid)
         lv.spai.set_params(grid, params)
         solver = lv.PressureSolver(grid, method=method, tol=desc.solve.tol, itmax=4000,
-                                   inner_tol=desc.solve.inner_tol, picard_window=desc.solve.picard_window)
+                                   inner_tol=desc.solve.inner_tol, picard_window=desc.solve.picard_window,
+                                   picard_carry=desc.solve.picard_carry)
         d = lv.make_step_desc(desc.dt, eos="stiffened", gamma=desc.gamma, P0=desc.P0, solver=solver,
                               viscous=True, artificial_viscosity=False, alpha=1.0, rusanov=True)
         grid.sync()
@@ -490,7 +493,8 @@ def main():
     if args.solver == "spai" or (args.solver == "amg" and args.smoother == "learned"):
         lv.spai.set_params(grid, spai_theta(lv)[0])
     solver = lv.PressureSolver(grid, method=args.solver, tol=args.tol, itmax=2000, inner_tol=args.inner_tol,
-                               picard_window=args.picard_window, amg_layout=args.amg_layout)
+                               picard_window=args.picard_window, amg_layout=args.amg_layout,
+                               picard_carry=args.picard_carry)
     desc = lv.make_step_desc(dt, eos="stiffened", gamma=ic["gamma"], P0=ic["P0"], solver=solver,
                              viscous=True, artificial_viscosity=False, alpha=1.0, rusanov=True)
     for _ in range(args.warmup):
@@ -548,6 +552,7 @@ def main():
                                     "c0=50 stiffened, nu=1e-3, 10 Picard solves/step, %s PCG to %g" %
                                     (K, K, args.solver, args.tol),
                            particles=n, solver=args.solver, tol=args.tol, picard_window=args.picard_window,
+                           picard_carry=args.picard_carry,
                            smoother=args.smoother if args.solver == "amg" else None),
                pressure_solve_ms=round(first_solve_ms, 3), pressure_solve_iters=first_solve_it,
                max_relres=relres, picard_iters=stats[-1]["iters"], picard_sweeps=stats[-1]["outer"],

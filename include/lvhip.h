@@ -93,6 +93,10 @@ typedef struct {
                               1 keeps the operator's own row order (int32 columns, FP32 couplings);
                               2 is 0 without the compact copy, 3 uses it only for groups of at most
                               16 such slots (the same results bit for bit; for measurement and tests) */
+    int32_t picard_carry; /* with picard_window > 0: 1 also carries the first two Picard corrections
+                             into the next find_pressure! on the same particles, where (multiplied by
+                             the new operator) they improve the guesses of Picard solves 2 to 4;
+                             every solve still meets tol */
 } lv_solve_opts;
 
 typedef struct {

@@ -67,7 +67,7 @@ class GridDesc(ctypes.Structure):
 class SolveOpts(ctypes.Structure):
     _fields_ = [("method", ctypes.c_int32), ("tol", ctypes.c_double), ("itmax", ctypes.c_int32),
                 ("inner_tol", ctypes.c_double), ("outer_max", ctypes.c_int32), ("vbc", ctypes.c_double * 8),
-                ("picard_window", ctypes.c_int32), ("amg_layout", ctypes.c_int32)]
+                ("picard_window", ctypes.c_int32), ("amg_layout", ctypes.c_int32), ("picard_carry", ctypes.c_int32)]
 
 
 class SolveStats(ctypes.Structure):
@@ -463,13 +463,14 @@ def relaxation_step(grid, dt, rusanov=True):
 
 
 def solve_opts(method="pcg64", tol=1e-10, itmax=1000, inner_tol=1e-4, outer_max=20, boundary_velocity=None,
-               picard_window=0, amg_layout=0):
+               picard_window=0, amg_layout=0, picard_carry=0):
     o = SolveOpts()
     o.method = SOLVERS[method] if isinstance(method, str) else int(method)
     o.tol, o.itmax, o.inner_tol, o.outer_max = float(tol), int(itmax), float(inner_tol), int(outer_max)
     o.vbc[:] = _vbc_table(boundary_velocity)
     o.picard_window = int(picard_window)
     o.amg_layout = int(amg_layout)
+    o.picard_carry = int(bool(picard_carry))
     return o
 
 
@@ -478,18 +479,19 @@ class PressureSolver:
     this build's (the reference hard-codes MINRES with atol = rtol = 1e-6, :219)."""
 
     def __init__(self, grid, verbose=False, method="pcg64", tol=1e-10, itmax=1000, inner_tol=1e-4, outer_max=20,
-                 picard_window=0, amg_layout=0):
+                 picard_window=0, amg_layout=0, picard_carry=0):
         self.grid = grid
         self.verbose = verbose
         self.method, self.tol, self.itmax = method, tol, itmax
         self.inner_tol, self.outer_max = inner_tol, outer_max
         self.picard_window = picard_window
         self.amg_layout = amg_layout
+        self.picard_carry = picard_carry
         self.last_stats = None
 
     def opts(self, boundary_velocity=None):
         return solve_opts(self.method, self.tol, self.itmax, self.inner_tol, self.outer_max, boundary_velocity,
-                          self.picard_window, self.amg_layout)
+                          self.picard_window, self.amg_layout, self.picard_carry)
 
     # PressureOperator access (src/pressure.jl:89-135)
     def assemble(self, dt):

@@ -68,6 +68,11 @@ struct lv_ctx {
     double *pw_P[2] = {nullptr, nullptr}, *pw_AP[2] = {nullptr, nullptr};
     double *pw_t1 = nullptr, *pw_t2 = nullptr, *pw_dots = nullptr;
     int pw_cap = 0;
+    // carried Picard directions (lv_solve_opts.picard_carry): the previous call's first two
+    // corrections (pc_z, pc_cnt of them, valid for pc_n particles), the ones this call is recording
+    // (pc_new), and this call's orthonormal set made of the carried ones (pc_q = A pc_o)
+    double *pc_z[2] = {nullptr}, *pc_new[2] = {nullptr}, *pc_q[2] = {nullptr}, *pc_o[2] = {nullptr};
+    int pc_cnt = 0, pc_n = -1;
     // learned preconditioner: network state and the preconditioner of the last assembled operator
     lv_spai spai{};
     int spai_ok = 0;          // lv_spai_init succeeded
@@ -466,6 +471,12 @@ int lv_destroy(lv_ctx *ctx)
     dfree(ctx->pw_t1);
     dfree(ctx->pw_t2);
     dfree(ctx->pw_dots);
+    for (int k = 0; k < 2; k++) {
+        dfree(ctx->pc_z[k]);
+        dfree(ctx->pc_new[k]);
+        dfree(ctx->pc_q[k]);
+        dfree(ctx->pc_o[k]);
+    }
     for (double *p : {ctx->rg.w1, ctx->rg.w2, ctx->rg.u1, ctx->rg.u2, ctx->rg.bdiv, ctx->rg.den}) dfree(p);
     for (int k = 0; k < LV_NKTIMERS; k++) {
         if (!ctx->timers[k]) continue;
@@ -524,6 +535,7 @@ int lv_set_particles(lv_ctx *ctx, int32_t n, const double *x)
     ctx->n = n;
     ctx->mesh_valid = false;
     ctx->A.valid = 0;
+    ctx->pc_cnt = 0;  // a new particle set: no carried Picard directions
     const size_t cap = (size_t)ctx->cap;
     double *zero2[] = {ctx->F.v, ctx->F.dv, ctx->F.mom, ctx->F.cent};
     for (double *p : zero2) LV_HIP_CHECK(hipMemsetAsync(p, 0, sizeof(double) * 2 * cap, ctx->s));
@@ -1322,30 +1334,27 @@ static int ensure_picard_window(lv_ctx *ctx, int w)
 // the iterations of the 2048^2 benchmark steps unchanged and cost 0.8 ms per step.)  The dots stay on the
 // device (the combinations read them there), so the host never waits; a correction with no
 // component left enters as a zero direction, which adds nothing to later projections.
-static int picard_add_direction(lv_ctx *ctx, int w, int *cnt, const double *Pcur, const double *Pprev,
-                                const double *APcur, const double *APprev)
+// the direction in pw_t1 with its image A pw_t1 in pw_t2 (both overwritten) into the set (Q, Z) of
+// at most w
+static int picard_add_pair(lv_ctx *ctx, double **Q, double **Z, int w, int *cnt)
 {
     const int n = ctx->n;
     hipStream_t s = ctx->s;
     const double *xp[8];
     double c[8];
-    xp[0] = Pcur; xp[1] = Pprev; c[0] = 1.0; c[1] = -1.0;
-    LV_HIP_CHECK(lv_combine(n, ctx->pw_t1, 0.0, xp, c, 2, s));
-    xp[0] = APcur; xp[1] = APprev;
-    LV_HIP_CHECK(lv_combine(n, ctx->pw_t2, 0.0, xp, c, 2, s));
     if (*cnt > 0) {
         const double *a[8], *b[8];
         for (int k = 0; k < *cnt; k++) {
-            a[k] = ctx->pw_q[k];
+            a[k] = Q[k];
             b[k] = ctx->pw_t2;
         }
         LV_HIP_CHECK(lv_dots_device(n, a, b, *cnt, ctx->W.partials, ctx->pw_dots, s));
         for (int k = 0; k < *cnt; k++) {
             c[k] = -1.0;  // -<q_k, t2>
-            xp[k] = ctx->pw_q[k];
+            xp[k] = Q[k];
         }
         LV_HIP_CHECK(lv_combine(n, ctx->pw_t2, 1.0, xp, c, *cnt, s, ctx->pw_dots, 1));
-        for (int k = 0; k < *cnt; k++) xp[k] = ctx->pw_z[k];
+        for (int k = 0; k < *cnt; k++) xp[k] = Z[k];
         LV_HIP_CHECK(lv_combine(n, ctx->pw_t1, 1.0, xp, c, *cnt, s, ctx->pw_dots, 1));
     }
     {
@@ -1353,41 +1362,71 @@ static int picard_add_direction(lv_ctx *ctx, int w, int *cnt, const double *Pcur
         LV_HIP_CHECK(lv_dots_device(n, a, b, 1, ctx->W.partials, ctx->pw_dots, s));
     }
     if (*cnt == w) {  // drop the oldest direction
-        double *q0 = ctx->pw_q[0], *z0 = ctx->pw_z[0];
+        double *q0 = Q[0], *z0 = Z[0];
         for (int k = 1; k < w; k++) {
-            ctx->pw_q[k - 1] = ctx->pw_q[k];
-            ctx->pw_z[k - 1] = ctx->pw_z[k];
+            Q[k - 1] = Q[k];
+            Z[k - 1] = Z[k];
         }
-        ctx->pw_q[w - 1] = q0;
-        ctx->pw_z[w - 1] = z0;
+        Q[w - 1] = q0;
+        Z[w - 1] = z0;
         (*cnt)--;
     }
     xp[0] = ctx->pw_t2;
     c[0] = 1.0;  // 1 / |t2|
-    LV_HIP_CHECK(lv_combine(n, ctx->pw_q[*cnt], 0.0, xp, c, 1, s, ctx->pw_dots, 2));
+    LV_HIP_CHECK(lv_combine(n, Q[*cnt], 0.0, xp, c, 1, s, ctx->pw_dots, 2));
     xp[0] = ctx->pw_t1;
-    LV_HIP_CHECK(lv_combine(n, ctx->pw_z[*cnt], 0.0, xp, c, 1, s, ctx->pw_dots, 2));
+    LV_HIP_CHECK(lv_combine(n, Z[*cnt], 0.0, xp, c, 1, s, ctx->pw_dots, 2));
     (*cnt)++;
     return LV_OK;
 }
 
-// P0 += sum_k <q_k, b - AP_cur> z_k: the guess whose residual is r0 minus its projection on the q's
-static int picard_extrapolate(lv_ctx *ctx, int cnt, const double *APcur)
+static int picard_add_direction(lv_ctx *ctx, int w, int *cnt, const double *Pcur, const double *Pprev,
+                                const double *APcur, const double *APprev)
+{
+    const double *xp[2] = {Pcur, Pprev};
+    const double c[2] = {1.0, -1.0};
+    LV_HIP_CHECK(lv_combine(ctx->n, ctx->pw_t1, 0.0, xp, c, 2, ctx->s));
+    xp[0] = APcur;
+    xp[1] = APprev;
+    LV_HIP_CHECK(lv_combine(ctx->n, ctx->pw_t2, 0.0, xp, c, 2, ctx->s));
+    return picard_add_pair(ctx, ctx->pw_q, ctx->pw_z, w, cnt);
+}
+
+// P_cur - P_prev into y
+static int picard_record(lv_ctx *ctx, double *y, const double *Pcur, const double *Pprev)
+{
+    const double *xp[2] = {Pcur, Pprev};
+    const double c[2] = {1.0, -1.0};
+    LV_HIP_CHECK(lv_combine(ctx->n, y, 0.0, xp, c, 2, ctx->s));
+    return LV_OK;
+}
+
+// P0 += sum_k <q_k, r> z_k over the set (Q, Z): the guess whose residual is r minus its projection
+// on the q's.  r is b - AP_cur (APcur given), or pw_t2 as a previous projection left it (APcur null);
+// with keep, pw_t2 is left holding the projected residual.
+static int picard_project(lv_ctx *ctx, double *const *Q, double *const *Z, int cnt, const double *APcur, bool keep)
 {
     const int n = ctx->n;
     hipStream_t s = ctx->s;
     const double *xp[8] = {ctx->b, APcur};
     double c[8] = {1.0, -1.0};
-    LV_HIP_CHECK(lv_combine(n, ctx->pw_t2, 0.0, xp, c, 2, s));
+    if (APcur) LV_HIP_CHECK(lv_combine(n, ctx->pw_t2, 0.0, xp, c, 2, s));
     const double *a[8], *b[8];
     for (int k = 0; k < cnt; k++) {
-        a[k] = ctx->pw_q[k];
+        a[k] = Q[k];
         b[k] = ctx->pw_t2;
-        c[k] = 1.0;  // <q_k, b - AP_cur>
+        c[k] = 1.0;  // <q_k, r>
     }
     LV_HIP_CHECK(lv_dots_device(n, a, b, cnt, ctx->W.partials, ctx->pw_dots, s));
-    for (int k = 0; k < cnt; k++) xp[k] = ctx->pw_z[k];
+    for (int k = 0; k < cnt; k++) xp[k] = Z[k];
     LV_HIP_CHECK(lv_combine(n, ctx->P0, 1.0, xp, c, cnt, s, ctx->pw_dots, 1));
+    if (keep) {
+        for (int k = 0; k < cnt; k++) {
+            xp[k] = Q[k];
+            c[k] = -1.0;
+        }
+        LV_HIP_CHECK(lv_combine(n, ctx->pw_t2, 1.0, xp, c, cnt, s, ctx->pw_dots, 1));
+    }
     return LV_OK;
 }
 
@@ -1413,6 +1452,31 @@ int lv_find_pressure(lv_ctx *ctx, double dt, int32_t niter, const lv_solve_opts 
         if (rc) return rc;
     }
     int cnt = 0, have = 0;  // directions in the window; finished Picard iterates remembered (<= 2)
+    // carried directions: the previous call's first two corrections, multiplied by this call's
+    // operator and made orthonormal, improve the guesses of Picard solves 2 to 4 by a projection of
+    // their own after the window's (the window's directions are not made orthogonal to them, so
+    // the later solves' window is the same as without them); this call records its own
+    const bool carry = w > 0 && o.picard_carry != 0 && niter >= 4;
+    int ccnt = 0;
+    if (carry) {
+        for (int k = 0; k < 2; k++) {
+            double **bufs[4] = {&ctx->pc_z[k], &ctx->pc_new[k], &ctx->pc_q[k], &ctx->pc_o[k]};
+            for (double **b : bufs)
+                if (!*b) LV_HIP_CHECK(dalloc(b, (size_t)ctx->cap));
+        }
+        if (ctx->pc_n == ctx->n) {
+            for (int k = 0; k < ctx->pc_cnt; k++) {
+                const double *xp[1] = {ctx->pc_z[k]};
+                const double c1[1] = {1.0};
+                LV_HIP_CHECK(lv_combine(ctx->n, ctx->pw_t1, 0.0, xp, c1, 1, ctx->s));
+                LV_HIP_CHECK(lv_spmv64(ctx->A, ctx->n, ctx->pc_z[k], ctx->pw_t2, ctx->s));
+                rc = picard_add_pair(ctx, ctx->pc_q, ctx->pc_o, 2, &ccnt);
+                if (rc) return rc;
+            }
+        }
+    } else {
+        ctx->pc_cnt = 0;
+    }
     int status = LV_OK;
     for (int it = 1; it <= niter; it++) {
         // refresh!(solver, dt, it > 1, boundary_velocity): the first iteration reads p.P (and takes it
@@ -1430,11 +1494,21 @@ int lv_find_pressure(lv_ctx *ctx, double dt, int32_t niter, const lv_solve_opts 
         }
         if (w > 0 && have >= 1) {
             if (have >= 2) {
+                // the corrections P_2 - P_1 and P_3 - P_2, recorded for the next call
+                if (carry && (it == 3 || it == 4)) {
+                    rc = picard_record(ctx, ctx->pc_new[it - 3], ctx->pw_P[0], ctx->pw_P[1]);
+                    if (rc) return rc;
+                }
                 rc = picard_add_direction(ctx, w, &cnt, ctx->pw_P[0], ctx->pw_P[1], ctx->pw_AP[0], ctx->pw_AP[1]);
                 if (rc) return rc;
             }
+            const bool use_carried = ccnt > 0 && it <= 4;
             if (cnt > 0) {
-                rc = picard_extrapolate(ctx, cnt, ctx->pw_AP[0]);
+                rc = picard_project(ctx, ctx->pw_q, ctx->pw_z, cnt, ctx->pw_AP[0], use_carried);
+                if (rc) return rc;
+            }
+            if (use_carried) {
+                rc = picard_project(ctx, ctx->pc_q, ctx->pc_o, ccnt, cnt > 0 ? nullptr : ctx->pw_AP[0], false);
                 if (rc) return rc;
             }
         }
@@ -1449,6 +1523,11 @@ int lv_find_pressure(lv_ctx *ctx, double dt, int32_t niter, const lv_solve_opts 
         }
     }
     LV_HIP_CHECK(lv_launch_write_P(ctx->F, ctx->n, ctx->P0, ctx->s));
+    if (carry) {  // this call's corrections become the carried ones
+        for (int k = 0; k < 2; k++) std::swap(ctx->pc_z[k], ctx->pc_new[k]);
+        ctx->pc_cnt = 2;
+        ctx->pc_n = ctx->n;
+    }
     if (status) lv_fail(ctx, status, "pressure solve did not reach its tolerance");
     return status;
 }

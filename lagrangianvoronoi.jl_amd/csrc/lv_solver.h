@@ -113,3 +113,5 @@ hipError_t lv_combine(int n, double *y, double alpha, const double *const *x, co
                       const double *dc = nullptr, int dmode = 0);
 hipError_t lv_apply_from_residual(int n, const double *b, const double *r, double *Ax, const lv_solver_work &W,
                                   hipStream_t s);
+// y = A x in FP64
+hipError_t lv_spmv64(const lv_pressure_system &A, int n, const double *x, double *y, hipStream_t s);

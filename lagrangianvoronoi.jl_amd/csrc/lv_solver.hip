@@ -676,6 +676,15 @@ __global__ __launch_bounds__(B) void k_resid64(int n, const int *__restrict__ de
     block_partials<3>(v, partials, gridDim.x);
 }
 
+// y = A x (FP64, the operator's own ELL)
+__global__ __launch_bounds__(B) void k_spmv64(int n, const int *__restrict__ deg, const int *__restrict__ ecol,
+                                              const double *__restrict__ ev, const double *__restrict__ diag,
+                                              const double *__restrict__ x, double *__restrict__ y)
+{
+    const lv_ell_own<double> A{deg, ecol, ev};
+    for (int i = blockIdx.x * B + threadIdx.x; i < n; i += gridDim.x * B) y[i] = spmv_row<double>(i, n, A, diag, x);
+}
+
 __global__ __launch_bounds__(B) void k_sum1(int n, const double *__restrict__ x, double *__restrict__ partials)
 {
     double v[3] = {0.0, 0.0, 0.0};
@@ -1229,6 +1238,13 @@ hipError_t lv_combine(int n, double *y, double alpha, const double *const *x, co
         C.c[q] = (q < m) ? c[q] : 0.0;
     }
     hipLaunchKernelGGL(k_combine, dim3(nblocks_of(n)), dim3(B), 0, s, n, y, alpha, C, m, dc, dmode);
+    return hipGetLastError();
+}
+
+hipError_t lv_spmv64(const lv_pressure_system &A, int n, const double *x, double *y, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_spmv64, dim3(nblocks_of(n)), dim3(B), 0, s, n, (const int *)A.deg, (const int *)A.ecol,
+                       (const double *)A.ev64, (const double *)A.diag64, x, y);
     return hipGetLastError();
 }
 

@@ -567,7 +567,10 @@ int lv_upload(lv_ctx *ctx, int32_t field, const void *host)
     if (count == 0) return LV_OK;
     LV_HIP_CHECK(hipMemcpyAsync(p, host, count * elem, hipMemcpyHostToDevice, ctx->s));
     LV_HIP_CHECK(hipStreamSynchronize(ctx->s));
-    if (field == LV_FIELD_X) ctx->mesh_valid = false;
+    if (field == LV_FIELD_X) {
+        ctx->mesh_valid = false;
+        ctx->pc_cnt = 0;  // positions from elsewhere: no carried Picard corrections
+    }
     return LV_OK;
 }
 

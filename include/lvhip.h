@@ -91,8 +91,10 @@ typedef struct {
                               (16-bit column offsets, BF16 couplings in the cycle) when the offsets fit,
                               the slots past the sixth of each 64-row group read from a compact copy;
                               1 keeps the operator's own row order (int32 columns, FP32 couplings);
-                              2 is 0 without the compact copy, 3 uses it only for groups of at most
-                              16 such slots (the same results bit for bit; for measurement and tests) */
+                              2 is 0 without the compact copies (also the one of the operator's own
+                              ELL that the FP64 kernels read), 3 uses the aggregate order's only for
+                              groups of at most 16 such slots (the same results bit for bit; for
+                              measurement and tests).  Systems below 65536 rows use no compact copy. */
     int32_t picard_carry; /* with picard_window > 0: 1 also carries the first two Picard corrections
                              into the next find_pressure! on the same particles, where (multiplied by
                              the new operator) they improve the guesses of Picard solves 2 to 4;

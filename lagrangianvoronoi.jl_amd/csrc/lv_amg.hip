@@ -1368,7 +1368,8 @@ hipError_t lv_amg_setup(lv_amg *G, const lv_grid_params &g, const double *x, con
         const size_t mo = ((size_t)n / 64 + 1) * LV_OVF_CAP;
         AMG_CHECK(grow(&G->ooff, &G->cap_ooff, mo));
         AMG_CHECK(grow(&G->oev32, &G->cap_oev32, mo));
-        G->ovf_cap = layout == 0 ? LV_OVF_CAP : layout == 3 ? 16 : 0;
+        // (below 65536 rows launches, not the padding, set the kernels' time: no compact copy)
+        G->ovf_cap = (layout == 0 && n >= 65536) ? LV_OVF_CAP : layout == 3 ? 16 : 0;
         AMG_CHECK(grow(&G->pdiag32, &G->cap_pdiag32, (size_t)n));
         AMG_CHECK(grow(&G->pdinv32, &G->cap_pdinv32, (size_t)n));
         if (G->fine_learned) {

@@ -390,6 +390,9 @@ int lv_create(const lv_grid_desc *d, lv_ctx **out)
     ALLOC(ctx->A.ecol, (size_t)LV_MAXE * cap);
     ALLOC(ctx->A.ev64, (size_t)LV_MAXE * cap);
     ALLOC(ctx->A.ev32, (size_t)LV_MAXE * cap);
+    ALLOC(ctx->A.ocol, ((size_t)cap / 64 + 1) * LV_OVF_CAP);
+    ALLOC(ctx->A.oev64, ((size_t)cap / 64 + 1) * LV_OVF_CAP);
+    ALLOC(ctx->A.oev32, ((size_t)cap / 64 + 1) * LV_OVF_CAP);
     ALLOC(ctx->b, cap);
     ALLOC(ctx->P0, cap);
     ALLOC(ctx->GP, 2 * cap);
@@ -451,7 +454,8 @@ int lv_destroy(lv_ctx *ctx)
                     ctx->cell_x,
                     ctx->cl_temp, ctx->d_nitems, ctx->d_flags, ctx->path_i, ctx->path_rr, ctx->A.row_ptr,
                     ctx->A.cols, ctx->A.val64, ctx->A.val32, ctx->A.diag64, ctx->A.dinv64, ctx->A.diag32,
-                    ctx->A.dinv32, ctx->deg, ctx->A.ecol, ctx->A.ev64, ctx->A.ev32, ctx->b, ctx->P0, ctx->GP, ctx->W.r, ctx->W.z, ctx->W.p,
+                    ctx->A.dinv32, ctx->deg, ctx->A.ecol, ctx->A.ev64, ctx->A.ev32, ctx->A.ocol, ctx->A.oev64,
+                    ctx->A.oev32, ctx->b, ctx->P0, ctx->GP, ctx->W.r, ctx->W.z, ctx->W.p,
                     ctx->W.q, ctx->W.e32, ctx->W.r32, ctx->W.z32, ctx->W.p32, ctx->W.q32, ctx->W.S,
                     ctx->W.partials, ctx->scan_temp};
     for (void *p : ptrs) dfree(p);

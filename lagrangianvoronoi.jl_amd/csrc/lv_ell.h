@@ -129,9 +129,11 @@ __device__ __forceinline__ int lv_ell_head_with(const V &A, const T *__restrict_
 // fits (else slots LV_OVF_BASE.. slot-major).  The first 8 slots are loaded unconditionally, as in
 // lv_ell_head; the overflow entries of rows below degree 8 are clamped into the group's region and
 // dropped.
+// (with shift, every gathered x_j is read as x_j - shift, for a product with the stored x - shift;
+// shift 0 leaves the values exactly as they are)
 template <typename T, class V>
 __device__ __forceinline__ T lv_ell_row_ovf(const V &A, int i, int n, int d, lv_ovf_pos q, T di, T xi,
-                                            const T *__restrict__ x)
+                                            const T *__restrict__ x, T shift = (T)0)
 {
     constexpr int U = 8;
     int c[U];
@@ -171,25 +173,25 @@ __device__ __forceinline__ T lv_ell_row_ovf(const V &A, int i, int n, int d, lv_
     }
     T acc = di * xi;
 #pragma unroll
-    for (int k = 0; k < U; k++) xj[k] = x[c[k]];
+    for (int k = 0; k < U; k++) xj[k] = x[c[k]] - shift;
 #pragma unroll
     for (int k = 0; k < U; k++) acc += v[k] * (xi - xj[k]);
     for (int k = U; k < d; k++) {
         const size_t e = compact ? g0 + (size_t)(q.pos + (k - LV_OVF_BASE)) : (size_t)k * n + i;
         const int ck = compact ? lv_ell_col(A.ocol, e, i, n) : A.column(e, i, n);
         const T vk = compact ? (T)lv_ell_val(A.oval, e) : (T)A.value(e);
-        acc += vk * (xi - x[ck]);
+        acc += vk * (xi - (x[ck] - shift));
     }
     return acc;
 }
 
-// views that carry the compact overflow (the aggregate order's encodings)
+// views that can carry the compact overflow (every ELL view; ocap == 0 when it is absent)
 template <class V>
 struct lv_ell_compact {
     static constexpr bool value = false;
 };
-template <typename VAL>
-struct lv_ell_compact<lv_ell_view<uint8_t, int16_t, VAL>> {
+template <typename DEG, typename COL, typename VAL>
+struct lv_ell_compact<lv_ell_view<DEG, COL, VAL>> {
     static constexpr bool value = true;
 };
 

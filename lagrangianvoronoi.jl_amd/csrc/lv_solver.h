@@ -2,6 +2,7 @@
 #pragma once
 
 #include "lv_internal.h"
+#include "lv_ell.h"
 
 // Device-resident CG scalars: written by the last block of each reduction, read by the next launch.
 // rz is double-buffered by iteration parity: the p-update of iteration k reads rz[k&1] in every
@@ -26,6 +27,11 @@ struct lv_pressure_system {
     int *deg, *ecol;
     double *ev64;
     float *ev32;
+    // the compact copy of slots LV_OVF_BASE.. per 64-row group (lv_ell.h): columns, FP64 and FP32
+    // values, for the groups with at most LV_OVF_CAP such slots
+    int *ocol;
+    double *oev64;
+    float *oev32;
     int64_t nnz, nnz_cap;
     int singular;  // every d_i == 0 (c2 = Inf): a graph Laplacian with the constants as null space
     int valid;

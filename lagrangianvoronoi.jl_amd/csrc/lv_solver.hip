@@ -77,10 +77,11 @@ __global__ __launch_bounds__(B) void k_assemble(lv_grid_params g, lv_fields F, l
                                                 double *__restrict__ dinv64, float *__restrict__ dinv32,
                                                 int *__restrict__ ecol, double *__restrict__ ev64,
                                                 float *__restrict__ ev32, int *__restrict__ nonsingular, lv_vbc vbc,
-                                                lv_rhs_geom R)
+                                                lv_rhs_geom R, int *__restrict__ ocol, double *__restrict__ oev64,
+                                                float *__restrict__ oev32)
 {
     const int i = blockIdx.x * B + threadIdx.x;
-    if (i >= n) return;
+    if (i >= n) return;  // rows past n hold no slots: their lanes leave the group's ballots below
     const double x1 = F.x[i], x2 = F.x[n + i];
     const double rho = F.rho[i];
     const double d = (mode == 0) ? F.mass[i] / (((rho * rho) * F.c2[i]) * (dt * dt)) : F.mass[i] / dt;
@@ -130,6 +131,18 @@ __global__ __launch_bounds__(B) void k_assemble(lv_grid_params g, lv_fields F, l
         ecol[es] = sc[k];
         ev64[es] = sv[k];
         ev32[es] = (float)sv[k];
+    }
+    // slots LV_OVF_BASE.. also into the compact copy of the row's 64-row group (a block's
+    // wavefronts are whole groups), when the group has at most LV_OVF_CAP of them
+    const lv_ovf_pos q = lv_ovf_prefix(m);
+    if (ocol && q.tot <= LV_OVF_CAP) {
+        const size_t g0 = (size_t)(i >> 6) * LV_OVF_CAP;
+        for (int k = LV_OVF_BASE; k < m; k++) {
+            const size_t eo = g0 + (size_t)(q.pos + (k - LV_OVF_BASE));
+            ocol[eo] = sc[k];
+            oev64[eo] = sv[k];
+            oev32[eo] = (float)sv[k];
+        }
     }
     if (geom) {
         R.bdiv[i] = bd;
@@ -392,8 +405,30 @@ __device__ __forceinline__ T spmv_row(int i, int n, const V &A, const T *__restr
 }
 
 // r = (b - bmean) - A x; z = dinv r; p = z; partials r.z, r.r, b'.b'
-__global__ __launch_bounds__(B) void k_cg64_init(int n, const int *__restrict__ deg, const int *__restrict__ ecol,
-                                                 const double *__restrict__ ev, const double *__restrict__ diag,
+// Grid-stride passes over the rows whose wavefronts are whole 64-row groups: the compact overflow's
+// prefix of the next pass is formed during this one (lv_ell.h)
+// (COMPACT false: no compact copy, and no ballots)
+#define LV_OVF_PASSES(A, n, i0, stride, body)                                                 \
+    if constexpr (COMPACT) {                                                                    \
+        int dn_ = (i0) < (n) ? (A).degree(i0) : 0;                                              \
+        lv_ovf_pos qn_ = lv_ovf_prefix(dn_);                                                    \
+        for (int i = (i0); i < (n); i += (stride)) {                                            \
+            const int d_ = dn_;                                                                 \
+            const lv_ovf_pos q_ = qn_;                                                          \
+            dn_ = i + (stride) < (n) ? (A).degree(i + (stride)) : 0;                            \
+            body;                                                                               \
+            qn_ = lv_ovf_prefix(dn_);                                                           \
+        }                                                                                       \
+    } else {                                                                                    \
+        for (int i = (i0); i < (n); i += (stride)) {                                            \
+            const int d_ = (A).degree(i);                                                       \
+            const lv_ovf_pos q_{0, 0};                                                          \
+            body;                                                                               \
+        }                                                                                       \
+    }
+
+template <bool COMPACT>
+__global__ __launch_bounds__(B) void k_cg64_init(int n, lv_ell_own<double> A, const double *__restrict__ diag,
                                                  const double *__restrict__ dinv, const double *__restrict__ b,
                                                  const double *__restrict__ x, double *__restrict__ r,
                                                  double *__restrict__ z, double *__restrict__ Prr,
@@ -401,16 +436,16 @@ __global__ __launch_bounds__(B) void k_cg64_init(int n, const int *__restrict__ 
 {
     const double bmean = S->bmean;
     double v[2] = {0.0, 0.0}, w[3] = {0.0, 0.0, 0.0};
-    for (int i = blockIdx.x * B + threadIdx.x; i < n; i += gridDim.x * B) {
+    LV_OVF_PASSES(A, n, blockIdx.x * B + threadIdx.x, gridDim.x * B, {
         const double bi = b[i] - bmean;
-        const double ri = bi - spmv_row<double>(i, n, lv_ell_own<double>{deg, ecol, ev}, diag, x);
+        const double ri = bi - lv_ell_row_ovf<double>(A, i, n, d_, q_, diag[i], x[i], x);
         const double zi = dinv[i] * ri;
         r[i] = ri;
         z[i] = zi;
         v[0] += ri * zi;
         v[1] += ri * ri;
         w[0] += bi * bi;
-    }
+    })
     block_partials<2>(v, Prr, gridDim.x);
     block_partials<3>(w, P3, gridDim.x);
 }
@@ -422,10 +457,24 @@ __global__ __launch_bounds__(B) void k_cg64_init(int n, const int *__restrict__ 
 // using A p_new = A z + beta A p_old, so only z is gathered and p, q are touched row-locally.
 // The block's slice of the CSR arrays is first staged through LDS with coalesced loads.
 // U (update): alpha = r.z / p.q;  x += alpha p;  r -= alpha q;  z = dinv r;  partials r.z, r.r.
-// (8 wavefronts per SIMD asked of the register allocator: with the compact overflow's prefix the
-// kernel would otherwise take 66 VGPRs and 7 wavefronts, and run 25% slower)
-template <typename T, class V>
-__global__ __launch_bounds__(B, 8) void k_cg_s(int n, V A, const T *__restrict__ diag,
+// (wavefronts per SIMD asked of the register allocator: 8 for FP32, where with the compact
+// overflow's prefix the kernel would otherwise take 66 VGPRs and 7 wavefronts and run 25% slower;
+// 7 for FP64 with the compact copy (74 VGPRs and 6 otherwise), nothing for FP64 without it (72 VGPRs,
+// 7 wavefronts; asking 7 there spills))
+template <typename T, bool COMPACT>
+struct cg_waves {
+    static constexpr int value = 8;
+};
+template <>
+struct cg_waves<double, true> {
+    static constexpr int value = 7;
+};
+template <>
+struct cg_waves<double, false> {
+    static constexpr int value = 1;
+};
+template <typename T, class V, bool COMPACT>
+__global__ __launch_bounds__(B, (cg_waves<T, COMPACT>::value)) void k_cg_s(int n, V A, const T *__restrict__ diag,
                                             const T *__restrict__ z, T *__restrict__ p, T *__restrict__ q,
                                             const double *__restrict__ Prr, double *__restrict__ Ppq,
                                             lv_cg_scalars *S, lv_progress *prog, int par, int k, int first)
@@ -463,7 +512,7 @@ __global__ __launch_bounds__(B, 8) void k_cg_s(int n, V A, const T *__restrict__
     // of the next pass is formed during this one, so that its overflow loads issue with the slots
     int dn0 = 0, dn1 = 0;
     lv_ovf_pos qn0{0, 0}, qn1{0, 0};
-    if constexpr (lv_ell_compact<V>::value) {
+    if constexpr (COMPACT) {
         dn0 = istart < n ? A.degree(istart) : 0;
         dn1 = istart + stride < n ? A.degree(istart + stride) : 0;
         qn0 = lv_ovf_prefix(dn0);
@@ -473,7 +522,7 @@ __global__ __launch_bounds__(B, 8) void k_cg_s(int n, V A, const T *__restrict__
         const int i1 = i0 + stride;
         const T z0 = z[i0];
         T w0, z1 = (T)0, w1 = (T)0;
-        if constexpr (lv_ell_compact<V>::value) {
+        if constexpr (COMPACT) {
             const int d0 = dn0, d1 = dn1;
             const lv_ovf_pos q0 = qn0, q1 = qn1;
             const int j0 = i0 + 2 * stride, j1 = j0 + stride;
@@ -650,8 +699,8 @@ __device__ __forceinline__ double spmv_row_shifted(int i, int n, const lv_ell_ow
 // true residual r = (b - bmean) - A x; partials |r|^2, |b'|^2 (FP64).  With Pxs (the partial sums
 // of x written by k_correct, singular systems) the residual is that of x - d, d = mean(x) - xmean0,
 // the x that k_shift stores at the end of the solve; block 0 records d.
-__global__ __launch_bounds__(B) void k_resid64(int n, const int *__restrict__ deg, const int *__restrict__ ecol,
-                                               const double *__restrict__ ev, const double *__restrict__ diag,
+template <bool COMPACT>
+__global__ __launch_bounds__(B) void k_resid64(int n, lv_ell_own<double> A, const double *__restrict__ diag,
                                                const double *__restrict__ b, const double *__restrict__ x,
                                                double *__restrict__ r, double *__restrict__ partials,
                                                const double *__restrict__ Pxs, lv_cg_scalars *S)
@@ -664,25 +713,26 @@ __global__ __launch_bounds__(B) void k_resid64(int n, const int *__restrict__ de
         d = t[0] / n - S->xmean0;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) S->xshift = d;
-    const lv_ell_own<double> A{deg, ecol, ev};
     double v[3] = {0.0, 0.0, 0.0};
-    for (int i = blockIdx.x * B + threadIdx.x; i < n; i += gridDim.x * B) {
+    LV_OVF_PASSES(A, n, blockIdx.x * B + threadIdx.x, gridDim.x * B, {
         const double bi = b[i] - bmean;
-        const double ri = bi - (Pxs ? spmv_row_shifted(i, n, A, diag, x, d) : spmv_row<double>(i, n, A, diag, x));
+        // (with Pxs, the product with the stored x - d: every x read shifted, as k_shift will store it)
+        const double ri = bi - (Pxs ? lv_ell_row_ovf<double>(A, i, n, d_, q_, diag[i], x[i] - d, x, d)
+                                    : lv_ell_row_ovf<double>(A, i, n, d_, q_, diag[i], x[i], x));
         r[i] = ri;
         v[0] += ri * ri;
         v[1] += bi * bi;
-    }
+    })
     block_partials<3>(v, partials, gridDim.x);
 }
 
 // y = A x (FP64, the operator's own ELL)
-__global__ __launch_bounds__(B) void k_spmv64(int n, const int *__restrict__ deg, const int *__restrict__ ecol,
-                                              const double *__restrict__ ev, const double *__restrict__ diag,
+template <bool COMPACT>
+__global__ __launch_bounds__(B) void k_spmv64(int n, lv_ell_own<double> A, const double *__restrict__ diag,
                                               const double *__restrict__ x, double *__restrict__ y)
 {
-    const lv_ell_own<double> A{deg, ecol, ev};
-    for (int i = blockIdx.x * B + threadIdx.x; i < n; i += gridDim.x * B) y[i] = spmv_row<double>(i, n, A, diag, x);
+    LV_OVF_PASSES(A, n, blockIdx.x * B + threadIdx.x, gridDim.x * B,
+                  { y[i] = lv_ell_row_ovf<double>(A, i, n, d_, q_, diag[i], x[i], x); })
 }
 
 __global__ __launch_bounds__(B) void k_sum1(int n, const double *__restrict__ x, double *__restrict__ partials)
@@ -839,7 +889,7 @@ hipError_t lv_launch_assemble(const lv_grid_params &g, lv_fields F, lv_mesh_view
     const lv_rhs_geom r = (R && mode == 0) ? *R : lv_rhs_geom{};
     hipLaunchKernelGGL(k_assemble, grid_of(n), dim3(B), 0, s, g, F, M, n, dt, mode, (const int *)A.row_ptr, A.cols,
                        A.val64, A.val32, A.diag64, A.diag32, A.dinv64, A.dinv32, A.ecol, A.ev64, A.ev32, nonsingular,
-                       v, r);
+                       v, r, A.ocol, A.oev64, A.oev32);
     return hipGetLastError();
 }
 
@@ -919,6 +969,42 @@ __global__ void k_init_scalars(lv_cg_scalars *S, double tol2, int itmax, int sin
     *S = z;
 }
 
+// the operator's own ELL as the kernels read it, with its compact overflow when it has one
+// (systems below LV_OVF_MIN_ROWS rows leave it out: there the kernels' launch latency, not the
+// padding's bytes, sets their time, and the ballots cost more than they save: Gresho at 100^2 took
+// 11.8 against 11.2 ms per step with it)
+#define LV_OVF_MIN_ROWS 65536
+// (amg_layout 2 leaves it out too, for measurement and tests)
+static int own_ocap(const lv_pressure_system &A, int n, int layout)
+{
+    return (A.ocol && n >= LV_OVF_MIN_ROWS && layout != 2) ? LV_OVF_CAP : 0;
+}
+static lv_ell_own<double> own64(const lv_pressure_system &A, int n, int layout = 0)
+{
+    lv_ell_own<double> v{A.deg, A.ecol, A.ev64};
+    v.ocol = A.ocol;
+    v.oval = A.oev64;
+    v.ocap = own_ocap(A, n, layout);
+    return v;
+}
+static lv_ell_own<float> own32(const lv_pressure_system &A, int n, int layout = 0)
+{
+    lv_ell_own<float> v{A.deg, A.ecol, A.ev32};
+    v.ocol = A.ocol;
+    v.oval = A.oev32;
+    v.ocap = own_ocap(A, n, layout);
+    return v;
+}
+
+// kern<true> (with the compact copy) or kern<false>, by the view's ocap
+#define LV_LAUNCH_BY_OCAP(kern, view, ...)                 \
+    do {                                                   \
+        if ((view).ocap > 0)                               \
+            hipLaunchKernelGGL((kern<true>), __VA_ARGS__);  \
+        else                                               \
+            hipLaunchKernelGGL((kern<false>), __VA_ARGS__); \
+    } while (0)
+
 // partial-sum regions: P3 = one-off reductions (3 values), Ppq (1), Prr (2)
 struct parts {
     double *p3, *pq, *rr, *ur;
@@ -968,8 +1054,12 @@ hipError_t cg_iterate(int n, V A, const T *diag, const T *dinv, T *x, T *r, T *z
         const int par = k & 1;
         {
             timer_scope ts(tm, s);
-            hipLaunchKernelGGL((k_cg_s<T, V>), dim3(nb), dim3(B), 0, s, n, A, diag, (const T *)z, p, q,
-                               (const double *)P.rr, P.pq, dS, dprog, par, k, k == 0 ? 1 : 0);
+            if (A.ocap > 0)
+                hipLaunchKernelGGL((k_cg_s<T, V, true>), dim3(nb), dim3(B), 0, s, n, A, diag, (const T *)z, p, q,
+                                   (const double *)P.rr, P.pq, dS, dprog, par, k, k == 0 ? 1 : 0);
+            else
+                hipLaunchKernelGGL((k_cg_s<T, V, false>), dim3(nb), dim3(B), 0, s, n, A, diag, (const T *)z, p, q,
+                                   (const double *)P.rr, P.pq, dS, dprog, par, k, k == 0 ? 1 : 0);
         }
         if (M) {
             // the cycle's pre-smoothed z0 stored with the residual update (FP32 solves only)
@@ -1027,13 +1117,12 @@ hipError_t lv_solve_device(const lv_pressure_system &A, lv_solver_work &W, int n
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(B), 0, s, (const double *)P.p3, nb, (int)FIN_MEANS, n, 0.0, W.S);
     int total = 0, nsweep = 0, skipped = 0;
     if (opt.method == LV_SOLVER_PCG64) {
-        hipLaunchKernelGGL(k_cg64_init, G, dim3(B), 0, s, n, (const int *)A.deg, (const int *)A.ecol,
-                           (const double *)A.ev64, (const double *)A.diag64, (const double *)A.dinv64, b,
+        LV_LAUNCH_BY_OCAP(k_cg64_init, own64(A, n, opt.amg_layout), G, dim3(B), 0, s, n, own64(A, n, opt.amg_layout), (const double *)A.diag64, (const double *)A.dinv64, b,
                            (const double *)x, W.r, W.z, P.rr, P.p3, (const lv_cg_scalars *)W.S);
         hipLaunchKernelGGL(k_finalize, dim3(1), dim3(B), 0, s, (const double *)P.p3, nb, (int)FIN_INIT64, n, 0.0,
                            W.S);
         int done = 0;
-        e = cg_iterate<double>(n, lv_ell_own<double>{A.deg, A.ecol, A.ev64}, (const double *)A.diag64,
+        e = cg_iterate<double>(n, own64(A, n, opt.amg_layout), (const double *)A.diag64,
                                (const double *)A.dinv64, x, W.r, W.z, W.p, W.q, P, W.S, W.hprog,
                                W.dprog, W.hS, s, t64, h, (const lv_precond *)nullptr, &done);
         if (e != hipSuccess) return e;
@@ -1056,8 +1145,7 @@ hipError_t lv_solve_device(const lv_pressure_system &A, lv_solver_work &W, int n
         bool pending = false;
         double itol = 0.0;
         for (;;) {
-            hipLaunchKernelGGL(k_resid64, G, dim3(B), 0, s, n, (const int *)A.deg, (const int *)A.ecol,
-                               (const double *)A.ev64, (const double *)A.diag64, b, (const double *)x, W.r, P.p3,
+            LV_LAUNCH_BY_OCAP(k_resid64, own64(A, n, opt.amg_layout), G, dim3(B), 0, s, n, own64(A, n, opt.amg_layout), (const double *)A.diag64, b, (const double *)x, W.r, P.p3,
                                nsweep > 0 ? (const double *)Pxs : (const double *)nullptr, W.S);
             if (nsweep == 0) {
                 hipLaunchKernelGGL(k_finalize, dim3(1), dim3(B), 0, s, (const double *)P.p3, nb, (int)FIN_RESID0,
@@ -1114,7 +1202,7 @@ hipError_t lv_solve_device(const lv_pressure_system &A, lv_solver_work &W, int n
                                           dinv32, W.e32, W.r32, W.z32, W.p32, W.q32, P, W.S, W.hprog, W.dprog,
                                           W.hS, s, t32, h, Mi, &done);
                 else
-                    e = cg_iterate<float>(n, lv_ell_own<float>{A.deg, A.ecol, A.ev32}, (const float *)A.diag32,
+                    e = cg_iterate<float>(n, own32(A, n, opt.amg_layout), (const float *)A.diag32,
                                           dinv32, W.e32, W.r32, W.z32, W.p32, W.q32, P, W.S, W.hprog, W.dprog,
                                           W.hS, s, t32, h, Mi, &done);
                 if (e != hipSuccess) return e;
@@ -1143,8 +1231,7 @@ hipError_t lv_solve_device(const lv_pressure_system &A, lv_solver_work &W, int n
                                W.S);
             hipLaunchKernelGGL(k_shift, G, dim3(B), 0, s, n, x, (const lv_cg_scalars *)W.S);
         }
-        hipLaunchKernelGGL(k_resid64, G, dim3(B), 0, s, n, (const int *)A.deg, (const int *)A.ecol,
-                           (const double *)A.ev64, (const double *)A.diag64, b, (const double *)x, W.r, P.p3,
+        LV_LAUNCH_BY_OCAP(k_resid64, own64(A, n, opt.amg_layout), G, dim3(B), 0, s, n, own64(A, n, opt.amg_layout), (const double *)A.diag64, b, (const double *)x, W.r, P.p3,
                            (const double *)nullptr, W.S);
         hipLaunchKernelGGL(k_finalize, dim3(1), dim3(B), 0, s, (const double *)P.p3, nb, (int)FIN_RESID, n, 0.0,
                            W.S);
@@ -1243,8 +1330,7 @@ hipError_t lv_combine(int n, double *y, double alpha, const double *const *x, co
 
 hipError_t lv_spmv64(const lv_pressure_system &A, int n, const double *x, double *y, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_spmv64, dim3(nblocks_of(n)), dim3(B), 0, s, n, (const int *)A.deg, (const int *)A.ecol,
-                       (const double *)A.ev64, (const double *)A.diag64, x, y);
+    LV_LAUNCH_BY_OCAP(k_spmv64, own64(A, n), dim3(nblocks_of(n)), dim3(B), 0, s, n, own64(A, n), (const double *)A.diag64, x, y);
     return hipGetLastError();
 }
 

@@ -115,7 +115,7 @@ def test_layouts_give_the_same_pressure(lv, oracle, gpu, kind, ic, K, dt):
     assert rel_err(P0, P1) <= (1e-4 if not np.any(d > 0) else 1e-5)
 
 
-@pytest.mark.parametrize("kind,ic,K,dt", [("periodic", "tg", 128, 4e-3), ("walls", "cavity", 96, 1e-3)])
+@pytest.mark.parametrize("kind,ic,K,dt", [("periodic", "tg", 256, 2e-3), ("walls", "cavity", 256, 1e-3)])
 def test_compact_overflow_is_bit_identical(lv, oracle, gpu, kind, ic, K, dt):
     """The aggregate order with the compact copy of the slots past the sixth (layout 0), without it
     (2), and with it only for groups of at most 16 such slots (3, so that both paths run in one
@@ -138,6 +138,26 @@ def test_compact_overflow_is_bit_identical(lv, oracle, gpu, kind, ic, K, dt):
     for layout in (2, 3):
         assert np.array_equal(out[0][0], out[layout][0]), layout
         assert list(out[0][1]) == list(out[layout][1]) and list(out[0][2]) == list(out[layout][2])
+
+
+@pytest.mark.parametrize("method", ["pcg64", "mixed"])
+def test_own_order_compact_overflow_is_bit_identical(lv, oracle, gpu, method):
+    """The operator's own ELL read with the compact copy of the slots past the sixth (systems of at
+    least 65536 rows, layout 0) and without it (layout 2): the FP64 and the own-order FP32 solves give
+    the same pressure bit for bit and the same iterations."""
+    _flush("own-order compact overflow %s" % method)
+    grid, op, solver, x, dr, d, rp, cols, coef = _system(lv, oracle, "periodic", 256, "tg", 2e-3)
+    assert grid.n >= 65536 and np.diff(rp).max() >= 8
+    b, _ = solver.rhs(2e-3, False)
+    x0 = grid.get("P")
+    out = {}
+    for layout in (0, 2):
+        P = solver.solve(b, x0, method=method, tol=1e-10, itmax=4000, amg_layout=layout)
+        st = solver.last_stats
+        assert max(st["relres"]) <= 1e-10, st
+        out[layout] = (P, st["iters"], st["outer"])
+    assert np.array_equal(out[0][0], out[2][0])
+    assert out[0][1] == out[2][1] and out[0][2] == out[2][2]
 
 
 def test_cycle_matches_oracle_learned_smoother(lv, oracle, gpu):

@@ -82,7 +82,8 @@ def run_ldc_profiles(lv, args, torch):
     torch.cuda.synchronize()
     t_lloyd = time.perf_counter() - t0
     solver = lv.PressureSolver(grid, method=args.solver if args.solver_given else "amg", tol=1e-8, itmax=4000,
-                               picard_window=8)
+                               picard_window=8, picard_carry=args.picard_carry,
+                               amg_level_kernels=args.amg_level_kernels)
     # As written, the example's step! (examples/cavity.jl:70-80) never recomputes the density of its
     # incompressible fluid, so the cells' mass drifts away from their area; relaxation_step!'s mass
     # flux then drains a wall cell to negative mass and the run ends in "The Voronoi Mesh has been
@@ -176,7 +177,8 @@ def run_config(lv, args, torch):
     if method == "spai" or (method == "amg" and args.smoother == "learned"):
         lv.spai.set_params(grid, spai_theta(lv)[0])
     solver = lv.PressureSolver(grid, method=method, tol=c["tol"], itmax=4000, inner_tol=args.inner_tol,
-                               picard_window=c["window"])
+                               picard_window=c["window"], amg_layout=args.amg_layout,
+                               picard_carry=args.picard_carry, amg_level_kernels=args.amg_level_kernels)
     desc = lv.make_step_desc(c["dt"], solver=solver, **kw)
     steps = args.steps if args.steps_given else c["steps"]
     warmup = args.warmup if args.warmup_given else c["warmup"]
@@ -229,6 +231,8 @@ def parse():
                     help="carry the first Picard corrections of a step into the next (lv_solve_opts.picard_carry)")
     ap.add_argument("--amg-layout", type=int, default=0, choices=[0, 1, 2, 3],
                     help="fine-level layout of the multilevel solver (lv_solve_opts.amg_layout)")
+    ap.add_argument("--amg-level-kernels", type=int, default=0, choices=[0, 1],
+                    help="1: every coarse level above the tail as its own kernels (lv_solve_opts.amg_level_kernels)")
     ap.add_argument("--picard-window", type=int, default=8,
                     help="Picard corrections used to extrapolate each solve's initial guess (0: previous P)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -494,7 +498,7 @@ def main():
         lv.spai.set_params(grid, spai_theta(lv)[0])
     solver = lv.PressureSolver(grid, method=args.solver, tol=args.tol, itmax=2000, inner_tol=args.inner_tol,
                                picard_window=args.picard_window, amg_layout=args.amg_layout,
-                               picard_carry=args.picard_carry)
+                               picard_carry=args.picard_carry, amg_level_kernels=args.amg_level_kernels)
     desc = lv.make_step_desc(dt, eos="stiffened", gamma=ic["gamma"], P0=ic["P0"], solver=solver,
                              viscous=True, artificial_viscosity=False, alpha=1.0, rusanov=True)
     for _ in range(args.warmup):
@@ -552,7 +556,7 @@ def main():
                                     "c0=50 stiffened, nu=1e-3, 10 Picard solves/step, %s PCG to %g" %
                                     (K, K, args.solver, args.tol),
                            particles=n, solver=args.solver, tol=args.tol, picard_window=args.picard_window,
-                           picard_carry=args.picard_carry,
+                           picard_carry=args.picard_carry, amg_level_kernels=args.amg_level_kernels,
                            smoother=args.smoother if args.solver == "amg" else None),
                pressure_solve_ms=round(first_solve_ms, 3), pressure_solve_iters=first_solve_it,
                max_relres=relres, picard_iters=stats[-1]["iters"], picard_sweeps=stats[-1]["outer"],

@@ -99,6 +99,10 @@ typedef struct {
                              into the next find_pressure! on the same particles, where (multiplied by
                              the new operator) they improve the guesses of Picard solves 2 to 4;
                              every solve still meets tol */
+    int32_t amg_level_kernels; /* AMG: 0 runs pairs of the coarse levels above the single-workgroup
+                                  tail as one down and one up kernel each, 1 every such level as its
+                                  own two kernels (the same results bit for bit; for measurement and
+                                  tests) */
 } lv_solve_opts;
 
 typedef struct {
@@ -229,7 +233,8 @@ int lv_solve_pressure(lv_ctx *ctx, const double *b, double *x, const lv_solve_op
  * LV_SOLVER_SPAI: the learned M, LV_SOLVER_AMG: the multilevel V-cycle) on the assembled operator;
  * host vectors of n doubles.  No counterpart in the reference; it is the hook a Julia caller uses
  * to hand this preconditioner to Krylov.jl (src/pressure.jl:219 runs minres! unpreconditioned).
- * The multilevel cycle keeps the amg_layout of the last solve on this context (0 before any). */
+ * The multilevel cycle keeps the amg_layout and amg_level_kernels of the last solve on this context
+ * (0 before any). */
 int lv_apply_preconditioner(lv_ctx *ctx, int32_t method, const double *r, double *z);
 
 /* ---- initialisation and interpolation ------------------------------------------------------- */

@@ -67,7 +67,8 @@ class GridDesc(ctypes.Structure):
 class SolveOpts(ctypes.Structure):
     _fields_ = [("method", ctypes.c_int32), ("tol", ctypes.c_double), ("itmax", ctypes.c_int32),
                 ("inner_tol", ctypes.c_double), ("outer_max", ctypes.c_int32), ("vbc", ctypes.c_double * 8),
-                ("picard_window", ctypes.c_int32), ("amg_layout", ctypes.c_int32), ("picard_carry", ctypes.c_int32)]
+                ("picard_window", ctypes.c_int32), ("amg_layout", ctypes.c_int32), ("picard_carry", ctypes.c_int32),
+                ("amg_level_kernels", ctypes.c_int32)]
 
 
 class SolveStats(ctypes.Structure):
@@ -463,7 +464,7 @@ def relaxation_step(grid, dt, rusanov=True):
 
 
 def solve_opts(method="pcg64", tol=1e-10, itmax=1000, inner_tol=1e-4, outer_max=20, boundary_velocity=None,
-               picard_window=0, amg_layout=0, picard_carry=0):
+               picard_window=0, amg_layout=0, picard_carry=0, amg_level_kernels=0):
     o = SolveOpts()
     o.method = SOLVERS[method] if isinstance(method, str) else int(method)
     o.tol, o.itmax, o.inner_tol, o.outer_max = float(tol), int(itmax), float(inner_tol), int(outer_max)
@@ -471,6 +472,7 @@ def solve_opts(method="pcg64", tol=1e-10, itmax=1000, inner_tol=1e-4, outer_max=
     o.picard_window = int(picard_window)
     o.amg_layout = int(amg_layout)
     o.picard_carry = int(bool(picard_carry))
+    o.amg_level_kernels = int(bool(amg_level_kernels))
     return o
 
 
@@ -479,7 +481,7 @@ class PressureSolver:
     this build's (the reference hard-codes MINRES with atol = rtol = 1e-6, :219)."""
 
     def __init__(self, grid, verbose=False, method="pcg64", tol=1e-10, itmax=1000, inner_tol=1e-4, outer_max=20,
-                 picard_window=0, amg_layout=0, picard_carry=0):
+                 picard_window=0, amg_layout=0, picard_carry=0, amg_level_kernels=0):
         self.grid = grid
         self.verbose = verbose
         self.method, self.tol, self.itmax = method, tol, itmax
@@ -487,11 +489,12 @@ class PressureSolver:
         self.picard_window = picard_window
         self.amg_layout = amg_layout
         self.picard_carry = picard_carry
+        self.amg_level_kernels = amg_level_kernels
         self.last_stats = None
 
     def opts(self, boundary_velocity=None):
         return solve_opts(self.method, self.tol, self.itmax, self.inner_tol, self.outer_max, boundary_velocity,
-                          self.picard_window, self.amg_layout, self.picard_carry)
+                          self.picard_window, self.amg_layout, self.picard_carry, self.amg_level_kernels)
 
     # PressureOperator access (src/pressure.jl:89-135)
     def assemble(self, dt):

@@ -34,6 +34,9 @@ struct lv_amg {
     float *pv, *pu;      // power-iteration vectors (n0)
     size_t cap_pv, cap_pu;
     int ltail;           // first level of the single-workgroup kernel (levels ltail..nlev)
+    int pair[LV_AMG_MAXLEV + 1];  // pair[l]: levels l and l+1 (both above ltail) run as one down and
+                                  // one up kernel
+    int level_kernels;   // 1: every level above ltail runs as its own kernels (pairs not used)
     int wcycle;          // 1: the levels of that kernel are visited by a W-cycle
     int *prog;           // its operations (device), nprog of them; hprog: the host copy
     int nprog;

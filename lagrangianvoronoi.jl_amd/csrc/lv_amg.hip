@@ -782,6 +782,288 @@ __global__ __launch_bounds__(B) void k_st_up(st_level L, int Kyc, const float *_
     }
 }
 
+// ---- two coarse levels in one kernel ------------------------------------------------------------
+//
+// Levels l and l+1 above the tail run their down passes as one kernel and their up passes as another.
+// A block owns a tile of cells and recomputes in LDS what its stencils read from the cells around the
+// tile (a halo), with the expressions and the order of st_down_lane, sum_children and st_up_cell, so
+// every value is bit for bit the one the one-level kernels give.  Cells are addressed by unwrapped
+// coordinates; on a periodic axis the width of level l is a multiple of 4, so that the parent map
+// x -> x >> 1 commutes with the wrap on levels l and l+1.  Level l is at least PAIR_MIN_K cells wide
+// on both axes, so the unwrapped coordinates of every region stay within one width of the level and
+// one conditional add or subtract wraps them.  The down tile is small (PD = 4: 4 lanes of work per
+// cell of level l+2) so that the small levels spread over many workgroups; 8 x 8 tiles left one
+// wavefront per SIMD working through six halo cells a lane, no faster than two one-level kernels.
+
+constexpr int PD = 4;   // down: a block owns PD x PD cells of level l+2
+constexpr int PU = 16;  // up: a block owns PU x PU cells of level l, one a lane
+constexpr int PAIR_MIN_K = 24;
+static_assert(PAIR_MIN_K >= 4 * PD + 8 && PAIR_MIN_K >= PU + 8, "every region within one width of its level");
+
+struct st_pair {
+    st_level L0, L1;  // levels l and l+1
+    int Kx2, Ky2;     // level l+2
+};
+
+// cell (x, y) of a Kx x Ky level, unwrapped within one width: its index, or 0 with ok false outside a
+// non-periodic axis (so that every load below has a valid address and none waits behind a branch)
+__device__ __forceinline__ int cell_of(int x, int y, int Kx, int Ky, int xper, int yper, bool &ok)
+{
+    bool okx, oky;
+    x = wrap_near(x, Kx, xper, okx);
+    y = wrap_near(y, Ky, yper, oky);
+    ok = okx && oky;
+    return ok ? x * Ky + y : 0;
+}
+
+// the couplings of cell f = (x, y) (unwrapped) of level L and the mask of its neighbours inside the
+// level; a neighbour outside gets coupling 0 (st_nbrs)
+__device__ __forceinline__ unsigned st_coefs(const st_level &L, int x, int y, int f, float ck[8])
+{
+    const int n = L.Kx * L.Ky;
+    unsigned ok = 0;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        int ox, oy;
+        offset_of(k, ox, oy);
+        const bool in = (L.xper || (x + ox >= 0 && x + ox < L.Kx)) && (L.yper || (y + oy >= 0 && y + oy < L.Ky));
+        ok |= (unsigned)in << k;
+        const float c = L.c[(size_t)k * n + f];
+        ck[k] = in ? c : 0.0f;
+    }
+    return ok;
+}
+
+// the residual r - A z of one cell from the z of its 3 x 3 neighbourhood in an LDS region of row
+// length SR (zc at offset 0), as st_down_lane forms it
+template <int SR>
+__device__ __forceinline__ float st_resid_lds(float rf, float d, const float ck[8], unsigned ok,
+                                              const float *__restrict__ zc)
+{
+    const float zf = zc[0];
+    float acc = d * zf;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        int ox, oy;
+        offset_of(k, ox, oy);
+        const float zj = ((ok >> k) & 1) ? zc[ox * SR + oy] : zf;
+        acc += ck[k] * (zf - zj);
+    }
+    return rf - acc;
+}
+
+// down on levels l and l+1: the pre-smoothed z of both, the right-hand side r1 of level l+1 and r2
+// of level l+2, as k_st_down<false> on level l and then on level l+1.  Regions of a block (level,
+// first cell relative to its tile's first cell, side): A (l, -3, 4PD+6) z and r, B (l, -2, 4PD+4)
+// the residual, C (l+1, -1, 2PD+2) r and z, D (l+1, 0, 2PD) the residual.
+__global__ __launch_bounds__(B) void k_st_down2(st_pair Q, const float *__restrict__ r0, float *__restrict__ z0,
+                                                float *__restrict__ r1, float *__restrict__ z1,
+                                                float *__restrict__ r2, const lv_cg_scalars *S)
+{
+    constexpr int SA = 4 * PD + 6, SB = 4 * PD + 4, SC = 2 * PD + 2, SD = 2 * PD;
+    constexpr int NA = SA * SA, NB = SB * SB, NC = SC * SC;
+    constexpr int IA = (NA + B - 1) / B, IB = (NB + B - 1) / B, IC = (NC + B - 1) / B;
+    static_assert(SD * SD <= B, "region D: at most one cell a lane");
+    __shared__ float zA[NA], rA[NA], tB[NB], rC[NC], zC[NC], tD[SD * SD];
+    if (S && S->done) return;
+    const st_level &L0 = Q.L0, &L1 = Q.L1;
+    const int qx = blockIdx.x * PD, qy = blockIdx.y * PD;  // the tile's first cell on level l+2
+    const int ax = 4 * qx - 3, ay = 4 * qy - 3, cx = 2 * qx - 1, cy = 2 * qy - 1;
+    const int tid = threadIdx.x;
+    // every global load first, unconditionally (fA..: -1 outside the level or past the region)
+    int fA[IA], fB[IB], fC[IC];
+    float ra[IA], da[IA], dB[IB], cB[IB][8], dvC[IC], cD[8];
+    unsigned okB[IB];
+#pragma unroll
+    for (int i = 0; i < IA; i++) {
+        const int s = tid + i * B, sa = s < NA ? s : NA - 1;  // lanes past the region load a cell of it
+        bool ok;
+        const int f = cell_of(ax + sa / SA, ay + sa % SA, L0.Kx, L0.Ky, L0.xper, L0.yper, ok);
+        ra[i] = r0[f];
+        da[i] = L0.dinv[f];
+        fA[i] = (ok && s < NA) ? f : -1;
+    }
+#pragma unroll
+    for (int i = 0; i < IB; i++) {
+        const int s = tid + i * B, sb = s < NB ? s : NB - 1, x = ax + 1 + sb / SB, y = ay + 1 + sb % SB;
+        bool ok;
+        const int f = cell_of(x, y, L0.Kx, L0.Ky, L0.xper, L0.yper, ok);
+        dB[i] = L0.d[f];
+        okB[i] = st_coefs(L0, x, y, f, cB[i]);
+        fB[i] = (ok && s < NB) ? f : -1;
+    }
+#pragma unroll
+    for (int i = 0; i < IC; i++) {
+        const int s = tid + i * B, sc = s < NC ? s : NC - 1;
+        bool ok;
+        const int f = cell_of(cx + sc / SC, cy + sc % SC, L1.Kx, L1.Ky, L1.xper, L1.yper, ok);
+        dvC[i] = L1.dinv[f];
+        fC[i] = (ok && s < NC) ? f : -1;
+    }
+    const int td = tid < SD * SD ? tid : 0, dxs = td / SD, dys = td % SD, Xd = cx + 1 + dxs, Yd = cy + 1 + dys;
+    bool okd;
+    int fD = cell_of(Xd, Yd, L1.Kx, L1.Ky, L1.xper, L1.yper, okd);
+    const float dD = L1.d[fD];
+    const unsigned okD = st_coefs(L1, Xd, Yd, fD, cD);
+    fD = (okd && tid < SD * SD) ? fD : -1;
+    // A: z = w D^-1 r of level l
+#pragma unroll
+    for (int i = 0; i < IA; i++) {
+        const int s = tid + i * B;
+        if (s < NA) {
+            zA[s] = fA[i] >= 0 ? OMEGA * da[i] * ra[i] : 0.0f;
+            rA[s] = fA[i] >= 0 ? ra[i] : 0.0f;
+        }
+    }
+    __syncthreads();
+    // B: the residual of level l; the tile's own cells store their z
+#pragma unroll
+    for (int i = 0; i < IB; i++) {
+        const int s = tid + i * B;
+        if (s >= NB) continue;
+        const int sx = s / SB, sy = s % SB;
+        if (fB[i] < 0) {
+            tB[s] = 0.0f;
+            continue;
+        }
+        const int a = (sx + 1) * SA + sy + 1;
+        tB[s] = st_resid_lds<SA>(rA[a], dB[i], cB[i], okB[i], zA + a);
+        const int x = ax + 1 + sx, y = ay + 1 + sy;
+        if (sx >= 2 && sx < 4 * PD + 2 && sy >= 2 && sy < 4 * PD + 2 && x < L0.Kx && y < L0.Ky) z0[fB[i]] = zA[a];
+    }
+    __syncthreads();
+    // C: r of level l+1, the sum of its children's residuals in sum_children's order, and its z
+#pragma unroll
+    for (int i = 0; i < IC; i++) {
+        const int s = tid + i * B;
+        if (s >= NC) continue;
+        const int sx = s / SC, sy = s % SC;
+        if (fC[i] < 0) {
+            rC[s] = 0.0f;
+            zC[s] = 0.0f;
+            continue;
+        }
+        const int b = 2 * sx * SB + 2 * sy;
+        const float r = (tB[b] + tB[b + 1]) + (tB[b + SB] + tB[b + SB + 1]);
+        rC[s] = r;
+        zC[s] = OMEGA * dvC[i] * r;
+        const int X = cx + sx, Y = cy + sy;
+        if (sx >= 1 && sx < 2 * PD + 1 && sy >= 1 && sy < 2 * PD + 1 && X < L1.Kx && Y < L1.Ky) r1[fC[i]] = r;
+    }
+    __syncthreads();
+    // D: the residual of level l+1 on the tile's children; they store their z
+    if (tid < SD * SD) {
+        const int c = (dxs + 1) * SC + dys + 1;
+        tD[tid] = fD >= 0 ? st_resid_lds<SC>(rC[c], dD, cD, okD, zC + c) : 0.0f;
+        if (fD >= 0 && Xd < L1.Kx && Yd < L1.Ky) z1[fD] = zC[c];
+    }
+    __syncthreads();
+    // E: r of level l+2
+    if (tid < PD * PD) {
+        const int sx = tid / PD, sy = tid % PD, X = qx + sx, Y = qy + sy;
+        if (X < Q.Kx2 && Y < Q.Ky2) {
+            const int b = 2 * sx * SD + 2 * sy;
+            r2[X * Q.Ky2 + Y] = (tD[b] + tD[b + 1]) + (tD[b + SD] + tD[b + SD + 1]);
+        }
+    }
+}
+
+// up on levels l+1 and l: the final correction t0 of level l from the final correction t2 of level
+// l+2, as k_st_up<false> on level l+1 and then on level l (level l+1's own correction stays in LDS).
+// Regions of a block (level, first cell relative to the tile's, side): Z0 (l, -1, PU+2) z, V (l+1,
+// -1, PU/2+2) the correction, Z1 (l+1, -2, PU/2+4) z, E (l+2, -1, PU/4+2) t2.
+__global__ __launch_bounds__(B) void k_st_up2(st_pair Q, const float *__restrict__ r0, const float *__restrict__ z0,
+                                              const float *__restrict__ r1, const float *__restrict__ z1,
+                                              const float *__restrict__ t2, float *__restrict__ t0,
+                                              const lv_cg_scalars *S)
+{
+    constexpr int SZ0 = PU + 2, SV = PU / 2 + 2, SZ1 = PU / 2 + 4, SE = PU / 4 + 2;
+    constexpr int NZ0 = SZ0 * SZ0, IZ0 = (NZ0 + B - 1) / B;
+    static_assert(PU * PU == B && SZ1 * SZ1 <= B && SV * SV <= B, "one cell of each region a lane");
+    __shared__ float zl[NZ0], vm[SV * SV], zm[SZ1 * SZ1], e2[SE * SE];
+    if (S && S->done) return;
+    const st_level &L0 = Q.L0, &L1 = Q.L1;
+    const int x0 = blockIdx.x * PU, y0 = blockIdx.y * PU;  // the tile's first cell on level l
+    const int vx = x0 / 2 - 1, vy = y0 / 2 - 1, ex = x0 / 4 - 1, ey = y0 / 4 - 1;
+    const int tid = threadIdx.x;
+    // every global load first, unconditionally (from cell 0 outside a level or a region)
+    float zv[IZ0];
+#pragma unroll
+    for (int i = 0; i < IZ0; i++) {
+        const int s = tid + i * B, sz = s < NZ0 ? s : NZ0 - 1;  // lanes past the region load a cell of it
+        bool ok;
+        const int f = cell_of(x0 - 1 + sz / SZ0, y0 - 1 + sz % SZ0, L0.Kx, L0.Ky, L0.xper, L0.yper, ok);
+        const float v = z0[f];
+        zv[i] = ok ? v : 0.0f;
+    }
+    bool okm, oke;
+    const int tm = tid < SZ1 * SZ1 ? tid : 0, te = tid < SE * SE ? tid : 0;
+    const int fm = cell_of(vx - 1 + tm / SZ1, vy - 1 + tm % SZ1, L1.Kx, L1.Ky, L1.xper, L1.yper, okm);
+    const int fe = cell_of(ex + te / SE, ey + te % SE, Q.Kx2, Q.Ky2, L1.xper, L1.yper, oke);
+    const float zm_ = z1[fm], ev_ = t2[fe];
+    const float zmv = okm ? zm_ : 0.0f, ev = oke ? ev_ : 0.0f;
+    const int tv = tid < SV * SV ? tid : 0, vxs = tv / SV, vys = tv % SV, Xv = vx + vxs, Yv = vy + vys;
+    bool okv;
+    int fV = cell_of(Xv, Yv, L1.Kx, L1.Ky, L1.xper, L1.yper, okv);
+    float cV[8];
+    const float dV = L1.d[fV], dvV = L1.dinv[fV], rV = r1[fV];
+    const unsigned okV = st_coefs(L1, Xv, Yv, fV, cV);
+    fV = (okv && tid < SV * SV) ? fV : -1;
+    const int txs = tid / PU, tys = tid % PU, x = x0 + txs, y = y0 + tys;
+    const bool own = x < L0.Kx && y < L0.Ky;
+    const int f = own ? x * L0.Ky + y : 0;
+    float cT[8];
+    const float dT = L0.d[f], dvT = L0.dinv[f], rT = r0[f];
+    const unsigned okT = st_coefs(L0, x, y, f, cT);
+#pragma unroll
+    for (int i = 0; i < IZ0; i++)
+        if (tid + i * B < NZ0) zl[tid + i * B] = zv[i];
+    if (tid < SZ1 * SZ1) zm[tid] = zmv;
+    if (tid < SE * SE) e2[tid] = ev;
+    __syncthreads();
+    // V: level l+1's final correction (st_up_cell with ec = t2)
+    if (tid < SV * SV) {
+        float v = 0.0f;
+        if (fV >= 0) {
+            const float a = L1.alpha;
+            const int zc = (vxs + 1) * SZ1 + vys + 1;
+            const float pe = e2[((Xv >> 1) - ex) * SE + ((Yv >> 1) - ey)];
+            const float z1c = zm[zc] + a * pe;
+            float acc = dV * z1c;
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                int ox, oy;
+                offset_of(k, ox, oy);
+                const bool in = (okV >> k) & 1;
+                const float zj = in ? zm[zc + ox * SZ1 + oy] : zm[zc];
+                const float ej = a * (in ? e2[(((Xv + ox) >> 1) - ex) * SE + (((Yv + oy) >> 1) - ey)] : pe);
+                acc += cV[k] * (z1c - (zj + ej));
+            }
+            v = z1c + OMEGA * dvV * (rV - acc);
+        }
+        vm[tid] = v;
+    }
+    __syncthreads();
+    // the tile: level l's final correction (st_up_cell with ec = level l+1's)
+    if (own) {
+        const float a = L0.alpha;
+        const int zc = (txs + 1) * SZ0 + tys + 1;
+        const float pv = vm[((x >> 1) - vx) * SV + ((y >> 1) - vy)];
+        const float z1c = zl[zc] + a * pv;
+        float acc = dT * z1c;
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            int ox, oy;
+            offset_of(k, ox, oy);
+            const bool in = (okT >> k) & 1;
+            const float zj = in ? zl[zc + ox * SZ0 + oy] : zl[zc];
+            const float ej = a * (in ? vm[(((x + ox) >> 1) - vx) * SV + (((y + oy) >> 1) - vy)] : pv);
+            acc += cT[k] * (z1c - (zj + ej));
+        }
+        t0[f] = z1c + OMEGA * dvT * (rT - acc);
+    }
+}
+
 // The small levels of the cycle in one workgroup: a program of operations on levels first..last
 // (last: the dense one), built at set-up for a V- or W-cycle, each operation followed by a barrier
 // (the levels' vectors stay in global memory, which the barrier makes visible across the
@@ -1336,6 +1618,16 @@ hipError_t lv_amg_setup(lv_amg *G, const lv_grid_params &g, const double *x, con
     int lt = G->nlev;
     while (lt > 1 && G->lev[lt - 1].n <= TAIL_MAX) lt--;
     G->ltail = lt;
+    // pairs of levels above the tail that run as one kernel each way, from the tail upwards (the
+    // smallest levels gain most): level l at least PAIR_MIN_K wide, a multiple of 4 on a periodic axis
+    memset(G->pair, 0, sizeof(G->pair));
+    for (int k = lt - 2; k >= 2;) {
+        const lv_amg_level &L = G->lev[k];
+        const bool ok = L.Kx >= PAIR_MIN_K && L.Ky >= PAIR_MIN_K && (!G->xper || L.Kx % 4 == 0) &&
+                        (!G->yper || L.Ky % 4 == 0);
+        G->pair[k] = ok ? 1 : 0;
+        k -= ok ? 2 : 1;
+    }
     G->wcycle = A.singular ? 1 : 0;
     G->nprog = tail_program(lt, G->nlev, G->wcycle != 0, G->hprog);
     AMG_CHECK(hipMemcpyAsync(G->prog, G->hprog, sizeof(int) * G->nprog, hipMemcpyHostToDevice, s));
@@ -1415,6 +1707,14 @@ hipError_t lv_amg_setup(lv_amg *G, const lv_grid_params &g, const double *x, con
     return hipGetLastError();
 }
 
+// levels l and l+1 for the pair kernels
+static st_pair st_pair_of(const lv_amg *G, int l)
+{
+    const lv_amg_level &A = G->lev[l], &C = G->lev[l + 1], &C2 = G->lev[l + 2];
+    return st_pair{st_level{A.Kx, A.Ky, G->xper, G->yper, A.d, A.c, A.dinv, A.alpha},
+                   st_level{C.Kx, C.Ky, G->xper, G->yper, C.d, C.c, C.dinv, C.alpha}, C2.Kx, C2.Ky};
+}
+
 // the cycle on a fine view F; agg: level-1 cell of each fine row; list: the fine rows of each cell
 // (null when they are consecutive)
 template <class V>
@@ -1450,9 +1750,18 @@ static hipError_t vcycle(const lv_amg *G, const fine_ops<V> &F, const int *agg, 
     // coarse levels above the tail, down: each level's pre-smoothed z and the next level's
     // right-hand side
     const int lt = G->ltail;
+    const bool pairs = !G->level_kernels;
     for (int l = 1; l < lt; l++) {
         const lv_amg_level &L = G->lev[l], &C = G->lev[l + 1];
         const st_level sl{L.Kx, L.Ky, G->xper, G->yper, L.d, L.c, L.dinv, L.alpha};
+        if (pairs && G->pair[l]) {
+            const lv_amg_level &C2 = G->lev[l + 2];
+            const st_pair q = st_pair_of(G, l);
+            hipLaunchKernelGGL(k_st_down2, dim3((C2.Kx + PD - 1) / PD, (C2.Ky + PD - 1) / PD), dim3(B), 0, s, q,
+                               (const float *)L.r, L.z, C.r, C.z, C2.r, S);
+            l++;
+            continue;
+        }
         if (l == 1 && z1_given)
             hipLaunchKernelGGL(k_st_down<true>, g1((int64_t)C.n * CF2), dim3(B), 0, s, sl, C.Ky, C.n,
                                (const float *)L.r, L.z, C.r, S);
@@ -1473,6 +1782,14 @@ static hipError_t vcycle(const lv_amg *G, const fine_ops<V> &F, const int *agg, 
     // order with the Jacobi smoother also prolongates into z0
     const bool prolong_in_up = lt > 1 && !G->fine_learned && list == nullptr;
     for (int l = lt - 1; l >= 1; l--) {
+        if (pairs && l >= 3 && G->pair[l - 1]) {
+            const lv_amg_level &L = G->lev[l - 1], &C = G->lev[l], &C2 = G->lev[l + 1];
+            hipLaunchKernelGGL(k_st_up2, dim3((L.Kx + PU - 1) / PU, (L.Ky + PU - 1) / PU), dim3(B), 0, s,
+                               st_pair_of(G, l - 1), (const float *)L.r, (const float *)L.z, (const float *)C.r,
+                               (const float *)C.z, (const float *)C2.t, L.t, S);
+            l--;
+            continue;
+        }
         const lv_amg_level &L = G->lev[l], &C = G->lev[l + 1];
         const st_level sl{L.Kx, L.Ky, G->xper, G->yper, L.d, L.c, L.dinv, L.alpha};
         if (l == 1 && prolong_in_up)

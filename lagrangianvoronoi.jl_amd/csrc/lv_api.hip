@@ -1189,6 +1189,7 @@ static int spai_prepare(lv_ctx *ctx, const lv_solve_opts &opt, lv_precond *M, bo
         ctx->amg_learned = (int)learned;
         ctx->amg_layout = opt.amg_layout;
     }
+    ctx->amg.level_kernels = opt.amg_level_kernels ? 1 : 0;
     M->amg = &ctx->amg;
     M->A = &ctx->A;
     *have = true;
@@ -1283,6 +1284,7 @@ int lv_apply_preconditioner(lv_ctx *ctx, int32_t method, const double *r, double
     lv_solve_opts opt{};
     opt.method = method;
     opt.amg_layout = ctx->amg_layout > 0 ? ctx->amg_layout : 0;  // the layout of the last multilevel solve
+    opt.amg_level_kernels = ctx->amg.level_kernels;               // and its coarse-level kernels
     lv_precond M{};
     bool have = false;
     int rc = spai_prepare(ctx, opt, &M, &have);

@@ -140,6 +140,33 @@ def test_compact_overflow_is_bit_identical(lv, oracle, gpu, kind, ic, K, dt):
         assert list(out[0][1]) == list(out[layout][1]) and list(out[0][2]) == list(out[layout][2])
 
 
+@pytest.mark.parametrize("kind,ic,K,dt", [("periodic", "tg", 512, 2e-3), ("walls", "cavity", 300, 1e-3)])
+def test_paired_coarse_levels_are_bit_identical(lv, oracle, gpu, kind, ic, K, dt):
+    """Pairs of coarse levels above the single-workgroup tail run as one down and one up kernel each
+    (amg_level_kernels 0) or every level as its own kernels (1): the same cycle output, pressure and
+    iterations bit for bit.  At 512^2 periodic levels 2 and 3 (128^2 and 64^2 cells) are paired, at
+    300^2 with walls levels 2 and 3 (75^2 and 38^2 cells: odd widths, partial tiles at the walls)."""
+    _flush("amg paired coarse levels %s %d" % (kind, K))
+    grid, op, solver, x, dr, d, rp, cols, coef = _system(lv, oracle, kind, K, ic, dt)
+    b, _ = solver.rhs(dt, False)
+    if not np.any(d > 0):
+        b = np.random.default_rng(6).standard_normal(grid.n)
+        b -= b.mean()
+    x0 = grid.get("P")
+    r = np.random.default_rng(13).standard_normal(grid.n)
+    out = {}
+    for lk in (1, 0):
+        P = solver.solve(b, x0, method="amg", amg_level_kernels=lk)
+        st = solver.last_stats
+        assert max(st["relres"]) <= 1e-8, st
+        z = solver.apply_preconditioner(r, "amg")  # the cycle with the last solve's kernels
+        out[lk] = (P, z, list(st["iters"][:st["nsolves"]]), list(st["outer"][:st["nsolves"]]))
+    assert np.all(np.isfinite(out[0][1])) and np.linalg.norm(out[0][1]) > 0
+    assert np.array_equal(out[0][1], out[1][1])
+    assert np.array_equal(out[0][0], out[1][0])
+    assert out[0][2] == out[1][2] and out[0][3] == out[1][3]
+
+
 @pytest.mark.parametrize("method", ["pcg64", "mixed"])
 def test_own_order_compact_overflow_is_bit_identical(lv, oracle, gpu, method):
     """The operator's own ELL read with the compact copy of the slots past the sixth (systems of at

@@ -477,7 +477,8 @@ template <typename T, class V, bool COMPACT>
 __global__ __launch_bounds__(B, (cg_waves<T, COMPACT>::value)) void k_cg_s(int n, V A, const T *__restrict__ diag,
                                             const T *__restrict__ z, T *__restrict__ p, T *__restrict__ q,
                                             const double *__restrict__ Prr, double *__restrict__ Ppq,
-                                            lv_cg_scalars *S, lv_progress *prog, int par, int k, int first)
+                                            lv_cg_scalars *S, lv_progress *prog, int par, int k, int first,
+                                            int seen_here)
 {
     if (S->done) {
         // a sweep that starts done (nothing left to solve): its first launch tells the host
@@ -498,7 +499,7 @@ __global__ __launch_bounds__(B, (cg_waves<T, COMPACT>::value)) void k_cg_s(int n
         if (stop) S->done = conv ? 1 : 2;
         if (prog) {
             if (stop) __hip_atomic_store(&prog->done, conv ? 1 : 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-            __hip_atomic_store(&prog->seen, k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            if (seen_here) __hip_atomic_store(&prog->seen, k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
         }
     }
     if (stop) return;
@@ -600,18 +601,23 @@ __global__ __launch_bounds__(B) void k_cg_u(int n, T *__restrict__ x, T *__restr
 }
 
 // the convergence test of k_cg_s, made on the residual k_cg_u has just updated (after k+1
-// updates), so that a converged iteration skips the preconditioner
+// updates), so that a converged iteration skips the preconditioner; it also marks iteration k as
+// seen (its test ran), which lets the host launch iteration k+1 while the device applies the
+// preconditioner
 __global__ __launch_bounds__(B) void k_conv_check(const double *__restrict__ Pur, int nb, lv_cg_scalars *S,
                                                   lv_progress *prog, int k)
 {
     if (S->done) return;
     double o[1];
     reduce_all<1>(Pur, nb, o);
-    if (threadIdx.x == 0 && o[0] <= S->thr2) {
-        S->rr = o[0];
-        S->iters = k + 1;
-        S->done = 1;
-        if (prog) __hip_atomic_store(&prog->done, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    if (threadIdx.x == 0) {
+        if (o[0] <= S->thr2) {
+            S->rr = o[0];
+            S->iters = k + 1;
+            S->done = 1;
+            if (prog) __hip_atomic_store(&prog->done, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
+        if (prog) __hip_atomic_store(&prog->seen, k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     }
 }
 
@@ -1013,11 +1019,14 @@ inline parts parts_of(double *base) { return parts{base, base + 3 * NBMAX, base 
 
 // CG iterations (two launches each; with a preconditioner also the convergence test and its
 // application) until the device flags completion; *dflag is the done flag (1 converged, 2 stopped).
-// It returns without synchronising: S (iterations, residuals) is read by the caller's next read.  The host launches iteration k once the device
-// has begun iteration k - 1 (prog->seen, in pinned host memory), so the queue stays one iteration
-// deep without synchronising, and stops launching as soon as prog->done is set: at most about one
-// iteration runs as no-ops after convergence.  If the record does not advance for 20 ms the host
-// synchronises and reads the scalars instead, so the loop ends whatever the record shows.
+// It returns without synchronising: S (iterations, residuals) is read by the caller's next read.
+// The host launches iteration k once the device has marked iteration k - 1 seen in prog (pinned
+// host memory): k_cg_s marks it when it starts, or, with a preconditioner, k_conv_check once its
+// test has run, so the host launches the next iteration while the device applies the
+// preconditioner.  It stops launching as soon as prog->done is set: at most one iteration (with a
+// preconditioner, one application of it) runs as no-ops after convergence.  If the record does not
+// advance for 20 ms the host synchronises and reads the scalars instead, so the loop ends whatever
+// the record shows.
 template <typename T, class V>
 hipError_t cg_iterate(int n, V A, const T *diag, const T *dinv, T *x, T *r, T *z, T *p, T *q, const parts &P,
                       lv_cg_scalars *dS, lv_progress *hprog, lv_progress *dprog, lv_cg_scalars *pin, hipStream_t s,
@@ -1056,10 +1065,10 @@ hipError_t cg_iterate(int n, V A, const T *diag, const T *dinv, T *x, T *r, T *z
             timer_scope ts(tm, s);
             if (A.ocap > 0)
                 hipLaunchKernelGGL((k_cg_s<T, V, true>), dim3(nb), dim3(B), 0, s, n, A, diag, (const T *)z, p, q,
-                                   (const double *)P.rr, P.pq, dS, dprog, par, k, k == 0 ? 1 : 0);
+                                   (const double *)P.rr, P.pq, dS, dprog, par, k, k == 0 ? 1 : 0, M ? 0 : 1);
             else
                 hipLaunchKernelGGL((k_cg_s<T, V, false>), dim3(nb), dim3(B), 0, s, n, A, diag, (const T *)z, p, q,
-                                   (const double *)P.rr, P.pq, dS, dprog, par, k, k == 0 ? 1 : 0);
+                                   (const double *)P.rr, P.pq, dS, dprog, par, k, k == 0 ? 1 : 0, M ? 0 : 1);
         }
         if (M) {
             // the cycle's pre-smoothed z0 stored with the residual update (FP32 solves only)

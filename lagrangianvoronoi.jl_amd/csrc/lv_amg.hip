@@ -62,6 +62,12 @@ __device__ __forceinline__ float sum_lanes(float s)
     for (int m = 1; m < RL; m <<= 1) s += __shfl_xor(s, m);
     return s;
 }
+__device__ __forceinline__ double sum_lanes(double s)
+{
+#pragma unroll
+    for (int m = 1; m < RL; m <<= 1) s += __shfl_xor(s, m);
+    return s;
+}
 
 inline dim3 g1(int64_t count) { return dim3((unsigned)std::max<int64_t>(1, (count + B - 1) / B)); }
 
@@ -121,14 +127,9 @@ __global__ __launch_bounds__(B) void k_agg_fill(int n, const int *__restrict__ a
     list[start[a] + atomicAdd(&cursor[a], 1)] = i;
 }
 
-// per cell: sort its rows (fixed summation order), then the Galerkin sums of d and of the couplings
-// to the 8 neighbouring cells, accumulated in FP64
-__global__ __launch_bounds__(B) void k_coarse1(int na, int Kx, int Ky, int xper, int yper,
-                                               const int *__restrict__ start, int *__restrict__ list,
-                                               const int *__restrict__ agg, int n, const int *__restrict__ deg,
-                                               const int *__restrict__ ecol, const double *__restrict__ ev,
-                                               const double *__restrict__ d64, float *__restrict__ dout,
-                                               float *__restrict__ cout, float *__restrict__ dinv)
+// per cell: its rows sorted (a fixed order for every sum over a cell's rows, here and in the
+// aggregate order)
+__global__ __launch_bounds__(B) void k_agg_sort(int na, const int *__restrict__ start, int *__restrict__ list)
 {
     const int I = blockIdx.x * B + threadIdx.x;
     if (I >= na) return;
@@ -142,21 +143,42 @@ __global__ __launch_bounds__(B) void k_coarse1(int na, int Kx, int Ky, int xper,
         }
         list[b] = v;
     }
-    const int Ix = I / Ky, Iy = I % Ky;
+}
+
+// per cell, RL lanes: the Galerkin sums of d and of the couplings to the 8 neighbouring cells in
+// FP64, lane q over the cell's sorted rows q, q + RL, ..., then the lanes' sums added by sum_lanes'
+// fixed tree
+__global__ __launch_bounds__(B) void k_coarse1(int na, int Kx, int Ky, int xper, int yper,
+                                               const int *__restrict__ start, const int *__restrict__ list,
+                                               const int *__restrict__ agg, int n, const int *__restrict__ deg,
+                                               const int *__restrict__ ecol, const double *__restrict__ ev,
+                                               const double *__restrict__ d64, float *__restrict__ dout,
+                                               float *__restrict__ cout, float *__restrict__ dinv)
+{
+    const int t = blockIdx.x * B + threadIdx.x;
+    const int I = t / RL, q = t % RL;
+    const bool live = I < na;  // every lane takes part in the shuffles below
+    const int Ic = live ? I : 0;
+    const int s0 = start[Ic], s1 = live ? start[Ic + 1] : s0;
+    const int Ix = Ic / Ky, Iy = Ic % Ky;
     double d = 0.0, c[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    for (int a = s0; a < s1; a++) {
+    for (int a = s0 + q; a < s1; a += RL) {
         const int i = list[a];
         d += d64[i];
         const int dg = deg[i];
         for (int k = 0; k < dg; k++) {
             const size_t e = (size_t)k * n + i;
             const int J = agg[ecol[e]];
-            if (J == I) continue;
+            if (J == Ic) continue;
             const int dx = fold(J / Ky - Ix, Kx, xper), dy = fold(J % Ky - Iy, Ky, yper);
             if (dx < -1 || dx > 1 || dy < -1 || dy > 1 || (dx == 0 && dy == 0)) continue;
             c[slot_of(dx, dy)] += ev[e];
         }
     }
+    d = sum_lanes(d);
+#pragma unroll
+    for (int k = 0; k < 8; k++) c[k] = sum_lanes(c[k]);
+    if (!live || q != 0) return;
     if (s1 == s0) d = 1.0;  // an empty cell is a decoupled identity row
     double sum = d;
 #pragma unroll
@@ -210,59 +232,71 @@ __global__ __launch_bounds__(B) void k_coarsen(int Kxc, int Kyc, int Kxf, int Ky
 // the coarsest level as a dense FP64 matrix (plus sigma/nc everywhere for a singular system, which
 // makes it definite without changing its action on mean-free vectors), inverted in place in LDS by
 // Gauss-Jordan in one workgroup (an SPD matrix needs no pivoting); the inverse is stored
-// column-major in FP32
-__global__ __launch_bounds__(256) void k_dense_inverse(int Kx, int Ky, int xper, int yper, const float *__restrict__ d,
+// column-major in FP32.  In LDS the rows have the fixed stride DM, so that an entry's row and column
+// are a shift and a mask of its index, and the 1024 lanes take DM * DM / 1024 entries each.
+constexpr int DM = LV_AMG_DENSE_MAX, DTB = 1024;
+static_assert(DM == 64 && (DM * DM) % DTB == 0, "dense level: 64 x 64 entries over the workgroup");
+__global__ __launch_bounds__(DTB) void k_dense_inverse(int Kx, int Ky, int xper, int yper, const float *__restrict__ d,
                                                        const float *__restrict__ c, int singular,
                                                        float *__restrict__ inv)
 {
     const int nc = Kx * Ky;
-    __shared__ double M[LV_AMG_DENSE_MAX * LV_AMG_DENSE_MAX];
-    __shared__ double colp[LV_AMG_DENSE_MAX];
-    for (int t = threadIdx.x; t < nc * nc; t += 256) M[t] = 0.0;
+    __shared__ double M[DM * DM];
+    __shared__ double colp[DM], rowdiag[DM];
+    for (int t = threadIdx.x; t < DM * DM; t += DTB) M[t] = 0.0;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        double dsum = 0.0;
-        for (int I = 0; I < nc; I++) {
-            const int Ix = I / Ky, Iy = I % Ky;
-            double diag = d[I];
-            for (int k = 0; k < 8; k++) {
-                const double ck = c[(size_t)k * nc + I];
-                if (ck == 0.0) continue;
-                int ox, oy;
-                offset_of(k, ox, oy);
-                bool okx, oky;
-                const int jx = wrap(Ix + ox, Kx, xper, okx), jy = wrap(Iy + oy, Ky, yper, oky);
-                if (!okx || !oky) continue;
-                diag += ck;
-                M[I * nc + jx * Ky + jy] -= ck;
-            }
-            M[I * nc + I] += diag;
-            dsum += diag;
+    if (threadIdx.x < nc) {  // row I by lane I
+        const int I = threadIdx.x, Ix = I / Ky, Iy = I % Ky;
+        double diag = d[I];
+        for (int k = 0; k < 8; k++) {
+            const double ck = c[(size_t)k * nc + I];
+            if (ck == 0.0) continue;
+            int ox, oy;
+            offset_of(k, ox, oy);
+            bool okx, oky;
+            const int jx = wrap(Ix + ox, Kx, xper, okx), jy = wrap(Iy + oy, Ky, yper, oky);
+            if (!okx || !oky) continue;
+            diag += ck;
+            M[I * DM + jx * Ky + jy] -= ck;
         }
-        if (singular) {
-            const double sig = dsum / nc / nc;
-            for (int t = 0; t < nc * nc; t++) M[t] += sig;
-        }
+        M[I * DM + I] += diag;
+        rowdiag[I] = diag;
     }
     __syncthreads();
+    if (singular) {
+        __shared__ double sig;
+        if (threadIdx.x == 0) {
+            double dsum = 0.0;
+            for (int I = 0; I < nc; I++) dsum += rowdiag[I];
+            sig = dsum / nc / nc;
+        }
+        __syncthreads();
+        for (int t = threadIdx.x; t < DM * DM; t += DTB)
+            if (t / DM < nc && t % DM < nc) M[t] += sig;
+        __syncthreads();
+    }
     // in place: after pivot p, row p holds row p of the partial inverse and column p the
     // eliminated multipliers
     for (int p = 0; p < nc; p++) {
-        const double piv = M[p * nc + p];
-        for (int i = threadIdx.x; i < nc; i += 256) colp[i] = M[i * nc + p];
+        const double piv = M[p * DM + p];
+        if (threadIdx.x < nc) colp[threadIdx.x] = M[threadIdx.x * DM + p];
         __syncthreads();
-        for (int j = threadIdx.x; j < nc; j += 256) M[p * nc + j] = (j == p ? 1.0 : M[p * nc + j]) / piv;
+        if (threadIdx.x < nc) {
+            const int j = threadIdx.x;
+            M[p * DM + j] = (j == p ? 1.0 : M[p * DM + j]) / piv;
+        }
         __syncthreads();
-        for (int t = threadIdx.x; t < nc * nc; t += 256) {
-            const int i = t / nc, j = t % nc;
-            if (i == p) continue;
-            M[t] = (j == p ? 0.0 : M[t]) - colp[i] * M[p * nc + j];
+#pragma unroll
+        for (int q = 0; q < DM * DM / DTB; q++) {
+            const int t = threadIdx.x + q * DTB, i = t / DM, j = t % DM;
+            if (i >= nc || j >= nc || i == p) continue;
+            M[t] = (j == p ? 0.0 : M[t]) - colp[i] * M[p * DM + j];
         }
         __syncthreads();
     }
-    for (int t = threadIdx.x; t < nc * nc; t += 256) {
-        const int i = t / nc, j = t % nc;
-        inv[j * nc + i] = (float)M[t];
+    for (int t = threadIdx.x; t < DM * DM; t += DTB) {
+        const int i = t / DM, j = t % DM;
+        if (i < nc && j < nc) inv[j * nc + i] = (float)M[t];
     }
 }
 
@@ -1596,7 +1630,9 @@ hipError_t lv_amg_setup(lv_amg *G, const lv_grid_params &g, const double *x, con
     hipLaunchKernelGGL(k_agg_fill, g1(n), dim3(B), 0, s, n, (const int *)G->agg, (const int *)G->start, G->cursor,
                        G->list);
     AMG_CHECK(level_alloc(G->lev[1], Kx, Ky));
-    hipLaunchKernelGGL(k_coarse1, g1(na), dim3(B), 0, s, na, Kx, Ky, G->xper, G->yper, (const int *)G->start, G->list,
+    hipLaunchKernelGGL(k_agg_sort, g1(na), dim3(B), 0, s, na, (const int *)G->start, G->list);
+    hipLaunchKernelGGL(k_coarse1, g1((int64_t)na * RL), dim3(B), 0, s, na, Kx, Ky, G->xper, G->yper,
+                       (const int *)G->start, (const int *)G->list,
                        (const int *)G->agg, n, (const int *)A.deg, (const int *)A.ecol, (const double *)A.ev64,
                        (const double *)A.diag64, G->lev[1].d, G->lev[1].c, G->lev[1].dinv);
     int l = 1;
@@ -1632,7 +1668,7 @@ hipError_t lv_amg_setup(lv_amg *G, const lv_grid_params &g, const double *x, con
     G->nprog = tail_program(lt, G->nlev, G->wcycle != 0, G->hprog);
     AMG_CHECK(hipMemcpyAsync(G->prog, G->hprog, sizeof(int) * G->nprog, hipMemcpyHostToDevice, s));
     const lv_amg_level &D = G->lev[l];
-    hipLaunchKernelGGL(k_dense_inverse, dim3(1), dim3(256), 0, s, D.Kx, D.Ky, G->xper, G->yper, (const float *)D.d,
+    hipLaunchKernelGGL(k_dense_inverse, dim3(1), dim3(DTB), 0, s, D.Kx, D.Ky, G->xper, G->yper, (const float *)D.d,
                        (const float *)D.c, A.singular, G->dense_inv);
     {
         const tail_layout Y = tail_layout_of(G);

@@ -1358,14 +1358,28 @@ __global__ __launch_bounds__(B) void k_perm_ell(int n, const int *__restrict__ l
     pdinv[p] = dinv[i];
     if (mdg) pmdg[p] = mdg[i];
     bool wide = false;
-    for (int k = 0; k < d; k++) {
+    // the first U slots' columns and couplings loaded together, then their rows' new positions
+    // gathered together (a slot past the degree reads the allocated slot, and its column is taken
+    // as the row's own so that the gather stays in bounds; neither is used)
+    constexpr int U = 8;
+    int pj[U];
+    float vj[U];
+#pragma unroll
+    for (int k = 0; k < U; k++) {
+        const size_t e = (size_t)k * n + i;
+        const int c = ecol[e];
+        vj[k] = ev[e];
+        pj[k] = k < d ? c : i;
+    }
+#pragma unroll
+    for (int k = 0; k < U; k++) pj[k] = iperm[pj[k]];
+    auto put = [&](int k, int col_new, float v) {
         const size_t e = (size_t)k * n + i, ep = (size_t)k * n + p;
-        int off = iperm[ecol[e]] - p;
+        int off = col_new - p;
         if (off > n / 2) off -= n;
         if (off < -(n / 2)) off += n;
         wide |= (off < -32768 || off > 32767);
         poff[ep] = (int16_t)off;
-        const float v = ev[e];
         pev32[ep] = v;
         pev16[ep] = lv_bf16_bits(v);
         if (mval) pmval[ep] = mval[e];
@@ -1374,6 +1388,13 @@ __global__ __launch_bounds__(B) void k_perm_ell(int n, const int *__restrict__ l
             ooff[eo] = (int16_t)off;
             oev32[eo] = v;
         }
+    };
+#pragma unroll
+    for (int k = 0; k < U; k++)
+        if (k < d) put(k, pj[k], vj[k]);
+    for (int k = U; k < d; k++) {
+        const size_t e = (size_t)k * n + i;
+        put(k, iperm[ecol[e]], ev[e]);
     }
     if (wide) atomicOr(flag, 1);
 }

@@ -63,6 +63,31 @@ def test_headline_mesh_bit_exact(lv, oracle, headline):
 
 
 @pytest.mark.timeout(600)
+def test_headline_paired_coarse_levels_bit_identical(lv, oracle, headline):
+    """The benchmark's pressure system solved with pairs of coarse levels in one kernel each way
+    (levels 2-3 and 4-5 of this hierarchy) and with one kernel per level: the same pressure and
+    iterations bit for bit."""
+    grid, op, x = headline
+    ic = lv.inputs.taylor_green(x, op.area())
+    for k in ("v", "rho", "mass", "P", "e", "mu"):
+        grid.set_soa(k, ic[k])
+    lv.stiffened_eos(grid, ic["gamma"], ic["P0"])
+    solver = lv.PressureSolver(grid, method="amg", tol=1e-8, itmax=4000)
+    solver.assemble(DT)
+    b, _ = solver.rhs(DT, False)
+    x0 = grid.get("P")
+    out = {}
+    for lk in (1, 0):
+        _say("headline %d^2: pressure solve, amg_level_kernels %d" % (K, lk))
+        P = solver.solve(b, x0, amg_level_kernels=lk)
+        st = solver.last_stats
+        assert max(st["relres"]) <= 1e-8, st
+        out[lk] = (P, list(st["iters"][:st["nsolves"]]), list(st["outer"][:st["nsolves"]]))
+    assert np.array_equal(out[0][0], out[1][0])
+    assert out[0][1] == out[1][1] and out[0][2] == out[1][2]
+
+
+@pytest.mark.timeout(600)
 def test_headline_full_step(lv, oracle, headline):
     grid, op, x = headline
     ic = lv.inputs.taylor_green(x, op.area())

@@ -64,14 +64,15 @@ struct lv_ctx {
     lv_timer *timers[LV_NKTIMERS] = {nullptr, nullptr, nullptr, nullptr};
     // Picard extrapolation: orthonormal A-images q_k of past Picard corrections and their
     // preimages z_k (A z_k = q_k), the last two Picard iterates and their A-images
-    double *pw_q[8] = {nullptr}, *pw_z[8] = {nullptr};
+    float *pw_q[8] = {nullptr}, *pw_z[8] = {nullptr};  // orthonormal directions, FP32
     double *pw_P[2] = {nullptr, nullptr}, *pw_AP[2] = {nullptr, nullptr};
     double *pw_t1 = nullptr, *pw_t2 = nullptr, *pw_dots = nullptr;
     int pw_cap = 0;
     // carried Picard directions (lv_solve_opts.picard_carry): the previous call's first two
     // corrections (pc_z, pc_cnt of them, valid for pc_n particles), the ones this call is recording
     // (pc_new), and this call's orthonormal set made of the carried ones (pc_q = A pc_o)
-    double *pc_z[2] = {nullptr}, *pc_new[2] = {nullptr}, *pc_q[2] = {nullptr}, *pc_o[2] = {nullptr};
+    double *pc_z[2] = {nullptr}, *pc_new[2] = {nullptr};
+    float *pc_q[2] = {nullptr}, *pc_o[2] = {nullptr};
     int pc_cnt = 0, pc_n = -1;
     // learned preconditioner: network state and the preconditioner of the last assembled operator
     lv_spai spai{};
@@ -1345,33 +1346,31 @@ static int ensure_picard_window(lv_ctx *ctx, int w)
 // component left enters as a zero direction, which adds nothing to later projections.
 // the direction in pw_t1 with its image A pw_t1 in pw_t2 (both overwritten) into the set (Q, Z) of
 // at most w
-static int picard_add_pair(lv_ctx *ctx, double **Q, double **Z, int w, int *cnt)
+static int picard_add_pair(lv_ctx *ctx, float **Q, float **Z, int w, int *cnt)
 {
     const int n = ctx->n;
     hipStream_t s = ctx->s;
     const double *xp[8];
+    const float *xf[8];
     double c[8];
     if (*cnt > 0) {
-        const double *a[8], *b[8];
+        const double *b[8];
         for (int k = 0; k < *cnt; k++) {
-            a[k] = Q[k];
+            xf[k] = Q[k];
             b[k] = ctx->pw_t2;
         }
-        LV_HIP_CHECK(lv_dots_device(n, a, b, *cnt, ctx->W.partials, ctx->pw_dots, s));
-        for (int k = 0; k < *cnt; k++) {
-            c[k] = -1.0;  // -<q_k, t2>
-            xp[k] = Q[k];
-        }
-        LV_HIP_CHECK(lv_combine(n, ctx->pw_t2, 1.0, xp, c, *cnt, s, ctx->pw_dots, 1));
-        for (int k = 0; k < *cnt; k++) xp[k] = Z[k];
-        LV_HIP_CHECK(lv_combine(n, ctx->pw_t1, 1.0, xp, c, *cnt, s, ctx->pw_dots, 1));
+        LV_HIP_CHECK(lv_dots_device(n, xf, b, *cnt, ctx->W.partials, ctx->pw_dots, s));
+        for (int k = 0; k < *cnt; k++) c[k] = -1.0;  // -<q_k, t2>
+        LV_HIP_CHECK(lv_combine(n, ctx->pw_t2, 1.0, xf, c, *cnt, s, ctx->pw_dots, 1));
+        for (int k = 0; k < *cnt; k++) xf[k] = Z[k];
+        LV_HIP_CHECK(lv_combine(n, ctx->pw_t1, 1.0, xf, c, *cnt, s, ctx->pw_dots, 1));
     }
     {
         const double *a[1] = {ctx->pw_t2}, *b[1] = {ctx->pw_t2};
         LV_HIP_CHECK(lv_dots_device(n, a, b, 1, ctx->W.partials, ctx->pw_dots, s));
     }
     if (*cnt == w) {  // drop the oldest direction
-        double *q0 = Q[0], *z0 = Z[0];
+        float *q0 = Q[0], *z0 = Z[0];
         for (int k = 1; k < w; k++) {
             Q[k - 1] = Q[k];
             Z[k - 1] = Z[k];
@@ -1413,28 +1412,30 @@ static int picard_record(lv_ctx *ctx, double *y, const double *Pcur, const doubl
 // P0 += sum_k <q_k, r> z_k over the set (Q, Z): the guess whose residual is r minus its projection
 // on the q's.  r is b - AP_cur (APcur given), or pw_t2 as a previous projection left it (APcur null);
 // with keep, pw_t2 is left holding the projected residual.
-static int picard_project(lv_ctx *ctx, double *const *Q, double *const *Z, int cnt, const double *APcur, bool keep)
+static int picard_project(lv_ctx *ctx, float *const *Q, float *const *Z, int cnt, const double *APcur, bool keep)
 {
     const int n = ctx->n;
     hipStream_t s = ctx->s;
     const double *xp[8] = {ctx->b, APcur};
     double c[8] = {1.0, -1.0};
     if (APcur) LV_HIP_CHECK(lv_combine(n, ctx->pw_t2, 0.0, xp, c, 2, s));
-    const double *a[8], *b[8];
+    const float *xf[8];
+    const double *b[8];
     for (int k = 0; k < cnt; k++) {
-        a[k] = Q[k];
+        xf[k] = Q[k];
         b[k] = ctx->pw_t2;
         c[k] = 1.0;  // <q_k, r>
     }
-    LV_HIP_CHECK(lv_dots_device(n, a, b, cnt, ctx->W.partials, ctx->pw_dots, s));
-    for (int k = 0; k < cnt; k++) xp[k] = Z[k];
-    LV_HIP_CHECK(lv_combine(n, ctx->P0, 1.0, xp, c, cnt, s, ctx->pw_dots, 1));
+    if (cnt == 0) return LV_OK;
+    LV_HIP_CHECK(lv_dots_device(n, xf, b, cnt, ctx->W.partials, ctx->pw_dots, s));
+    for (int k = 0; k < cnt; k++) xf[k] = Z[k];
+    LV_HIP_CHECK(lv_combine(n, ctx->P0, 1.0, xf, c, cnt, s, ctx->pw_dots, 1));
     if (keep) {
         for (int k = 0; k < cnt; k++) {
-            xp[k] = Q[k];
+            xf[k] = Q[k];
             c[k] = -1.0;
         }
-        LV_HIP_CHECK(lv_combine(n, ctx->pw_t2, 1.0, xp, c, cnt, s, ctx->pw_dots, 1));
+        LV_HIP_CHECK(lv_combine(n, ctx->pw_t2, 1.0, xf, c, cnt, s, ctx->pw_dots, 1));
     }
     return LV_OK;
 }
@@ -1469,8 +1470,11 @@ int lv_find_pressure(lv_ctx *ctx, double dt, int32_t niter, const lv_solve_opts 
     int ccnt = 0;
     if (carry) {
         for (int k = 0; k < 2; k++) {
-            double **bufs[4] = {&ctx->pc_z[k], &ctx->pc_new[k], &ctx->pc_q[k], &ctx->pc_o[k]};
+            double **bufs[2] = {&ctx->pc_z[k], &ctx->pc_new[k]};
             for (double **b : bufs)
+                if (!*b) LV_HIP_CHECK(dalloc(b, (size_t)ctx->cap));
+            float **fbufs[2] = {&ctx->pc_q[k], &ctx->pc_o[k]};
+            for (float **b : fbufs)
                 if (!*b) LV_HIP_CHECK(dalloc(b, (size_t)ctx->cap));
         }
         if (ctx->pc_n == ctx->n) {

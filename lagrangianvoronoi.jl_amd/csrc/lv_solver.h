@@ -109,13 +109,20 @@ hipError_t lv_solve_device(const lv_pressure_system &A, lv_solver_work &W, int n
                            int *sweeps, double *relres, double *relres0, bool *converged, const lv_precond *M,
                            int *fallbacks);
 
-// Picard extrapolation helpers (FP64 vectors of length n on stream s)
+// Picard extrapolation helpers (vectors of length n on stream s; the window's stored directions
+// are FP32, everything else FP64; sums in FP64)
 // the dots into the device array dout[0..m) without a host copy
 hipError_t lv_dots_device(int n, const double *const *a, const double *const *b, int m, double *partials,
+                          double *dout, hipStream_t s);
+hipError_t lv_dots_device(int n, const float *const *a, const double *const *b, int m, double *partials,
                           double *dout, hipStream_t s);
 // y = alpha y + sum_q c_q x_q, the c_q scaled by device values: dmode 0 none, 1 c_q dc[q],
 // 2 c_q / sqrt(dc[0]) (0 unless dc[0] is positive and finite)
 hipError_t lv_combine(int n, double *y, double alpha, const double *const *x, const double *c, int m, hipStream_t s,
+                      const double *dc = nullptr, int dmode = 0);
+hipError_t lv_combine(int n, double *y, double alpha, const float *const *x, const double *c, int m, hipStream_t s,
+                      const double *dc = nullptr, int dmode = 0);
+hipError_t lv_combine(int n, float *y, double alpha, const double *const *x, const double *c, int m, hipStream_t s,
                       const double *dc = nullptr, int dmode = 0);
 hipError_t lv_apply_from_residual(int n, const double *b, const double *r, double *Ax, const lv_solver_work &W,
                                   hipStream_t s);

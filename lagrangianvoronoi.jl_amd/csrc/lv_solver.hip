@@ -807,22 +807,28 @@ __global__ __launch_bounds__(B) void k_correct(int n, double *__restrict__ x, co
 
 // ---- small FP64 vector algebra for the Picard extrapolation -----------------------------------
 
+// operands of the multi-dot and combination kernels; the a's and x's may be FP32 (the Picard
+// window's stored directions), everything is accumulated in FP64
+template <typename TA>
 struct lv_vec8 {
-    const double *a[8], *b[8];
+    const TA *a[8];
+    const double *b[8];
 };
+template <typename TX>
 struct lv_comb8 {
-    const double *x[8];
+    const TX *x[8];
     double c[8];
 };
 
 // up to 8 dot products a_q . b_q in one pass
-__global__ __launch_bounds__(B) void k_multidot(int n, lv_vec8 V, int m, double *__restrict__ partials)
+template <typename TA>
+__global__ __launch_bounds__(B) void k_multidot(int n, lv_vec8<TA> V, int m, double *__restrict__ partials)
 {
     double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (int i = blockIdx.x * B + threadIdx.x; i < n; i += gridDim.x * B) {
 #pragma unroll
         for (int q = 0; q < 8; q++)
-            if (q < m) v[q] += V.a[q][i] * V.b[q][i];
+            if (q < m) v[q] += (double)V.a[q][i] * V.b[q][i];
     }
     block_partials<8>(v, partials, gridDim.x);
 }
@@ -837,8 +843,10 @@ __global__ __launch_bounds__(B) void k_multidot_fin(const double *__restrict__ p
 
 // y = alpha*y + sum_q c_q x_q  (y may be one of the x_q only if its c is applied via alpha).  The
 // coefficients are C.c scaled by device values dc (the output of k_multidot_fin): dmode 0 none,
-// 1 c_q dc[q], 2 c_q / sqrt(dc[0]) (0 when dc[0] is not positive and finite).
-__global__ __launch_bounds__(B) void k_combine(int n, double *__restrict__ y, double alpha, lv_comb8 C, int m,
+// 1 c_q dc[q], 2 c_q / sqrt(dc[0]) (0 when dc[0] is not positive and finite).  Accumulated in FP64
+// and rounded once to y's type.
+template <typename TX, typename TY>
+__global__ __launch_bounds__(B) void k_combine(int n, TY *__restrict__ y, double alpha, lv_comb8<TX> C, int m,
                                                const double *__restrict__ dc, int dmode)
 {
     double c[8];
@@ -855,11 +863,11 @@ __global__ __launch_bounds__(B) void k_combine(int n, double *__restrict__ y, do
         for (int q = 0; q < 8; q++) c[q] = C.c[q] * f;
     }
     for (int i = blockIdx.x * B + threadIdx.x; i < n; i += gridDim.x * B) {
-        double acc = (alpha != 0.0) ? alpha * y[i] : 0.0;  // alpha = 0 never reads y (may be uninitialised)
+        double acc = (alpha != 0.0) ? alpha * (double)y[i] : 0.0;  // alpha = 0 never reads y (may be uninitialised)
 #pragma unroll
         for (int q = 0; q < 8; q++)
-            if (q < m) acc += c[q] * C.x[q][i];
-        y[i] = acc;
+            if (q < m) acc += c[q] * (double)C.x[q][i];
+        y[i] = (TY)acc;
     }
 }
 
@@ -1311,30 +1319,61 @@ hipError_t lv_apply_precond(const lv_pressure_system &A, lv_solver_work &W, int 
 
 // ---- host helpers for the Picard extrapolation ------------------------------------------------
 
-hipError_t lv_dots_device(int n, const double *const *a, const double *const *b, int m, double *partials,
-                          double *dout, hipStream_t s)
+namespace {
+template <typename TA>
+hipError_t dots_device(int n, const TA *const *a, const double *const *b, int m, double *partials, double *dout,
+                       hipStream_t s)
 {
-    lv_vec8 V;
+    lv_vec8<TA> V;
     for (int q = 0; q < 8; q++) {
         V.a[q] = (q < m) ? a[q] : a[0];
         V.b[q] = (q < m) ? b[q] : b[0];
     }
     const int nb = nblocks_of(n);
-    hipLaunchKernelGGL(k_multidot, dim3(nb), dim3(B), 0, s, n, V, m, partials);
+    hipLaunchKernelGGL(k_multidot<TA>, dim3(nb), dim3(B), 0, s, n, V, m, partials);
     hipLaunchKernelGGL(k_multidot_fin, dim3(1), dim3(B), 0, s, (const double *)partials, nb, dout);
     return hipGetLastError();
+}
+
+template <typename TX, typename TY>
+hipError_t combine(int n, TY *y, double alpha, const TX *const *x, const double *c, int m, hipStream_t s,
+                   const double *dc, int dmode)
+{
+    lv_comb8<TX> C;
+    for (int q = 0; q < 8; q++) {  // unused slots (q >= m, never read) hold a valid device pointer
+        C.x[q] = (q < m) ? x[q] : (m > 0 ? x[0] : reinterpret_cast<const TX *>(y));
+        C.c[q] = (q < m) ? c[q] : 0.0;
+    }
+    hipLaunchKernelGGL((k_combine<TX, TY>), dim3(nblocks_of(n)), dim3(B), 0, s, n, y, alpha, C, m, dc, dmode);
+    return hipGetLastError();
+}
+}  // namespace
+
+hipError_t lv_dots_device(int n, const double *const *a, const double *const *b, int m, double *partials,
+                          double *dout, hipStream_t s)
+{
+    return dots_device(n, a, b, m, partials, dout, s);
+}
+hipError_t lv_dots_device(int n, const float *const *a, const double *const *b, int m, double *partials,
+                          double *dout, hipStream_t s)
+{
+    return dots_device(n, a, b, m, partials, dout, s);
 }
 
 hipError_t lv_combine(int n, double *y, double alpha, const double *const *x, const double *c, int m, hipStream_t s,
                       const double *dc, int dmode)
 {
-    lv_comb8 C;
-    for (int q = 0; q < 8; q++) {
-        C.x[q] = (q < m) ? x[q] : y;
-        C.c[q] = (q < m) ? c[q] : 0.0;
-    }
-    hipLaunchKernelGGL(k_combine, dim3(nblocks_of(n)), dim3(B), 0, s, n, y, alpha, C, m, dc, dmode);
-    return hipGetLastError();
+    return combine(n, y, alpha, x, c, m, s, dc, dmode);
+}
+hipError_t lv_combine(int n, double *y, double alpha, const float *const *x, const double *c, int m, hipStream_t s,
+                      const double *dc, int dmode)
+{
+    return combine(n, y, alpha, x, c, m, s, dc, dmode);
+}
+hipError_t lv_combine(int n, float *y, double alpha, const double *const *x, const double *c, int m, hipStream_t s,
+                      const double *dc, int dmode)
+{
+    return combine(n, y, alpha, x, c, m, s, dc, dmode);
 }
 
 hipError_t lv_spmv64(const lv_pressure_system &A, int n, const double *x, double *y, hipStream_t s)

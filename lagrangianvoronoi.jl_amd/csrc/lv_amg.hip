@@ -663,19 +663,20 @@ struct st_level {
 
 // wrap(i, K, per) for i in [-1, K] (a cell index plus an offset of at most one): a compare and an
 // add instead of two remainders
+// (selects, not branches: a branch here splits the stencil kernels' loads into basic blocks that
+// the compiler then waits between)
 __device__ __forceinline__ int wrap_near(int i, int K, int per, bool &ok)
 {
-    if (per) {
-        ok = true;
-        return i < 0 ? i + K : (i >= K ? i - K : i);
-    }
-    ok = (i >= 0 && i < K);
-    return i;
+    const int w = i < 0 ? i + K : (i >= K ? i - K : i);
+    ok = per || (i >= 0 && i < K);
+    return per ? w : i;
 }
 
 // the 8 neighbour cells of (cx, cy) on level L and their couplings; a neighbour outside a
 // non-periodic level gets coupling 0 and index f, so every load below is in bounds and all of them
-// can be issued at once.  With PJ, also each neighbour's parent on the next level (Kyc columns).
+// can be issued at once (the coupling slot is loaded whatever the neighbour, and then selected: a
+// load under the condition would make the gathers wait for it).  With PJ, also each neighbour's
+// parent on the next level (Kyc columns).
 __device__ __forceinline__ void st_nbrs(const st_level &L, int cx, int cy, int f, float ck[8], int J[8],
                                         int *PJ = nullptr, int Kyc = 0)
 {
@@ -688,7 +689,8 @@ __device__ __forceinline__ void st_nbrs(const st_level &L, int cx, int cy, int f
         const int jx = wrap_near(cx + ox, L.Kx, L.xper, okx), jy = wrap_near(cy + oy, L.Ky, L.yper, oky);
         const bool ok = okx && oky;
         J[k] = ok ? jx * L.Ky + jy : f;
-        ck[k] = ok ? L.c[(size_t)k * n + f] : 0.0f;
+        const float c = L.c[(size_t)k * n + f];
+        ck[k] = ok ? c : 0.0f;
         if (PJ) PJ[k] = ok ? (jx / CF) * Kyc + jy / CF : (cx / CF) * Kyc + cy / CF;
     }
 }

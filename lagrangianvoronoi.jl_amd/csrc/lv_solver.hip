@@ -820,15 +820,15 @@ struct lv_comb8 {
     double c[8];
 };
 
-// up to 8 dot products a_q . b_q in one pass
-template <typename TA>
-__global__ __launch_bounds__(B) void k_multidot(int n, lv_vec8<TA> V, int m, double *__restrict__ partials)
+// M <= 8 dot products a_q . b_q in one pass (M a template parameter, so that a row's loads are
+// issued together rather than each behind a branch on the count)
+template <typename TA, int M>
+__global__ __launch_bounds__(B) void k_multidot(int n, lv_vec8<TA> V, double *__restrict__ partials)
 {
     double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (int i = blockIdx.x * B + threadIdx.x; i < n; i += gridDim.x * B) {
 #pragma unroll
-        for (int q = 0; q < 8; q++)
-            if (q < m) v[q] += (double)V.a[q][i] * V.b[q][i];
+        for (int q = 0; q < M; q++) v[q] += (double)V.a[q][i] * V.b[q][i];
     }
     block_partials<8>(v, partials, gridDim.x);
 }
@@ -845,8 +845,9 @@ __global__ __launch_bounds__(B) void k_multidot_fin(const double *__restrict__ p
 // coefficients are C.c scaled by device values dc (the output of k_multidot_fin): dmode 0 none,
 // 1 c_q dc[q], 2 c_q / sqrt(dc[0]) (0 when dc[0] is not positive and finite).  Accumulated in FP64
 // and rounded once to y's type.
-template <typename TX, typename TY>
-__global__ __launch_bounds__(B) void k_combine(int n, TY *__restrict__ y, double alpha, lv_comb8<TX> C, int m,
+// (M, the number of x's, a template parameter as in k_multidot)
+template <typename TX, typename TY, int M>
+__global__ __launch_bounds__(B) void k_combine(int n, TY *__restrict__ y, double alpha, lv_comb8<TX> C,
                                                const double *__restrict__ dc, int dmode)
 {
     double c[8];
@@ -854,8 +855,7 @@ __global__ __launch_bounds__(B) void k_combine(int n, TY *__restrict__ y, double
     for (int q = 0; q < 8; q++) c[q] = C.c[q];
     if (dmode == 1) {
 #pragma unroll
-        for (int q = 0; q < 8; q++)
-            if (q < m) c[q] = C.c[q] * dc[q];
+        for (int q = 0; q < M; q++) c[q] = C.c[q] * dc[q];
     } else if (dmode == 2) {
         const double h = dc[0];
         const double f = (h > 0.0 && h <= 1.7976931348623157e308) ? 1.0 / sqrt(h) : 0.0;
@@ -865,8 +865,7 @@ __global__ __launch_bounds__(B) void k_combine(int n, TY *__restrict__ y, double
     for (int i = blockIdx.x * B + threadIdx.x; i < n; i += gridDim.x * B) {
         double acc = (alpha != 0.0) ? alpha * (double)y[i] : 0.0;  // alpha = 0 never reads y (may be uninitialised)
 #pragma unroll
-        for (int q = 0; q < 8; q++)
-            if (q < m) acc += c[q] * (double)C.x[q][i];
+        for (int q = 0; q < M; q++) acc += c[q] * (double)C.x[q][i];
         y[i] = (TY)acc;
     }
 }
@@ -1330,7 +1329,16 @@ hipError_t dots_device(int n, const TA *const *a, const double *const *b, int m,
         V.b[q] = (q < m) ? b[q] : b[0];
     }
     const int nb = nblocks_of(n);
-    hipLaunchKernelGGL(k_multidot<TA>, dim3(nb), dim3(B), 0, s, n, V, m, partials);
+    switch (m) {
+#define LV_MD(M)                                                                                         \
+    case M:                                                                                              \
+        hipLaunchKernelGGL((k_multidot<TA, M>), dim3(nb), dim3(B), 0, s, n, V, partials);             \
+        break;
+        LV_MD(1) LV_MD(2) LV_MD(3) LV_MD(4) LV_MD(5) LV_MD(6) LV_MD(7) LV_MD(8)
+#undef LV_MD
+    default:
+        hipLaunchKernelGGL((k_multidot<TA, 0>), dim3(nb), dim3(B), 0, s, n, V, partials);
+    }
     hipLaunchKernelGGL(k_multidot_fin, dim3(1), dim3(B), 0, s, (const double *)partials, nb, dout);
     return hipGetLastError();
 }
@@ -1344,7 +1352,17 @@ hipError_t combine(int n, TY *y, double alpha, const TX *const *x, const double 
         C.x[q] = (q < m) ? x[q] : (m > 0 ? x[0] : reinterpret_cast<const TX *>(y));
         C.c[q] = (q < m) ? c[q] : 0.0;
     }
-    hipLaunchKernelGGL((k_combine<TX, TY>), dim3(nblocks_of(n)), dim3(B), 0, s, n, y, alpha, C, m, dc, dmode);
+    const dim3 G(nblocks_of(n));
+    switch (m) {
+#define LV_CB(M)                                                                                         \
+    case M:                                                                                              \
+        hipLaunchKernelGGL((k_combine<TX, TY, M>), G, dim3(B), 0, s, n, y, alpha, C, dc, dmode);      \
+        break;
+        LV_CB(1) LV_CB(2) LV_CB(3) LV_CB(4) LV_CB(5) LV_CB(6) LV_CB(7) LV_CB(8)
+#undef LV_CB
+    default:
+        hipLaunchKernelGGL((k_combine<TX, TY, 0>), G, dim3(B), 0, s, n, y, alpha, C, dc, dmode);
+    }
     return hipGetLastError();
 }
 }  // namespace

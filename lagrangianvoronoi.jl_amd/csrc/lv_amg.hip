@@ -20,7 +20,9 @@
 //            ~40 with 2 x 2);
 //   the coarsest level (<= LV_AMG_DENSE_MAX cells) is inverted densely once per assembly.
 // The levels of at most TAIL_MAX cells run inside one single-workgroup kernel per cycle (k_tail):
-// they are too small to fill the device, and one launch replaces two per level.
+// they are too small to fill the device, and one launch replaces two per level.  Above them, levels
+// 2 and up run in pairs, each pair as one down and one up kernel (k_st_down2, k_st_up2) that
+// recompute a halo around each workgroup's tile instead of waiting for a second launch.
 // Every sum is a gather in a fixed order (per-aggregate row lists are sorted), so the hierarchy
 // and the preconditioner are deterministic.
 //

@@ -794,7 +794,9 @@ __global__ __launch_bounds__(B) void k_st_down(st_level L, int Kyc, int nc, cons
                                                float *__restrict__ z, float *__restrict__ rc, const lv_cg_scalars *S)
 {
     if (S && S->done) return;
-    const int tid = blockIdx.x * B + threadIdx.x;
+    // (workgroups in the XCD-aware order: the parent rows above and below a workgroup's, whose
+    // children it gathers, are mostly on its own XCD)
+    const int tid = lv_xcd_block(blockIdx.x, gridDim.x) * B + threadIdx.x;
     const int P = tid / CF2, q = tid % CF2;
     const float t = sum_children(st_down_lane<ZGIVEN>(L, Kyc, nc, P, q, r, z));
     if (q == 0 && P < nc) rc[P] = t;
@@ -810,7 +812,7 @@ __global__ __launch_bounds__(B) void k_st_up(st_level L, int Kyc, const float *_
                                              const int *__restrict__ start, float alpha0, float *__restrict__ z0)
 {
     if (S && S->done) return;
-    const int f = blockIdx.x * B + threadIdx.x;
+    const int f = lv_xcd_block(blockIdx.x, gridDim.x) * B + threadIdx.x;  // (as k_st_down)
     if (f >= L.Kx * L.Ky) return;
     const float v = st_up_cell(L, Kyc, f, r, z, ec, out);
     if (PROLONG) {

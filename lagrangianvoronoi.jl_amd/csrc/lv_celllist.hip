@@ -20,31 +20,54 @@ constexpr int RT = RB * RR;             // items per tile
 constexpr int NW = RB / 64;
 
 // In-grid images of seed x in insert_periodic! order; returns how many, writes their cell ids.
+// the keys of a seed and of its periodic images, in the order insert_periodic! visits them.  An
+// image's coordinate is x1, x1 + P or x1 - P (and x2, x2 + Q or x2 - Q; the reference's
+// zero-weighted terms add 0.0, which changes no floor), so the exact divisions of findkey are made
+// once per distinct coordinate: 6 instead of 18
+__device__ __forceinline__ bool lv_key_axis(double x, double origin, double h, int nmax, int &i)
+{
+    const double f = floor((x - origin) / h);
+    if (!(f == f) || f < -2.0e9 || f > 2.0e9) return false;
+    i = (int)f + 1;
+    return i >= 1 && i <= nmax;
+}
+
 __device__ __forceinline__ int seed_images(const lv_grid_params &g, double x1, double x2, int *cells)
 {
-    double px[9], py[9];
-    int k = 0;
     const double P = g.xperiod, Q = g.yperiod;
-    px[k] = x1; py[k] = x2; k++;
+    int ix[3], iy[3];
+    bool ox[3], oy[3];
+    ox[0] = lv_key_axis(x1, g.origin0, g.h, g.n1, ix[0]);
+    oy[0] = lv_key_axis(x2, g.origin1, g.h, g.n2, iy[0]);
+    ox[1] = ox[2] = oy[1] = oy[2] = false;
     if (g.xperiodic) {
-        px[k] = x1 + P * 1.0; py[k] = x2 + P * 0.0; k++;
-        px[k] = x1 - P * 1.0; py[k] = x2 - P * 0.0; k++;
+        ox[1] = lv_key_axis(x1 + P * 1.0, g.origin0, g.h, g.n1, ix[1]);
+        ox[2] = lv_key_axis(x1 - P * 1.0, g.origin0, g.h, g.n1, ix[2]);
     }
     if (g.yperiodic) {
-        px[k] = x1 + Q * 0.0; py[k] = x2 + Q * 1.0; k++;
-        px[k] = x1 - Q * 0.0; py[k] = x2 - Q * 1.0; k++;
+        oy[1] = lv_key_axis(x2 + Q * 1.0, g.origin1, g.h, g.n2, iy[1]);
+        oy[2] = lv_key_axis(x2 - Q * 1.0, g.origin1, g.h, g.n2, iy[2]);
+    }
+    // (a, b): the image's x and y variants, in the order of the reference's image list
+    int pa[9], pb[9], k = 0;
+    pa[k] = 0; pb[k] = 0; k++;
+    if (g.xperiodic) {
+        pa[k] = 1; pb[k] = 0; k++;
+        pa[k] = 2; pb[k] = 0; k++;
+    }
+    if (g.yperiodic) {
+        pa[k] = 0; pb[k] = 1; k++;
+        pa[k] = 0; pb[k] = 2; k++;
     }
     if (g.xperiodic && g.yperiodic) {
-        px[k] = (x1 + P * 1.0) + Q * 0.0; py[k] = (x2 + P * 0.0) + Q * 1.0; k++;
-        px[k] = (x1 + P * 1.0) - Q * 0.0; py[k] = (x2 + P * 0.0) - Q * 1.0; k++;
-        px[k] = (x1 - P * 1.0) + Q * 0.0; py[k] = (x2 - P * 0.0) + Q * 1.0; k++;
-        px[k] = (x1 - P * 1.0) - Q * 0.0; py[k] = (x2 - P * 0.0) - Q * 1.0; k++;
+        pa[k] = 1; pb[k] = 1; k++;
+        pa[k] = 1; pb[k] = 2; k++;
+        pa[k] = 2; pb[k] = 1; k++;
+        pa[k] = 2; pb[k] = 2; k++;
     }
     int m = 0;
-    for (int j = 0; j < k; j++) {
-        int i1, i2;
-        if (lv_findkey(g, px[j], py[j], i1, i2)) cells[m++] = (i1 - 1) + (i2 - 1) * g.n1;
-    }
+    for (int j = 0; j < k; j++)
+        if (ox[pa[j]] && oy[pb[j]]) cells[m++] = (ix[pa[j]] - 1) + (iy[pb[j]] - 1) * g.n1;
     return m;
 }
 

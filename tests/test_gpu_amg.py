@@ -140,12 +140,15 @@ def test_compact_overflow_is_bit_identical(lv, oracle, gpu, kind, ic, K, dt):
         assert list(out[0][1]) == list(out[layout][1]) and list(out[0][2]) == list(out[layout][2])
 
 
-@pytest.mark.parametrize("kind,ic,K,dt", [("periodic", "tg", 512, 2e-3), ("walls", "cavity", 300, 1e-3)])
+@pytest.mark.parametrize("kind,ic,K,dt", [("periodic", "tg", 512, 2e-3), ("walls", "cavity", 300, 1e-3),
+                                          ("tall", "cavity", 200, 1e-3)])
 def test_paired_coarse_levels_are_bit_identical(lv, oracle, gpu, kind, ic, K, dt):
     """Pairs of coarse levels above the single-workgroup tail run as one down and one up kernel each
     (amg_level_kernels 0) or every level as its own kernels (1): the same cycle output, pressure and
     iterations bit for bit.  At 512^2 periodic levels 2 and 3 (128^2 and 64^2 cells) are paired, at
-    300^2 with walls levels 2 and 3 (75^2 and 38^2 cells: odd widths, partial tiles at the walls)."""
+    300^2 with walls levels 2 and 3 (75^2 and 38^2 cells: odd widths, partial tiles at the walls), and
+    on the walled 1 x 2 domain at 200 x 400 levels 2 and 3 (50 x 100 and 25 x 50 cells: unequal
+    axes)."""
     _flush("amg paired coarse levels %s %d" % (kind, K))
     grid, op, solver, x, dr, d, rp, cols, coef = _system(lv, oracle, kind, K, ic, dt)
     b, _ = solver.rhs(dt, False)

@@ -92,7 +92,8 @@ typedef struct {
                               the slots past the sixth of each 64-row group read from a compact copy;
                               1 keeps the operator's own row order (int32 columns, FP32 couplings);
                               2 is 0 without the compact copies (also the one of the operator's own
-                              ELL that the FP64 kernels read), 3 uses the aggregate order's only for
+                              ELL that the FP64 kernels read, and find_pressure!'s of the right-hand
+                              side's edge geometry), 3 uses the aggregate order's only for
                               groups of at most 16 such slots (the same results bit for bit; for
                               measurement and tests).  Systems below 65536 rows use no compact copy. */
     int32_t picard_carry; /* with picard_window > 0: 1 also carries the first two Picard corrections

@@ -482,7 +482,10 @@ int lv_destroy(lv_ctx *ctx)
         dfree(ctx->pc_q[k]);
         dfree(ctx->pc_o[k]);
     }
-    for (double *p : {ctx->rg.w1, ctx->rg.w2, ctx->rg.u1, ctx->rg.u2, ctx->rg.bdiv, ctx->rg.den}) dfree(p);
+    for (double *p : {ctx->rg.w1, ctx->rg.w2, ctx->rg.u1, ctx->rg.u2, ctx->rg.bdiv, ctx->rg.den, ctx->rg.ow1,
+                      ctx->rg.ow2, ctx->rg.ou1, ctx->rg.ou2})
+        dfree(p);
+    dfree(ctx->rg.oel);
     for (int k = 0; k < LV_NKTIMERS; k++) {
         if (!ctx->timers[k]) continue;
         for (int q = 0; q < 2 * LV_TPOOL; q++)
@@ -1126,6 +1129,10 @@ static int ensure_rhs_geom(lv_ctx *ctx)
     LV_HIP_CHECK(dalloc(&ctx->rg.u2, (size_t)LV_MAXE * cap));
     LV_HIP_CHECK(dalloc(&ctx->rg.bdiv, cap));
     LV_HIP_CHECK(dalloc(&ctx->rg.den, cap));
+    const size_t mo = (cap / 64 + 1) * LV_OVF_CAP;
+    LV_HIP_CHECK(dalloc(&ctx->rg.oel, mo));
+    for (double **p : {&ctx->rg.ow1, &ctx->rg.ow2, &ctx->rg.ou1, &ctx->rg.ou2}) LV_HIP_CHECK(dalloc(p, mo));
+    ctx->rg.ocap = 0;
     ctx->rg_alloc = true;
     return LV_OK;
 }
@@ -1139,6 +1146,7 @@ int lv_pressure_rhs(lv_ctx *ctx, double dt, int32_t gp_step, const double vbc[8]
     // the same split evaluation as find_pressure!: the P-independent parts, GP, then b
     rc = ensure_rhs_geom(ctx);
     if (rc) return rc;
+    ctx->rg.ocap = ctx->n >= 65536 ? LV_OVF_CAP : 0;
     const int n = ctx->n;
     const lv_mesh_view M = mesh_view(ctx);
     LV_HIP_CHECK(lv_launch_rhs_geom(ctx->g, ctx->F, M, n, dt, v, ctx->rg, ctx->s));
@@ -1454,6 +1462,9 @@ int lv_find_pressure(lv_ctx *ctx, double dt, int32_t niter, const lv_solve_opts 
     for (int k = 0; k < 8; k++) v.v[k] = o.vbc[k];
     rc = ensure_rhs_geom(ctx);
     if (rc) return rc;
+    // the compact copy of the right-hand side's edge slots past the sixth, as the operator's (not below
+    // 65536 rows; amg_layout 2 turns it off with the operator's)
+    ctx->rg.ocap = (ctx->n >= 65536 && o.amg_layout != 2) ? LV_OVF_CAP : 0;
     rc = assemble_system(ctx, dt, 0, nullptr, &v);  // refresh!(A), and the fixed parts of refresh!(solver)
     if (rc) return rc;
     const int w = std::min(std::max(o.picard_window, 0), 8);

@@ -92,6 +92,12 @@ hipError_t lv_launch_assemble(const lv_grid_params &g, lv_fields F, lv_mesh_view
 // per row the divergence and boundary-flux part of b (bdiv) and rho c2 dt^2 (den).
 struct lv_rhs_geom {
     double *w1, *w2, *u1, *u2, *bdiv, *den;
+    // the mesh slots LV_OVF_BASE.. of each 64-row group in a compact copy (lv_ell.h), labels and the
+    // four planes, for the groups with at most ocap of them (ocap 0: no copy; the writer and the
+    // readers of one set of planes see the same ocap)
+    int *oel;
+    double *ow1, *ow2, *ou1, *ou2;
+    int ocap;
 };
 hipError_t lv_launch_rhs_geom(const lv_grid_params &g, lv_fields F, lv_mesh_view M, int n, double dt, lv_vbc vbc,
                               const lv_rhs_geom &R, hipStream_t s);

@@ -67,6 +67,25 @@ __device__ __forceinline__ double rhs_bdary(const lv_vbc &vbc, int lab, double e
     return (dS1 * (vb1 - pv1) + dS2 * (vb2 - pv2)) / dt;
 }
 
+// the right-hand side's edge slots LV_OVF_BASE.. of row i (just written to the planes by this
+// thread; a boundary slot's values are whatever the planes hold, and unused) into the compact copy
+// of its 64-row group, when the group has at most R.ocap of them.  Every active lane of the
+// wavefront takes part in the ballots.
+__device__ __forceinline__ void rhs_geom_compact(const lv_mesh_view &M, const lv_rhs_geom &R, int i, int n, int ne)
+{
+    const lv_ovf_pos q = lv_ovf_prefix(ne);
+    if (q.tot > R.ocap) return;
+    const size_t g0 = (size_t)(i >> 6) * LV_OVF_CAP;
+    for (int k = LV_OVF_BASE; k < ne; k++) {
+        const size_t e = (size_t)k * n + i, eo = g0 + (size_t)(q.pos + (k - LV_OVF_BASE));
+        R.oel[eo] = LV_EL(M, k, i);
+        R.ow1[eo] = R.w1[e];
+        R.ow2[eo] = R.w2[e];
+        R.ou1[eo] = R.u1[e];
+        R.ou2[eo] = R.u2[e];
+    }
+}
+
 // refresh!(A) (src/pressure.jl:104-117): the CSR copy in edge order, the ELL copy with each row's
 // columns sorted, diagonals and Jacobi inverses.  With R.w1 (mode 0, find_pressure!) the same edge
 // loop also stores the P-independent parts of refresh!(solver) (vbc: boundary_velocity per label).
@@ -147,6 +166,7 @@ __global__ __launch_bounds__(B) void k_assemble(lv_grid_params g, lv_fields F, l
     if (geom) {
         R.bdiv[i] = bd;
         R.den[i] = (rho * F.c2[i]) * (dt * dt);
+        if (R.ocap > 0) rhs_geom_compact(M, R, i, n, ne);
     }
     // the system is singular iff every d_i == 0 (c2 = Inf): one flag write per wavefront at most
     if (__any(d != 0.0) && (threadIdx.x & 63) == 0 && *nonsingular == 0) atomicOr(nonsingular, 1);
@@ -194,27 +214,55 @@ __global__ __launch_bounds__(B) void k_rhs_geom(lv_grid_params g, lv_fields F, l
     }
     R.bdiv[i] = bd;
     R.den[i] = (F.rho[i] * F.c2[i]) * (dt * dt);
+    if (R.ocap > 0) rhs_geom_compact(M, R, i, n, ne);
 }
 
-// the first U mesh slots of row i: neighbour (the row itself on a boundary slot or past ne) and the
-// two weight planes (0 there), loaded unconditionally (the planes hold LV_MAXE slots per row, so
-// every load is in bounds) so that the gathers that follow are issued together
-template <int U>
-__device__ __forceinline__ int rhs_head(const lv_mesh_view &M, const double *__restrict__ a, const double *__restrict__ b,
-                                        int i, int n, int (&c)[U], double (&va)[U], double (&vb)[U])
+// The edge slots of row i for the Picard right-hand side: neighbour (the row itself on a boundary
+// slot or past ne) and two weight planes (0 there).  Slots 0..LV_OVF_BASE-1 come from the planes,
+// the later ones from the compact copy of the row's 64-row group when it holds them (q: the group's
+// ballot prefix), else from the planes.  The first U slots are loaded unconditionally (both copies
+// hold every slot they may read, the compact index clamped into the group's region), the pointers
+// chosen by selects, so that the loads and the gathers that follow are issued together.
+struct rhs_slots {
+    const int *el;
+    const double *a, *b;
+    size_t g0;
+    int pos, n, i;
+    bool compact;
+    __device__ size_t index(int k) const
+    {
+        if (k < LV_OVF_BASE || !compact) return (size_t)k * n + i;
+        const int o = pos + (k - LV_OVF_BASE);
+        return g0 + (size_t)(o < LV_OVF_CAP ? o : LV_OVF_CAP - 1);
+    }
+};
+__device__ __forceinline__ rhs_slots rhs_slots_of(const lv_mesh_view &M, const double *a, const double *b,
+                                                  const int *oel, const double *oa, const double *ob, int ocap,
+                                                  lv_ovf_pos q, int i, int n)
 {
-    const int ne = M.ne[i];
+    const bool compact = ocap > 0 && q.tot <= ocap;
+    return rhs_slots{M.el, a, b, (size_t)(i >> 6) * LV_OVF_CAP, q.pos, n, i, compact};
+}
+template <int U>
+__device__ __forceinline__ void rhs_head(const rhs_slots &S, const int *oel, const double *oa, const double *ob,
+                                         int ne, int (&c)[U], double (&va)[U], double (&vb)[U])
+{
+    int j[U];
 #pragma unroll
     for (int k = 0; k < U; k++) {
-        const size_t e = (size_t)k * n + i;
-        const int j = M.el[e];
-        const double ta = a[e], tb = b[e];
-        const bool on = k < ne && j >= 0;
-        c[k] = on ? j : i;
-        va[k] = on ? ta : 0.0;
-        vb[k] = on ? tb : 0.0;
+        const bool ov = k >= LV_OVF_BASE && S.compact;
+        const size_t e = S.index(k);
+        j[k] = (ov ? oel : S.el)[e];
+        va[k] = (ov ? oa : S.a)[e];
+        vb[k] = (ov ? ob : S.b)[e];
     }
-    return ne;
+#pragma unroll
+    for (int k = 0; k < U; k++) {
+        const bool on = (k < ne) & (j[k] >= 0);  // (not &&: a branch here would sink the loads above into it)
+        c[k] = on ? j[k] : S.i;
+        va[k] = on ? va[k] : 0.0;
+        vb[k] = on ? vb[k] : 0.0;
+    }
 }
 
 // GP from P; with Pout also the write-back of P into the particles (find_pressure!'s p.P = P), with
@@ -224,11 +272,14 @@ __global__ __launch_bounds__(B) void k_gp(lv_mesh_view M, int n, lv_rhs_geom R, 
                                           double *__restrict__ Pout, double *__restrict__ Pout2)
 {
     const int i = blockIdx.x * B + threadIdx.x;
+    const int ne = i < n ? M.ne[i] : 0;
+    const lv_ovf_pos q = R.ocap > 0 ? lv_ovf_prefix(ne) : lv_ovf_pos{0, 0};  // the whole wavefront, before any lane leaves
     if (i >= n) return;
     constexpr int U = 8;
     int c[U];
     double a1[U], a2[U], pj[U];
-    const int ne = rhs_head<U>(M, R.w1, R.w2, i, n, c, a1, a2);
+    const rhs_slots S = rhs_slots_of(M, R.w1, R.w2, R.oel, R.ow1, R.ow2, R.ocap, q, i, n);
+    rhs_head<U>(S, R.oel, R.ow1, R.ow2, ne, c, a1, a2);
     const double Pi = P[i];
 #pragma unroll
     for (int k = 0; k < U; k++) pj[k] = P[c[k]];
@@ -240,12 +291,12 @@ __global__ __launch_bounds__(B) void k_gp(lv_mesh_view M, int n, lv_rhs_geom R, 
         gy -= dp * a2[k];
     }
     for (int k = U; k < ne; k++) {
-        const size_t e = (size_t)k * n + i;
-        const int j = M.el[e];
+        const size_t e = S.index(k);
+        const int j = (S.compact ? R.oel : M.el)[e];
         if (j < 0) continue;
         const double dp = Pi - P[j];
-        gx -= dp * R.w1[e];
-        gy -= dp * R.w2[e];
+        gx -= dp * (S.compact ? R.ow1 : R.w1)[e];
+        gy -= dp * (S.compact ? R.ow2 : R.w2)[e];
     }
     const double mi = mass[i];
     GP[i] = gx / mi;
@@ -261,6 +312,8 @@ __global__ __launch_bounds__(B) void k_rhs_b(lv_mesh_view M, int n, int gp, lv_r
                                              double *__restrict__ P0)
 {
     const int i = blockIdx.x * B + threadIdx.x;
+    const int ne = i < n ? M.ne[i] : 0;
+    const lv_ovf_pos q = (gp && R.ocap > 0) ? lv_ovf_prefix(ne) : lv_ovf_pos{0, 0};  // before any lane leaves
     if (i >= n) return;
     const double Pi = P[i];
     double bi = (area[i] * Pi) / R.den[i] + R.bdiv[i];
@@ -268,7 +321,8 @@ __global__ __launch_bounds__(B) void k_rhs_b(lv_mesh_view M, int n, int gp, lv_r
         constexpr int U = 8;
         int c[U];
         double a1[U], a2[U], g1[U], g2[U];
-        const int ne = rhs_head<U>(M, R.u1, R.u2, i, n, c, a1, a2);
+        const rhs_slots S = rhs_slots_of(M, R.u1, R.u2, R.oel, R.ou1, R.ou2, R.ocap, q, i, n);
+        rhs_head<U>(S, R.oel, R.ou1, R.ou2, ne, c, a1, a2);
 #pragma unroll
         for (int k = 0; k < U; k++) {
             g1[k] = GP[c[k]];
@@ -279,10 +333,10 @@ __global__ __launch_bounds__(B) void k_rhs_b(lv_mesh_view M, int n, int gp, lv_r
 #pragma unroll
         for (int k = 0; k < U; k++) corr += (gi1 - g1[k]) * a1[k] + (gi2 - g2[k]) * a2[k];
         for (int k = U; k < ne; k++) {
-            const size_t e = (size_t)k * n + i;
-            const int j = M.el[e];
+            const size_t e = S.index(k);
+            const int j = (S.compact ? R.oel : M.el)[e];
             if (j < 0) continue;
-            corr += (gi1 - GP[j]) * R.u1[e] + (gi2 - GP[n + j]) * R.u2[e];
+            corr += (gi1 - GP[j]) * (S.compact ? R.ou1 : R.u1)[e] + (gi2 - GP[n + j]) * (S.compact ? R.ou2 : R.u2)[e];
         }
         bi += corr;
     }

@@ -170,6 +170,25 @@ def test_paired_coarse_levels_are_bit_identical(lv, oracle, gpu, kind, ic, K, dt
     assert out[0][2] == out[1][2] and out[0][3] == out[1][3]
 
 
+def test_find_pressure_compact_copies_are_bit_identical(lv, oracle, gpu):
+    """find_pressure! (10 Picard iterations with the window) with the compact copies of the slots past
+    the sixth, the operator's and the right-hand side's edge geometry (amg_layout 0), and without them
+    (2): the same pressure bit for bit at 256^2 (65536 rows, where the copies are used)."""
+    _flush("find_pressure compact copies 256")
+    grid, op, x, dr = make_pair(lv, oracle, "periodic", K=256, seed=0x5EED0507, ic="tg")
+    lv.stiffened_eos(grid, 1.4, 2500.0 / 1.4)
+    P0 = grid.get("P")
+    out = {}
+    for layout in (0, 2):
+        grid.set("P", P0)
+        solver = lv.PressureSolver(grid, method="amg", tol=1e-10, itmax=4000, picard_window=4, amg_layout=layout)
+        st = lv.find_pressure(solver, 2e-3, 10)
+        assert max(st["relres"][:st["nsolves"]]) <= 1e-10, st
+        out[layout] = (grid.get("P"), list(st["iters"][:st["nsolves"]]))
+    assert np.array_equal(out[0][0], out[2][0])
+    assert out[0][1] == out[2][1]
+
+
 @pytest.mark.parametrize("method", ["pcg64", "mixed"])
 def test_own_order_compact_overflow_is_bit_identical(lv, oracle, gpu, method):
     """The operator's own ELL read with the compact copy of the slots past the sixth (systems of at

@@ -116,6 +116,33 @@ __global__ __launch_bounds__(B) void k_eos(lv_fields F, int n, int kind, double 
     }
 }
 
+// Edge slots of a row in batches of EB: the batch's labels and edge vertices are loaded together
+// (a slot past the row's last reads the row's last slot again, a cache hit), then the
+// neighbours' fields together (a boundary slot or one past the row gathers the row itself), and
+// only then is the batch applied, edge by edge in the reference's order.  A loop that loads each
+// edge's label, then its vertices and its neighbour's fields, waits for three or four round trips
+// per edge.
+constexpr int EB = 4;
+struct edge_batch {
+    int j[EB];
+    bool on[EB];
+    double e1x[EB], e1y[EB], e2x[EB], e2y[EB];
+};
+__device__ __forceinline__ void load_edges(const lv_mesh_view &M, int i, int k0, int ne, edge_batch &E)
+{
+#pragma unroll
+    for (int u = 0; u < EB; u++) {
+        const int k = k0 + u < ne ? k0 + u : ne - 1;
+        const int j = LV_EL(M, k, i);
+        E.e1x[u] = LV_EV(M, 0, k, i);
+        E.e1y[u] = LV_EV(M, 1, k, i);
+        E.e2x[u] = LV_EV(M, 2, k, i);
+        E.e2y[u] = LV_EV(M, 3, k, i);
+        E.on[u] = (k0 + u < ne) & (j >= 0);
+        E.j[u] = E.on[u] ? j : i;
+    }
+}
+
 // pressure_step! pass 1, src/pressure.jl:11-17: v += dt/m * lrr * (P_i - P_j) * (m - x)
 __global__ __launch_bounds__(B) void k_pstep1(lv_grid_params g, lv_fields F, lv_mesh_view M, int n, double dt)
 {
@@ -125,17 +152,28 @@ __global__ __launch_bounds__(B) void k_pstep1(lv_grid_params g, lv_fields F, lv_
     double v1 = F.v[i], v2 = F.v[n + i];
     const double Pi = F.P[i], dtm = dt / F.mass[i];
     const int ne = M.ne[i];
-    for (int k = 0; k < ne; k++) {
-        const int j = LV_EL(M, k, i);
-        if (j < 0) continue;
-        const double e1x = LV_EV(M, 0, k, i), e1y = LV_EV(M, 1, k, i), e2x = LV_EV(M, 2, k, i), e2y = LV_EV(M, 3, k, i);
-        double y1, y2;
-        nbr_y(g, F, n, j, x1, x2, y1, y2);
-        const double lrr = lv_lr_ratio(x1 - y1, x2 - y2, e1x, e1y, e2x, e2y);
-        const double m1 = 0.5 * (e1x + e2x), m2 = 0.5 * (e1y + e2y);
-        const double w = (dtm * lrr) * (Pi - F.P[j]);
-        v1 = v1 + w * (m1 - x1);
-        v2 = v2 + w * (m2 - x2);
+    for (int k0 = 0; k0 < ne; k0 += EB) {
+        edge_batch E;
+        load_edges(M, i, k0, ne, E);
+        double xj1[EB], xj2[EB], Pj[EB];
+#pragma unroll
+        for (int u = 0; u < EB; u++) {
+            const int j = E.j[u];
+            xj1[u] = F.x[j];
+            xj2[u] = F.x[n + j];
+            Pj[u] = F.P[j];
+        }
+#pragma unroll
+        for (int u = 0; u < EB; u++) {
+            if (!E.on[u]) continue;
+            double y1, y2;
+            lv_arrow_target(g, xj1[u], xj2[u], x1, x2, y1, y2);
+            const double lrr = lv_lr_ratio(x1 - y1, x2 - y2, E.e1x[u], E.e1y[u], E.e2x[u], E.e2y[u]);
+            const double m1 = 0.5 * (E.e1x[u] + E.e2x[u]), m2 = 0.5 * (E.e1y[u] + E.e2y[u]);
+            const double w = (dtm * lrr) * (Pi - Pj[u]);
+            v1 = v1 + w * (m1 - x1);
+            v2 = v2 + w * (m2 - x2);
+        }
     }
     F.v[i] = v1;
     F.v[n + i] = v2;
@@ -156,27 +194,40 @@ __global__ __launch_bounds__(B) void k_pstep2(lv_grid_params g, lv_fields F, lv_
     double ei = F.e[i];
     double D11 = 0.0, D21 = 0.0, D12 = 0.0, D22 = 0.0;
     const int ne = M.ne[i];
-    for (int k = 0; k < ne; k++) {
-        const int j = LV_EL(M, k, i);
-        if (j < 0) continue;
-        const double e1x = LV_EV(M, 0, k, i), e1y = LV_EV(M, 1, k, i), e2x = LV_EV(M, 2, k, i), e2y = LV_EV(M, 3, k, i);
-        double y1, y2;
-        nbr_y(g, F, n, j, x1, x2, y1, y2);
-        const double lrr = lv_lr_ratio(x1 - y1, x2 - y2, e1x, e1y, e2x, e2y);
-        const double m1 = 0.5 * (e1x + e2x), m2 = 0.5 * (e1y + e2y);
-        const double Pj = F.P[j];
-        const double vj1 = F.v[j], vj2 = F.v[n + j];
-        const double qv1 = Pj * vj1, qv2 = Pj * vj2;
-        const double a = (m1 - x1) * pv1 + (m2 - x2) * pv2;
-        const double b = (m1 - y1) * qv1 + (m2 - y2) * qv2;
-        ei = ei - ((dt * lrr) / mi) * (a - b);
-        if (WITH_D) {
-            const double a1 = vi1 - vj1, a2 = vi2 - vj2;
-            const double b1 = m1 - y1, b2 = m2 - y2;
-            D11 = D11 + lrr * (a1 * b1);
-            D21 = D21 + lrr * (a2 * b1);
-            D12 = D12 + lrr * (a1 * b2);
-            D22 = D22 + lrr * (a2 * b2);
+    for (int k0 = 0; k0 < ne; k0 += EB) {
+        edge_batch E;
+        load_edges(M, i, k0, ne, E);
+        double xj1[EB], xj2[EB], Pq[EB], vq1[EB], vq2[EB];
+#pragma unroll
+        for (int u = 0; u < EB; u++) {
+            const int j = E.j[u];
+            xj1[u] = F.x[j];
+            xj2[u] = F.x[n + j];
+            Pq[u] = F.P[j];
+            vq1[u] = F.v[j];
+            vq2[u] = F.v[n + j];
+        }
+#pragma unroll
+        for (int u = 0; u < EB; u++) {
+            if (!E.on[u]) continue;
+            double y1, y2;
+            lv_arrow_target(g, xj1[u], xj2[u], x1, x2, y1, y2);
+            const double lrr = lv_lr_ratio(x1 - y1, x2 - y2, E.e1x[u], E.e1y[u], E.e2x[u], E.e2y[u]);
+            const double m1 = 0.5 * (E.e1x[u] + E.e2x[u]), m2 = 0.5 * (E.e1y[u] + E.e2y[u]);
+            const double Pj = Pq[u];
+            const double vj1 = vq1[u], vj2 = vq2[u];
+            const double qv1 = Pj * vj1, qv2 = Pj * vj2;
+            const double a = (m1 - x1) * pv1 + (m2 - x2) * pv2;
+            const double b = (m1 - y1) * qv1 + (m2 - y2) * qv2;
+            ei = ei - ((dt * lrr) / mi) * (a - b);
+            if (WITH_D) {
+                const double a1 = vi1 - vj1, a2 = vi2 - vj2;
+                const double b1 = m1 - y1, b2 = m2 - y2;
+                D11 = D11 + lrr * (a1 * b1);
+                D21 = D21 + lrr * (a2 * b1);
+                D12 = D12 + lrr * (a1 * b2);
+                D22 = D22 + lrr * (a2 * b2);
+            }
         }
     }
     F.e[i] = ei;
@@ -228,13 +279,13 @@ __global__ __launch_bounds__(B) void k_find_D(lv_grid_params g, lv_fields F, lv_
     F.D[3 * n + i] = 0.5 * (D22 + D22);
 }
 
-// getS, src/diffusion.jl:22-29 (column-major S)
-__device__ __forceinline__ void getS(const lv_fields &F, int n, int i, double dr, double S[4])
+// getS, src/diffusion.jl:22-29 (column-major S), from a cell's D, mu and rho
+__device__ __forceinline__ void getS(double D11, double D21, double D12, double D22, double mu0, double rho,
+                                        double dr, double S[4])
 {
-    const double D11 = F.D[i], D21 = F.D[n + i], D12 = F.D[2 * n + i], D22 = F.D[3 * n + i];
     const double divv = ((D11 * 1.0 + D21 * 0.0) + D12 * 0.0) + D22 * 1.0;
-    double mu = F.mu[i];
-    if (divv < 0.0) mu -= (divv * F.rho[i]) * (dr * dr);
+    double mu = mu0;
+    if (divv < 0.0) mu -= (divv * rho) * (dr * dr);
     const double tm = 2.0 * mu;
     S[0] = tm * (D11 - (divv * 1.0) / 3.0);
     S[1] = tm * (D21 - (divv * 0.0) / 3.0);
@@ -252,23 +303,39 @@ __global__ __launch_bounds__(B) void k_visc1(lv_grid_params g, lv_fields F, lv_m
     const double x1 = F.x[i], x2 = F.x[n + i];
     double v1 = F.v[i], v2 = F.v[n + i];
     double Sp[4];
-    getS(F, n, i, avdr, Sp);
+    getS(F.D[i], F.D[n + i], F.D[2 * n + i], F.D[3 * n + i], F.mu[i], F.rho[i], avdr, Sp);
     const double mi = F.mass[i];
     const int ne = M.ne[i];
-    for (int k = 0; k < ne; k++) {
-        const int j = LV_EL(M, k, i);
-        if (j < 0) continue;
-        const double e1x = LV_EV(M, 0, k, i), e1y = LV_EV(M, 1, k, i), e2x = LV_EV(M, 2, k, i), e2y = LV_EV(M, 3, k, i);
-        double y1, y2;
-        nbr_y(g, F, n, j, x1, x2, y1, y2);
-        const double m1 = 0.5 * (e1x + e2x), m2 = 0.5 * (e1y + e2y);
-        double Sq[4];
-        getS(F, n, j, avdr, Sq);
-        const double sc = (dt * lv_lr_ratio(x1 - y1, x2 - y2, e1x, e1y, e2x, e2y)) / mi;
-        const double M11 = Sp[0] - Sq[0], M21 = Sp[1] - Sq[1], M12 = Sp[2] - Sq[2], M22 = Sp[3] - Sq[3];
-        const double w1 = (m1 - x1) * sc, w2 = (m2 - x2) * sc;
-        v1 = v1 - (M11 * w1 + M12 * w2);
-        v2 = v2 - (M21 * w1 + M22 * w2);
+    for (int k0 = 0; k0 < ne; k0 += EB) {
+        edge_batch E;
+        load_edges(M, i, k0, ne, E);
+        double xj1[EB], xj2[EB], D11[EB], D21[EB], D12[EB], D22[EB], muj[EB], rhoj[EB];
+#pragma unroll
+        for (int u = 0; u < EB; u++) {
+            const int j = E.j[u];
+            xj1[u] = F.x[j];
+            xj2[u] = F.x[n + j];
+            D11[u] = F.D[j];
+            D21[u] = F.D[n + j];
+            D12[u] = F.D[2 * n + j];
+            D22[u] = F.D[3 * n + j];
+            muj[u] = F.mu[j];
+            rhoj[u] = F.rho[j];
+        }
+#pragma unroll
+        for (int u = 0; u < EB; u++) {
+            if (!E.on[u]) continue;
+            double y1, y2;
+            lv_arrow_target(g, xj1[u], xj2[u], x1, x2, y1, y2);
+            const double m1 = 0.5 * (E.e1x[u] + E.e2x[u]), m2 = 0.5 * (E.e1y[u] + E.e2y[u]);
+            double Sq[4];
+            getS(D11[u], D21[u], D12[u], D22[u], muj[u], rhoj[u], avdr, Sq);
+            const double sc = (dt * lv_lr_ratio(x1 - y1, x2 - y2, E.e1x[u], E.e1y[u], E.e2x[u], E.e2y[u])) / mi;
+            const double M11 = Sp[0] - Sq[0], M21 = Sp[1] - Sq[1], M12 = Sp[2] - Sq[2], M22 = Sp[3] - Sq[3];
+            const double w1 = (m1 - x1) * sc, w2 = (m2 - x2) * sc;
+            v1 = v1 - (M11 * w1 + M12 * w2);
+            v2 = v2 - (M21 * w1 + M22 * w2);
+        }
     }
     F.v[i] = v1;
     F.v[n + i] = v2;
@@ -300,31 +367,49 @@ __global__ __launch_bounds__(B) void k_visc2(lv_grid_params g, lv_fields F, lv_m
     const double x1 = F.x[i], x2 = F.x[n + i];
     double ei = F.e[i];
     double Sp[4];
-    getS(F, n, i, avdr, Sp);
+    getS(F.D[i], F.D[n + i], F.D[2 * n + i], F.D[3 * n + i], F.mu[i], F.rho[i], avdr, Sp);
     const double pv1 = F.v[i], pv2 = F.v[n + i];
     const double sv1 = Sp[0] * pv1 + Sp[2] * pv2;
     const double sv2 = Sp[1] * pv1 + Sp[3] * pv2;
     const double mi = F.mass[i];
     double rmax = 0.0, rmin = __builtin_inf();
     const int ne = M.ne[i];
-    for (int k = 0; k < ne; k++) {
-        const int j = LV_EL(M, k, i);
-        if (j < 0) continue;
-        const double e1x = LV_EV(M, 0, k, i), e1y = LV_EV(M, 1, k, i), e2x = LV_EV(M, 2, k, i), e2y = LV_EV(M, 3, k, i);
-        double y1, y2;
-        nbr_y(g, F, n, j, x1, x2, y1, y2);
-        const double m1 = 0.5 * (e1x + e2x), m2 = 0.5 * (e1y + e2y);
-        double Sq[4];
-        getS(F, n, j, avdr, Sq);
-        const double qv1 = Sq[0] * F.v[j] + Sq[2] * F.v[n + j];
-        const double qv2 = Sq[1] * F.v[j] + Sq[3] * F.v[n + j];
-        const double a = (m1 - x1) * sv1 + (m2 - x2) * sv2;
-        const double b = (m1 - y1) * qv1 + (m2 - y2) * qv2;
-        ei = ei + ((dt * lv_lr_ratio(x1 - y1, x2 - y2, e1x, e1y, e2x, e2y)) / mi) * (a - b);
-        if (WITH_DV) {
-            const double r = lv_norm2(x1 - y1, x2 - y2);
-            if (r > rmax) rmax = r;
-            if (r < rmin) rmin = r;
+    for (int k0 = 0; k0 < ne; k0 += EB) {
+        edge_batch E;
+        load_edges(M, i, k0, ne, E);
+        double xj1[EB], xj2[EB], D11[EB], D21[EB], D12[EB], D22[EB], muj[EB], rhoj[EB], vq1[EB], vq2[EB];
+#pragma unroll
+        for (int u = 0; u < EB; u++) {
+            const int j = E.j[u];
+            xj1[u] = F.x[j];
+            xj2[u] = F.x[n + j];
+            D11[u] = F.D[j];
+            D21[u] = F.D[n + j];
+            D12[u] = F.D[2 * n + j];
+            D22[u] = F.D[3 * n + j];
+            muj[u] = F.mu[j];
+            rhoj[u] = F.rho[j];
+            vq1[u] = F.v[j];
+            vq2[u] = F.v[n + j];
+        }
+#pragma unroll
+        for (int u = 0; u < EB; u++) {
+            if (!E.on[u]) continue;
+            double y1, y2;
+            lv_arrow_target(g, xj1[u], xj2[u], x1, x2, y1, y2);
+            const double m1 = 0.5 * (E.e1x[u] + E.e2x[u]), m2 = 0.5 * (E.e1y[u] + E.e2y[u]);
+            double Sq[4];
+            getS(D11[u], D21[u], D12[u], D22[u], muj[u], rhoj[u], avdr, Sq);
+            const double qv1 = Sq[0] * vq1[u] + Sq[2] * vq2[u];
+            const double qv2 = Sq[1] * vq1[u] + Sq[3] * vq2[u];
+            const double a = (m1 - x1) * sv1 + (m2 - x2) * sv2;
+            const double b = (m1 - y1) * qv1 + (m2 - y2) * qv2;
+            ei = ei + ((dt * lv_lr_ratio(x1 - y1, x2 - y2, E.e1x[u], E.e1y[u], E.e2x[u], E.e2y[u])) / mi) * (a - b);
+            if (WITH_DV) {
+                const double r = lv_norm2(x1 - y1, x2 - y2);
+                if (r > rmax) rmax = r;
+                if (r < rmin) rmin = r;
+            }
         }
     }
     F.e[i] = ei;
@@ -406,38 +491,57 @@ __global__ __launch_bounds__(B) void k_relax1(lv_grid_params g, lv_fields F, lv_
     double en = mass * ep;
     const double na = lv_norm2(pd1, pd2);
     const int ne = M.ne[i];
-    for (int k = 0; k < ne; k++) {
-        const int j = LV_EL(M, k, i);
-        if (j < 0) continue;
-        if (!(php == F.phase[j])) continue;
-        const double e1x = LV_EV(M, 0, k, i), e1y = LV_EV(M, 1, k, i), e2x = LV_EV(M, 2, k, i), e2y = LV_EV(M, 3, k, i);
-        double y1, y2;
-        nbr_y(g, F, n, j, x1, x2, y1, y2);
-        const double lrr = lv_lr_ratio(x1 - y1, x2 - y2, e1x, e1y, e2x, e2y);
-        const double m1 = 0.5 * (e1x + e2x), m2 = 0.5 * (e1y + e2y);
-        const double z1 = 0.5 * (x1 + y1), z2 = 0.5 * (x2 + y2);
-        const double rq = F.rho[j], eq = F.e[j];
-        const double qv1 = F.v[j], qv2 = F.v[n + j];
-        const double qd1 = F.dv[j], qd2 = F.dv[n + j];
-        const double pdvpq = pd1 * (x1 - y1) + pd2 * (x2 - y2);
-        const double qdvpq = qd1 * (x1 - y1) + qd2 * (x2 - y2);
-        const double pdvmz = pd1 * (m1 - z1) + pd2 * (m2 - z2);
-        const double qdvmz = qd1 * (m1 - z1) + qd2 * (m2 - z2);
-        const double dl = dt * lrr;
-        const double pr_mz = pdvmz * rp, qr_mz = qdvmz * rq, pr_pq = pdvpq * rp, qr_pq = qdvpq * rq;
-        mass += dl * ((pr_mz - qr_mz) - 0.5 * (pr_pq + qr_pq));
-        mo1 += dl * ((pr_mz * pv1 - qr_mz * qv1) - 0.5 * (pr_pq * pv1 + qr_pq * qv1));
-        mo2 += dl * ((pr_mz * pv2 - qr_mz * qv2) - 0.5 * (pr_pq * pv2 + qr_pq * qv2));
-        en += dl * ((pr_mz * ep - qr_mz * eq) - 0.5 * (pr_pq * ep + qr_pq * eq));
-        if (rusanov) {  // src/relaxation.jl:52-58
-            const double nb = lv_norm2(qd1, qd2);
-            const double a = (nb > na) ? nb : na;
-            const double l = lv_norm2(e1x - e2x, e1y - e2y);
-            const double w = ((0.5 * dt) * l) * a;
-            mass += w * (rq - rp);
-            mo1 += w * (rq * qv1 - rp * pv1);
-            mo2 += w * (rq * qv2 - rp * pv2);
-            en += w * (rq * eq - rp * ep);
+    for (int k0 = 0; k0 < ne; k0 += EB) {
+        edge_batch E;
+        load_edges(M, i, k0, ne, E);
+        double xj1[EB], xj2[EB], rq_[EB], eq_[EB], vq1[EB], vq2[EB], dq1[EB], dq2[EB];
+        int phq[EB];
+#pragma unroll
+        for (int u = 0; u < EB; u++) {
+            const int j = E.j[u];
+            phq[u] = F.phase[j];
+            xj1[u] = F.x[j];
+            xj2[u] = F.x[n + j];
+            rq_[u] = F.rho[j];
+            eq_[u] = F.e[j];
+            vq1[u] = F.v[j];
+            vq2[u] = F.v[n + j];
+            dq1[u] = F.dv[j];
+            dq2[u] = F.dv[n + j];
+        }
+#pragma unroll
+        for (int u = 0; u < EB; u++) {
+            if (!E.on[u]) continue;
+            if (!(php == phq[u])) continue;
+            const double e1x = E.e1x[u], e1y = E.e1y[u], e2x = E.e2x[u], e2y = E.e2y[u];
+            double y1, y2;
+            lv_arrow_target(g, xj1[u], xj2[u], x1, x2, y1, y2);
+            const double lrr = lv_lr_ratio(x1 - y1, x2 - y2, e1x, e1y, e2x, e2y);
+            const double m1 = 0.5 * (e1x + e2x), m2 = 0.5 * (e1y + e2y);
+            const double z1 = 0.5 * (x1 + y1), z2 = 0.5 * (x2 + y2);
+            const double rq = rq_[u], eq = eq_[u];
+            const double qv1 = vq1[u], qv2 = vq2[u];
+            const double qd1 = dq1[u], qd2 = dq2[u];
+            const double pdvpq = pd1 * (x1 - y1) + pd2 * (x2 - y2);
+            const double qdvpq = qd1 * (x1 - y1) + qd2 * (x2 - y2);
+            const double pdvmz = pd1 * (m1 - z1) + pd2 * (m2 - z2);
+            const double qdvmz = qd1 * (m1 - z1) + qd2 * (m2 - z2);
+            const double dl = dt * lrr;
+            const double pr_mz = pdvmz * rp, qr_mz = qdvmz * rq, pr_pq = pdvpq * rp, qr_pq = qdvpq * rq;
+            mass += dl * ((pr_mz - qr_mz) - 0.5 * (pr_pq + qr_pq));
+            mo1 += dl * ((pr_mz * pv1 - qr_mz * qv1) - 0.5 * (pr_pq * pv1 + qr_pq * qv1));
+            mo2 += dl * ((pr_mz * pv2 - qr_mz * qv2) - 0.5 * (pr_pq * pv2 + qr_pq * qv2));
+            en += dl * ((pr_mz * ep - qr_mz * eq) - 0.5 * (pr_pq * ep + qr_pq * eq));
+            if (rusanov) {  // src/relaxation.jl:52-58
+                const double nb = lv_norm2(qd1, qd2);
+                const double a = (nb > na) ? nb : na;
+                const double l = lv_norm2(e1x - e2x, e1y - e2y);
+                const double w = ((0.5 * dt) * l) * a;
+                mass += w * (rq - rp);
+                mo1 += w * (rq * qv1 - rp * pv1);
+                mo2 += w * (rq * qv2 - rp * pv2);
+                en += w * (rq * eq - rp * ep);
+            }
         }
     }
     F.mass[i] = mass;

@@ -56,7 +56,7 @@ struct lv_amg {
     size_t cap_ooff, cap_oev32;
     int *pflag;          // [0] set when a column offset does not fit 16 bits, [1] when a level-1 cell
                          // is too large for the fused down pass
-    int *bfirst;         // fused down pass: the first level-1 cell of each block (nbfuse + 1 entries)
+    int *bfirst;         // fused down pass: (first level-1 cell, first row) of each block (nbfuse + 1 pairs), then (0, 0)
     size_t cap_bfirst;
     int nbfuse;          // its blocks; 0: the down pass and the restriction run as two kernels
     int z0_in;           // 1: the fused down pass can take z0 = fine_scale D^-1 r from the caller

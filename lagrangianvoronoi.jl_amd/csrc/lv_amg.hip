@@ -449,7 +449,7 @@ __global__ __launch_bounds__(B) void k_fine_down_j(fine_ops<V> F, const float *_
 
 // The down pass and the restriction in one kernel, for the aggregate order (the rows of level-1
 // cell I are start[I]..start[I+1]-1): block b owns the cells whose first row lies in
-// [FUSE_ROWS b, FUSE_ROWS (b+1)) (bfirst[b]..bfirst[b+1]-1), keeps their residuals t in LDS and sums
+// [FUSE_ROWS b, FUSE_ROWS (b+1)) (blk[b].x..blk[b+1].x-1), keeps their residuals t in LDS and sums
 // each cell's as k_restrict1 does (RL lanes, the same order and shuffle tree, so rc is the same).
 // A block holds at most FUSE_ROWS - 1 + (largest cell) rows; set-up checks that this fits FUSE_CAP.
 // With z0 given each thread takes FUSE_RPT rows (FUSE_ROWS a little under FUSE_RPT B), whose loads
@@ -483,18 +483,39 @@ __device__ __forceinline__ float down_row_z0(const fine_ops<V> &F, int i, const 
 
 // With z1 (level 1 above the tail), each cell's pre-smoothed z1 = w D1^-1 rc is stored too, as
 // level 1's down pass would form it.
+//
+// blk[b] = (first cell, first row) of block b, so the block's bounds take one round trip, with the
+// done flag (S->done, or without a solve the zero pair blk[nbf + 1]).  The
+// restriction's row-list bounds and level-1 diagonals for its first FUSE_RPT trips of B lanes (a
+// block's cells have about as many lanes as rows) are loaded before the rows, not after the
+// barrier: a block's life is a chain of dependent round trips, and the blocks on a CU take several
+// turns.  A lane past the block's cells loads cell na - 1's and drops them; dinv1 (level 1's
+// diagonal) is read whether or not z1 is stored.
 template <class V, bool Z0>
 __global__ __launch_bounds__(B) void k_fine_down_jr(fine_ops<V> F, const float *__restrict__ r,
                                                     float *__restrict__ z0, const int *__restrict__ start,
-                                                    const int *__restrict__ bfirst, float *__restrict__ rc,
+                                                    const int2 *__restrict__ blk, int na, float *__restrict__ rc,
                                                     const float *__restrict__ dinv1, float *__restrict__ z1,
-                                                    const lv_cg_scalars *S)
+                                                    const int *__restrict__ done)
 {
     __shared__ float tl[FUSE_CAP];
-    if (S && S->done) return;
     const int b = lv_xcd_block(blockIdx.x, gridDim.x);
-    const int c0 = bfirst[b], c1 = bfirst[b + 1];
-    const int R0 = start[c0], R1 = start[c1];
+    const int2 p0 = blk[b], p1 = blk[b + 1];
+    const int dn = *done;
+    // the bounds and the flag in one round trip: without this the bounds' loads sink below the test
+    asm volatile("" ::"s"(p0.x), "s"(p0.y), "s"(p1.x), "s"(p1.y), "s"(dn));
+    if (dn) return;
+    const int c0 = p0.x, c1 = p1.x, R0 = p0.y, R1 = p1.y;
+    int sa[FUSE_RPT], sb[FUSE_RPT];
+    float dv[FUSE_RPT];
+#pragma unroll
+    for (int p = 0; p < FUSE_RPT; p++) {
+        const int I = c0 + (p * B + (int)threadIdx.x) / RL;
+        const int Ic = I < na ? I : na - 1;
+        sa[p] = start[Ic];
+        sb[p] = start[Ic + 1];
+        dv[p] = dinv1[Ic];
+    }
     if constexpr (Z0) {
         // rows i0 + q B; all but the first are clamped into the block's rows so that every row's
         // loads are issued without a branch (a clamped duplicate is computed and dropped)
@@ -514,35 +535,48 @@ __global__ __launch_bounds__(B) void k_fine_down_jr(fine_ops<V> F, const float *
     }
     __syncthreads();
     const int lanes = (c1 - c0) * RL;
-    for (int base = 0; base < lanes; base += B) {  // uniform trip count: every lane shuffles
+    // one trip of B lanes, RL per cell; the trip count is uniform, so every lane shuffles
+    auto restrict_trip = [&](int base, int a0, int s1, float di) {
         const int idx = base + (int)threadIdx.x;
         const int I = c0 + idx / RL, q = idx % RL;
         float s = 0.0f;
-        if (idx < lanes) {
-            const int s1 = start[I + 1];
-            for (int a = start[I] + q; a < s1; a += RL) s += tl[a - R0];
-        }
+        if (idx < lanes)
+            for (int a = a0 + q; a < s1; a += RL) s += tl[a - R0];
         s = sum_lanes(s);
         if (q == 0 && idx < lanes) {
             rc[I] = s;
-            if (z1) z1[I] = OMEGA * dinv1[I] * s;
+            if (z1) z1[I] = OMEGA * di * s;
         }
+    };
+#pragma unroll
+    for (int p = 0; p < FUSE_RPT; p++)
+        if (p * B < lanes) restrict_trip(p * B, sa[p], sb[p], dv[p]);
+    for (int base = FUSE_RPT * B; base < lanes; base += B) {
+        const int I = c0 + (base + (int)threadIdx.x) / RL;
+        const int Ic = I < na ? I : na - 1;
+        restrict_trip(base, start[Ic], start[Ic + 1], dinv1[Ic]);
     }
 }
 
-// bfirst[b] = the first cell I with start[I] >= FUSE_ROWS b, for b = 0..nbf-1, and bfirst[nbf] = na
-// (start[na] = n, nbf = n / FUSE_ROWS + 1).  Thread I writes the b in (start[I-1], start[I]] /
-// FUSE_ROWS; flag[1] is set when a cell has more than FUSE_CAP - FUSE_ROWS rows.
+// blk[b] = (the first cell I with start[I] >= FUSE_ROWS b, start[I]) for b = 0..nbf-1,
+// blk[nbf] = (na, n) (start[na] = n, nbf = n / FUSE_ROWS + 1) and blk[nbf + 1] = (0, 0), the done flag
+// of a cycle outside a solve.  Thread I writes the b in
+// (start[I-1], start[I]] / FUSE_ROWS; flag[1] is set when a cell has more than FUSE_CAP - FUSE_ROWS
+// rows.
 __global__ __launch_bounds__(B) void k_fuse_blocks(int na, int nbf, const int *__restrict__ start,
-                                                   int *__restrict__ bfirst, int *__restrict__ flag)
+                                                   int2 *__restrict__ blk, int *__restrict__ flag)
 {
     const int I = blockIdx.x * B + threadIdx.x;
     if (I > na) return;
+    const int sI = start[I];
     const int lo = I == 0 ? 0 : start[I - 1] / FUSE_ROWS + 1;
-    const int hi = start[I] / FUSE_ROWS;
-    for (int b = lo; b <= hi && b < nbf; b++) bfirst[b] = I;
-    if (I == na) bfirst[nbf] = na;
-    if (I < na && start[I + 1] - start[I] > FUSE_CAP - FUSE_ROWS) atomicOr(flag + 1, 1);
+    const int hi = sI / FUSE_ROWS;
+    for (int b = lo; b <= hi && b < nbf; b++) blk[b] = make_int2(I, sI);
+    if (I == na) {
+        blk[nbf] = make_int2(na, sI);
+        blk[nbf + 1] = make_int2(0, 0);
+    }
+    if (I < na && start[I + 1] - sI > FUSE_CAP - FUSE_ROWS) atomicOr(flag + 1, 1);
 }
 
 // up: z1 = z0 + alpha P ec and t = r - A z1 in one pass (z1 of a neighbour from its z0 and its
@@ -1741,9 +1775,9 @@ hipError_t lv_amg_setup(lv_amg *G, const lv_grid_params &g, const double *x, con
                            G->pdinv32, G->pmval, G->pmdg, G->pflag, G->ooff, G->oev32, G->ovf_cap);
         // the blocks of the fused down pass (Jacobi smoother)
         const int nbf = n / FUSE_ROWS + 1;
-        AMG_CHECK(grow(&G->bfirst, &G->cap_bfirst, (size_t)nbf + 1));
+        AMG_CHECK(grow(&G->bfirst, &G->cap_bfirst, 2 * ((size_t)nbf + 2)));
         hipLaunchKernelGGL(k_fuse_blocks, g1((int64_t)na + 1), dim3(B), 0, s, na, nbf, (const int *)G->start,
-                           G->bfirst, G->pflag);
+                           reinterpret_cast<int2 *>(G->bfirst), G->pflag);
         int flags[2] = {1, 1};
         AMG_CHECK(hipMemcpyAsync(flags, G->pflag, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
         AMG_CHECK(hipStreamSynchronize(s));
@@ -1797,14 +1831,15 @@ static hipError_t vcycle(const lv_amg *G, const fine_ops<V> &F, const int *agg, 
         // level 1 above the tail: its pre-smoothed z stored here, gathered by its down pass
         z1_given = G->ltail > 1;
         float *z1 = z1_given ? L1.z : nullptr;
+        const int *done = S ? &S->done : G->bfirst + 2 * (G->nbfuse + 1);
         if (z0_ready && G->z0_in)
             hipLaunchKernelGGL((k_fine_down_jr<V, true>), dim3(G->nbfuse), dim3(B), 0, s, F, r, G->z0,
-                               (const int *)G->start, (const int *)G->bfirst, L1.r, (const float *)L1.dinv, z1,
-                               S);
+                               (const int *)G->start, reinterpret_cast<const int2 *>(G->bfirst), G->na, L1.r,
+                               (const float *)L1.dinv, z1, done);
         else
             hipLaunchKernelGGL((k_fine_down_jr<V, false>), dim3(G->nbfuse), dim3(B), 0, s, F, r, G->z0,
-                               (const int *)G->start, (const int *)G->bfirst, L1.r, (const float *)L1.dinv, z1,
-                               S);
+                               (const int *)G->start, reinterpret_cast<const int2 *>(G->bfirst), G->na, L1.r,
+                               (const float *)L1.dinv, z1, done);
     } else {
         hipLaunchKernelGGL(k_fine_down_j<V>, g1(n), dim3(B), 0, s, F, r, G->z0, G->t0, S);
         hipLaunchKernelGGL(k_restrict1, g1((int64_t)G->na * RL), dim3(B), 0, s, G->na, (const int *)G->start, list,

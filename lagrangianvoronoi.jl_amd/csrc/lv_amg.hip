@@ -848,11 +848,29 @@ __global__ __launch_bounds__(B) void k_st_up(st_level L, int Kyc, const float *_
     if (S && S->done) return;
     const int f = lv_xcd_block(blockIdx.x, gridDim.x) * B + threadIdx.x;  // (as k_st_down)
     if (f >= L.Kx * L.Ky) return;
+    // the prolongation's rows: bounds first, then the first PH rows' z0 loaded together (a row past
+    // the cell's reloads its last, or row 0 for an empty cell at the end), so that the cell's rows
+    // take one round trip after the stencil's loads instead of one each
+    constexpr int PH = 4;
+    int b0 = 0, e = 0;
+    float zz[PH];
+    if (PROLONG) {
+        b0 = start[f];
+        e = start[f + 1];
+        const int last = e > 0 ? e - 1 : 0;
+#pragma unroll
+        for (int h = 0; h < PH; h++) zz[h] = z0[b0 + h < e ? b0 + h : last];
+    }
     const float v = st_up_cell(L, Kyc, f, r, z, ec, out);
     if (PROLONG) {
+        // (the rows' values in registers before the stores' branches, which would otherwise take
+        // the first row's load with them)
+        asm volatile("" : "+v"(zz[0]), "+v"(zz[1]), "+v"(zz[2]), "+v"(zz[3]));
         const float a = alpha0 * v;
-        const int e = start[f + 1];
-        for (int i = start[f]; i < e; i++) z0[i] += a;
+#pragma unroll
+        for (int h = 0; h < PH; h++)
+            if (b0 + h < e) z0[b0 + h] = zz[h] + a;
+        for (int i = b0 + PH; i < e; i++) z0[i] += a;
     }
 }
 

@@ -86,21 +86,21 @@ __device__ __forceinline__ void rhs_geom_compact(const lv_mesh_view &M, const lv
     }
 }
 
-// refresh!(A) (src/pressure.jl:104-117): the CSR copy in edge order, the ELL copy with each row's
-// columns sorted, diagonals and Jacobi inverses.  With R.w1 (mode 0, find_pressure!) the same edge
-// loop also stores the P-independent parts of refresh!(solver) (vbc: boundary_velocity per label).
-__global__ __launch_bounds__(B) void k_assemble(lv_grid_params g, lv_fields F, lv_mesh_view M, int n, double dt, int mode,
-                                                const int *__restrict__ row_ptr, int *__restrict__ cols,
-                                                double *__restrict__ val64, float *__restrict__ val32,
-                                                double *__restrict__ diag64, float *__restrict__ diag32,
-                                                double *__restrict__ dinv64, float *__restrict__ dinv32,
-                                                int *__restrict__ ecol, double *__restrict__ ev64,
-                                                float *__restrict__ ev32, int *__restrict__ nonsingular, lv_vbc vbc,
-                                                lv_rhs_geom R, int *__restrict__ ocol, double *__restrict__ oev64,
-                                                float *__restrict__ oev32)
+constexpr int ACSR = 2048;  // CSR entries of a k_assemble block staged in LDS (32 KB)
+
+// one row of k_assemble
+__device__ __forceinline__ void assemble_row(const lv_grid_params &g, const lv_fields &F, const lv_mesh_view &M, int n,
+                                             double dt, int mode, int i, const int *__restrict__ row_ptr, bool stage,
+                                             int cbase, int *s_col, double *s_v64, float *s_v32,
+                                             int *__restrict__ cols, double *__restrict__ val64,
+                                             float *__restrict__ val32, double *__restrict__ diag64,
+                                             float *__restrict__ diag32, double *__restrict__ dinv64,
+                                             float *__restrict__ dinv32, int *__restrict__ ecol,
+                                             double *__restrict__ ev64, float *__restrict__ ev32,
+                                             int *__restrict__ nonsingular, const lv_vbc &vbc, const lv_rhs_geom &R,
+                                             int *__restrict__ ocol, double *__restrict__ oev64,
+                                             float *__restrict__ oev32)
 {
-    const int i = blockIdx.x * B + threadIdx.x;
-    if (i >= n) return;  // rows past n hold no slots: their lanes leave the group's ballots below
     const double x1 = F.x[i], x2 = F.x[n + i];
     const double rho = F.rho[i];
     const double d = (mode == 0) ? F.mass[i] / (((rho * rho) * F.c2[i]) * (dt * dt)) : F.mass[i] / dt;
@@ -126,9 +126,15 @@ __global__ __launch_bounds__(B) void k_assemble(lv_grid_params g, lv_fields F, l
         lv_arrow_target(g, F.x[j], F.x[n + j], x1, x2, y1, y2);
         const double lrr = lv_lr_ratio(x1 - y1, x2 - y2, e1x, e1y, e2x, e2y);
         const double c = (mode == 0) ? lrr * (0.5 / rho + 0.5 / F.rho[j]) : (0.5 * lrr) * (mui + F.mu[j]);
-        cols[o] = j;
-        val64[o] = c;
-        val32[o] = (float)c;
+        if (stage) {
+            s_col[o - cbase] = j;
+            s_v64[o - cbase] = c;
+            s_v32[o - cbase] = (float)c;
+        } else {
+            cols[o] = j;
+            val64[o] = c;
+            val32[o] = (float)c;
+        }
         dd += c;
         o++;
         if (geom) {
@@ -174,6 +180,43 @@ __global__ __launch_bounds__(B) void k_assemble(lv_grid_params g, lv_fields F, l
     diag32[i] = (float)d;
     dinv64[i] = 1.0 / dd;
     dinv32[i] = (float)(1.0 / dd);
+}
+
+// refresh!(A) (src/pressure.jl:104-117): the CSR copy in edge order, the ELL copy with each row's
+// columns sorted, diagonals and Jacobi inverses.  With R.w1 (mode 0, find_pressure!) the same edge
+// loop also stores the P-independent parts of refresh!(solver) (vbc: boundary_velocity per label).
+__global__ __launch_bounds__(B) void k_assemble(lv_grid_params g, lv_fields F, lv_mesh_view M, int n, double dt, int mode,
+                                                const int *__restrict__ row_ptr, int *__restrict__ cols,
+                                                double *__restrict__ val64, float *__restrict__ val32,
+                                                double *__restrict__ diag64, float *__restrict__ diag32,
+                                                double *__restrict__ dinv64, float *__restrict__ dinv32,
+                                                int *__restrict__ ecol, double *__restrict__ ev64,
+                                                float *__restrict__ ev32, int *__restrict__ nonsingular, lv_vbc vbc,
+                                                lv_rhs_geom R, int *__restrict__ ocol, double *__restrict__ oev64,
+                                                float *__restrict__ oev32)
+{
+    // The block's rows own one contiguous span of the CSR copy; it is formed in LDS and stored with
+    // coalesced writes after the barrier (each thread's own entries would be 64 scattered stores an
+    // edge), unless it is longer than ACSR entries.
+    __shared__ int s_col[ACSR];
+    __shared__ double s_v64[ACSR];
+    __shared__ float s_v32[ACSR];
+    const int i = blockIdx.x * B + threadIdx.x;
+    const int rb1 = (int)blockIdx.x * B + B < n ? (int)blockIdx.x * B + B : n;
+    const int cbase = row_ptr[blockIdx.x * B], clen = row_ptr[rb1] - cbase;
+    const bool stage = clen <= ACSR;
+    if (i < n) {  // rows past n hold no slots: their lanes stay out of the group's ballots below
+        assemble_row(g, F, M, n, dt, mode, i, row_ptr, stage, cbase, s_col, s_v64, s_v32, cols, val64, val32,
+                     diag64, diag32, dinv64, dinv32, ecol, ev64, ev32, nonsingular, vbc, R, ocol, oev64, oev32);
+    }
+    if (stage) {
+        __syncthreads();
+        for (int t = threadIdx.x; t < clen; t += B) {
+            cols[cbase + t] = s_col[t];
+            val64[cbase + t] = s_v64[t];
+            val32[cbase + t] = s_v32[t];
+        }
+    }
 }
 
 // ---- refresh!(solver) split by what depends on P ---------------------------------------------------

@@ -39,9 +39,13 @@ struct lv_pressure_system {
 
 // Progress of a running CG solve, stored by the device in pinned host memory: seen = the last
 // iteration whose convergence test ran, done = the solve's done flag.  The host reads it without
-// synchronising to keep its launches one iteration ahead of the device.
+// synchronising to keep its launches one iteration ahead of the device.  After a refinement
+// sweep's FP64 residual, k_finalize also publishes there what the host tests (the scalars of
+// lv_cg_scalars of those names) and then fseq, the number the host asked for.
 struct lv_progress {
     int seen, done;
+    int fseq, iters, skip, pad;
+    double rr64, rr64_0, bb, thr2;
 };
 
 struct lv_solver_work {

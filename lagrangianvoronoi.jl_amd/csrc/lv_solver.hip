@@ -397,7 +397,8 @@ __global__ __launch_bounds__(B) void k_write_P(lv_fields F, int n, const double 
 enum { FIN_MEANS = 0, FIN_INIT64 = 1, FIN_RESID = 2, FIN_SUM1 = 3, FIN_INIT32 = 4, FIN_RESID0 = 5 };
 
 __global__ __launch_bounds__(B) void k_finalize(const double *__restrict__ partials, int nb, int mode, int n,
-                                                double inner_tol2, lv_cg_scalars *S)
+                                                double inner_tol2, lv_cg_scalars *S, lv_progress *pub = nullptr,
+                                                int seq = 0)
 {
     double o[3];
     {
@@ -423,6 +424,16 @@ __global__ __launch_bounds__(B) void k_finalize(const double *__restrict__ parti
         S->rr64 = o[0];
         S->bb = o[1];
         S->thr2 = S->tol2 * o[1];
+        if (pub) {  // the host's tests after a sweep, without a copy and a synchronisation
+            pub->rr64 = S->rr64;
+            pub->rr64_0 = S->rr64_0;
+            pub->bb = S->bb;
+            pub->thr2 = S->thr2;
+            pub->iters = S->iters;
+            pub->skip = S->skip;
+            __threadfence_system();
+            __hip_atomic_store(&pub->fseq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
         break;
     case FIN_RESID0: {  // the initial residual: the host's tests before the first sweep, made here
         // (inner_tol2 carries the per-sweep floor of the inner tolerance, not squared)
@@ -1069,6 +1080,28 @@ hipError_t read_scalars(lv_cg_scalars *dS, lv_cg_scalars *pin, lv_cg_scalars *h,
     return hipSuccess;
 }
 
+// The scalars k_finalize published for sweep decision seq into h (the fields the host tests).  The
+// host spins on the pinned record; after 20 ms without it (a kernel failed, or the device is
+// busy elsewhere) it falls back to the synchronising read, whose values are then current.
+hipError_t await_published(lv_progress *hprog, int seq, lv_cg_scalars *dS, lv_cg_scalars *pin, lv_cg_scalars *h,
+                           hipStream_t s)
+{
+    const auto t0 = std::chrono::steady_clock::now();
+    for (int spin = 1;; spin++) {
+        if (__atomic_load_n(&hprog->fseq, __ATOMIC_ACQUIRE) == seq) {
+            h->rr64 = hprog->rr64;
+            h->rr64_0 = hprog->rr64_0;
+            h->bb = hprog->bb;
+            h->thr2 = hprog->thr2;
+            h->iters = hprog->iters;
+            h->skip = hprog->skip;
+            return hipSuccess;
+        }
+        if ((spin & 255) == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(20))
+            return read_scalars(dS, pin, h, s);
+    }
+}
+
 __global__ void k_init_scalars(lv_cg_scalars *S, double tol2, int itmax, int singular)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
@@ -1137,8 +1170,9 @@ hipError_t cg_iterate(int n, V A, const T *diag, const T *dinv, T *x, T *r, T *z
                       lv_timer *tm, lv_cg_scalars &h, const lv_precond *M, int *dflag)
 {
     const int nb = nblocks_of(n);
-    // no kernel of this solve that stores progress is in flight: the caller synchronised since the
-    // previous sweep (its read of the FP64 residual, or the read before a fallback sweep)
+    // no kernel of this solve that stores progress is in flight: since the previous sweep the caller
+    // has read the FP64 residual (published by k_finalize, which runs after every earlier kernel of
+    // the stream has finished) or synchronised before a fallback sweep
     __atomic_store_n(&hprog->seen, -1, __ATOMIC_RELAXED);
     __atomic_store_n(&hprog->done, 0, __ATOMIC_RELEASE);
     for (int k = 0;; k++) {
@@ -1264,9 +1298,10 @@ hipError_t lv_solve_device(const lv_pressure_system &A, lv_solver_work &W, int n
                 hipLaunchKernelGGL(k_finalize, dim3(1), dim3(B), 0, s, (const double *)P.p3, nb, (int)FIN_RESID0,
                                    n, floor_tol, W.S);
             } else {
+                const int seq = __atomic_load_n(&W.hprog->fseq, __ATOMIC_ACQUIRE) + 1;
                 hipLaunchKernelGGL(k_finalize, dim3(1), dim3(B), 0, s, (const double *)P.p3, nb, (int)FIN_RESID, n,
-                                   0.0, W.S);
-                e = read_scalars(W.S, W.hS, &h, s);
+                                   0.0, W.S, W.dprog, seq);
+                e = await_published(W.hprog, seq, W.S, W.hS, &h, s);
                 if (e != hipSuccess) return e;
                 if (pending) total += h.iters;
                 pending = false;

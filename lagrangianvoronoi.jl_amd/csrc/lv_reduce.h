@@ -58,28 +58,46 @@ __device__ __forceinline__ void block_partials(const double (&v)[NV], double *__
     }
 }
 
+// reduce_all in two parts: the loads of a thread's first LV_RED_PER partials per value, then the
+// sums, so that a kernel can issue its other loads before it needs the sums
+constexpr int LV_RED_PER = (LV_NBMAX + 255) / 256;
+
 template <int NV>
-__device__ __forceinline__ void reduce_all(const double *__restrict__ partials, int nb, double (&out)[NV])
+__device__ __forceinline__ void reduce_all_load(const double *__restrict__ partials, int nb,
+                                                double (&p)[NV][LV_RED_PER])
 {
-    __shared__ double sm[NV][256 / 64];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     // each thread sums partials threadIdx.x, +256, ... in that order; the first LV_NBMAX of them in
     // an unrolled loop without branches (a slot past nb loads a valid partial and is not added), so
     // that all of a thread's loads are in flight together: a rolled loop waits for one L2 round trip
     // per partial
-    constexpr int PER = (LV_NBMAX + 255) / 256;
     const int last = nb > 0 ? nb - 1 : 0;
+#pragma unroll
+    for (int q = 0; q < NV; q++) {
+        const double *pq = partials + (size_t)q * nb;
+#pragma unroll
+        for (int j = 0; j < LV_RED_PER; j++) {
+            const int k = (int)threadIdx.x + 256 * j;
+            p[q][j] = pq[k < nb ? k : last];
+        }
+    }
+}
+
+template <int NV>
+__device__ __forceinline__ void reduce_all_sum(const double *__restrict__ partials, int nb,
+                                               const double (&p)[NV][LV_RED_PER], double (&out)[NV])
+{
+    __shared__ double sm[NV][256 / 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
     for (int q = 0; q < NV; q++) {
         const double *pq = partials + (size_t)q * nb;
         double s = 0.0;
 #pragma unroll
-        for (int j = 0; j < PER; j++) {
+        for (int j = 0; j < LV_RED_PER; j++) {
             const int k = (int)threadIdx.x + 256 * j;
-            const double p = pq[k < nb ? k : last];
-            s = (k < nb) ? s + p : s;
+            s = (k < nb) ? s + p[q][j] : s;
         }
-        for (int k = (int)threadIdx.x + 256 * PER; k < nb; k += 256) s += pq[k];
+        for (int k = (int)threadIdx.x + 256 * LV_RED_PER; k < nb; k += 256) s += pq[k];
         s = lv_wave_sum(s);
         if (lane == 0) sm[q][w] = s;
     }
@@ -91,6 +109,25 @@ __device__ __forceinline__ void reduce_all(const double *__restrict__ partials, 
         out[q] = t;
     }
     __syncthreads();
+}
+
+template <int NV>
+__device__ __forceinline__ void reduce_all(const double *__restrict__ partials, int nb, double (&out)[NV])
+{
+    double p[NV][LV_RED_PER];
+    reduce_all_load<NV>(partials, nb, p);
+    reduce_all_sum<NV>(partials, nb, p, out);
+}
+
+// the loaded partials in registers here: the loads are issued before a following branch (which
+// the compiler would otherwise take them behind)
+template <int NV>
+__device__ __forceinline__ void reduce_all_pin(double (&p)[NV][LV_RED_PER])
+{
+#pragma unroll
+    for (int q = 0; q < NV; q++)
+#pragma unroll
+        for (int j = 0; j < LV_RED_PER; j++) asm volatile("" : "+v"(p[q][j]));
 }
 
 }  // namespace

@@ -588,17 +588,24 @@ __global__ __launch_bounds__(B, (cg_waves<T, COMPACT>::value)) void k_cg_s(int n
                                             lv_cg_scalars *S, lv_progress *prog, int par, int k, int first,
                                             int seen_here)
 {
-    if (S->done) {
+    // the partials and the scalars the test and beta need in one round trip
+    double pp[2][LV_RED_PER];
+    reduce_all_load<2>(Prr, gridDim.x, pp);
+    const int dn = S->done, itmax = S->itmax;
+    const double thr2 = S->thr2, rzp = S->rz[par];
+    reduce_all_pin<2>(pp);
+    asm volatile("" ::"s"(dn), "s"(itmax), "s"(thr2), "s"(rzp));
+    if (dn) {
         // a sweep that starts done (nothing left to solve): its first launch tells the host
         if (k == 0 && prog && blockIdx.x == 0 && threadIdx.x == 0)
-            __hip_atomic_store(&prog->done, S->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            __hip_atomic_store(&prog->done, dn, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
         return;
     }
     double o[2];
-    reduce_all<2>(Prr, gridDim.x, o);
+    reduce_all_sum<2>(Prr, gridDim.x, pp, o);
     const double rz = o[0], rr = o[1];
-    const bool conv = rr <= S->thr2;
-    const bool stop = conv || k >= S->itmax || !(rr == rr);
+    const bool conv = rr <= thr2;
+    const bool stop = conv || k >= itmax || !(rr == rr);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         S->rz[par ^ 1] = rz;
         S->rr = rr;
@@ -611,7 +618,7 @@ __global__ __launch_bounds__(B, (cg_waves<T, COMPACT>::value)) void k_cg_s(int n
         }
     }
     if (stop) return;
-    const T beta = first ? (T)0 : (T)(rz / S->rz[par]);
+    const T beta = first ? (T)0 : (T)(rz / rzp);
     double pq = 0.0;
     // grid-stride passes over the rows, each pass one contiguous window whose eighth per XCD is
     // contiguous too (lv_xcd_block); two rows per thread per pass keep twice the loads in flight
@@ -715,11 +722,18 @@ __global__ __launch_bounds__(B) void k_cg_u(int n, T *__restrict__ x, T *__restr
 __global__ __launch_bounds__(B) void k_conv_check(const double *__restrict__ Pur, int nb, lv_cg_scalars *S,
                                                   lv_progress *prog, int k)
 {
-    if (S->done) return;
+    // the partials, the flag and the threshold in one round trip
+    double p[1][LV_RED_PER];
+    reduce_all_load<1>(Pur, nb, p);
+    const int dn = S->done;
+    const double thr2 = S->thr2;
+    reduce_all_pin<1>(p);
+    asm volatile("" ::"s"(dn), "s"(thr2));
+    if (dn) return;
     double o[1];
-    reduce_all<1>(Pur, nb, o);
+    reduce_all_sum<1>(Pur, nb, p, o);
     if (threadIdx.x == 0) {
-        if (o[0] <= S->thr2) {
+        if (o[0] <= thr2) {
             S->rr = o[0];
             S->iters = k + 1;
             S->done = 1;

@@ -685,10 +685,17 @@ __global__ __launch_bounds__(B) void k_cg_u(int n, T *__restrict__ x, T *__restr
                                             double *__restrict__ Prr, double *__restrict__ Pur, lv_cg_scalars *S,
                                             int par, T *__restrict__ z0, T sc)
 {
-    if (S->done) return;
+    // the partials, the flag and rz in one round trip (every block starts here at once)
+    double pp[1][LV_RED_PER];
+    reduce_all_load<1>(Ppq, gridDim.x, pp);
+    const int dn = S->done;
+    const double rzn = S->rz[par ^ 1];
+    reduce_all_pin<1>(pp);
+    asm volatile("" ::"s"(dn), "s"(rzn));
+    if (dn) return;
     double pq[1];
-    reduce_all<1>(Ppq, gridDim.x, pq);
-    const double alpha = S->rz[par ^ 1] / pq[0];
+    reduce_all_sum<1>(Ppq, gridDim.x, pp, pq);
+    const double alpha = rzn / pq[0];
     const T a = (T)alpha;
     double v[2] = {0.0, 0.0};
     for (int i = blockIdx.x * B + threadIdx.x; i < n; i += gridDim.x * B) {

@@ -60,7 +60,7 @@ struct lv_ctx {
     bool rg_alloc = false;
     int *deg = nullptr;
     void *scan_temp = nullptr;
-    int *d_flags = nullptr;   // [0] error word, [1] nonsingular flag
+    int *d_flags = nullptr;   // [0] error word, [1] nonsingular flag, [2] a column offset wider than 16 bits
     lv_timer *timers[LV_NKTIMERS] = {nullptr, nullptr, nullptr, nullptr};
     // Picard extrapolation: orthonormal A-images q_k of past Picard corrections and their
     // preimages z_k (A z_k = q_k), the last two Picard iterates and their A-images
@@ -249,7 +249,7 @@ int read_flags(lv_ctx *ctx, int *flags2)
 
 int clear_flags(lv_ctx *ctx)
 {
-    LV_HIP_CHECK(hipMemsetAsync(ctx->d_flags, 0, 2 * sizeof(int), ctx->s));
+    LV_HIP_CHECK(hipMemsetAsync(ctx->d_flags, 0, 3 * sizeof(int), ctx->s));
     return LV_OK;
 }
 
@@ -378,7 +378,7 @@ int lv_create(const lv_grid_desc *d, lv_ctx **out)
     ALLOC(ctx->clip_retry, cap + 1);
     ALLOC(ctx->cell_start, ncells + 1);
     ALLOC(ctx->d_nitems, 1);
-    ALLOC(ctx->d_flags, 2);
+    ALLOC(ctx->d_flags, 3);
     ALLOC(ctx->path_i, 2 * (size_t)std::max(1, g.npath));
     ALLOC(ctx->path_rr, (size_t)std::max(1, g.npath));
     // pressure system and solver work
@@ -394,6 +394,9 @@ int lv_create(const lv_grid_desc *d, lv_ctx **out)
     ALLOC(ctx->A.ocol, ((size_t)cap / 64 + 1) * LV_OVF_CAP);
     ALLOC(ctx->A.oev64, ((size_t)cap / 64 + 1) * LV_OVF_CAP);
     ALLOC(ctx->A.oev32, ((size_t)cap / 64 + 1) * LV_OVF_CAP);
+    ALLOC(ctx->A.deg8, cap);
+    ALLOC(ctx->A.ecol16, (size_t)LV_MAXE * cap);
+    ALLOC(ctx->A.ocol16, ((size_t)cap / 64 + 1) * LV_OVF_CAP);
     ALLOC(ctx->b, cap);
     ALLOC(ctx->P0, cap);
     ALLOC(ctx->GP, 2 * cap);
@@ -422,7 +425,7 @@ int lv_create(const lv_grid_desc *d, lv_ctx **out)
         if (e == hipSuccess)
             e = hipMemcpy(ctx->path_rr, ctx->h_path_rr.data(), sizeof(double) * g.npath, hipMemcpyHostToDevice);
     }
-    if (e == hipSuccess) e = hipMemset(ctx->d_flags, 0, 2 * sizeof(int));
+    if (e == hipSuccess) e = hipMemset(ctx->d_flags, 0, 3 * sizeof(int));
     if (e == hipSuccess) {
         e = lv_spai_init(&ctx->spai);
         ctx->spai_ok = (e == hipSuccess);
@@ -456,7 +459,7 @@ int lv_destroy(lv_ctx *ctx)
                     ctx->cl_temp, ctx->d_nitems, ctx->d_flags, ctx->path_i, ctx->path_rr, ctx->A.row_ptr,
                     ctx->A.cols, ctx->A.val64, ctx->A.val32, ctx->A.diag64, ctx->A.dinv64, ctx->A.diag32,
                     ctx->A.dinv32, ctx->deg, ctx->A.ecol, ctx->A.ev64, ctx->A.ev32, ctx->A.ocol, ctx->A.oev64,
-                    ctx->A.oev32, ctx->b, ctx->P0, ctx->GP, ctx->W.r, ctx->W.z, ctx->W.p,
+                    ctx->A.oev32, ctx->A.deg8, ctx->A.ecol16, ctx->A.ocol16, ctx->b, ctx->P0, ctx->GP, ctx->W.r, ctx->W.z, ctx->W.p,
                     ctx->W.q, ctx->W.e32, ctx->W.r32, ctx->W.z32, ctx->W.p32, ctx->W.q32, ctx->W.S,
                     ctx->W.partials, ctx->scan_temp};
     for (void *p : ptrs) dfree(p);
@@ -1081,12 +1084,13 @@ static int assemble_system(lv_ctx *ctx, double dt, int mode, int64_t *nnz, const
     ctx->A.deg = ctx->deg;
     rc = clear_flags(ctx);
     if (rc) return rc;
-    LV_HIP_CHECK(lv_launch_assemble(ctx->g, ctx->F, M, n, dt, mode, ctx->A, ctx->d_flags + 1, ctx->s, rhs_vbc,
-                                    rhs_vbc ? &ctx->rg : nullptr));
-    int flags[2];
-    rc = read_flags(ctx, flags);
-    if (rc) return rc;
+    LV_HIP_CHECK(lv_launch_assemble(ctx->g, ctx->F, M, n, dt, mode, ctx->A, ctx->d_flags + 1, ctx->d_flags + 2,
+                                    ctx->s, rhs_vbc, rhs_vbc ? &ctx->rg : nullptr));
+    int flags[3];
+    LV_HIP_CHECK(hipMemcpyAsync(flags, ctx->d_flags, 3 * sizeof(int), hipMemcpyDeviceToHost, ctx->s));
+    LV_HIP_CHECK(hipStreamSynchronize(ctx->s));
     ctx->A.singular = (flags[1] == 0);
+    ctx->A.narrow = (flags[2] == 0);
     ctx->assembly_id++;
     ctx->A.valid = 1;
     ctx->A_kind = mode;

@@ -32,6 +32,11 @@ struct lv_pressure_system {
     int *ocol;
     double *oev64;
     float *oev32;
+    // the same columns as 16-bit offsets j - i (taken modulo n, lv_ell.h) with uint8 degrees, for the
+    // FP64 passes; narrow: every offset of the last assembly fits, so they hold the operator
+    uint8_t *deg8;
+    int16_t *ecol16, *ocol16;
+    int narrow;
     int64_t nnz, nnz_cap;
     int singular;  // every d_i == 0 (c2 = Inf): a graph Laplacian with the constants as null space
     int valid;
@@ -88,9 +93,10 @@ hipError_t lv_launch_degree(lv_mesh_view M, int n, int *deg, hipStream_t s);
 struct lv_rhs_geom;
 // refresh!(A) into A (mode 0 the pressure operator, 1 ViscousSolver's); with R (mode 0) the same pass
 // also stores the P-independent parts of refresh!(solver) for boundary velocity vbc
+// (wide: set when a column offset does not fit the 16-bit copy)
 hipError_t lv_launch_assemble(const lv_grid_params &g, lv_fields F, lv_mesh_view M, int n, double dt, int mode,
-                              lv_pressure_system &A, int *nonsingular, hipStream_t s, const lv_vbc *vbc = nullptr,
-                              const lv_rhs_geom *R = nullptr);
+                              lv_pressure_system &A, int *nonsingular, int *wide, hipStream_t s,
+                              const lv_vbc *vbc = nullptr, const lv_rhs_geom *R = nullptr);
 // The P-independent parts of refresh!(solver) for one find_pressure!: per mesh edge slot
 // w = lrr (m - x_i) and u = lrr (m - z) (each two planes of LV_MAXE x ld doubles, interior edges only),
 // per row the divergence and boundary-flux part of b (bdiv) and rho c2 dt^2 (den).

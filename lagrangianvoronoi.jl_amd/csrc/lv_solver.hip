@@ -99,7 +99,9 @@ __device__ __forceinline__ void assemble_row(const lv_grid_params &g, const lv_f
                                              double *__restrict__ ev64, float *__restrict__ ev32,
                                              int *__restrict__ nonsingular, const lv_vbc &vbc, const lv_rhs_geom &R,
                                              int *__restrict__ ocol, double *__restrict__ oev64,
-                                             float *__restrict__ oev32)
+                                             float *__restrict__ oev32, uint8_t *__restrict__ deg8,
+                                             int16_t *__restrict__ ecol16, int16_t *__restrict__ ocol16,
+                                             int *__restrict__ wide)
 {
     const double x1 = F.x[i], x2 = F.x[n + i];
     const double rho = F.rho[i];
@@ -151,12 +153,24 @@ __device__ __forceinline__ void assemble_row(const lv_grid_params &g, const lv_f
         sc[q] = j;
         sv[q] = c;
     }
+    // the 16-bit copy's offsets j - i, taken into [-(n/2), n/2] modulo n (lv_ell_col undoes it)
+    auto off16 = [&](int j) {
+        int off = j - i;
+        if (off > n / 2) off -= n;
+        if (off < -(n / 2)) off += n;
+        return off;
+    };
+    bool w16 = false;
     for (int k = 0; k < m; k++) {
         const size_t es = (size_t)k * n + i;
         ecol[es] = sc[k];
         ev64[es] = sv[k];
         ev32[es] = (float)sv[k];
+        const int off = off16(sc[k]);
+        w16 |= (off < -32768 || off > 32767);
+        ecol16[es] = (int16_t)off;
     }
+    deg8[i] = (uint8_t)m;
     // slots LV_OVF_BASE.. also into the compact copy of the row's 64-row group (a block's
     // wavefronts are whole groups), when the group has at most LV_OVF_CAP of them
     const lv_ovf_pos q = lv_ovf_prefix(m);
@@ -167,8 +181,10 @@ __device__ __forceinline__ void assemble_row(const lv_grid_params &g, const lv_f
             ocol[eo] = sc[k];
             oev64[eo] = sv[k];
             oev32[eo] = (float)sv[k];
+            ocol16[eo] = (int16_t)off16(sc[k]);
         }
     }
+    if (__any(w16) && (threadIdx.x & 63) == 0) atomicOr(wide, 1);
     if (geom) {
         R.bdiv[i] = bd;
         R.den[i] = (rho * F.c2[i]) * (dt * dt);
@@ -193,7 +209,9 @@ __global__ __launch_bounds__(B) void k_assemble(lv_grid_params g, lv_fields F, l
                                                 int *__restrict__ ecol, double *__restrict__ ev64,
                                                 float *__restrict__ ev32, int *__restrict__ nonsingular, lv_vbc vbc,
                                                 lv_rhs_geom R, int *__restrict__ ocol, double *__restrict__ oev64,
-                                                float *__restrict__ oev32)
+                                                float *__restrict__ oev32, uint8_t *__restrict__ deg8,
+                                                int16_t *__restrict__ ecol16, int16_t *__restrict__ ocol16,
+                                                int *__restrict__ wide)
 {
     // The block's rows own one contiguous span of the CSR copy; it is formed in LDS and stored with
     // coalesced writes after the barrier (each thread's own entries would be 64 scattered stores an
@@ -207,7 +225,8 @@ __global__ __launch_bounds__(B) void k_assemble(lv_grid_params g, lv_fields F, l
     const bool stage = clen <= ACSR;
     if (i < n) {  // rows past n hold no slots: their lanes stay out of the group's ballots below
         assemble_row(g, F, M, n, dt, mode, i, row_ptr, stage, cbase, s_col, s_v64, s_v32, cols, val64, val32,
-                     diag64, diag32, dinv64, dinv32, ecol, ev64, ev32, nonsingular, vbc, R, ocol, oev64, oev32);
+                     diag64, diag32, dinv64, dinv32, ecol, ev64, ev32, nonsingular, vbc, R, ocol, oev64, oev32,
+                     deg8, ecol16, ocol16, wide);
     }
     if (stage) {
         __syncthreads();
@@ -834,8 +853,8 @@ __device__ __forceinline__ double spmv_row_shifted(int i, int n, const lv_ell_ow
 // true residual r = (b - bmean) - A x; partials |r|^2, |b'|^2 (FP64).  With Pxs (the partial sums
 // of x written by k_correct, singular systems) the residual is that of x - d, d = mean(x) - xmean0,
 // the x that k_shift stores at the end of the solve; block 0 records d.
-template <bool COMPACT>
-__global__ __launch_bounds__(B) void k_resid64(int n, lv_ell_own<double> A, const double *__restrict__ diag,
+template <bool COMPACT, class V = lv_ell_own<double>>
+__global__ __launch_bounds__(B) void k_resid64(int n, V A, const double *__restrict__ diag,
                                                const double *__restrict__ b, const double *__restrict__ x,
                                                double *__restrict__ r, double *__restrict__ partials,
                                                const double *__restrict__ Pxs, lv_cg_scalars *S)
@@ -862,8 +881,8 @@ __global__ __launch_bounds__(B) void k_resid64(int n, lv_ell_own<double> A, cons
 }
 
 // y = A x (FP64, the operator's own ELL)
-template <bool COMPACT>
-__global__ __launch_bounds__(B) void k_spmv64(int n, lv_ell_own<double> A, const double *__restrict__ diag,
+template <bool COMPACT, class V = lv_ell_own<double>>
+__global__ __launch_bounds__(B) void k_spmv64(int n, V A, const double *__restrict__ diag,
                                               const double *__restrict__ x, double *__restrict__ y)
 {
     LV_OVF_PASSES(A, n, blockIdx.x * B + threadIdx.x, gridDim.x * B,
@@ -1024,14 +1043,14 @@ hipError_t lv_launch_degree(lv_mesh_view M, int n, int *deg, hipStream_t s)
 }
 
 hipError_t lv_launch_assemble(const lv_grid_params &g, lv_fields F, lv_mesh_view M, int n, double dt, int mode,
-                              lv_pressure_system &A, int *nonsingular, hipStream_t s, const lv_vbc *vbc,
+                              lv_pressure_system &A, int *nonsingular, int *wide, hipStream_t s, const lv_vbc *vbc,
                               const lv_rhs_geom *R)
 {
     const lv_vbc v = vbc ? *vbc : lv_vbc{};
     const lv_rhs_geom r = (R && mode == 0) ? *R : lv_rhs_geom{};
     hipLaunchKernelGGL(k_assemble, grid_of(n), dim3(B), 0, s, g, F, M, n, dt, mode, (const int *)A.row_ptr, A.cols,
                        A.val64, A.val32, A.diag64, A.diag32, A.dinv64, A.dinv32, A.ecol, A.ev64, A.ev32, nonsingular,
-                       v, r, A.ocol, A.oev64, A.oev32);
+                       v, r, A.ocol, A.oev64, A.oev32, A.deg8, A.ecol16, A.ocol16, wide);
     return hipGetLastError();
 }
 
@@ -1151,6 +1170,16 @@ static lv_ell_own<double> own64(const lv_pressure_system &A, int n, int layout =
     v.ocap = own_ocap(A, n, layout);
     return v;
 }
+// the FP64 operator through its 16-bit offsets and uint8 degrees (valid when A.narrow)
+using lv_ell_own64s = lv_ell_view<uint8_t, int16_t, double>;
+static lv_ell_own64s own64s(const lv_pressure_system &A, int n, int layout = 0)
+{
+    lv_ell_own64s v{A.deg8, A.ecol16, A.ev64};
+    v.ocol = A.ocol16;
+    v.oval = A.oev64;
+    v.ocap = own_ocap(A, n, layout);
+    return v;
+}
 static lv_ell_own<float> own32(const lv_pressure_system &A, int n, int layout = 0)
 {
     lv_ell_own<float> v{A.deg, A.ecol, A.ev32};
@@ -1168,6 +1197,34 @@ static lv_ell_own<float> own32(const lv_pressure_system &A, int n, int layout = 
         else                                               \
             hipLaunchKernelGGL((kern<false>), __VA_ARGS__); \
     } while (0)
+
+// r = b - A x in FP64 (k_resid64) and y = A x (k_spmv64), through the 16-bit copy of the operator's
+// columns when every offset fits (2 bytes a slot and 1 a row instead of 4 and 4)
+template <class V>
+static void launch_resid64_v(const V &Av, dim3 G, hipStream_t s, int n, const double *diag, const double *b,
+                             const double *x, double *r, double *p3, const double *Pxs, lv_cg_scalars *S)
+{
+    if (Av.ocap > 0)
+        hipLaunchKernelGGL((k_resid64<true, V>), G, dim3(B), 0, s, n, Av, diag, b, x, r, p3, Pxs, S);
+    else
+        hipLaunchKernelGGL((k_resid64<false, V>), G, dim3(B), 0, s, n, Av, diag, b, x, r, p3, Pxs, S);
+}
+static void launch_resid64(const lv_pressure_system &A, int layout, dim3 G, hipStream_t s, int n, const double *b,
+                           const double *x, double *r, double *p3, const double *Pxs, lv_cg_scalars *S)
+{
+    if (A.narrow)
+        launch_resid64_v(own64s(A, n, layout), G, s, n, (const double *)A.diag64, b, x, r, p3, Pxs, S);
+    else
+        launch_resid64_v(own64(A, n, layout), G, s, n, (const double *)A.diag64, b, x, r, p3, Pxs, S);
+}
+template <class V>
+static void launch_spmv64_v(const V &Av, dim3 G, hipStream_t s, int n, const double *diag, const double *x, double *y)
+{
+    if (Av.ocap > 0)
+        hipLaunchKernelGGL((k_spmv64<true, V>), G, dim3(B), 0, s, n, Av, diag, x, y);
+    else
+        hipLaunchKernelGGL((k_spmv64<false, V>), G, dim3(B), 0, s, n, Av, diag, x, y);
+}
 
 // partial-sum regions: P3 = one-off reductions (3 values), Ppq (1), Prr (2)
 struct parts {
@@ -1313,8 +1370,8 @@ hipError_t lv_solve_device(const lv_pressure_system &A, lv_solver_work &W, int n
         bool pending = false;
         double itol = 0.0;
         for (;;) {
-            LV_LAUNCH_BY_OCAP(k_resid64, own64(A, n, opt.amg_layout), G, dim3(B), 0, s, n, own64(A, n, opt.amg_layout), (const double *)A.diag64, b, (const double *)x, W.r, P.p3,
-                               nsweep > 0 ? (const double *)Pxs : (const double *)nullptr, W.S);
+            launch_resid64(A, opt.amg_layout, G, s, n, b, (const double *)x, W.r, P.p3,
+                           nsweep > 0 ? (const double *)Pxs : (const double *)nullptr, W.S);
             if (nsweep == 0) {
                 hipLaunchKernelGGL(k_finalize, dim3(1), dim3(B), 0, s, (const double *)P.p3, nb, (int)FIN_RESID0,
                                    n, floor_tol, W.S);
@@ -1400,8 +1457,7 @@ hipError_t lv_solve_device(const lv_pressure_system &A, lv_solver_work &W, int n
                                W.S);
             hipLaunchKernelGGL(k_shift, G, dim3(B), 0, s, n, x, (const lv_cg_scalars *)W.S);
         }
-        LV_LAUNCH_BY_OCAP(k_resid64, own64(A, n, opt.amg_layout), G, dim3(B), 0, s, n, own64(A, n, opt.amg_layout), (const double *)A.diag64, b, (const double *)x, W.r, P.p3,
-                           (const double *)nullptr, W.S);
+        launch_resid64(A, opt.amg_layout, G, s, n, b, (const double *)x, W.r, P.p3, (const double *)nullptr, W.S);
         hipLaunchKernelGGL(k_finalize, dim3(1), dim3(B), 0, s, (const double *)P.p3, nb, (int)FIN_RESID, n, 0.0,
                            W.S);
         e = read_scalars(W.S, W.hS, &h, s);
@@ -1549,7 +1605,10 @@ hipError_t lv_combine(int n, float *y, double alpha, const double *const *x, con
 
 hipError_t lv_spmv64(const lv_pressure_system &A, int n, const double *x, double *y, hipStream_t s)
 {
-    LV_LAUNCH_BY_OCAP(k_spmv64, own64(A, n), dim3(nblocks_of(n)), dim3(B), 0, s, n, own64(A, n), (const double *)A.diag64, x, y);
+    if (A.narrow)
+        launch_spmv64_v(own64s(A, n), dim3(nblocks_of(n)), s, n, (const double *)A.diag64, x, y);
+    else
+        launch_spmv64_v(own64(A, n), dim3(nblocks_of(n)), s, n, (const double *)A.diag64, x, y);
     return hipGetLastError();
 }
 

@@ -209,6 +209,54 @@ def test_own_order_compact_overflow_is_bit_identical(lv, oracle, gpu, method):
     assert out[0][1] == out[2][1] and out[0][2] == out[2][2]
 
 
+def _wrapped_offsets(rp, cols, n):
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    off = cols.astype(np.int64) - rows
+    off = np.where(off > n // 2, off - n, off)
+    return np.where(off < -(n // 2), off + n, off)
+
+
+def test_fp64_passes_with_column_offsets_wider_than_16_bits(lv, gpu):
+    """The FP64 residual and products read the operator's columns as 16-bit offsets j - i (modulo
+    n) when every one fits, and the 32-bit columns otherwise.  The same 300^2 particles (90000 rows)
+    in lattice order (every offset fits) and in a random order (most do not): each solve meets its
+    tolerance on the residual recomputed on the host from the exported operator, and the two give
+    the same pressure to 1e3 tol."""
+    _flush("fp64 passes, 16- and 32-bit column offsets")
+    K, dt = 300, 2e-3
+    dr = 1.0 / K
+    x = lv.inputs.jittered_lattice(K, K, dr, seed=0x5EED0508)
+    n = K * K
+    perm = np.random.default_rng(8).permutation(n)
+    orders = {"lattice": x, "random": np.concatenate([x[:n][perm], x[n:][perm]])}
+    P_of = {}
+    for name, xx in orders.items():
+        grid = lv.VoronoiGrid(lv.UnitRectangle(), dr, xperiodic=True, yperiodic=True, positions=xx)
+        lv.remesh(grid)
+        ic = lv.inputs.taylor_green(xx, grid.get("area"))
+        for k in ("v", "rho", "mass", "P", "e", "mu"):
+            grid.set_soa(k, ic[k])
+        lv.stiffened_eos(grid, ic["gamma"], ic["P0"])
+        solver = lv.PressureSolver(grid, method="amg", tol=1e-10, itmax=4000)
+        solver.assemble(dt)
+        d, rp, cols, coef = solver.operator()
+        wide = np.abs(_wrapped_offsets(rp, cols, n)).max() > 32767
+        assert wide == (name == "random")
+        b, _ = solver.rhs(dt, False)
+        P = solver.solve(b, grid.get("P"), method="amg")
+        st = solver.last_stats
+        assert max(st["relres"]) <= 1e-10 and st["fallbacks"] == 0, st
+        A = sp.csr_matrix((coef, cols, rp), shape=(n, n))
+        AP = d * P + np.asarray(A.sum(axis=1)).ravel() * P - A @ P
+        assert np.linalg.norm(b - AP) <= 1.05e-10 * np.linalg.norm(b)
+        if name == "random":
+            Pl = np.empty(n)
+            Pl[perm] = P
+            P = Pl
+        P_of[name] = P
+    assert rel_err(P_of["random"], P_of["lattice"]) <= 1e-7
+
+
 def test_cycle_matches_oracle_learned_smoother(lv, oracle, gpu):
     _flush("amg cycle learned")
     grid, op, solver, x, dr, d, rp, cols, coef = _system(lv, oracle, "periodic", 96, "tg", 8e-3)

@@ -634,7 +634,7 @@ int lv_remesh(lv_ctx *ctx)
         hipError_t e = lv_build_cell_list(ctx->g, n, ctx->F.x, ctx->cell_start, ctx->cell_labels, ctx->cell_x,
                                           ctx->max_items, ctx->d_nitems, ctx->cl_temp, ctx->s,
                                           tc ? tc->ev[2 * tc->used] : nullptr, tc ? tc->ev[2 * tc->used + 1] : nullptr,
-                                          tc != nullptr);
+                                          tc != nullptr, &ctx->nitems);
         if (e == hipErrorInvalidValue && attempt == 0) {
             // more periodic images than buffered: grow to the count the scan produced and rebuild
             int m = 0;
@@ -657,7 +657,6 @@ int lv_remesh(lv_ctx *ctx)
         if (tc) tc->used++;
         break;
     }
-    LV_HIP_CHECK(hipMemcpyAsync(&ctx->nitems, ctx->d_nitems, sizeof(int), hipMemcpyDeviceToHost, ctx->s));
     lv_timer *tk = timer_if_on(ctx, 2);
     if (tk) lv_timer_begin(tk, ctx->s);
     LV_HIP_CHECK(lv_launch_clip(ctx->g, n, ctx->F.x, ctx->cell_start, ctx->cell_labels, ctx->cell_x, ctx->path_i,

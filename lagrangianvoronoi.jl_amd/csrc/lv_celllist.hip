@@ -208,10 +208,11 @@ size_t lv_celllist_temp_bytes(int64_t max_items, int64_t ncells)
 }
 
 // Builds the cell list of n seeds.  d_nitems receives the item count (host reads it to size the
-// sort).  Returns hipErrorInvalidValue if the items exceed max_items.
+// sort, and returns it in *h_nitems when given).  Returns hipErrorInvalidValue if the items exceed
+// max_items.
 hipError_t lv_build_cell_list(const lv_grid_params &g, int n, const double *x, int *cell_start, int *cell_labels,
                               double *cell_x, int64_t max_items, int *d_nitems, void *temp, hipStream_t s,
-                              hipEvent_t ev0, hipEvent_t ev1, bool timed)
+                              hipEvent_t ev0, hipEvent_t ev1, bool timed, int *h_nitems)
 {
     const int64_t ncells = (int64_t)g.n1 * g.n2;
     char *t = (char *)temp;
@@ -232,6 +233,7 @@ hipError_t lv_build_cell_list(const lv_grid_params &g, int n, const double *x, i
     void *scan_tmp = t;
 
     if (n <= 0) {
+        if (h_nitems) *h_nitems = 0;
         hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((ncells + 1 + 255) / 256)), dim3(256), 0, s, cell_start,
                            ncells + 1, 0);
         return hipGetLastError();
@@ -244,6 +246,7 @@ hipError_t lv_build_cell_list(const lv_grid_params &g, int n, const double *x, i
     if (e != hipSuccess) return e;
     e = hipStreamSynchronize(s);
     if (e != hipSuccess) return e;
+    if (h_nitems) *h_nitems = m;
     if (m > max_items || m <= 0) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_img_emit, dim3((n + 255) / 256), dim3(256), 0, s, g, n, x, (const int *)cnt, kA, vA,
                        (int64_t)max_items);

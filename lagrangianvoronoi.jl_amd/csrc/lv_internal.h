@@ -129,7 +129,7 @@ hipError_t lv_exclusive_scan_i32(const int *in, int *out, int64_t n, int *d_tota
 size_t lv_celllist_temp_bytes(int64_t max_items, int64_t ncells);
 hipError_t lv_build_cell_list(const lv_grid_params &g, int n, const double *x, int *cell_start, int *cell_labels,
                               double *cell_x, int64_t max_items, int *d_nitems, void *temp, hipStream_t s,
-                              hipEvent_t ev0, hipEvent_t ev1, bool timed);
+                              hipEvent_t ev0, hipEvent_t ev1, bool timed, int *h_nitems = nullptr);
 
 // lv_clip.hip.  retry: n + 1 ints of work space (the seeds whose polygon needs the full capacity).
 hipError_t lv_launch_clip(const lv_grid_params &g, int n, const double *x, const int *cell_start,

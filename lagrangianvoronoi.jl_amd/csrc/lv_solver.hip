@@ -1338,8 +1338,13 @@ hipError_t lv_solve_device(const lv_pressure_system &A, lv_solver_work &W, int n
     hipLaunchKernelGGL(k_init_scalars, dim3(1), dim3(1), 0, s, W.S, opt.tol * opt.tol,
                        opt.itmax > 0 ? opt.itmax : 1000, (int)A.singular);
     hipError_t e = hipSuccess;
-    hipLaunchKernelGGL(k_means, G, dim3(B), 0, s, n, b, (const double *)x, P.p3);
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(B), 0, s, (const double *)P.p3, nb, (int)FIN_MEANS, n, 0.0, W.S);
+    // the means of b and x serve singular systems only (bmean stays the 0 k_init_scalars set, and xmean0
+    // is read only where x's mean is restored)
+    if (A.singular) {
+        hipLaunchKernelGGL(k_means, G, dim3(B), 0, s, n, b, (const double *)x, P.p3);
+        hipLaunchKernelGGL(k_finalize, dim3(1), dim3(B), 0, s, (const double *)P.p3, nb, (int)FIN_MEANS, n, 0.0,
+                           W.S);
+    }
     int total = 0, nsweep = 0, skipped = 0;
     if (opt.method == LV_SOLVER_PCG64) {
         LV_LAUNCH_BY_OCAP(k_cg64_init, own64(A, n, opt.amg_layout), G, dim3(B), 0, s, n, own64(A, n, opt.amg_layout), (const double *)A.diag64, (const double *)A.dinv64, b,

@@ -106,27 +106,57 @@ __device__ __forceinline__ int wrap(int i, int K, int per, bool &ok)
 
 // ---- level 1: aggregation of the particles -----------------------------------------------------
 
+// ctr[key] += 1 for every active lane, by runs: lanes of a wavefront with equal keys in consecutive
+// lanes (the active lanes being a prefix) form a run, whose first lane adds the run's length in one
+// atomic; each lane gets the old value plus its rank in the run, its own slot among the key's.
+// Consecutive rows mostly share a level-1 cell, so this halves the atomics, which bound both
+// kernels below.  Every lane of the wavefront calls it.
+__device__ __forceinline__ int run_atomic_add(int *ctr, int key, bool active)
+{
+    const int lane = threadIdx.x & 63;
+    const int prev = __shfl_up(key, 1);
+    const unsigned long long am = __ballot(active);
+    const bool pact = lane > 0 && ((am >> (lane - 1)) & 1ull);
+    const bool head = active && (!pact || prev != key);
+    const unsigned long long heads = __ballot(head);
+    const unsigned long long upto = (2ull << lane) - 1ull;  // lanes 0..lane (every lane at lane 63)
+    const int hb = 63 - __clzll((long long)(heads & upto));
+    const int h = active ? hb : lane;
+    const unsigned long long above = heads & ~upto;
+    const int nexth = above ? __ffsll((long long)above) - 1 : 64;
+    const int nact = 64 - __clzll((long long)am);
+    const int runlen = (nexth < nact ? nexth : nact) - lane;
+    int old = 0;
+    if (head) old = atomicAdd(&ctr[key], runlen);
+    return __shfl(old, h) + (lane - h);
+}
+
 __global__ __launch_bounds__(B) void k_agg_key(int n, const double *__restrict__ x, double x0, double y0, double hx,
                                                double hy, int Kx, int Ky, int *__restrict__ agg,
                                                int *__restrict__ cnt)
 {
     const int i = blockIdx.x * B + threadIdx.x;
-    if (i >= n) return;
-    int ix = (int)floor((x[i] - x0) / hx), iy = (int)floor((x[n + i] - y0) / hy);
-    ix = ix < 0 ? 0 : (ix >= Kx ? Kx - 1 : ix);
-    iy = iy < 0 ? 0 : (iy >= Ky ? Ky - 1 : iy);
-    const int a = ix * Ky + iy;
-    agg[i] = a;
-    atomicAdd(&cnt[a], 1);
+    const bool act = i < n;
+    int a = 0;
+    if (act) {
+        int ix = (int)floor((x[i] - x0) / hx), iy = (int)floor((x[n + i] - y0) / hy);
+        ix = ix < 0 ? 0 : (ix >= Kx ? Kx - 1 : ix);
+        iy = iy < 0 ? 0 : (iy >= Ky ? Ky - 1 : iy);
+        a = ix * Ky + iy;
+        agg[i] = a;
+    }
+    (void)run_atomic_add(cnt, a, act);
 }
 
+// (the order of a cell's rows in list is then fixed by k_agg_sort)
 __global__ __launch_bounds__(B) void k_agg_fill(int n, const int *__restrict__ agg, const int *__restrict__ start,
                                                 int *__restrict__ cursor, int *__restrict__ list)
 {
     const int i = blockIdx.x * B + threadIdx.x;
-    if (i >= n) return;
-    const int a = agg[i];
-    list[start[a] + atomicAdd(&cursor[a], 1)] = i;
+    const bool act = i < n;
+    const int a = act ? agg[i] : 0;
+    const int p = run_atomic_add(cursor, a, act);
+    if (act) list[start[a] + p] = i;
 }
 
 // per cell: its rows sorted (a fixed order for every sum over a cell's rows, here and in the

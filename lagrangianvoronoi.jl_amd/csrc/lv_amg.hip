@@ -1435,30 +1435,36 @@ __global__ __launch_bounds__(B) void k_perm_ell(int n, const int *__restrict__ l
                                                 int16_t *__restrict__ ooff, float *__restrict__ oev32, int ocap)
 {
     const int p = blockIdx.x * B + threadIdx.x;
-    const int i = p < n ? list[p] : 0;
-    const int d = p < n ? deg[i] : 0;
+    // everything of row i loaded as soon as i is known, unconditionally (a lane past n takes row
+    // list[n-1] and drops it): the degree, the diagonals and the first U slots' columns and
+    // couplings; then their rows' new positions gathered together (a slot past the degree reads
+    // the allocated slot, and its column is taken as the row's own so that the gather stays in
+    // bounds; neither is used)
+    const int i = list[p < n ? p : n - 1];
+    constexpr int U = 8;
+    const int di = deg[i];
+    const float dgi = diag[i], dvi = dinv[i];
+    int cj[U];
+    float vj[U];
+#pragma unroll
+    for (int k = 0; k < U; k++) {
+        const size_t e = (size_t)k * n + i;
+        cj[k] = ecol[e];
+        vj[k] = ev[e];
+    }
+    const int d = p < n ? di : 0;
     const lv_ovf_pos q = lv_ovf_prefix(d);  // the block's wavefronts are whole 64-row groups
     if (p >= n) return;
     const bool compact = ocap > 0 && q.tot <= ocap;
     const size_t g0 = (size_t)(p >> 6) * LV_OVF_CAP;
     pdeg[p] = (uint8_t)d;
-    pdiag[p] = diag[i];
-    pdinv[p] = dinv[i];
+    pdiag[p] = dgi;
+    pdinv[p] = dvi;
     if (mdg) pmdg[p] = mdg[i];
     bool wide = false;
-    // the first U slots' columns and couplings loaded together, then their rows' new positions
-    // gathered together (a slot past the degree reads the allocated slot, and its column is taken
-    // as the row's own so that the gather stays in bounds; neither is used)
-    constexpr int U = 8;
     int pj[U];
-    float vj[U];
 #pragma unroll
-    for (int k = 0; k < U; k++) {
-        const size_t e = (size_t)k * n + i;
-        const int c = ecol[e];
-        vj[k] = ev[e];
-        pj[k] = k < d ? c : i;
-    }
+    for (int k = 0; k < U; k++) pj[k] = k < d ? cj[k] : i;
 #pragma unroll
     for (int k = 0; k < U; k++) pj[k] = iperm[pj[k]];
     auto put = [&](int k, int col_new, float v) {
